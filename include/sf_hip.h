@@ -1,0 +1,159 @@
+/*
+ * sf_hip.h -- C ABI of libsfhip, the gfx950 (MI355X) implementation of the
+ * StokesFlow.jl per-timestep hot path: pseudo-transient Stokes solve, marker -> grid
+ * interpolation and RK4 marker advection, on one GPU.
+ *
+ * Conventions
+ *   - Every array pointer is a DEVICE pointer to float64 data in column-major order with
+ *     the first (x) index fastest -- the memory layout of the reference's Julia
+ *     Data.Array / ROCArray.  A(i, j) = A[i + j * size1] with 0-based (i, j).
+ *   - Node counts Nx, Ny and array shapes follow StokesFlow2D_multixpu.jl:89-114:
+ *       P, mu_p, txx, tyy, divV, dtauP : (Nx-1, Ny-1)    txy, mu_b : (Nx, Ny)
+ *       Vx_s : (Nx, Ny+1)   Vx_pad : (Nx+2, Ny+1)        rho_vy : (Nx+1, Ny)
+ *       Vy_s : (Nx+1, Ny)   Vy_pad : (Nx+1, Ny+2)
+ *       Rx, dVxdtau, dtauVx : (Nx-2, Ny-1)    Ry, dVydtau, dtauVy : (Nx-1, Ny-2)
+ *   - Work is enqueued on the context's HIP stream; calls that return results on the
+ *     host (solve outputs, counters) synchronise that stream before returning.
+ *   - Every function returns SF_OK (0) or a negative status; sf_last_error() returns a
+ *     message for the calling thread's last failure.  One context per stream/thread; a
+ *     context is not re-entrant.
+ *   - Data problems the reference only prints about ("erroneous marker position",
+ *     "no markers close enough", the velocity-interpolation failsafe clamps) come back as
+ *     counters.  Inputs the reference would throw on (floor(Int, NaN)) return
+ *     SF_ERR_NONFINITE.
+ */
+#ifndef SF_HIP_H
+#define SF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SF_OK 0
+#define SF_ERR_ARG (-1)       /* invalid argument (shape, NULL pointer, size) */
+#define SF_ERR_HIP (-2)       /* HIP runtime error */
+#define SF_ERR_NONFINITE (-3) /* non-finite / unrepresentable marker coordinate */
+#define SF_ERR_SIZE (-4)      /* problem larger than the 32-bit index limits */
+
+typedef struct sf_ctx sf_ctx;
+
+typedef struct {
+    double dt_new;      /* compute_dt(Vx_s, Vy_s) after the solve: 1st return value */
+    double err;         /* last err (max of mean |Rx|, |Ry|, |divV|) */
+    double t_eff;       /* reference T_eff = 6*Nx*Ny*8 B / t_it, GB/s: 2nd return value */
+    double runtime;     /* seconds from PT iteration 11 to the end: 3rd return value */
+    int64_t niter;      /* PT iterations performed */
+    int32_t converged;  /* err <= eps at exit */
+    int32_t reserved;
+    double dt_last;     /* stabilisation dt in effect during the last iteration */
+    double t_it;        /* seconds per PT iteration over the timed window */
+    double gbs_alg;     /* algorithmic GB/s = 13-array bytes per iteration / t_it */
+} sf_solve_out;
+
+typedef struct {
+    int64_t n_bad_markers; /* markers outside the target grid (skipped) */
+    int64_t n_empty_nodes; /* nodes with zero weight (value left unchanged) */
+} sf_m2g_out;
+
+typedef struct {
+    int64_t n_clamped; /* failsafe index clamps taken in interpolateV */
+} sf_rk4_out;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int sf_version(void);
+/* Message describing the calling thread's last failed call. */
+const char *sf_last_error(void);
+
+/* Context: owns device scratch for an Nx x Ny node grid and the HIP stream to use
+ * (hip_stream may be NULL for the default stream).  Scratch for markers grows on demand.
+ * nx = ny = 0 creates a marker-only context (marker -> grid, RK4, set-up; no solve). */
+int sf_ctx_create(sf_ctx **ctx, int nx, int ny, void *hip_stream);
+int sf_ctx_destroy(sf_ctx *ctx);
+/* Replace the stream work is enqueued on. */
+int sf_ctx_set_stream(sf_ctx *ctx, void *hip_stream);
+
+/* solveStokes! -- replaces scripts/StokesSolver_multixpu.jl:14-96
+ * (solveStokes!(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtauPt, Rx, Ry,
+ *  dVxdtau, dVydtau, dtauVx, dtauVy, Vx_s, Vy_s, g_y, dx, dy, Nx, Ny, dt, maxdisp, comm;
+ *  use_free_surface_stabilization, eps, print_info, iterMax) -> (dt, T_eff, runtime)).
+ * In/out: P, Vx_s, Vy_s, dVxdtau, dVydtau (state), Vx_pad, Vy_pad (output).
+ * Scratch arguments txx, tyy, txy, divV, dtauP, Rx, Ry, dtauVx, dtauVy may be NULL; when
+ * given they are left holding what the reference leaves in them.  comm is absent: one
+ * rank.  The caller's rho_vy, mu_b, mu_p are read only. */
+int sf_solve_stokes(sf_ctx *ctx, double *P, double *Vx_pad, double *Vy_pad, const double *rho_vy,
+                    const double *mu_b, const double *mu_p, double *txx, double *tyy, double *txy,
+                    double *divV, double *dtauP, double *Rx, double *Ry, double *dVxdtau, double *dVydtau,
+                    double *dtauVx, double *dtauVy, double *Vx_s, double *Vy_s, double g_y, double dx,
+                    double dy, int nx, int ny, double dt, double maxdisp, int use_fss, double eps,
+                    int64_t iter_max, int print_info, sf_solve_out *out);
+
+/* n_iter iterations of the PT loop body (StokesSolver_multixpu.jl:48-58) at a fixed
+ * stabilisation dt, with no dt refresh and no convergence check.  Building block for
+ * fixed-iteration benchmarks and for bit-exact per-iteration parity. */
+int sf_pt_iterations(sf_ctx *ctx, int64_t n_iter, double *P, double *Vx_s, double *Vy_s, double *dVxdtau,
+                     double *dVydtau, const double *rho_vy, const double *mu_b, const double *mu_p,
+                     double g_y, double dx, double dy, int nx, int ny, double dt);
+
+/* bilinearMarkerToGrid! -- replaces scripts/MarkerToGrid.jl:18-38
+ * (bilinearMarkerToGrid!(x_grid_min, y_grid_min, val_grid, x_m, y_m, val_m, dx, dy,
+ *  val_wt_sum, wt_sum, grid)).  Deterministic: bit-identical to the reference's sequential
+ * CPU path (:110-155).  val_wt_sum / wt_sum (shape ws_nx x ws_ny >= gnx x gny) may be
+ * NULL; when given they are zeroed and hold the weight sums like the reference's helpers.
+ * Marker arrays are read only and keep their order. */
+int sf_marker_to_grid(sf_ctx *ctx, double x_grid_min, double y_grid_min, double *val_grid, int gnx, int gny,
+                      const double *x_m, const double *y_m, const double *val_m, int64_t nm, double dx,
+                      double dy, double *val_wt_sum, double *wt_sum, int ws_nx, int ws_ny, sf_m2g_out *out);
+
+/* moveMarkersRK4! -- replaces scripts/MoveMarkers.jl:15-23
+ * (moveMarkersRK4!(x_m, y_m, Vx, Vy, x_vx_min, y_vx_min, x_vy_min, y_vy_min, dt, lx, ly,
+ *  dx, dy)).  Vx is (vx_n1, vx_n2) = size(Vx), normally the padded (Nx+2, Ny+1) array;
+ * Vy likewise (Nx+1, Ny+2).  x_m / y_m are updated in place, order preserved. */
+int sf_move_markers_rk4(sf_ctx *ctx, double *x_m, double *y_m, int64_t nm, const double *Vx, int vx_n1,
+                        int vx_n2, const double *Vy, int vy_n1, int vy_n2, double x_vx_min, double y_vx_min,
+                        double x_vy_min, double y_vy_min, double dt, double lx, double ly, double dx,
+                        double dy, sf_rk4_out *out);
+
+/* One timestep of StokesFlow2D at one rank -- replaces the loop body
+ * scripts/StokesFlow2D_multixpu.jl:164-181 (three bilinearMarkerToGrid! calls on the
+ * rho_vy, mu_b and mu_p grids, solveStokes!, moveMarkersRK4!; exchangeMarkers is the
+ * identity at one rank).  dt on input is the previous step's dt (0 on the first step)
+ * and on output the new one, as the reference's loop variable. */
+typedef struct {
+    int nx, ny;
+    double dx, dy;
+    double g_y, maxdisp, eps;
+    int64_t iter_max;
+    int use_fss;
+    double dt;
+    double *P, *Vx_pad, *Vy_pad, *rho_vy, *mu_b, *mu_p, *dVxdtau, *dVydtau, *Vx_s, *Vy_s;
+    double *x_m, *y_m, *rho_m, *mu_m;
+    int64_t nm;
+} sf_step_args;
+
+int sf_step(sf_ctx *ctx, sf_step_args *args, sf_solve_out *solve_out, sf_m2g_out m2g_out[3],
+            sf_rk4_out *rk4_out);
+
+/* Seeded marker set-up: md x md markers per cell, x outer / y inner (the order of
+ * common.jl:76-83), uniform jitter of +-jitter_frac cells from splitmix64(seed, index),
+ * clamped to the domain.  x_m, y_m hold md^2 (nx-1)(ny-1) values.  No reference
+ * counterpart (the reference seeds Julia's RNG, common.jl:85-87). */
+int sf_init_markers(sf_ctx *ctx, double *x_m, double *y_m, int nx, int ny, int md, double dx, double dy,
+                    double jitter_frac, uint64_t seed);
+
+/* Material descriptors replacing the reference's density/viscosity callbacks
+ * (setInitialMarkerProperties!, common.jl:104-110).
+ * plume: mat = {air_h, cx, cy, r, rho_air, rho_plume, rho_bg, mu_air, mu_plume, mu_bg}:
+ *   y < air_h -> air, else (x-cx)^2+(y-cy)^2 < r^2 -> plume, else background.
+ * rt:    mat = {y0, amp, k, lx, rho_up, mu_up, rho_low, mu_low}: y < y0 + amp sin(2 pi k x/lx)
+ *   -> upper layer, else lower layer. */
+int sf_plume_properties(sf_ctx *ctx, const double *x_m, const double *y_m, int64_t nm, const double mat[10],
+                        double *rho_m, double *mu_m);
+int sf_rt_properties(sf_ctx *ctx, const double *x_m, const double *y_m, int64_t nm, const double mat[8],
+                     double *rho_m, double *mu_m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SF_HIP_H */

@@ -1,0 +1,640 @@
+/*
+ * sf_oracle.c -- CPU restatement of the StokesFlow.jl FP64 hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker that the HIP product is compared against.
+ * It is loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * and by nothing else.  It is deliberately written like the reference (separate
+ * unfused passes over whole arrays, in the reference's call order) so that it is
+ * an independent check of the fused device kernels.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fopenmp (no -ffast-math).  The reference is
+ * Julia, which never contracts a*b+c into an FMA without @fastmath/muladd, and
+ * every expression below keeps the reference's left-to-right evaluation order.
+ *
+ * Pinning (how each function is checked beyond retyping its formula):
+ *   solve (compute_timesteps .. compute_dt, sfo_solve_stokes): golden vectors of
+ *       test/test_StokesSolver.jl:49-51 plus the BC identities :116,:128-140;
+ *       hydrostatic invariant (uniform rho -> V = 0, dP/dy = rho g) and mirror
+ *       symmetry in tests/test_oracle_solver.py.
+ *   marker_to_grid: known answer test/test_MarkerToGrid.jl:36; brute-force direct
+ *       sum; constant field reproduced exactly.
+ *   interpolate_v / move_markers_rk4: closed forms of test/test_MoveMarkers.jl
+ *       :43-47 and :54-152 (all four quadrants); a linear field is reproduced.
+ *   full timestep loop: golden vectors test/test_StokesFlow2D.jl:86-88.
+ *   lattice_markers: test/test_InitialMarkerCoords.jl:101-131 lattice identity.
+ *   mean_abs: exact on integer-valued data; Julia's SIMD lane order inside a
+ *       1024 block is not reproduced (ulp level; documented).
+ */
+#include "sf_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#define IX(i, j, n1) ((int64_t)(i) + (int64_t)(j) * (int64_t)(n1))
+
+static double wall_seconds(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* Julia's Base.min / Base.max for Float64 (NaN-propagating). */
+static double jl_min(double a, double b)
+{
+    if (isnan(a) || isnan(b)) return NAN;
+    if (a == b) return signbit(a) ? a : b;
+    return a < b ? a : b;
+}
+
+static double jl_max(double a, double b)
+{
+    if (isnan(a) || isnan(b)) return NAN;
+    if (a == b) return signbit(a) ? b : a;
+    return a > b ? a : b;
+}
+
+/* ---------------------------------------------------------------------------------------- */
+/* StokesSolver_multixpu.jl:128-133.  @av_xa(A) = (A[ix,iy] + A[ix+1,iy]) * 0.5 and
+ * @av_ya(A) = (A[ix,iy] + A[ix,iy+1]) * 0.5 (ParallelStencil FiniteDifferences2D).        */
+void sfo_compute_timesteps(int nx, int ny, const double *mu_p, double *dtauVx, double *dtauVy,
+                           double *dtauP, double Vsc, double Ptsc, double min_dxy2, int max_nxy)
+{
+    const int np1 = nx - 1;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny - 1; ++j) {
+        for (int i = 0; i < nx - 2; ++i) {
+            double av = (mu_p[IX(i, j, np1)] + mu_p[IX(i + 1, j, np1)]) * 0.5;
+            dtauVx[IX(i, j, nx - 2)] = ((Vsc * min_dxy2) / av) / 4.1;
+        }
+        if (j < ny - 2) {
+            for (int i = 0; i < nx - 1; ++i) {
+                double av = (mu_p[IX(i, j, np1)] + mu_p[IX(i, j + 1, np1)]) * 0.5;
+                dtauVy[IX(i, j, nx - 1)] = ((Vsc * min_dxy2) / av) / 4.1;
+            }
+        }
+        for (int i = 0; i < nx - 1; ++i)
+            dtauP[IX(i, j, np1)] = ((Ptsc * 4.1) * mu_p[IX(i, j, np1)]) / (double)max_nxy;
+    }
+}
+
+/* StokesSolver_multixpu.jl:140-169 (launched over (Nx+1, Ny+1) with guards size(txy)). */
+void sfo_compute_P_tau(int nx, int ny, double *divV, double *P, const double *Vx, const double *Vy,
+                       const double *dtauP, double *txx, double *tyy, double *txy,
+                       const double *mu_p, const double *mu_b, double _dx, double _dy)
+{
+    const int lvx = nx, lvy = nx + 1, lp = nx - 1;
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < ny; ++b) {
+        for (int a = 0; a < nx; ++a) {
+            double vx0 = Vx[IX(a, b, lvx)];
+            double vx1 = Vx[IX(a, b + 1, lvx)];
+            double vy0 = Vy[IX(a, b, lvy)];
+            double vy1 = Vy[IX(a + 1, b, lvy)];
+            if (a < nx - 1 && b < ny - 1) {
+                double vx2 = Vx[IX(a + 1, b + 1, lvx)];
+                double vy2 = Vy[IX(a + 1, b + 1, lvy)];
+                double dVx_dx = (vx2 - vx1) * _dx;
+                double dVy_dy = (vy2 - vy1) * _dy;
+                double div = dVx_dx + dVy_dy;
+                int64_t p = IX(a, b, lp);
+                divV[p] = div;
+                P[p] = P[p] - dtauP[p] * div;
+                double mu = mu_p[p];
+                txx[p] = (2.0 * mu) * (dVx_dx - (1.0 / 3.0) * div);
+                tyy[p] = (2.0 * mu) * (dVy_dy - (1.0 / 3.0) * div);
+            }
+            double dVx_dy = (vx1 - vx0) * _dy;
+            double dVy_dx = (vy1 - vy0) * _dx;
+            txy[IX(a, b, nx)] = ((2.0 * mu_b[IX(a, b, nx)]) * 0.5) * (dVx_dy + dVy_dx);
+        }
+    }
+}
+
+/* compute_ResX, StokesSolver_multixpu.jl:176-182; (i, j) is the 0-based dVx index. */
+static inline double res_x(int nx, int i, int j, const double *txx, const double *txy,
+                           const double *P, double _dx, double _dy)
+{
+    const int lp = nx - 1;
+    return ((txx[IX(i + 1, j, lp)] - txx[IX(i, j, lp)]) * _dx +
+            (txy[IX(i + 1, j + 1, nx)] - txy[IX(i + 1, j, nx)]) * _dy) -
+           (P[IX(i + 1, j, lp)] - P[IX(i, j, lp)]) * _dx;
+}
+
+/* compute_ResY, StokesSolver_multixpu.jl:189-202; (i, j) is the 0-based dVy index. */
+static inline double res_y(int nx, int i, int j, const double *tyy, const double *txy,
+                           const double *P, const double *rho, const double *Vx, const double *Vy,
+                           double g_y, double _dx, double _dy, double dt)
+{
+    const int lp = nx - 1, lvx = nx, lvy = nx + 1, lr = nx + 1;
+    double av = 0.25 * (((Vx[IX(i, j + 1, lvx)] + Vx[IX(i + 1, j + 1, lvx)]) + Vx[IX(i, j + 2, lvx)]) +
+                        Vx[IX(i + 1, j + 2, lvx)]);
+    double d_xi = rho[IX(i + 2, j + 1, lr)] - rho[IX(i, j + 1, lr)];
+    double d_yi = rho[IX(i + 1, j + 2, lr)] - rho[IX(i + 1, j, lr)];
+    return (((tyy[IX(i, j + 1, lp)] - tyy[IX(i, j, lp)]) * _dy +
+             (txy[IX(i + 1, j + 1, nx)] - txy[IX(i, j + 1, nx)]) * _dx) -
+            (P[IX(i, j + 1, lp)] - P[IX(i, j, lp)]) * _dy) +
+           g_y * (rho[IX(i + 1, j + 1, lr)] -
+                  dt * (((av * d_xi) * 0.5) * _dx + ((Vy[IX(i + 1, j + 1, lvy)] * d_yi) * 0.5) * _dy));
+}
+
+void sfo_compute_dV(int nx, int ny, double *dVx, double *dVy, const double *P, const double *rho,
+                    const double *txx, const double *tyy, const double *txy, const double *Vx,
+                    const double *Vy, double g_y, double dampX, double dampY, double _dx, double _dy,
+                    double dt)
+{
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny - 1; ++j) {
+        for (int i = 0; i < nx - 2; ++i) {
+            double r = res_x(nx, i, j, txx, txy, P, _dx, _dy);
+            int64_t k = IX(i, j, nx - 2);
+            dVx[k] = dampX * dVx[k] + r;
+        }
+        if (j < ny - 2) {
+            for (int i = 0; i < nx - 1; ++i) {
+                double r = res_y(nx, i, j, tyy, txy, P, rho, Vx, Vy, g_y, _dx, _dy, dt);
+                int64_t k = IX(i, j, nx - 1);
+                dVy[k] = dampY * dVy[k] + r;
+            }
+        }
+    }
+}
+
+void sfo_compute_V(int nx, int ny, double *Vx, double *Vy, const double *dVx, const double *dVy,
+                   const double *dtauVx, const double *dtauVy)
+{
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny - 1; ++j) {
+        for (int i = 0; i < nx - 2; ++i) {
+            int64_t k = IX(i, j, nx - 2);
+            int64_t v = IX(i + 1, j + 1, nx);
+            Vx[v] = Vx[v] + dtauVx[k] * dVx[k];
+        }
+        if (j < ny - 2) {
+            for (int i = 0; i < nx - 1; ++i) {
+                int64_t k = IX(i, j, nx - 1);
+                int64_t v = IX(i + 1, j + 1, nx + 1);
+                Vy[v] = Vy[v] + dtauVy[k] * dVy[k];
+            }
+        }
+    }
+}
+
+void sfo_apply_free_slip(int nx, int ny, double *Vx, double *Vy)
+{
+    /* bc_y_noflux!(Vx_s): A[ix,1] = A[ix,2]; A[ix,end] = A[ix,end-1]   (:299-303) */
+    for (int i = 0; i < nx; ++i) {
+        Vx[IX(i, 0, nx)] = Vx[IX(i, 1, nx)];
+        Vx[IX(i, ny, nx)] = Vx[IX(i, ny - 1, nx)];
+    }
+    /* bc_x_zero!(Vx_s): A[1,iy] = A[end,iy] = 0                       (:256-260) */
+    for (int j = 0; j <= ny; ++j) {
+        Vx[IX(0, j, nx)] = 0.0;
+        Vx[IX(nx - 1, j, nx)] = 0.0;
+    }
+    /* bc_y_zero!(Vy_s): A[ix,1] = A[ix,end] = 0                       (:289-293) */
+    for (int i = 0; i <= nx; ++i) {
+        Vy[IX(i, 0, nx + 1)] = 0.0;
+        Vy[IX(i, ny - 1, nx + 1)] = 0.0;
+    }
+    /* bc_x_noflux!(Vy_s): A[1,iy] = A[2,iy]; A[end,iy] = A[end-1,iy]  (:267-271) */
+    for (int j = 0; j < ny; ++j) {
+        Vy[IX(0, j, nx + 1)] = Vy[IX(1, j, nx + 1)];
+        Vy[IX(nx, j, nx + 1)] = Vy[IX(nx - 1, j, nx + 1)];
+    }
+}
+
+void sfo_compute_residuals(int nx, int ny, double *Rx, double *Ry, const double *P, const double *rho,
+                           const double *txx, const double *tyy, const double *txy, const double *Vx,
+                           const double *Vy, double g_y, double _dx, double _dy, double dt)
+{
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny - 1; ++j) {
+        for (int i = 0; i < nx - 2; ++i)
+            Rx[IX(i, j, nx - 2)] = res_x(nx, i, j, txx, txy, P, _dx, _dy);
+        if (j < ny - 2)
+            for (int i = 0; i < nx - 1; ++i)
+                Ry[IX(i, j, nx - 1)] = res_y(nx, i, j, tyy, txy, P, rho, Vx, Vy, g_y, _dx, _dy, dt);
+    }
+}
+
+/* Base.mapreduce_impl structure: pairwise halves down to blocks of < 1024, sequential inside. */
+static double pairwise_abs_sum(const double *a, int64_t first, int64_t last)
+{
+    if (first == last) return fabs(a[first]);
+    if (last - first < 1024) {
+        double v = fabs(a[first]) + fabs(a[first + 1]);
+        for (int64_t i = first + 2; i <= last; ++i) v = v + fabs(a[i]);
+        return v;
+    }
+    int64_t mid = first + ((last - first) >> 1);
+    double v1 = pairwise_abs_sum(a, first, mid);
+    double v2 = pairwise_abs_sum(a, mid + 1, last);
+    return v1 + v2;
+}
+
+double sfo_mean_abs(const double *a, int64_t n)
+{
+    if (n <= 0) return NAN; /* Statistics.mean of an empty array is NaN */
+    return pairwise_abs_sum(a, 0, n - 1) / (double)n;
+}
+
+double sfo_compute_err(int nx, int ny, const double *Rx, const double *Ry, const double *divV)
+{
+    double mRx = sfo_mean_abs(Rx, (int64_t)(nx - 2) * (ny - 1));
+    double mRy = sfo_mean_abs(Ry, (int64_t)(nx - 1) * (ny - 2));
+    double mdV = sfo_mean_abs(divV, (int64_t)(nx - 1) * (ny - 1));
+    return jl_max(jl_max(mRx, mRy), mdV);
+}
+
+static double max_abs(const double *a, int64_t n)
+{
+    double m = -INFINITY;
+    for (int64_t k = 0; k < n; ++k) m = jl_max(m, fabs(a[k]));
+    return m;
+}
+
+double sfo_compute_dt(int nx, int ny, const double *Vx, const double *Vy, double maxdisp,
+                      double dx, double dy)
+{
+    double mx = max_abs(Vx, (int64_t)nx * (ny + 1));
+    double my = max_abs(Vy, (int64_t)(nx + 1) * ny);
+    double min_l = jl_min(dx / mx, dy / my);
+    return maxdisp * min_l;
+}
+
+/* One iteration of the loop body, StokesSolver_multixpu.jl:48-58 (update_halo! is the
+ * identity at one rank; @hide_communication does not change the result). */
+static void pt_iteration(int nx, int ny, double *P, double *Vx, double *Vy, double *dVx,
+                         double *dVy, const double *rho, const double *mu_b, const double *mu_p,
+                         double *txx, double *tyy, double *txy, double *divV, const double *dtauP,
+                         const double *dtauVx, const double *dtauVy, double g_y, double dampX,
+                         double dampY, double _dx, double _dy, double dt)
+{
+    sfo_compute_P_tau(nx, ny, divV, P, Vx, Vy, dtauP, txx, tyy, txy, mu_p, mu_b, _dx, _dy);
+    sfo_compute_dV(nx, ny, dVx, dVy, P, rho, txx, tyy, txy, Vx, Vy, g_y, dampX, dampY, _dx, _dy, dt);
+    sfo_compute_V(nx, ny, Vx, Vy, dVx, dVy, dtauVx, dtauVy);
+    sfo_apply_free_slip(nx, ny, Vx, Vy);
+}
+
+void sfo_pt_iterations(int nx, int ny, int64_t n_iter, double *P, double *Vx_s, double *Vy_s,
+                       double *dVx, double *dVy, const double *rho, const double *mu_b,
+                       const double *mu_p, double *txx, double *tyy, double *txy, double *divV,
+                       const double *dtauP, const double *dtauVx, const double *dtauVy,
+                       double g_y, double dx, double dy, double dt)
+{
+    const double dampX = 1.0 - 4.0 / (double)nx;
+    const double dampY = 1.0 - 4.0 / (double)ny;
+    for (int64_t it = 0; it < n_iter; ++it)
+        pt_iteration(nx, ny, P, Vx_s, Vy_s, dVx, dVy, rho, mu_b, mu_p, txx, tyy, txy, divV, dtauP,
+                     dtauVx, dtauVy, g_y, dampX, dampY, 1.0 / dx, 1.0 / dy, dt);
+}
+
+int sfo_solve_stokes(double *P, double *Vx_pad, double *Vy_pad, const double *rho_vy,
+                     const double *mu_b, const double *mu_p, double *txx, double *tyy, double *txy,
+                     double *divV, double *dtauP, double *Rx, double *Ry, double *dVx, double *dVy,
+                     double *dtauVx, double *dtauVy, double *Vx_s, double *Vy_s, double g_y,
+                     double dx, double dy, int nx, int ny, double dt, double maxdisp, int use_fss,
+                     double eps, int64_t iter_max, sfo_solve_out *out)
+{
+    if (nx < 3 || ny < 3) return -1;
+    /* constants, StokesSolver_multixpu.jl:17-27 (nx_g() = Nx at one rank) */
+    const double Vdmp = 4.0, Vsc = 0.45, Ptsc = 1.0 / 2.0;
+    const double mdxy = jl_min(dx, dy);
+    const double min_dxy2 = mdxy * mdxy;
+    const int max_nxy = nx > ny ? nx : ny;
+    const double dampX = 1.0 - Vdmp / (double)nx;
+    const double dampY = 1.0 - Vdmp / (double)ny;
+    const double _dx = 1.0 / dx, _dy = 1.0 / dy;
+    if (!use_fss) dt = 0.0;
+
+    sfo_compute_timesteps(nx, ny, mu_p, dtauVx, dtauVy, dtauP, Vsc, Ptsc, min_dxy2, max_nxy);
+    const int mn = nx < ny ? nx : ny;
+    const int64_t ncheck = (5 * mn < 2000) ? 5 * mn : 2000;
+    const int64_t ndtupdate = 10 * (int64_t)mn;
+    const int64_t itert1 = 11;
+    double t1 = 0.0;
+    double err = 2.0 * eps;
+    int64_t iter = 1, niter = 0;
+    while (err > eps && iter <= iter_max) {
+        if (iter == itert1) t1 = wall_seconds();
+        pt_iteration(nx, ny, P, Vx_s, Vy_s, dVx, dVy, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtauP,
+                     dtauVx, dtauVy, g_y, dampX, dampY, _dx, _dy, dt);
+        if (use_fss && (iter % ndtupdate == 0)) dt = sfo_compute_dt(nx, ny, Vx_s, Vy_s, maxdisp, dx, dy);
+        if (iter % ncheck == 0) {
+            double dt_check = use_fss ? sfo_compute_dt(nx, ny, Vx_s, Vy_s, maxdisp, dx, dy) : 0.0;
+            sfo_compute_residuals(nx, ny, Rx, Ry, P, rho_vy, txx, tyy, txy, Vx_s, Vy_s, g_y, _dx, _dy,
+                                  dt_check);
+            err = sfo_compute_err(nx, ny, Rx, Ry, divV);
+        }
+        iter += 1;
+        niter += 1;
+    }
+    double runtime = (t1 > 0.0) ? wall_seconds() - t1 : 0.0;
+    double t_it = runtime / (double)(niter - itert1 + 1);
+    double A_eff = (3.0 * 2.0) / 1e9 * (double)nx * (double)ny * 8.0;
+
+    /* padded copies and mirror BCs, StokesSolver_multixpu.jl:89-92 (:278-282, :309-313) */
+    for (int j = 0; j <= ny; ++j) {
+        memcpy(Vx_pad + IX(1, j, nx + 2), Vx_s + IX(0, j, nx), sizeof(double) * (size_t)nx);
+        Vx_pad[IX(0, j, nx + 2)] = -Vx_pad[IX(2, j, nx + 2)];
+        Vx_pad[IX(nx + 1, j, nx + 2)] = -Vx_pad[IX(nx - 1, j, nx + 2)];
+    }
+    memcpy(Vy_pad + IX(0, 1, nx + 1), Vy_s, sizeof(double) * (size_t)(nx + 1) * (size_t)ny);
+    for (int i = 0; i <= nx; ++i) {
+        Vy_pad[IX(i, 0, nx + 1)] = -Vy_pad[IX(i, 2, nx + 1)];
+        Vy_pad[IX(i, ny + 1, nx + 1)] = -Vy_pad[IX(i, ny - 1, nx + 1)];
+    }
+    if (out) {
+        out->dt_new = sfo_compute_dt(nx, ny, Vx_s, Vy_s, maxdisp, dx, dy);
+        out->err = err;
+        out->t_eff = A_eff / t_it;
+        out->runtime = runtime;
+        out->niter = niter;
+        out->converged = err <= eps;
+        out->reserved = 0;
+        out->dt_last = dt;
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------------- */
+/* common.jl:22-33 */
+void sfo_topleft_index(double x_grid_min, double y_grid_min, double x, double y, double dx,
+                       double dy, int64_t *ix, int64_t *iy, double *dxij, double *dyij)
+{
+    int64_t i = (int64_t)floor((x - x_grid_min) / dx) + 1;
+    int64_t j = (int64_t)floor((y - y_grid_min) / dy) + 1;
+    double x_ix = x_grid_min + (double)(i - 1) * dx;
+    double y_iy = y_grid_min + (double)(j - 1) * dy;
+    *ix = i;
+    *iy = j;
+    *dxij = (x - x_ix) / dx;
+    *dyij = (y - y_iy) / dy;
+}
+
+/* floor(Int, v) in Julia throws for values that do not fit an Int64 (NaN, Inf, huge). */
+static int key_representable(double v)
+{
+    return isfinite(v) && v > -9.2e18 && v < 9.2e18;
+}
+
+int sfo_marker_to_grid(double x_grid_min, double y_grid_min, double *val_grid, int gnx, int gny,
+                       const double *x_m, const double *y_m, const double *val_m, int64_t nm,
+                       double dx, double dy, double *val_wt_sum, double *wt_sum, int ws_nx,
+                       int ws_ny, int64_t *n_bad, int64_t *n_empty)
+{
+    if (ws_nx < gnx || ws_ny < gny) return -2;
+    memset(val_wt_sum, 0, sizeof(double) * (size_t)ws_nx * (size_t)ws_ny);
+    memset(wt_sum, 0, sizeof(double) * (size_t)ws_nx * (size_t)ws_ny);
+    int64_t bad = 0, empty = 0;
+    /* singleCoreInterpolationCPU!, MarkerToGrid.jl:110-155: markers in index order */
+    for (int64_t m = 0; m < nm; ++m) {
+        double xm = x_m[m], ym = y_m[m], val = val_m[m];
+        if (!key_representable((xm - x_grid_min) / dx) || !key_representable((ym - y_grid_min) / dy))
+            return -1;
+        int64_t ix, iy;
+        double dxm, dym;
+        sfo_topleft_index(x_grid_min, y_grid_min, xm, ym, dx, dy, &ix, &iy, &dxm, &dym);
+        if (!(ix >= 0 && ix <= gnx && iy >= 0 && iy <= gny)) {
+            ++bad; /* "ATTENTION: erroneous marker position" */
+            continue;
+        }
+        /* 1-based node (ix, iy) -> 0-based (ix-1, iy-1) in the helper arrays' own layout */
+        if (iy > 0) {
+            if (ix > 0) {
+                double w = (1.0 - dxm) * (1.0 - dym);
+                wt_sum[IX(ix - 1, iy - 1, ws_nx)] += w;
+                val_wt_sum[IX(ix - 1, iy - 1, ws_nx)] += w * val;
+            }
+            if (ix < gnx) {
+                double w = dxm * (1.0 - dym);
+                wt_sum[IX(ix, iy - 1, ws_nx)] += w;
+                val_wt_sum[IX(ix, iy - 1, ws_nx)] += w * val;
+            }
+        }
+        if (iy < gny) {
+            if (ix > 0) {
+                double w = (1.0 - dxm) * dym;
+                wt_sum[IX(ix - 1, iy, ws_nx)] += w;
+                val_wt_sum[IX(ix - 1, iy, ws_nx)] += w * val;
+            }
+            if (ix < gnx) {
+                double w = dxm * dym;
+                wt_sum[IX(ix, iy, ws_nx)] += w;
+                val_wt_sum[IX(ix, iy, ws_nx)] += w * val;
+            }
+        }
+    }
+    /* safeDivision, MarkerToGrid.jl:164-172: keep the old value where no weight arrived */
+    for (int j = 0; j < gny; ++j)
+        for (int i = 0; i < gnx; ++i) {
+            double den = wt_sum[IX(i, j, ws_nx)];
+            if (den != 0.0)
+                val_grid[IX(i, j, gnx)] = val_wt_sum[IX(i, j, ws_nx)] / den;
+            else
+                ++empty; /* "PROBLEM: no markers close enough ..." */
+        }
+    if (n_bad) *n_bad = bad;
+    if (n_empty) *n_empty = empty;
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------------- */
+/* bilinearInterp, MoveMarkers.jl:181-187 */
+static inline double bilinear(double v1, double v2, double v3, double v4, double dxij, double dyij)
+{
+    double s1 = ((1.0 - dxij) * (1.0 - dyij)) * v1;
+    double s2 = (dxij * (1.0 - dyij)) * v2;
+    double s3 = ((1.0 - dxij) * dyij) * v3;
+    double s4 = (dxij * dyij) * v4;
+    return ((s1 + s2) + s3) + s4;
+}
+
+/* interpolateV, MoveMarkers.jl:76-174.  Reference arrays are 1-based; A(i, j) below takes
+ * the reference's 1-based indices. */
+int sfo_interpolate_v(double x, double y, const double *Vx, int vx_n1, int vx_n2, const double *Vy,
+                      int vy_n1, int vy_n2, double x_vx_min, double y_vx_min, double x_vy_min,
+                      double y_vy_min, double dx, double dy, double *vx_out, double *vy_out,
+                      int64_t *n_clamp)
+{
+#define VX(i, j) Vx[IX((i) - 1, (j) - 1, vx_n1)]
+#define VY(i, j) Vy[IX((i) - 1, (j) - 1, vy_n1)]
+    int64_t ix, iy, nc = 0;
+    double dxij, dyij;
+    if (!key_representable((x - x_vx_min) / dx) || !key_representable((y - y_vx_min) / dy) ||
+        !key_representable((x - x_vy_min) / dx) || !key_representable((y - y_vy_min) / dy))
+        return -1;
+    /* Vx */
+    sfo_topleft_index(x_vx_min, y_vx_min, x, y, dx, dy, &ix, &iy, &dxij, &dyij);
+    if (ix < 2) { ix = 2; dxij = 0.0; ++nc; }
+    if (iy < 1) { iy = 1; dyij = 0.0; ++nc; }
+    if (ix >= vx_n1 - 1) { ix = vx_n1 - 2; dxij = 1.0; ++nc; }
+    if (iy >= vx_n2) { iy = vx_n2 - 1; dyij = 1.0; ++nc; }
+    double v1 = VX(ix, iy), v2 = VX(ix + 1, iy), v3 = VX(ix, iy + 1), v4 = VX(ix + 1, iy + 1);
+    double vx = bilinear(v1, v2, v3, v4, dxij, dyij);
+    double corr;
+    if (dxij >= 0.5) {
+        double v5 = VX(ix + 2, iy), v6 = VX(ix + 2, iy + 1);
+        corr = (0.5 * ((dxij - 0.5) * (dxij - 0.5))) *
+               ((1.0 - dyij) * ((v1 - 2.0 * v2) + v5) + dyij * ((v3 - 2.0 * v4) + v6));
+    } else {
+        double v5 = VX(ix - 1, iy), v6 = VX(ix - 1, iy + 1);
+        corr = (0.5 * ((dxij - 0.5) * (dxij - 0.5))) *
+               ((1.0 - dyij) * ((v5 - 2.0 * v1) + v2) + dyij * ((v6 - 2.0 * v3) + v4));
+    }
+    vx = vx + corr;
+    /* Vy */
+    sfo_topleft_index(x_vy_min, y_vy_min, x, y, dx, dy, &ix, &iy, &dxij, &dyij);
+    if (ix < 1) { ix = 1; dxij = 0.0; ++nc; }
+    if (iy < 2) { iy = 2; dyij = 0.0; ++nc; }
+    if (ix >= vy_n1) { ix = vy_n1 - 1; dxij = 1.0; ++nc; }
+    if (iy >= vy_n2 - 1) { iy = vy_n2 - 2; dyij = 1.0; ++nc; }
+    v1 = VY(ix, iy); v2 = VY(ix + 1, iy); v3 = VY(ix, iy + 1); v4 = VY(ix + 1, iy + 1);
+    double vy = bilinear(v1, v2, v3, v4, dxij, dyij);
+    if (dyij >= 0.5) {
+        double v5 = VY(ix, iy + 2), v6 = VY(ix + 1, iy + 2);
+        corr = (0.5 * ((dyij - 0.5) * (dyij - 0.5))) *
+               ((1.0 - dxij) * ((v1 - 2.0 * v3) + v5) + dxij * ((v2 - 2.0 * v4) + v6));
+    } else {
+        double v5 = VY(ix, iy - 1), v6 = VY(ix + 1, iy - 1);
+        corr = (0.5 * ((dyij - 0.5) * (dyij - 0.5))) *
+               ((1.0 - dxij) * ((v5 - 2.0 * v1) + v3) + dxij * ((v6 - 2.0 * v2) + v4));
+    }
+    vy = vy + corr;
+    *vx_out = vx;
+    *vy_out = vy;
+    if (n_clamp) *n_clamp += nc;
+    return 0;
+#undef VX
+#undef VY
+}
+
+/* min(max(v, 0), hi) with Julia semantics (MoveMarkers.jl:61-62) */
+static inline double clamp_domain(double v, double hi)
+{
+    return jl_min(jl_max(v, 0.0), hi);
+}
+
+int sfo_move_markers_rk4(double *x_m, double *y_m, int64_t nm, const double *Vx, int vx_n1,
+                         int vx_n2, const double *Vy, int vy_n1, int vy_n2, double x_vx_min,
+                         double y_vx_min, double x_vy_min, double y_vy_min, double dt, double lx,
+                         double ly, double dx, double dy, int64_t *n_clamp)
+{
+    const double rk_dt[4] = {0.0, 0.5 * dt, 0.5 * dt, dt};
+    const double rk_wt[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    int status = 0;
+    int64_t clamps = 0;
+#pragma omp parallel for schedule(static) reduction(+ : clamps) reduction(min : status)
+    for (int64_t m = 0; m < nm; ++m) {
+        double x_old = x_m[m], y_old = y_m[m];
+        double vx_eff = 0.0, vy_eff = 0.0, vx_rk = 0.0, vy_rk = 0.0;
+        int64_t nc = 0;
+        int bad = 0;
+        for (int it = 0; it < 4; ++it) {
+            double x_rk = x_old + rk_dt[it] * vx_rk;
+            double y_rk = y_old + rk_dt[it] * vy_rk;
+            if (sfo_interpolate_v(x_rk, y_rk, Vx, vx_n1, vx_n2, Vy, vy_n1, vy_n2, x_vx_min, y_vx_min,
+                                  x_vy_min, y_vy_min, dx, dy, &vx_rk, &vy_rk, &nc) != 0) {
+                bad = 1;
+                break;
+            }
+            vx_eff = vx_eff + rk_wt[it] * vx_rk;
+            vy_eff = vy_eff + rk_wt[it] * vy_rk;
+        }
+        if (bad) {
+            status = -1;
+            continue;
+        }
+        double x_new = x_old + vx_eff * dt;
+        double y_new = y_old + vy_eff * dt;
+        x_m[m] = clamp_domain(x_new, lx);
+        y_m[m] = clamp_domain(y_new, ly);
+        clamps += nc;
+    }
+    if (n_clamp) *n_clamp = clamps;
+    return status;
+}
+
+/* ---------------------------------------------------------------------------------------- */
+/* LinRange element lerpi(j, d, a, b) = fma(t, b, fma(-t, a, a)), t = j/d (Julia Base). */
+static double linrange_at(double a, double b, int64_t len, int64_t k)
+{
+    if (len == 1) return a;
+    double t = (double)k / (double)(len - 1);
+    return fma(t, b, fma(-t, a, a));
+}
+
+int64_t sfo_lattice_markers(int nx, int ny, int md, double lx, double ly, double dx, double dy,
+                            double *x_m, double *y_m)
+{
+    /* common.jl:40-83 with all four boundary flags true (one rank) */
+    double dxm = dx / md, dym = dy / md;
+    double xlo = dxm / 2, xhi = lx - dxm / 2, ylo = dym / 2, yhi = ly - dym / 2;
+    int64_t nmx = (int64_t)md * (nx - 1), nmy = (int64_t)md * (ny - 1);
+    if (x_m && y_m) {
+        int64_t m = 0;
+        for (int64_t i = 0; i < nmx; ++i)
+            for (int64_t j = 0; j < nmy; ++j, ++m) {
+                x_m[m] = linrange_at(xlo, xhi, nmx, i);
+                y_m[m] = linrange_at(ylo, yhi, nmy, j);
+            }
+    }
+    return nmx * nmy;
+}
+
+static uint64_t splitmix64(uint64_t seed, uint64_t counter)
+{
+    uint64_t z = seed + (counter + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+double sfo_uniform(uint64_t seed, uint64_t counter)
+{
+    return (double)(splitmix64(seed, counter) >> 11) * 0x1.0p-53;
+}
+
+void sfo_seeded_markers(int nx, int ny, int md, double dx, double dy, double jitter_frac,
+                        uint64_t seed, double *x_m, double *y_m)
+{
+    const int64_t nmx = (int64_t)md * (nx - 1), nmy = (int64_t)md * (ny - 1);
+    const double dxm = dx / md, dym = dy / md;
+    const double lx = (double)(nx - 1) * dx, ly = (double)(ny - 1) * dy;
+    const double jx = 2.0 * jitter_frac * dx, jy = 2.0 * jitter_frac * dy;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nmx; ++i)
+        for (int64_t j = 0; j < nmy; ++j) {
+            int64_t m = i * nmy + j;
+            double x = ((double)i + 0.5) * dxm + (sfo_uniform(seed, 2 * (uint64_t)m) - 0.5) * jx;
+            double y = ((double)j + 0.5) * dym + (sfo_uniform(seed, 2 * (uint64_t)m + 1) - 0.5) * jy;
+            x_m[m] = clamp_domain(x, lx);
+            y_m[m] = clamp_domain(y, ly);
+        }
+}
+
+void sfo_plume_properties(const double *x_m, const double *y_m, int64_t nm, double air_h,
+                          double cx, double cy, double r, double rho_air, double rho_plume,
+                          double rho_bg, double mu_air, double mu_plume, double mu_bg,
+                          double *rho_m, double *mu_m)
+{
+    const double r2 = r * r;
+#pragma omp parallel for schedule(static)
+    for (int64_t m = 0; m < nm; ++m) {
+        double x = x_m[m], y = y_m[m];
+        double ddx = x - cx, ddy = y - cy;
+        if (y < air_h) {
+            rho_m[m] = rho_air;
+            mu_m[m] = mu_air;
+        } else if (ddx * ddx + ddy * ddy < r2) {
+            rho_m[m] = rho_plume;
+            mu_m[m] = mu_plume;
+        } else {
+            rho_m[m] = rho_bg;
+            mu_m[m] = mu_bg;
+        }
+    }
+}

@@ -1,0 +1,609 @@
+// sf_capi.cpp -- C ABI of libsfhip (include/sf_hip.h): context, PT solve loop, marker
+// entry points and the one-rank timestep.
+#include "../../include/sf_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "sf_internal.h"
+
+using namespace sf;
+
+#define SF_VERSION_NUM 100  // 0.1.0
+
+static thread_local char g_err[512] = "";
+
+static int set_err(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                      \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) return set_err(SF_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                             __FILE__, __LINE__);                                         \
+    } while (0)
+
+struct sf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nx = 0, ny = 0;
+    // PT ping-pong state and check-iteration scratch
+    double *vx_b = nullptr, *vy_b = nullptr, *p_b = nullptr;
+    double *txx = nullptr, *tyy = nullptr, *txy = nullptr, *divv = nullptr, *rx = nullptr, *ry = nullptr;
+    double *part = nullptr;   // 3 * RED_MAX_BLOCKS
+    double *scal = nullptr;   // SC_COUNT device scalars
+    double *hscal = nullptr;  // pinned host mirror
+    unsigned long long *counters = nullptr;   // device counters (RK4)
+    unsigned long long *hcounters = nullptr;  // pinned host mirror
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    M2GScratch m2g;
+    int seg = 64;  // PT rows per tile
+};
+
+extern "C" int sf_version(void) { return SF_VERSION_NUM; }
+extern "C" const char* sf_last_error(void) { return g_err; }
+
+template <typename T>
+static hipError_t dalloc(T** p, size_t n)
+{
+    return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+static void free_ctx(sf_ctx* c)
+{
+    if (!c) return;
+    void* bufs[] = {c->vx_b, c->vy_b, c->p_b, c->txx, c->tyy, c->txy, c->divv, c->rx, c->ry, c->part, c->scal,
+                    c->counters, c->m2g.key, c->m2g.count, c->m2g.offset, c->m2g.fill, c->m2g.rec_idx,
+                    c->m2g.rec_dx, c->m2g.rec_dy, c->m2g.rec_val, c->m2g.blocksums, c->m2g.counters};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->hscal) (void)hipHostFree(c->hscal);
+    if (c->hcounters) (void)hipHostFree(c->hcounters);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    delete c;
+}
+
+extern "C" int sf_ctx_create(sf_ctx** out, int nx, int ny, void* stream)
+{
+    if (!out) return set_err(SF_ERR_ARG, "sf_ctx_create: ctx pointer is NULL");
+    *out = nullptr;
+    const bool markers_only = (nx == 0 && ny == 0);
+    if (!markers_only && (nx < 3 || ny < 3))
+        return set_err(SF_ERR_ARG, "sf_ctx_create: need Nx, Ny >= 3, or 0 x 0 for a marker-only context (got %d, %d)",
+                       nx, ny);
+    if ((int64_t)(nx + 2) * (ny + 2) >= (int64_t)1 << 31)
+        return set_err(SF_ERR_SIZE, "sf_ctx_create: grid %d x %d exceeds the 32-bit cell index", nx, ny);
+    sf_ctx* c = new sf_ctx();
+    c->nx = nx;
+    c->ny = ny;
+    c->stream = (hipStream_t)stream;
+    hipError_t e = hipGetDevice(&c->device);
+    // a marker-only context keeps 1-element placeholders for the grid scratch
+    const size_t gx = markers_only ? 3 : (size_t)nx, gy = markers_only ? 3 : (size_t)ny;
+    const size_t nvx = gx * (gy + 1), nvy = (gx + 1) * gy, np = (gx - 1) * (gy - 1);
+    if (e == hipSuccess) e = dalloc(&c->vx_b, nvx);
+    if (e == hipSuccess) e = dalloc(&c->vy_b, nvy);
+    if (e == hipSuccess) e = dalloc(&c->p_b, np);
+    if (e == hipSuccess) e = dalloc(&c->txx, np);
+    if (e == hipSuccess) e = dalloc(&c->tyy, np);
+    if (e == hipSuccess) e = dalloc(&c->txy, gx * gy);
+    if (e == hipSuccess) e = dalloc(&c->divv, np);
+    if (e == hipSuccess) e = dalloc(&c->rx, (gx - 2) * (gy - 1));
+    if (e == hipSuccess) e = dalloc(&c->ry, (gx - 1) * (gy - 2));
+    if (e == hipSuccess) e = dalloc(&c->part, 3 * (size_t)RED_MAX_BLOCKS);
+    if (e == hipSuccess) e = dalloc(&c->scal, (size_t)SC_COUNT);
+    if (e == hipSuccess) e = dalloc(&c->counters, 4);
+    if (e == hipSuccess) e = dalloc(&c->m2g.counters, 4);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->hscal, SC_COUNT * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->hcounters, 8 * sizeof(unsigned long long), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) {
+        free_ctx(c);
+        return set_err(SF_ERR_HIP, "sf_ctx_create: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return SF_OK;
+}
+
+extern "C" int sf_ctx_destroy(sf_ctx* c)
+{
+    if (!c) return SF_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    else (void)hipDeviceSynchronize();
+    free_ctx(c);
+    return SF_OK;
+}
+
+extern "C" int sf_ctx_set_stream(sf_ctx* c, void* stream)
+{
+    if (!c) return set_err(SF_ERR_ARG, "sf_ctx_set_stream: NULL ctx");
+    c->stream = (hipStream_t)stream;
+    return SF_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// PT solve
+
+namespace {
+
+struct SolveConsts {
+    double vsc_mdxy2, ptsc41, max_nxy, dampx, dampy, rdx, rdy;
+};
+
+// Constants of StokesSolver_multixpu.jl:17-27 with nx_g() = Nx, ny_g() = Ny.
+SolveConsts solve_consts(int nx, int ny, double dx, double dy)
+{
+    const double Vdmp = 4.0, Vsc = 0.45, Ptsc = 1.0 / 2.0;
+    const double m = std::fmin(dx, dy);
+    SolveConsts k;
+    k.vsc_mdxy2 = Vsc * (m * m);
+    k.ptsc41 = Ptsc * 4.1;
+    k.max_nxy = (double)std::max(nx, ny);
+    k.dampx = 1.0 - Vdmp / (double)nx;
+    k.dampy = 1.0 - Vdmp / (double)ny;
+    k.rdx = 1.0 / dx;
+    k.rdy = 1.0 / dy;
+    return k;
+}
+
+double bytes_per_iter(int nx, int ny)
+{
+    const double n_vx = (double)nx * (ny + 1), n_vy = (double)(nx + 1) * ny, n_p = (double)(nx - 1) * (ny - 1);
+    const double n_dvx = (double)(nx - 2) * (ny - 1), n_dvy = (double)(nx - 1) * (ny - 2);
+    const double n_mub = (double)nx * ny, n_rho = (double)(nx + 1) * ny;
+    return 8.0 * (2.0 * (n_vx + n_vy + n_p + n_dvx + n_dvy) + n_p + n_mub + n_rho);
+}
+
+}  // namespace
+
+static int check_ctx(sf_ctx* c, int nx, int ny, const char* fn)
+{
+    if (!c) return set_err(SF_ERR_ARG, "%s: NULL ctx", fn);
+    if (c->nx == 0) return set_err(SF_ERR_ARG, "%s: marker-only context has no grid scratch", fn);
+    if (nx != c->nx || ny != c->ny)
+        return set_err(SF_ERR_ARG, "%s: grid %d x %d does not match the context's %d x %d", fn, nx, ny, c->nx, c->ny);
+    return SF_OK;
+}
+
+// max|Vx|, max|Vy| -> dt_check (and dt if update) on the device
+static int enqueue_dt(sf_ctx* c, const double* vx, const double* vy, double dx, double dy, double maxdisp, int update)
+{
+    const int nx = c->nx, ny = c->ny;
+    RedParams rp{};
+    rp.a[0] = vx;
+    rp.n[0] = (int64_t)nx * (ny + 1);
+    rp.a[1] = vy;
+    rp.n[1] = (int64_t)(nx + 1) * ny;
+    rp.part = c->part;
+    rp.mode = RED_MAX;
+    const int nb = red_num_blocks(std::max(rp.n[0], rp.n[1]));
+    HIPCHK(launch_reduce(rp, 2, nb, c->stream));
+    HIPCHK(launch_finalize_dt(c->part, nb, dx, dy, maxdisp, c->scal, update, c->stream));
+    return SF_OK;
+}
+
+extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_pad, const double* rho_vy,
+                               const double* mu_b, const double* mu_p, double* txx, double* tyy, double* txy,
+                               double* divV, double* dtauP, double* Rx, double* Ry, double* dVx, double* dVy,
+                               double* dtauVx, double* dtauVy, double* Vx_s, double* Vy_s, double g_y, double dx,
+                               double dy, int nx, int ny, double dt, double maxdisp, int use_fss, double eps,
+                               int64_t iter_max, int print_info, sf_solve_out* out)
+{
+    int rc = check_ctx(c, nx, ny, "sf_solve_stokes");
+    if (rc) return rc;
+    if (!P || !Vx_pad || !Vy_pad || !rho_vy || !mu_b || !mu_p || !dVx || !dVy || !Vx_s || !Vy_s)
+        return set_err(SF_ERR_ARG, "sf_solve_stokes: a state or material array is NULL");
+    if (!(dx > 0.0) || !(dy > 0.0)) return set_err(SF_ERR_ARG, "sf_solve_stokes: dx, dy must be > 0");
+    hipStream_t s = c->stream;
+    const SolveConsts k = solve_consts(nx, ny, dx, dy);
+    if (!use_fss) dt = 0.0;
+
+    double* t_xx = txx ? txx : c->txx;
+    double* t_yy = tyy ? tyy : c->tyy;
+    double* t_xy = txy ? txy : c->txy;
+    double* d_v = divV ? divV : c->divv;
+    double* r_x = Rx ? Rx : c->rx;
+    double* r_y = Ry ? Ry : c->ry;
+    if (dtauP || dtauVx || dtauVy)
+        HIPCHK(launch_fill_timesteps(nx, ny, mu_p, dtauVx, dtauVy, dtauP, k.vsc_mdxy2, k.ptsc41, k.max_nxy, s));
+    HIPCHK(launch_set_scalar(c->scal + SC_DT, dt, s));
+
+    const int mn = std::min(nx, ny);
+    const int64_t ncheck = std::min<int64_t>(5 * (int64_t)mn, 2000);
+    const int64_t ndtupdate = 10 * (int64_t)mn;
+    const int64_t itert1 = 11;
+    const double n_rx = (double)(nx - 2) * (ny - 1), n_ry = (double)(nx - 1) * (ny - 2), n_dv = (double)(nx - 1) * (ny - 1);
+
+    PTParams pp{};
+    pp.nx = nx;
+    pp.ny = ny;
+    pp.seg = c->seg;
+    pp.dvx = dVx;
+    pp.dvy = dVy;
+    pp.rho = rho_vy;
+    pp.mu_b = mu_b;
+    pp.mu_p = mu_p;
+    pp.dt = c->scal + SC_DT;
+    pp.txx = t_xx;
+    pp.tyy = t_yy;
+    pp.txy = t_xy;
+    pp.divv = d_v;
+    pp.rx = r_x;
+    pp.g = g_y;
+    pp.dampx = k.dampx;
+    pp.dampy = k.dampy;
+    pp.rdx = k.rdx;
+    pp.rdy = k.rdy;
+    pp.vsc_mdxy2 = k.vsc_mdxy2;
+    pp.ptsc41 = k.ptsc41;
+    pp.max_nxy = k.max_nxy;
+
+    double* cur_vx = Vx_s;
+    double* cur_vy = Vy_s;
+    double* cur_p = P;
+    double* nxt_vx = c->vx_b;
+    double* nxt_vy = c->vy_b;
+    double* nxt_p = c->p_b;
+
+    double err = 2.0 * eps;
+    int64_t iter = 1, niter = 0;
+    bool timed = false;
+    while (err > eps && iter <= iter_max) {
+        if (iter == itert1) {
+            HIPCHK(hipEventRecord(c->ev0, s));
+            timed = true;
+        }
+        const bool check = (iter % ncheck) == 0;
+        const bool last = iter == iter_max;
+        pp.flags = ((check || last) ? PT_WRITE_TAU : 0) | (check ? PT_WRITE_RX : 0);
+        pp.vx_a = cur_vx;
+        pp.vy_a = cur_vy;
+        pp.p_a = cur_p;
+        pp.vx_b = nxt_vx;
+        pp.vy_b = nxt_vy;
+        pp.p_b = nxt_p;
+        HIPCHK(launch_pt_fused(pp, s));
+        std::swap(cur_vx, nxt_vx);
+        std::swap(cur_vy, nxt_vy);
+        std::swap(cur_p, nxt_p);
+
+        const bool dt_upd = use_fss && (iter % ndtupdate) == 0;
+        if (dt_upd) {
+            rc = enqueue_dt(c, cur_vx, cur_vy, dx, dy, maxdisp, 1);
+            if (rc) return rc;
+        }
+        if (check) {
+            if (use_fss) {
+                if (!dt_upd) {
+                    rc = enqueue_dt(c, cur_vx, cur_vy, dx, dy, maxdisp, 0);
+                    if (rc) return rc;
+                }
+            } else {
+                HIPCHK(launch_set_scalar(c->scal + SC_DT_CHECK, 0.0, s));
+            }
+            ResYParams rq{};
+            rq.nx = nx;
+            rq.ny = ny;
+            rq.tyy = t_yy;
+            rq.txy = t_xy;
+            rq.P = cur_p;
+            rq.rho = rho_vy;
+            rq.vx = cur_vx;
+            rq.vy = cur_vy;
+            rq.dt = c->scal + SC_DT_CHECK;
+            rq.ry = r_y;
+            rq.g = g_y;
+            rq.rdx = k.rdx;
+            rq.rdy = k.rdy;
+            HIPCHK(launch_pt_resid_y(rq, s));
+            RedParams rp{};
+            rp.a[0] = r_x;
+            rp.n[0] = (int64_t)n_rx;
+            rp.a[1] = r_y;
+            rp.n[1] = (int64_t)n_ry;
+            rp.a[2] = d_v;
+            rp.n[2] = (int64_t)n_dv;
+            rp.part = c->part;
+            rp.mode = RED_SUM;
+            const int nb = red_num_blocks((int64_t)n_dv);
+            HIPCHK(launch_reduce(rp, 3, nb, s));
+            HIPCHK(launch_finalize_err(c->part, nb, n_rx, n_ry, n_dv, c->scal, s));
+            HIPCHK(hipMemcpyAsync(c->hscal, c->scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            err = c->hscal[SC_ERR];
+        }
+        iter += 1;
+        niter += 1;
+    }
+    if (timed) HIPCHK(hipEventRecord(c->ev1, s));
+
+    // state back into the caller's arrays
+    if (cur_vx != Vx_s) {
+        HIPCHK(hipMemcpyAsync(Vx_s, cur_vx, sizeof(double) * (size_t)nx * (ny + 1), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(Vy_s, cur_vy, sizeof(double) * (size_t)(nx + 1) * ny, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(P, cur_p, sizeof(double) * (size_t)(nx - 1) * (ny - 1), hipMemcpyDeviceToDevice, s));
+    }
+    HIPCHK(launch_pad_mirror(nx, ny, Vx_s, Vy_s, Vx_pad, Vy_pad, s));
+    rc = enqueue_dt(c, Vx_s, Vy_s, dx, dy, maxdisp, 0);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->hscal, c->scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+
+    double runtime = 0.0;
+    if (timed) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        runtime = 1e-3 * (double)ms;
+    }
+    const double t_it = runtime / (double)(niter - itert1 + 1);
+    const double a_eff = (3.0 * 2.0) / 1e9 * (double)nx * (double)ny * 8.0;
+    if (print_info)
+        printf("Total steps = %lld, err = %1.3e, time = %1.3e sec (@ T_eff = %1.2f GB/s) \n", (long long)niter, err,
+               runtime, timed ? a_eff / t_it : 0.0);
+    if (out) {
+        out->dt_new = c->hscal[SC_DT_CHECK];
+        out->err = err;
+        out->t_eff = timed ? a_eff / t_it : 0.0;
+        out->runtime = runtime;
+        out->niter = niter;
+        out->converged = err <= eps;
+        out->reserved = 0;
+        out->dt_last = c->hscal[SC_DT];
+        out->t_it = timed ? t_it : 0.0;
+        out->gbs_alg = timed ? bytes_per_iter(nx, ny) / t_it / 1e9 : 0.0;
+    }
+    return SF_OK;
+}
+
+extern "C" int sf_pt_iterations(sf_ctx* c, int64_t n_iter, double* P, double* Vx_s, double* Vy_s, double* dVx,
+                                double* dVy, const double* rho_vy, const double* mu_b, const double* mu_p, double g_y,
+                                double dx, double dy, int nx, int ny, double dt)
+{
+    int rc = check_ctx(c, nx, ny, "sf_pt_iterations");
+    if (rc) return rc;
+    if (!P || !Vx_s || !Vy_s || !dVx || !dVy || !rho_vy || !mu_b || !mu_p)
+        return set_err(SF_ERR_ARG, "sf_pt_iterations: NULL array");
+    if (n_iter < 0) return set_err(SF_ERR_ARG, "sf_pt_iterations: n_iter < 0");
+    hipStream_t s = c->stream;
+    const SolveConsts k = solve_consts(nx, ny, dx, dy);
+    HIPCHK(launch_set_scalar(c->scal + SC_DT, dt, s));
+    PTParams pp{};
+    pp.nx = nx;
+    pp.ny = ny;
+    pp.seg = c->seg;
+    pp.flags = 0;
+    pp.dvx = dVx;
+    pp.dvy = dVy;
+    pp.rho = rho_vy;
+    pp.mu_b = mu_b;
+    pp.mu_p = mu_p;
+    pp.dt = c->scal + SC_DT;
+    pp.txx = c->txx;
+    pp.tyy = c->tyy;
+    pp.txy = c->txy;
+    pp.divv = c->divv;
+    pp.rx = c->rx;
+    pp.g = g_y;
+    pp.dampx = k.dampx;
+    pp.dampy = k.dampy;
+    pp.rdx = k.rdx;
+    pp.rdy = k.rdy;
+    pp.vsc_mdxy2 = k.vsc_mdxy2;
+    pp.ptsc41 = k.ptsc41;
+    pp.max_nxy = k.max_nxy;
+    double *cv = Vx_s, *cw = Vy_s, *cp = P, *nv = c->vx_b, *nw = c->vy_b, *np = c->p_b;
+    for (int64_t it = 0; it < n_iter; ++it) {
+        pp.vx_a = cv;
+        pp.vy_a = cw;
+        pp.p_a = cp;
+        pp.vx_b = nv;
+        pp.vy_b = nw;
+        pp.p_b = np;
+        HIPCHK(launch_pt_fused(pp, s));
+        std::swap(cv, nv);
+        std::swap(cw, nw);
+        std::swap(cp, np);
+    }
+    if (cv != Vx_s) {
+        HIPCHK(hipMemcpyAsync(Vx_s, cv, sizeof(double) * (size_t)nx * (ny + 1), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(Vy_s, cw, sizeof(double) * (size_t)(nx + 1) * ny, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(P, cp, sizeof(double) * (size_t)(nx - 1) * (ny - 1), hipMemcpyDeviceToDevice, s));
+    }
+    return SF_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// marker -> grid
+
+static int m2g_reserve(sf_ctx* c, int64_t nm, int64_t ncells)
+{
+    M2GScratch& S = c->m2g;
+    if (nm > S.cap_markers) {
+        for (void* p : {(void*)S.key, (void*)S.rec_idx, (void*)S.rec_dx, (void*)S.rec_dy, (void*)S.rec_val})
+            if (p) HIPCHK(hipFree(p));
+        S.key = S.rec_idx = nullptr;
+        S.rec_dx = S.rec_dy = S.rec_val = nullptr;
+        S.cap_markers = 0;
+        const int64_t cap = nm + nm / 8;
+        HIPCHK(dalloc(&S.key, (size_t)cap));
+        HIPCHK(dalloc(&S.rec_idx, (size_t)cap));
+        HIPCHK(dalloc(&S.rec_dx, (size_t)cap));
+        HIPCHK(dalloc(&S.rec_dy, (size_t)cap));
+        HIPCHK(dalloc(&S.rec_val, (size_t)cap));
+        S.cap_markers = cap;
+    }
+    if (ncells > S.cap_cells) {
+        for (void* p : {(void*)S.count, (void*)S.offset, (void*)S.fill, (void*)S.blocksums})
+            if (p) HIPCHK(hipFree(p));
+        S.count = S.offset = S.fill = S.blocksums = nullptr;
+        S.cap_cells = 0;
+        HIPCHK(dalloc(&S.count, (size_t)ncells));
+        HIPCHK(dalloc(&S.offset, (size_t)ncells + 1));
+        HIPCHK(dalloc(&S.fill, (size_t)ncells));
+        HIPCHK(dalloc(&S.blocksums, (size_t)(ncells / 2048 + 2)));
+        S.cap_cells = ncells;
+    }
+    return SF_OK;
+}
+
+static int m2g_call(sf_ctx* c, double xmin, double ymin, double* val_grid, int gnx, int gny, const double* x_m,
+                    const double* y_m, const double* val_m, int64_t nm, double dx, double dy, double* vws, double* ws,
+                    int ws_nx, int ws_ny, sf_m2g_out* out)
+{
+    if (!val_grid || (nm > 0 && (!x_m || !y_m || !val_m)))
+        return set_err(SF_ERR_ARG, "sf_marker_to_grid: NULL array");
+    if (gnx < 1 || gny < 1) return set_err(SF_ERR_ARG, "sf_marker_to_grid: empty grid %d x %d", gnx, gny);
+    if (nm < 0) return set_err(SF_ERR_ARG, "sf_marker_to_grid: nm < 0");
+    if ((vws == nullptr) != (ws == nullptr))
+        return set_err(SF_ERR_ARG, "sf_marker_to_grid: pass both helper arrays or neither");
+    if (ws && (ws_nx < gnx || ws_ny < gny))
+        return set_err(SF_ERR_ARG, "sf_marker_to_grid: helper arrays %d x %d smaller than grid %d x %d", ws_nx, ws_ny,
+                       gnx, gny);
+    if (!(dx > 0.0) || !(dy > 0.0)) return set_err(SF_ERR_ARG, "sf_marker_to_grid: dx, dy must be > 0");
+    const int64_t ncells = (int64_t)(gnx + 1) * (gny + 1);
+    if (nm >= ((int64_t)1 << 31) - 1 || ncells >= ((int64_t)1 << 31) - 1)
+        return set_err(SF_ERR_SIZE, "sf_marker_to_grid: %lld markers / %lld cells exceed the 32-bit index limit",
+                       (long long)nm, (long long)ncells);
+    int rc = m2g_reserve(c, nm, ncells);
+    if (rc) return rc;
+    HIPCHK(m2g_run(c->m2g, xmin, ymin, val_grid, gnx, gny, x_m, y_m, val_m, nm, dx, dy, vws, ws, ws_nx, ws_ny,
+                   c->stream));
+    HIPCHK(hipMemcpyAsync(c->hcounters, c->m2g.counters, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->hcounters[2] > 0)
+        return set_err(SF_ERR_NONFINITE, "sf_marker_to_grid: %llu markers have non-finite coordinates",
+                       (unsigned long long)c->hcounters[2]);
+    if (out) {
+        out->n_bad_markers = (int64_t)c->hcounters[0];
+        out->n_empty_nodes = (int64_t)c->hcounters[1];
+    }
+    return SF_OK;
+}
+
+extern "C" int sf_marker_to_grid(sf_ctx* c, double xmin, double ymin, double* val_grid, int gnx, int gny,
+                                 const double* x_m, const double* y_m, const double* val_m, int64_t nm, double dx,
+                                 double dy, double* val_wt_sum, double* wt_sum, int ws_nx, int ws_ny, sf_m2g_out* out)
+{
+    if (!c) return set_err(SF_ERR_ARG, "sf_marker_to_grid: NULL ctx");
+    return m2g_call(c, xmin, ymin, val_grid, gnx, gny, x_m, y_m, val_m, nm, dx, dy, val_wt_sum, wt_sum, ws_nx, ws_ny,
+                    out);
+}
+
+// ---------------------------------------------------------------------------------------
+// RK4
+
+extern "C" int sf_move_markers_rk4(sf_ctx* c, double* x_m, double* y_m, int64_t nm, const double* Vx, int vx_n1,
+                                   int vx_n2, const double* Vy, int vy_n1, int vy_n2, double x_vx_min, double y_vx_min,
+                                   double x_vy_min, double y_vy_min, double dt, double lx, double ly, double dx,
+                                   double dy, sf_rk4_out* out)
+{
+    if (!c) return set_err(SF_ERR_ARG, "sf_move_markers_rk4: NULL ctx");
+    if (nm < 0) return set_err(SF_ERR_ARG, "sf_move_markers_rk4: nm < 0");
+    if (nm > 0 && (!x_m || !y_m || !Vx || !Vy)) return set_err(SF_ERR_ARG, "sf_move_markers_rk4: NULL array");
+    // the failsafe clamps keep every access inside arrays of at least 3 x 2 / 2 x 3 (+ stencil)
+    if (vx_n1 < 4 || vx_n2 < 2 || vy_n1 < 2 || vy_n2 < 4)
+        return set_err(SF_ERR_ARG, "sf_move_markers_rk4: velocity arrays too small (%d x %d, %d x %d)", vx_n1, vx_n2,
+                       vy_n1, vy_n2);
+    if (!(dx > 0.0) || !(dy > 0.0)) return set_err(SF_ERR_ARG, "sf_move_markers_rk4: dx, dy must be > 0");
+    hipStream_t s = c->stream;
+    HIPCHK(launch_zero_u64(c->counters, 4, s));
+    HIPCHK(launch_rk4(x_m, y_m, nm, Vx, vx_n1, vx_n2, Vy, vy_n1, vy_n2, x_vx_min, y_vx_min, x_vy_min, y_vy_min, dt, lx,
+                      ly, dx, dy, c->counters, s));
+    HIPCHK(hipMemcpyAsync(c->hcounters, c->counters, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (c->hcounters[1] > 0)
+        return set_err(SF_ERR_NONFINITE, "sf_move_markers_rk4: %llu markers have non-finite positions",
+                       (unsigned long long)c->hcounters[1]);
+    if (out) out->n_clamped = (int64_t)c->hcounters[0];
+    return SF_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// timestep (StokesFlow2D_multixpu.jl:164-181 at one rank)
+
+extern "C" int sf_step(sf_ctx* c, sf_step_args* a, sf_solve_out* so, sf_m2g_out m2g_out[3], sf_rk4_out* ro)
+{
+    if (!c || !a) return set_err(SF_ERR_ARG, "sf_step: NULL argument");
+    int rc = check_ctx(c, a->nx, a->ny, "sf_step");
+    if (rc) return rc;
+    const int nx = a->nx, ny = a->ny;
+    const double dx = a->dx, dy = a->dy;
+    const double lx = (double)(nx - 1) * dx, ly = (double)(ny - 1) * dy;
+    // grid origins, StokesFlow2D_multixpu.jl:117-126
+    const double x_rho = 0.0 * dx - 0.5 * dx, y_rho = 0.0 * dy;
+    const double x_p = 0.0 * dx + 0.5 * dx, y_p = 0.0 * dy + 0.5 * dy;
+    const double x_vx = -1.0 * dx, y_vx = 0.0 * dy - 0.5 * dy;
+    const double x_vy = 0.0 * dx - 0.5 * dx, y_vy = -1.0 * dy;
+    sf_m2g_out mo[3] = {};
+    rc = m2g_call(c, x_rho, y_rho, a->rho_vy, nx + 1, ny, a->x_m, a->y_m, a->rho_m, a->nm, dx, dy, nullptr, nullptr, 0, 0, &mo[0]);
+    if (rc) return rc;
+    rc = m2g_call(c, 0.0, 0.0, a->mu_b, nx, ny, a->x_m, a->y_m, a->mu_m, a->nm, dx, dy, nullptr, nullptr, 0, 0, &mo[1]);
+    if (rc) return rc;
+    rc = m2g_call(c, x_p, y_p, a->mu_p, nx - 1, ny - 1, a->x_m, a->y_m, a->mu_m, a->nm, dx, dy, nullptr, nullptr, 0, 0, &mo[2]);
+    if (rc) return rc;
+    sf_solve_out sout{};
+    rc = sf_solve_stokes(c, a->P, a->Vx_pad, a->Vy_pad, a->rho_vy, a->mu_b, a->mu_p, nullptr, nullptr, nullptr,
+                         nullptr, nullptr, nullptr, nullptr, a->dVxdtau, a->dVydtau, nullptr, nullptr, a->Vx_s,
+                         a->Vy_s, a->g_y, dx, dy, nx, ny, a->dt, a->maxdisp, a->use_fss, a->eps, a->iter_max, 0, &sout);
+    if (rc) return rc;
+    a->dt = sout.dt_new;
+    sf_rk4_out rout{};
+    rc = sf_move_markers_rk4(c, a->x_m, a->y_m, a->nm, a->Vx_pad, nx + 2, ny + 1, a->Vy_pad, nx + 1, ny + 2, x_vx, y_vx,
+                             x_vy, y_vy, a->dt, lx, ly, dx, dy, &rout);
+    if (rc) return rc;
+    if (so) *so = sout;
+    if (m2g_out) std::memcpy(m2g_out, mo, sizeof(mo));
+    if (ro) *ro = rout;
+    return SF_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// marker set-up
+
+extern "C" int sf_init_markers(sf_ctx* c, double* x_m, double* y_m, int nx, int ny, int md, double dx, double dy,
+                               double jitter_frac, uint64_t seed)
+{
+    if (!c || !x_m || !y_m) return set_err(SF_ERR_ARG, "sf_init_markers: NULL argument");
+    if (nx < 2 || ny < 2 || md < 1) return set_err(SF_ERR_ARG, "sf_init_markers: bad sizes");
+    HIPCHK(launch_init_markers(x_m, y_m, nx, ny, md, dx, dy, jitter_frac, seed, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SF_OK;
+}
+
+extern "C" int sf_plume_properties(sf_ctx* c, const double* x_m, const double* y_m, int64_t nm, const double mat[10],
+                                   double* rho_m, double* mu_m)
+{
+    if (!c || !mat || (nm > 0 && (!x_m || !y_m || !rho_m || !mu_m)))
+        return set_err(SF_ERR_ARG, "sf_plume_properties: NULL argument");
+    Mat10 m{};
+    std::memcpy(m.v, mat, 10 * sizeof(double));
+    HIPCHK(launch_plume_properties(x_m, y_m, nm, m, rho_m, mu_m, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SF_OK;
+}
+
+extern "C" int sf_rt_properties(sf_ctx* c, const double* x_m, const double* y_m, int64_t nm, const double mat[8],
+                                double* rho_m, double* mu_m)
+{
+    if (!c || !mat || (nm > 0 && (!x_m || !y_m || !rho_m || !mu_m)))
+        return set_err(SF_ERR_ARG, "sf_rt_properties: NULL argument");
+    Mat10 m{};
+    std::memcpy(m.v, mat, 8 * sizeof(double));
+    HIPCHK(launch_rt_properties(x_m, y_m, nm, m, rho_m, mu_m, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SF_OK;
+}

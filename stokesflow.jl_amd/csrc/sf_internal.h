@@ -1,0 +1,112 @@
+// sf_internal.h -- kernel parameter blocks and host launchers shared inside libsfhip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sf {
+
+enum { PT_WRITE_TAU = 1, PT_WRITE_RX = 2 };
+enum { RED_SUM = 0, RED_MAX = 1 };
+enum { RED_MAX_BLOCKS = 1024 };
+// device scalar slots
+enum { SC_DT = 0, SC_DT_CHECK = 1, SC_ERR = 2, SC_COUNT = 8 };
+
+struct PTParams {
+    int nx, ny;           // node counts
+    int seg;              // basic-node rows per tile
+    int flags;            // PT_WRITE_TAU | PT_WRITE_RX
+    int nstrips, ntiles;  // filled by the launcher
+    const double* vx_a;   // Vx_s (nx, ny+1), state in
+    const double* vy_a;   // Vy_s (nx+1, ny)
+    const double* p_a;    // P (nx-1, ny-1)
+    double* vx_b;         // state out
+    double* vy_b;
+    double* p_b;
+    double* dvx;          // (nx-2, ny-1), in place
+    double* dvy;          // (nx-1, ny-2), in place
+    const double* rho;    // rho_vy (nx+1, ny)
+    const double* mu_b;   // (nx, ny)
+    const double* mu_p;   // (nx-1, ny-1)
+    const double* dt;     // device scalar: stabilisation dt
+    double* txx;          // check-iteration outputs
+    double* tyy;
+    double* txy;
+    double* divv;
+    double* rx;
+    double g, dampx, dampy, rdx, rdy, vsc_mdxy2, ptsc41, max_nxy;
+};
+
+struct ResYParams {
+    int nx, ny;
+    const double* tyy;
+    const double* txy;
+    const double* P;
+    const double* rho;
+    const double* vx;
+    const double* vy;
+    const double* dt;
+    double* ry;
+    double g, rdx, rdy;
+};
+
+struct RedParams {
+    const double* a[3];
+    int64_t n[3];
+    double* part;
+    int mode;
+};
+
+int pt_num_strips(int nx);
+int red_num_blocks(int64_t nmax);
+hipError_t launch_pt_fused(const PTParams& p, hipStream_t s);
+hipError_t launch_pt_resid_y(const ResYParams& p, hipStream_t s);
+hipError_t launch_reduce(const RedParams& p, int narr, int nb, hipStream_t s);
+hipError_t launch_finalize_dt(const double* part, int nb, double dx, double dy, double maxdisp, double* scal,
+                              int update, hipStream_t s);
+hipError_t launch_finalize_err(const double* part, int nb, double nrx, double nry, double ndv, double* scal,
+                               hipStream_t s);
+hipError_t launch_set_scalar(double* dst, double v, hipStream_t s);
+hipError_t launch_zero_u64(unsigned long long* dst, int n, hipStream_t s);
+hipError_t launch_pad_mirror(int nx, int ny, const double* vx, const double* vy, double* vx_pad, double* vy_pad,
+                             hipStream_t s);
+hipError_t launch_fill_timesteps(int nx, int ny, const double* mu_p, double* dtauvx, double* dtauvy, double* dtaup,
+                                 double vsc_mdxy2, double ptsc41, double max_nxy, hipStream_t s);
+
+// ---- marker -> grid ----
+struct M2GScratch {
+    int64_t cap_markers = 0;
+    int64_t cap_cells = 0;
+    int* key = nullptr;      // [nm] cell key or -1
+    int* count = nullptr;    // [cells]
+    int* offset = nullptr;   // [cells + 1]
+    int* fill = nullptr;     // [cells]
+    int* rec_idx = nullptr;  // [nm] marker index in cell order
+    double* rec_dx = nullptr;
+    double* rec_dy = nullptr;
+    double* rec_val = nullptr;
+    int* blocksums = nullptr;  // scan scratch
+    int64_t cap_blocks = 0;
+    unsigned long long* counters = nullptr;  // [4]: bad, empty, nonfinite, spare
+};
+
+hipError_t m2g_run(M2GScratch& s, double xmin, double ymin, double* val_grid, int gnx, int gny, const double* x_m,
+                   const double* y_m, const double* val_m, int64_t nm, double dx, double dy, double* val_wt_sum,
+                   double* wt_sum, int ws_nx, int ws_ny, hipStream_t st);
+
+// ---- RK4 / markers ----
+hipError_t launch_rk4(double* x_m, double* y_m, int64_t nm, const double* vx, int vx_n1, int vx_n2, const double* vy,
+                      int vy_n1, int vy_n2, double x_vx_min, double y_vx_min, double x_vy_min, double y_vy_min,
+                      double dt, double lx, double ly, double dx, double dy, unsigned long long* counters,
+                      hipStream_t s);
+hipError_t launch_init_markers(double* x_m, double* y_m, int nx, int ny, int md, double dx, double dy,
+                               double jitter_frac, uint64_t seed, hipStream_t s);
+struct Mat10 {
+    double v[10];
+};
+hipError_t launch_plume_properties(const double* x_m, const double* y_m, int64_t nm, Mat10 mat, double* rho_m,
+                                   double* mu_m, hipStream_t s);
+hipError_t launch_rt_properties(const double* x_m, const double* y_m, int64_t nm, Mat10 mat, double* rho_m,
+                                double* mu_m, hipStream_t s);
+
+}  // namespace sf

@@ -1,0 +1,351 @@
+"""Host-side mirror of the reference's hot-path API, running on libsfhip (gfx950).
+
+Functions keep the reference's names, argument order and meaning (Julia's trailing ``!``
+is dropped; arrays are still modified in place):
+
+* :func:`solveStokes`           -- scripts/StokesSolver_multixpu.jl:14-96
+* :func:`bilinearMarkerToGrid`  -- scripts/MarkerToGrid.jl:18-38
+* :func:`moveMarkersRK4`        -- scripts/MoveMarkers.jl:15-23
+* :func:`StokesFlow2D`          -- the timestep loop of scripts/StokesFlow2D_multixpu.jl:55-235
+                                   at one rank (no plotting / file output)
+
+Grid arrays are float64 CUDA (HIP) tensors in Julia's layout: shape ``(n1, n2)`` with
+strides ``(1, n1)`` (column-major, x fastest), so ``A[i, j]`` is the reference's
+``A[i+1, j+1]``.  :func:`fzeros` makes such arrays.  Marker arrays are 1-D float64 CUDA
+tensors.  Errors follow the reference: mismatched marker array lengths raise
+``AssertionError`` (its ``@assert``); data problems it only prints about are printed once
+with a count and returned in ``stats``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import M2GOut, RK4Out, SolveOut, StepArgs, check
+
+
+# --------------------------------------------------------------------------------------
+# arrays
+
+def fzeros(n1, n2, device="cuda"):
+    """Zeroed float64 (n1, n2) array in column-major (Julia) layout on ``device``."""
+    return torch.zeros((n2, n1), dtype=torch.float64, device=device).t()
+
+
+def from_numpy(a, device="cuda"):
+    """Copy a numpy array to the device, keeping Julia layout for 2-D arrays."""
+    a = np.asarray(a, dtype=np.float64)
+    if a.ndim == 1:
+        return torch.from_numpy(a.copy()).to(device)
+    f = np.asfortranarray(a)
+    return torch.from_numpy(np.ascontiguousarray(f.T)).to(device).t()
+
+
+def to_numpy(t):
+    """Device array -> numpy (Fortran order for 2-D arrays)."""
+    a = t.detach().cpu()
+    if a.dim() == 1:
+        return a.numpy().copy()
+    return np.asfortranarray(a.t().contiguous().numpy().T)
+
+
+def _grid_ptr(t, shape, name):
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a CUDA (HIP) tensor")
+    if t.dtype != torch.float64:
+        raise TypeError(f"{name} must be float64")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if t.numel() > 1 and t.stride() != (1, shape[0]):
+        raise ValueError(f"{name} must be column-major (strides (1, {shape[0]})), got {t.stride()}")
+    return t.data_ptr()
+
+
+def _grid_shape(t, name):
+    if not isinstance(t, torch.Tensor) or t.dim() != 2:
+        raise TypeError(f"{name} must be a 2-D tensor")
+    return tuple(t.shape)
+
+
+def _marker_ptr(t, name):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64:
+        raise TypeError(f"{name} must be a float64 CUDA tensor")
+    if t.dim() != 1 or (t.numel() > 1 and t.stride(0) != 1):
+        raise ValueError(f"{name} must be a contiguous 1-D tensor")
+    return t.data_ptr()
+
+
+# --------------------------------------------------------------------------------------
+# contexts
+
+class Context:
+    """Owns an ``sf_ctx``: device scratch for an Nx x Ny grid (0 x 0: markers only)."""
+
+    def __init__(self, nx=0, ny=0, stream=None):
+        self.lib = _lib.load()
+        self.nx, self.ny = nx, ny
+        h = C.c_void_p()
+        check(self.lib.sf_ctx_create(C.byref(h), nx, ny, stream))
+        self.handle = h
+
+    def use_current_stream(self):
+        s = torch.cuda.current_stream().cuda_stream
+        check(self.lib.sf_ctx_set_stream(self.handle, C.c_void_p(s)))
+        return self.handle
+
+    def close(self):
+        if self.handle:
+            self.lib.sf_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_contexts = {}
+
+
+def context(nx=0, ny=0):
+    key = (nx, ny, torch.cuda.current_device())
+    ctx = _contexts.get(key)
+    if ctx is None:
+        ctx = Context(nx, ny)
+        _contexts[key] = ctx
+    ctx.use_current_stream()
+    return ctx
+
+
+# --------------------------------------------------------------------------------------
+# solveStokes!
+
+def solveStokes(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtauPt, Rx, Ry, dVxdtau, dVydtau,
+                dtauVx, dtauVy, Vx_s, Vy_s, g_y, dx, dy, Nx, Ny, dt, maxdisp, comm=None, *,
+                use_free_surface_stabilization=True, eps=1e-8, print_info=True, iterMax=100000, stats=None):
+    """Pseudo-transient Stokes solve (StokesSolver_multixpu.jl:14-96).  Returns (dt, T_eff, runtime).
+
+    ``comm`` is accepted for signature compatibility and must be None (one rank).  The
+    scratch arrays txx, tyy, txy, divV, dtauPt, Rx, Ry, dtauVx, dtauVy may be None.
+    """
+    if comm is not None:
+        raise ValueError("one-rank build: comm must be None")
+    nx, ny = int(Nx), int(Ny)
+    sh = {
+        "P": (nx - 1, ny - 1), "Vx_pad": (nx + 2, ny + 1), "Vy_pad": (nx + 1, ny + 2), "rho_vy": (nx + 1, ny),
+        "mu_b": (nx, ny), "mu_p": (nx - 1, ny - 1), "txx": (nx - 1, ny - 1), "tyy": (nx - 1, ny - 1),
+        "txy": (nx, ny), "divV": (nx - 1, ny - 1), "dtauPt": (nx - 1, ny - 1), "Rx": (nx - 2, ny - 1),
+        "Ry": (nx - 1, ny - 2), "dVxdtau": (nx - 2, ny - 1), "dVydtau": (nx - 1, ny - 2),
+        "dtauVx": (nx - 2, ny - 1), "dtauVy": (nx - 1, ny - 2), "Vx_s": (nx, ny + 1), "Vy_s": (nx + 1, ny),
+    }
+    args = dict(P=P, Vx_pad=Vx_pad, Vy_pad=Vy_pad, rho_vy=rho_vy, mu_b=mu_b, mu_p=mu_p, txx=txx, tyy=tyy, txy=txy,
+                divV=divV, dtauPt=dtauPt, Rx=Rx, Ry=Ry, dVxdtau=dVxdtau, dVydtau=dVydtau, dtauVx=dtauVx,
+                dtauVy=dtauVy, Vx_s=Vx_s, Vy_s=Vy_s)
+    ptrs = [_grid_ptr(args[k], sh[k], k) for k in sh]
+    ctx = context(nx, ny)
+    out = SolveOut()
+    check(ctx.lib.sf_solve_stokes(ctx.handle, *ptrs, float(g_y), float(dx), float(dy), nx, ny, float(dt),
+                                  float(maxdisp), int(bool(use_free_surface_stabilization)), float(eps),
+                                  int(iterMax), int(bool(print_info)), C.byref(out)))
+    if stats is not None:
+        stats.update(out.as_dict())
+    return out.dt_new, out.t_eff, out.runtime
+
+
+def pt_iterations(n_iter, P, Vx_s, Vy_s, dVxdtau, dVydtau, rho_vy, mu_b, mu_p, g_y, dx, dy, Nx, Ny, dt):
+    """``n_iter`` PT loop bodies at a fixed stabilisation dt (no dt refresh, no check)."""
+    nx, ny = int(Nx), int(Ny)
+    ptrs = [
+        _grid_ptr(P, (nx - 1, ny - 1), "P"), _grid_ptr(Vx_s, (nx, ny + 1), "Vx_s"),
+        _grid_ptr(Vy_s, (nx + 1, ny), "Vy_s"), _grid_ptr(dVxdtau, (nx - 2, ny - 1), "dVxdtau"),
+        _grid_ptr(dVydtau, (nx - 1, ny - 2), "dVydtau"), _grid_ptr(rho_vy, (nx + 1, ny), "rho_vy"),
+        _grid_ptr(mu_b, (nx, ny), "mu_b"), _grid_ptr(mu_p, (nx - 1, ny - 1), "mu_p"),
+    ]
+    ctx = context(nx, ny)
+    check(ctx.lib.sf_pt_iterations(ctx.handle, int(n_iter), *ptrs, float(g_y), float(dx), float(dy), nx, ny,
+                                   float(dt)))
+
+
+# --------------------------------------------------------------------------------------
+# bilinearMarkerToGrid!
+
+def bilinearMarkerToGrid(x_grid_min, y_grid_min, val_grid, x_m, y_m, val_m, dx, dy, val_wt_sum=None,
+                         wt_sum=None, grid=None, *, stats=None, print_info=True):
+    """Marker -> grid interpolation (MarkerToGrid.jl:18-38), deterministic.
+
+    ``val_wt_sum`` / ``wt_sum`` may be None; ``grid`` (ImplicitGlobalGrid) must be None.
+    """
+    if grid is not None:
+        raise ValueError("one-rank build: grid must be None")
+    gnx, gny = _grid_shape(val_grid, "val_grid")
+    nm = x_m.numel()
+    assert y_m.numel() == nm and val_m.numel() == nm
+    pv = _grid_ptr(val_grid, (gnx, gny), "val_grid")
+    if (val_wt_sum is None) != (wt_sum is None):
+        raise ValueError("pass both helper arrays or neither")
+    ws_nx = ws_ny = 0
+    pvw = pw = None
+    if wt_sum is not None:
+        ws_nx, ws_ny = _grid_shape(wt_sum, "wt_sum")
+        pw = _grid_ptr(wt_sum, (ws_nx, ws_ny), "wt_sum")
+        pvw = _grid_ptr(val_wt_sum, (ws_nx, ws_ny), "val_wt_sum")
+    ctx = context()
+    out = M2GOut()
+    check(ctx.lib.sf_marker_to_grid(ctx.handle, float(x_grid_min), float(y_grid_min), pv, gnx, gny,
+                                    _marker_ptr(x_m, "x_m"), _marker_ptr(y_m, "y_m"), _marker_ptr(val_m, "val_m"),
+                                    nm, float(dx), float(dy), pvw, pw, ws_nx, ws_ny, C.byref(out)))
+    if print_info and out.n_bad_markers:
+        print(f"ATTENTION: erroneous marker position ({out.n_bad_markers} markers)")
+    if print_info and out.n_empty_nodes:
+        print(f"PROBLEM: no markers close enough for value interpolation at {out.n_empty_nodes} nodes")
+    if stats is not None:
+        stats.update(n_bad_markers=out.n_bad_markers, n_empty_nodes=out.n_empty_nodes)
+
+
+# --------------------------------------------------------------------------------------
+# moveMarkersRK4!
+
+def moveMarkersRK4(x_m, y_m, Vx, Vy, x_vx_min, y_vx_min, x_vy_min, y_vy_min, dt, lx, ly, dx, dy, *, stats=None,
+                   print_info=True):
+    """RK4 marker advection (MoveMarkers.jl:15-69), positions updated in place."""
+    nm = x_m.numel()
+    assert nm == y_m.numel(), "x and y Marker coords have not same length"
+    n1x, n2x = _grid_shape(Vx, "Vx")
+    n1y, n2y = _grid_shape(Vy, "Vy")
+    ctx = context()
+    out = RK4Out()
+    check(ctx.lib.sf_move_markers_rk4(ctx.handle, _marker_ptr(x_m, "x_m"), _marker_ptr(y_m, "y_m"), nm,
+                                      _grid_ptr(Vx, (n1x, n2x), "Vx"), n1x, n2x, _grid_ptr(Vy, (n1y, n2y), "Vy"),
+                                      n1y, n2y, float(x_vx_min), float(y_vx_min), float(x_vy_min), float(y_vy_min),
+                                      float(dt), float(lx), float(ly), float(dx), float(dy), C.byref(out)))
+    if print_info and out.n_clamped:
+        print(f"WARNING: V-interpolation index failsafe used {out.n_clamped} times")
+    if stats is not None:
+        stats.update(n_clamped=out.n_clamped)
+
+
+# --------------------------------------------------------------------------------------
+# markers
+
+def init_markers(Nx, Ny, md, dx, dy, jitter_frac=0.25, seed=0, device="cuda"):
+    """Seeded md x md markers per cell (x outer / y inner) with +-jitter_frac cell jitter."""
+    n = md * (Nx - 1) * md * (Ny - 1)
+    x = torch.empty(n, dtype=torch.float64, device=device)
+    y = torch.empty(n, dtype=torch.float64, device=device)
+    ctx = context()
+    check(ctx.lib.sf_init_markers(ctx.handle, x.data_ptr(), y.data_ptr(), int(Nx), int(Ny), int(md), float(dx),
+                                  float(dy), float(jitter_frac), int(seed) & 0xFFFFFFFFFFFFFFFF))
+    return x, y
+
+
+def plume_properties(x_m, y_m, air_h, cx, cy, r, rho_air, rho_plume, rho_bg, mu_air, mu_plume, mu_bg):
+    """Density/viscosity of the plume setups (air layer, disc, background) per marker."""
+    mat = (C.c_double * 10)(air_h, cx, cy, r, rho_air, rho_plume, rho_bg, mu_air, mu_plume, mu_bg)
+    rho = torch.empty_like(x_m)
+    mu = torch.empty_like(x_m)
+    ctx = context()
+    check(ctx.lib.sf_plume_properties(ctx.handle, _marker_ptr(x_m, "x_m"), _marker_ptr(y_m, "y_m"), x_m.numel(),
+                                      mat, rho.data_ptr(), mu.data_ptr()))
+    return rho, mu
+
+
+def rt_properties(x_m, y_m, y0, amp, k, lx, rho_up, mu_up, rho_low, mu_low):
+    """Two-layer Rayleigh-Taylor material: upper layer where y < y0 + amp sin(2 pi k x / lx)."""
+    mat = (C.c_double * 8)(y0, amp, k, lx, rho_up, mu_up, rho_low, mu_low)
+    rho = torch.empty_like(x_m)
+    mu = torch.empty_like(x_m)
+    ctx = context()
+    check(ctx.lib.sf_rt_properties(ctx.handle, _marker_ptr(x_m, "x_m"), _marker_ptr(y_m, "y_m"), x_m.numel(), mat,
+                                   rho.data_ptr(), mu.data_ptr()))
+    return rho, mu
+
+
+# --------------------------------------------------------------------------------------
+# timestep loop
+
+def grid_origins(dx, dy):
+    """Grid origins of StokesFlow2D_multixpu.jl:117-126."""
+    return {
+        "rho": (0 * dx - 0.5 * dx, 0 * dy), "mu_b": (0.0, 0.0), "mu_p": (0 * dx + 0.5 * dx, 0 * dy + 0.5 * dy),
+        "vx": (-1 * dx, 0 * dy - 0.5 * dy), "vy": (0 * dx - 0.5 * dx, -1 * dy),
+    }
+
+
+class FlowState:
+    """Device arrays of StokesFlow2D_multixpu.jl:89-109 (the persistent solver state)."""
+
+    def __init__(self, nx, ny, device="cuda"):
+        self.nx, self.ny = nx, ny
+        z = lambda a, b: fzeros(a, b, device)  # noqa: E731
+        self.P = z(nx - 1, ny - 1)
+        self.Vx = z(nx + 2, ny + 1)
+        self.Vy = z(nx + 1, ny + 2)
+        self.rho_vy = z(nx + 1, ny)
+        self.mu_b = z(nx, ny)
+        self.mu_p = z(nx - 1, ny - 1)
+        self.dVxdtau = z(nx - 2, ny - 1)
+        self.dVydtau = z(nx - 1, ny - 2)
+        self.Vx_small = z(nx, ny + 1)
+        self.Vy_small = z(nx + 1, ny)
+
+
+def step(state, x_m, y_m, rho_m, mu_m, dx, dy, dt, *, g_y=9.81, maxdisp=0.5, eps=1e-8, iterMax=100000,
+         use_free_surface_stabilization=True):
+    """One timestep (StokesFlow2D_multixpu.jl:164-181): M2G x3 -> solve -> RK4.  Returns (dt_new, stats)."""
+    nm = x_m.numel()
+    assert y_m.numel() == nm and rho_m.numel() == nm and mu_m.numel() == nm
+    s = state
+    ctx = context(s.nx, s.ny)
+    a = StepArgs()
+    a.nx, a.ny, a.dx, a.dy = s.nx, s.ny, float(dx), float(dy)
+    a.g_y, a.maxdisp, a.eps, a.iter_max = float(g_y), float(maxdisp), float(eps), int(iterMax)
+    a.use_fss, a.dt = int(bool(use_free_surface_stabilization)), float(dt)
+    nx, ny = s.nx, s.ny
+    a.P = _grid_ptr(s.P, (nx - 1, ny - 1), "P")
+    a.Vx_pad = _grid_ptr(s.Vx, (nx + 2, ny + 1), "Vx")
+    a.Vy_pad = _grid_ptr(s.Vy, (nx + 1, ny + 2), "Vy")
+    a.rho_vy = _grid_ptr(s.rho_vy, (nx + 1, ny), "rho_vy")
+    a.mu_b = _grid_ptr(s.mu_b, (nx, ny), "mu_b")
+    a.mu_p = _grid_ptr(s.mu_p, (nx - 1, ny - 1), "mu_p")
+    a.dVxdtau = _grid_ptr(s.dVxdtau, (nx - 2, ny - 1), "dVxdtau")
+    a.dVydtau = _grid_ptr(s.dVydtau, (nx - 1, ny - 2), "dVydtau")
+    a.Vx_s = _grid_ptr(s.Vx_small, (nx, ny + 1), "Vx_small")
+    a.Vy_s = _grid_ptr(s.Vy_small, (nx + 1, ny), "Vy_small")
+    a.x_m, a.y_m = _marker_ptr(x_m, "x_m"), _marker_ptr(y_m, "y_m")
+    a.rho_m, a.mu_m = _marker_ptr(rho_m, "rho_m"), _marker_ptr(mu_m, "mu_m")
+    a.nm = nm
+    so = SolveOut()
+    mo = (M2GOut * 3)()
+    ro = RK4Out()
+    check(ctx.lib.sf_step(ctx.handle, C.byref(a), C.byref(so), mo, C.byref(ro)))
+    stats = so.as_dict()
+    stats["m2g"] = [(m.n_bad_markers, m.n_empty_nodes) for m in mo]
+    stats["n_clamped"] = ro.n_clamped
+    return a.dt, stats
+
+
+def StokesFlow2D(Nt, Nx, Ny, Lx_glob, Ly_glob, x_m, y_m, rho_m, mu_m, *, stokes_eps=1e-5, g_y=9.81, maxdisp=0.5,
+                 iterMax=100000, print_info=False, state=None):
+    """Timestep loop of StokesFlow2D (StokesFlow2D_multixpu.jl:55-235) at one rank.
+
+    The reference evaluates density/viscosity callbacks on a marker lattice it builds
+    itself; here the caller passes the marker arrays (see :func:`init_markers` and the
+    material helpers).  Returns (Vx, Vy, t_tot) with Vx, Vy the padded velocity arrays.
+    """
+    dx = Lx_glob / (Nx - 1)
+    dy = Ly_glob / (Ny - 1)
+    st = state if state is not None else FlowState(Nx, Ny, device=x_m.device)
+    dt, t_tot = 0.0, 0.0
+    for t in range(Nt):
+        if print_info:
+            print(f"(t, t_tot) = ({t + 1}, {t_tot})")
+        dt, stats = step(st, x_m, y_m, rho_m, mu_m, dx, dy, dt, g_y=g_y, maxdisp=maxdisp, eps=stokes_eps,
+                         iterMax=iterMax)
+        t_tot += dt
+    return st.Vx, st.Vy, t_tot

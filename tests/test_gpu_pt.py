@@ -1,0 +1,154 @@
+"""PT solver on the GPU (C ABI through sfhip) against the CPU oracle and the reference's
+golden vectors.  FP64 iterations are compared bit for bit: the kernels evaluate the
+reference's expressions in the same order with no FMA contraction."""
+import numpy as np
+import pytest
+import torch
+
+import sfhip
+from oracle import sf_oracle as O
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("P", "Vx_s", "Vy_s", "dVx", "dVy")
+
+
+def to_gpu(S, keys):
+    return {k: sfhip.from_numpy(S[k]) for k in keys}
+
+
+def run_gpu_iterations(S, nx, ny, n_iter, g_y, dx, dy, dt):
+    G = to_gpu(S, STATE + ("rho_vy", "mu_b", "mu_p"))
+    sfhip.pt_iterations(n_iter, G["P"], G["Vx_s"], G["Vy_s"], G["dVx"], G["dVy"], G["rho_vy"], G["mu_b"], G["mu_p"],
+                        g_y, dx, dy, nx, ny, dt)
+    torch.cuda.synchronize()
+    return {k: sfhip.to_numpy(G[k]) for k in STATE}
+
+
+def oracle_iterations(S, nx, ny, n_iter, g_y, dx, dy, dt):
+    O.compute_timesteps(S, nx, ny, dx, dy)
+    O.pt_iterations(S, nx, ny, n_iter, g_y, dx, dy, dt)
+    return {k: S[k] for k in STATE}
+
+
+@pytest.mark.parametrize("n_iter", [1, 2, 7, 60])
+def test_iterations_bitwise_golden_inputs(gpu, golden, n_iter):
+    g = golden["stokes_solver"]
+    nx, ny = g["nx"], g["ny"]
+    S = cases.stokes_solver_inputs(g)
+    rng = np.random.default_rng(n_iter)
+    for k in STATE:  # a non-trivial state so every term is exercised
+        S[k][:] = rng.random(S[k].shape) - 0.5
+    got = run_gpu_iterations(S, nx, ny, n_iter, g["g_y"], g["dx"], g["dy"], 0.013)
+    exp = oracle_iterations(S, nx, ny, n_iter, g["g_y"], g["dx"], g["dy"], 0.013)
+    for k in STATE:
+        assert np.array_equal(got[k], exp[k]), (k, np.max(np.abs(got[k] - exp[k])))
+
+
+# sizes around the strip width (62 output columns) and the 64-row segments, minimal grids
+@pytest.mark.parametrize("nx,ny", [(3, 3), (4, 7), (5, 3), (62, 10), (63, 65), (64, 64), (65, 129), (125, 66),
+                                   (200, 31)])
+def test_iterations_bitwise_sizes(gpu, nx, ny):
+    S = cases.random_state(nx, ny, seed=nx * 1000 + ny)
+    got = run_gpu_iterations(S, nx, ny, 5, 1.7, 0.11, 0.07, 0.02)
+    exp = oracle_iterations(S, nx, ny, 5, 1.7, 0.11, 0.07, 0.02)
+    for k in STATE:
+        assert np.array_equal(got[k], exp[k]), (nx, ny, k)
+
+
+def gpu_solve(S, nx, ny, g_y, dx, dy, dt, maxdisp, use_fss, eps, iter_max, scratch=True):
+    keys = list(S.keys())
+    G = to_gpu(S, keys)
+    st = {}
+    opt = (lambda k: G[k]) if scratch else (lambda k: None)
+    res = sfhip.solveStokes(G["P"], G["Vx_pad"], G["Vy_pad"], G["rho_vy"], G["mu_b"], G["mu_p"], opt("txx"),
+                            opt("tyy"), opt("txy"), opt("divV"), opt("dtauP"), opt("Rx"), opt("Ry"), G["dVx"],
+                            G["dVy"], opt("dtauVx"), opt("dtauVy"), G["Vx_s"], G["Vy_s"], g_y, dx, dy, nx, ny, dt,
+                            maxdisp, use_free_surface_stabilization=use_fss, eps=eps, print_info=False,
+                            iterMax=iter_max, stats=st)
+    torch.cuda.synchronize()
+    return {k: sfhip.to_numpy(G[k]) for k in keys}, res, st
+
+
+def test_solve_golden_and_oracle(gpu, golden):
+    g = golden["stokes_solver"]
+    nx, ny = g["nx"], g["ny"]
+    S = cases.stokes_solver_inputs(g)
+    R, (dt, t_eff, runtime), st = gpu_solve(S, nx, ny, g["g_y"], g["dx"], g["dy"], 0.0, g["maxdisp"], True, g["eps"],
+                                            100000)
+    exp = O.solve_stokes(S, g["g_y"], g["dx"], g["dy"], nx, ny, 0.0, g["maxdisp"], True, g["eps"])
+    assert st["converged"] and st["niter"] == exp["niter"]
+    assert st["err"] == pytest.approx(exp["err"], rel=1e-12)
+    assert dt == exp["dt_new"]
+    # every array the reference leaves filled is bit-identical to the oracle's
+    for k in S:
+        assert np.array_equal(R[k], S[k]), k
+    inds = np.asarray(g["test_inds"]) - 1
+    vx = R["Vx_pad"].ravel(order="F")[inds]
+    vy = R["Vy_pad"].ravel(order="F")[inds]
+    for got, ref in ((vx, np.array(g["vx_ref"])), (vy, np.array(g["vy_ref"]))):
+        assert np.linalg.norm(got - ref) <= g["rtol"] * max(np.linalg.norm(got), np.linalg.norm(ref))
+    Vx, Vy = R["Vx_pad"], R["Vy_pad"]
+    assert np.all(Vx[0, :] == -Vx[2, :]) and np.all(Vx[1, :] == 0.0) and np.all(Vx[-2, :] == 0.0)
+    assert np.all(Vy[:, 0] == -Vy[:, 2]) and np.all(Vy[:, 1] == 0.0) and np.all(Vy[:, -2] == 0.0)
+    assert np.all(Vx[:, 0] == Vx[:, 1]) and np.all(Vy[0, :] == Vy[1, :])
+
+
+@pytest.mark.parametrize("iter_max,use_fss", [(37, True), (0, True), (400, False)])
+def test_solve_iter_cap_and_no_stabilisation(gpu, iter_max, use_fss):
+    nx, ny = 40, 33
+    S = cases.smooth_state(nx, ny, seed=5)
+    R, (dt, _, _), st = gpu_solve(S, nx, ny, 1.0, 0.03, 0.025, 0.004, 0.5, use_fss, 1e-14, iter_max)
+    exp = O.solve_stokes(S, 1.0, 0.03, 0.025, nx, ny, 0.004, 0.5, use_fss, 1e-14, iter_max)
+    assert np.isfinite(exp["err"]) or iter_max < 165
+    assert st["niter"] == exp["niter"] == iter_max and not st["converged"]
+    assert dt == exp["dt_new"]
+    for k in ("P", "Vx_s", "Vy_s", "dVx", "dVy", "Vx_pad", "Vy_pad", "txx", "tyy", "txy", "divV", "dtauP", "dtauVx",
+              "dtauVy"):
+        assert np.array_equal(R[k], S[k]), k
+    if iter_max >= 200:  # at least one check happened: residual arrays match too
+        assert np.array_equal(R["Rx"], S["Rx"]) and np.array_equal(R["Ry"], S["Ry"])
+
+
+def test_solve_without_scratch_arrays(gpu):
+    nx, ny = 30, 41
+    S = cases.smooth_state(nx, ny, seed=8, contrast=2.0)
+    R, (dt, _, _), st = gpu_solve(S, nx, ny, 1.0, 1 / 29, 1 / 40, 0.0, 0.5, True, 1e-8, 100000, scratch=False)
+    exp = O.solve_stokes(S, 1.0, 1 / 29, 1 / 40, nx, ny, 0.0, 0.5, True, 1e-8)
+    assert st["niter"] == exp["niter"] and st["converged"]
+    for k in ("P", "Vx_s", "Vy_s", "dVx", "dVy", "Vx_pad", "Vy_pad"):
+        assert np.array_equal(R[k], S[k]), k
+
+
+def test_solve_rejects_bad_shapes(gpu):
+    nx, ny = 10, 12
+    S = cases.random_state(nx, ny, seed=1)
+    G = to_gpu(S, list(S.keys()))
+    with pytest.raises(ValueError):
+        sfhip.solveStokes(G["P"], G["Vx_pad"], G["Vy_pad"], G["rho_vy"], G["mu_b"], G["mu_p"], None, None, None, None,
+                          None, None, None, G["dVx"], G["dVy"], None, None, G["Vx_s"], G["Vy_s"], 1.0, 0.1, 0.1,
+                          nx + 1, ny, 0.0, 0.5, print_info=False)
+    with pytest.raises(ValueError):
+        sfhip.solveStokes(G["P"].t().contiguous().t().contiguous(), G["Vx_pad"], G["Vy_pad"], G["rho_vy"],
+                          G["mu_b"], G["mu_p"], None, None, None, None, None, None, None, G["dVx"], G["dVy"], None,
+                          None, G["Vx_s"], G["Vy_s"], 1.0, 0.1, 0.1, nx, ny, 0.0, 0.5, print_info=False)
+
+
+def test_divergence_reproduced(gpu):
+    """Air viscosity 1e-3 against a matrix of 1 (63 x 63 cells): the reference PT scheme blows
+    up.  The GPU path stops at the same check iteration with the same non-finite err."""
+    nx = ny = 64
+    dx, dy, x, y, rho, mu = cases.plume_markers(nx, ny, 4, seed=1001, mu_air=1e-3)
+    S = O.solver_arrays(nx, ny)
+    vws, ws = O.zeros(nx + 1, ny), O.zeros(nx + 1, ny)
+    o = O.grid_origins(dx, dy)
+    O.marker_to_grid(*o["rho"], S["rho_vy"], x, y, rho, dx, dy, vws, ws)
+    O.marker_to_grid(*o["mu_b"], S["mu_b"], x, y, mu, dx, dy, vws, ws)
+    O.marker_to_grid(*o["mu_p"], S["mu_p"], x, y, mu, dx, dy, vws, ws)
+    R, (dt, _, _), st = gpu_solve(S, nx, ny, 1.0, dx, dy, 0.0, 0.5, True, 1e-8, 100000)
+    exp = O.solve_stokes(S, 1.0, dx, dy, nx, ny, 0.0, 0.5, True, 1e-8)
+    assert st["niter"] == exp["niter"] == 960
+    assert np.isnan(st["err"]) and np.isnan(exp["err"]) and not st["converged"]
+    assert np.isnan(dt) and np.isnan(exp["dt_new"])
