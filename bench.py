@@ -1,0 +1,160 @@
+"""Benchmark: pseudo-transient Stokes iterations on one MI355X (FP64).
+
+Workload (one GPU): 4095 x 4095 cells (Nx = Ny = 4096 nodes), 16 markers per cell
+(268,304,400 markers) with the plume material set-up.  Markers are generated on the device,
+and the three material grids are built by the marker -> grid pass (untimed set-up).  A
+"step" is one PT iteration of solveStokes! over the whole grid.  W untimed iterations come
+first, then exactly K timed ones.  This is the reference's own headline measurement
+(StokesSolver_scaling.jl:29-152: solveStokes! on a 4096^2 grid, PT iterations timed from
+iteration 11, T_eff = 6 Nx Ny 8 B / t_it).
+
+Printed: one JSON line with PT iterations/s (value), the reference's T_eff, the roofline of
+the fused iteration kernel (algorithmic bytes / average launch time against 8 TB/s), and the
+CPU oracle timed on a bounded sample of iterations on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "stokesflow.jl_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+REF_PT_ITERS_PER_S = 108.5  # reference at 4096^2 on 1 GPU: 190 timed iterations in 1.7516 s
+
+
+def bytes_per_iter(nx, ny):
+    """Algorithmic bytes of one fused PT iteration: read+write Vx, Vy, P, dVx, dVy; read mu_p,
+    mu_b, rho_vy (FP64)."""
+    n_vx, n_vy, n_p = nx * (ny + 1), (nx + 1) * ny, (nx - 1) * (ny - 1)
+    n_dvx, n_dvy = (nx - 2) * (ny - 1), (nx - 1) * (ny - 2)
+    return 8 * (2 * (n_vx + n_vy + n_p + n_dvx + n_dvy) + n_p + nx * ny + (nx + 1) * ny)
+
+
+def profiled_traffic(nx):
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3 summary of this
+    workload (FETCH_SIZE and WRITE_SIZE, each scaled by a copy kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", f"pt_fused_f64_n{nx}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(nx, ny, iters):
+    """The CPU oracle (C, OpenMP) timed on `iters` PT iterations of the same grid size."""
+    import numpy as np
+
+    from oracle import sf_oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    S = O.solver_arrays(nx, ny)
+    rng = np.random.default_rng(0)
+    for k in ("rho_vy", "mu_b", "mu_p"):
+        S[k][:] = 1.0 + rng.random(S[k].shape)
+    dx = dy = 1.0 / (nx - 1)
+    O.compute_timesteps(S, nx, ny, dx, dy)
+    O.pt_iterations(S, nx, ny, 1, 1.0, dx, dy, 0.0)  # first touch
+    t0 = time.perf_counter()
+    O.pt_iterations(S, nx, ny, iters, 1.0, dx, dy, 0.0)
+    t = time.perf_counter() - t0
+    return {"value": iters / t, "unit": "PT iters/s", "cores": threads, "kind": "port",
+            "sample": f"{iters} PT iterations of the oracle on {nx}x{ny} nodes (FP64), {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=4096, help="nodes per direction")
+    ap.add_argument("--cpu-iters", type=int, default=300, help="oracle iterations timed for cpu_baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import sfhip
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    nx = ny = a.n
+    dx = dy = 1.0 / (nx - 1)
+    md = 4
+
+    # set-up (untimed): markers on the device, material grids by marker -> grid
+    x_m, y_m = sfhip.init_markers(nx, ny, md, dx, dy, 0.25, seed=3000)
+    rho_m, mu_m = sfhip.plume_properties(x_m, y_m, 0.1, 0.5, 0.8, 0.1, 0.001, 0.9, 1.0, 0.01, 0.1, 1.0)
+    st = sfhip.FlowState(nx, ny)
+    o = sfhip.grid_origins(dx, dy)
+    sfhip.bilinearMarkerToGrid(*o["rho"], st.rho_vy, x_m, y_m, rho_m, dx, dy, print_info=False)
+    sfhip.bilinearMarkerToGrid(*o["mu_b"], st.mu_b, x_m, y_m, mu_m, dx, dy, print_info=False)
+    sfhip.bilinearMarkerToGrid(*o["mu_p"], st.mu_p, x_m, y_m, mu_m, dx, dy, print_info=False)
+    n_markers = x_m.numel()
+    del x_m, y_m, rho_m, mu_m
+    torch.cuda.empty_cache()
+
+    def solve(iters):
+        stats = {}
+        sfhip.solveStokes(st.P, st.Vx, st.Vy, st.rho_vy, st.mu_b, st.mu_p, None, None, None, None, None, None, None,
+                          st.dVxdtau, st.dVydtau, None, None, st.Vx_small, st.Vy_small, 1.0, dx, dy, nx, ny, 0.0, 0.5,
+                          eps=1e-300, print_info=False, iterMax=iters, stats=stats)
+        return stats
+
+    if a.warmup > 0:
+        solve(a.warmup)
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    stats = solve(a.steps)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_s = ev0.elapsed_time(ev1) * 1e-3
+    finite = bool(torch.isfinite(st.Vx_small).all().item())
+
+    per_iter = gpu_s / a.steps  # HIP-event time per launch of the fused kernel (plus launch gaps)
+    b_iter = bytes_per_iter(nx, ny)
+    achieved = b_iter / per_iter / 1e9
+    traffic = profiled_traffic(nx)
+    line = {
+        "metric": "PT iters/s + T_eff HBM GB/s; time-to-1e-8 FP32-refined vs FP64; batched solves/s",
+        "value": a.steps / wall,
+        "unit": "PT iters/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": (a.steps / wall) / REF_PT_ITERS_PER_S if nx == 4096 else None,
+        "dtype": "f64",
+        "data": "synthetic: seeded plume markers generated on the device",
+        "config": {"workload": f"solveStokes! PT iterations, {nx - 1}x{ny - 1} cells ({nx}x{ny} nodes), "
+                               f"{n_markers} markers (16 per cell), FP64",
+                   "nx": nx, "ny": ny, "markers": n_markers, "parallelism": "single GPU"},
+        "t_eff_gbs": 6 * nx * ny * 8 / (wall / a.steps) / 1e9,
+        "alg_gbs": b_iter / (wall / a.steps) / 1e9,
+        "state_finite": finite,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "pt_fused_f64", "bytes_alg_per_launch": b_iter, "launch_s": per_iter},
+        "cpu_baseline": None if a.no_cpu else cpu_baseline(nx, ny, a.cpu_iters),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -73,6 +73,12 @@ int sf_ctx_create(sf_ctx **ctx, int nx, int ny, void *hip_stream);
 int sf_ctx_destroy(sf_ctx *ctx);
 /* Replace the stream work is enqueued on. */
 int sf_ctx_set_stream(sf_ctx *ctx, void *hip_stream);
+/* Launch tuning of the PT iteration kernel; results do not depend on it.  rows: grid rows
+ * each tile marches through (0 = size heuristic); depth: rows of operands prefetched (1 or
+ * 2); max_waves: 0 = one wave per resident slot (persistent), > 0 = at most that many
+ * waves, < 0 = one wave per tile.  No reference counterpart (ParallelStencil picks launch
+ * shapes itself). */
+int sf_ctx_set_pt_tuning(sf_ctx *ctx, int rows, int depth, int max_waves);
 
 /* solveStokes! -- replaces scripts/StokesSolver_multixpu.jl:14-96
  * (solveStokes!(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtauPt, Rx, Ry,

@@ -48,7 +48,10 @@ struct sf_ctx {
     unsigned long long *hcounters = nullptr;  // pinned host mirror
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     M2GScratch m2g;
-    int seg = 64;  // PT rows per tile
+    // PT launch tuning (results do not depend on it)
+    int seg = 0;        // rows per tile; 0 = size heuristic
+    int depth = 1;      // rows of operands prefetched
+    int max_waves = -1; // 0 persistent (resident slots), > 0 cap, < 0 one wave per tile
 };
 
 extern "C" int sf_version(void) { return SF_VERSION_NUM; }
@@ -124,6 +127,17 @@ extern "C" int sf_ctx_destroy(sf_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     else (void)hipDeviceSynchronize();
     free_ctx(c);
+    return SF_OK;
+}
+
+extern "C" int sf_ctx_set_pt_tuning(sf_ctx* c, int rows, int depth, int max_waves)
+{
+    if (!c) return set_err(SF_ERR_ARG, "sf_ctx_set_pt_tuning: NULL ctx");
+    if (rows < 0 || rows > 4096) return set_err(SF_ERR_ARG, "sf_ctx_set_pt_tuning: rows %d outside [0, 4096]", rows);
+    if (depth < 1 || depth > 2) return set_err(SF_ERR_ARG, "sf_ctx_set_pt_tuning: depth %d not 1 or 2", depth);
+    c->seg = rows;
+    c->depth = depth;
+    c->max_waves = max_waves;
     return SF_OK;
 }
 
@@ -231,6 +245,8 @@ extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_
     pp.nx = nx;
     pp.ny = ny;
     pp.seg = c->seg;
+    pp.depth = c->depth;
+    pp.max_waves = c->max_waves;
     pp.dvx = dVx;
     pp.dvy = dVy;
     pp.rho = rho_vy;
@@ -384,6 +400,8 @@ extern "C" int sf_pt_iterations(sf_ctx* c, int64_t n_iter, double* P, double* Vx
     pp.nx = nx;
     pp.ny = ny;
     pp.seg = c->seg;
+    pp.depth = c->depth;
+    pp.max_waves = c->max_waves;
     pp.flags = 0;
     pp.dvx = dVx;
     pp.dvy = dVy;

@@ -14,8 +14,10 @@ enum { SC_DT = 0, SC_DT_CHECK = 1, SC_ERR = 2, SC_COUNT = 8 };
 
 struct PTParams {
     int nx, ny;           // node counts
-    int seg;              // basic-node rows per tile
+    int seg;              // basic-node rows per tile (<= 0: size heuristic)
     int flags;            // PT_WRITE_TAU | PT_WRITE_RX
+    int depth;            // rows of operands prefetched (1 or 2)
+    int max_waves;        // 0: one wave per resident slot, > 0: cap, < 0: one wave per tile
     int nstrips, ntiles;  // filled by the launcher
     const double* vx_a;   // Vx_s (nx, ny+1), state in
     const double* vy_a;   // Vy_s (nx+1, ny)
@@ -58,6 +60,8 @@ struct RedParams {
 };
 
 int pt_num_strips(int nx);
+int pt_default_rows(int nx, int ny);
+int pt_resident_waves(int depth);
 int red_num_blocks(int64_t nmax);
 hipError_t launch_pt_fused(const PTParams& p, hipStream_t s);
 hipError_t launch_pt_resid_y(const ResYParams& p, hipStream_t s);

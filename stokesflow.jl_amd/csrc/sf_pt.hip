@@ -23,6 +23,8 @@
 // pt_resid_y computes Ry with post-update V and dt_check exactly as compute_Residuals!
 // (:241-249).  Reductions use a fixed block count and a fixed combine order, so err and
 // dt are deterministic.
+#include <algorithm>
+
 #include "sf_device.h"
 #include "sf_internal.h"
 
@@ -40,20 +42,47 @@ __device__ inline int xcd_remap(int k, int total)
     return (xcd < rem) ? xcd * (bpx + 1) + idx : rem * (bpx + 1) + (xcd - rem) * bpx + idx;
 }
 
-// Always-in-bounds guarded load: out-of-range lanes read element 0 and get 0.
-__device__ inline double ldg0(const double* __restrict__ a, int64_t k, bool ok)
+
+// Operands one row step consumes (row r of the cell/basic quantities, row r+1 of V/rho,
+// row r-1 of dVx/dVy).
+struct RowIn {
+    double vx1, vy1, rho1, pold, mup, mub, dvx, dvy;
+};
+
+// Per-lane column offsets, clamped into each array so that every load stays in bounds.
+// Lanes whose column lies outside an array read a neighbouring in-array value; such
+// values only reach lanes and rows whose results are never stored (every stored output
+// reads in-array operands only), so no select is needed.
+struct LaneCols {
+    unsigned vx, vy, p, dvx, dvy;  // Vx/mu_b (nx), Vy/rho (nx+1), P/mu_p (nx-1), dVx (nx-2), dVy (nx-1)
+};
+
+// Uniform row index clamped into [0, nrows-1].
+__device__ inline int crow(int r, int nrows) { return r < 0 ? 0 : (r >= nrows ? nrows - 1 : r); }
+
+__device__ inline RowIn load_row(const PTParams& p, const LaneCols& lc, int r, bool need_dv)
 {
-    double v = a[ok ? k : 0];
-    return ok ? v : 0.0;
+    const int nx = p.nx, ny = p.ny;
+    RowIn in;
+    in.vx1 = (p.vx_a + (int64_t)crow(r + 1, ny + 1) * nx)[lc.vx];
+    in.vy1 = (p.vy_a + (int64_t)crow(r + 1, ny) * (nx + 1))[lc.vy];
+    in.rho1 = (p.rho + (int64_t)crow(r + 1, ny) * (nx + 1))[lc.vy];
+    in.pold = (p.p_a + (int64_t)crow(r, ny - 1) * (nx - 1))[lc.p];
+    in.mup = (p.mu_p + (int64_t)crow(r, ny - 1) * (nx - 1))[lc.p];
+    in.mub = (p.mu_b + (int64_t)crow(r, ny) * nx)[lc.vx];
+    if (need_dv) {
+        in.dvx = (p.dvx + (int64_t)crow(r - 1, ny - 1) * (nx - 2))[lc.dvx];
+        in.dvy = (p.dvy + (int64_t)crow(r - 1, ny - 2) * (nx - 1))[lc.dvy];
+    } else {
+        in.dvx = 0.0;
+        in.dvy = 0.0;
+    }
+    return in;
 }
 
-__global__ __launch_bounds__(64) void pt_fused_f64(PTParams p)
+template <int DEPTH, bool CHECK>
+__device__ inline void pt_tile(const PTParams& p, int strip, int segid)
 {
-    const int nstrips = p.nstrips;
-    const int logical = xcd_remap(blockIdx.x, p.ntiles);
-    const int strip = logical % nstrips;
-    const int segid = logical / nstrips;
-
     const int lane = threadIdx.x;
     const int c = strip * PT_OUT_LANES + lane;
     const int nx = p.nx, ny = p.ny;
@@ -61,16 +90,26 @@ __global__ __launch_bounds__(64) void pt_fused_f64(PTParams p)
     const int r0 = segid * p.seg;
     const int r1 = min(r0 + p.seg, ny);
     const double dt = *p.dt;
-    const int64_t lvx = nx, lvy = nx + 1, lp = nx - 1, lr = nx + 1;
+    const int64_t lvx = nx, lvy = nx + 1, lp = nx - 1;
     const bool c_vx = c < nx;      // Vx_s, mu_b, txy columns
-    const bool c_vy = c <= nx;     // Vy_s, rho_vy columns
     const bool c_p = c < nx - 1;   // P, mu_p columns
     const double rdx = p.rdx, rdy = p.rdy;
+    LaneCols lc;
+    lc.vx = (unsigned)min(c, nx - 1);
+    lc.vy = (unsigned)min(c, nx);
+    lc.p = (unsigned)min(c, nx - 2);
+    lc.dvx = (unsigned)min(c, nx - 3);
+    lc.dvy = (unsigned)min(c, nx - 2);
 
     int r = (r0 == 0) ? 0 : r0 - 1;
-    double vx_r = ldg0(p.vx_a, ix2(c, r, lvx), c_vx);
-    double vy_r = ldg0(p.vy_a, ix2(c, r, lvy), c_vy);
-    double rho_r = ldg0(p.rho, ix2(c, r, lr), c_vy);
+    double vx_r = (p.vx_a + (int64_t)r * nx)[lc.vx];
+    double vy_r = (p.vy_a + (int64_t)r * (nx + 1))[lc.vy];
+    double rho_r = (p.rho + (int64_t)r * (nx + 1))[lc.vy];
+    // the prologue row needs no dVx/dVy, and prefetch never reaches past the segment end
+    RowIn nxt = load_row(p, lc, r, r >= r0);
+    RowIn nxt2;
+    if constexpr (DEPTH > 1)
+        if (r + 1 < r1) nxt2 = load_row(p, lc, r + 1, r + 1 >= r0);
     double vx_r_p1 = lane_down(vx_r, 1);
     double vy_r_p1 = lane_down(vy_r, 1);
     double rho_r_p1 = lane_down(rho_r, 1);
@@ -82,22 +121,23 @@ __global__ __launch_bounds__(64) void pt_fused_f64(PTParams p)
     double mup_prev = 0.0, mup_prev_p1 = 0.0;
 
     for (; r < r1; ++r) {
+        const RowIn in = nxt;
+        if constexpr (DEPTH > 1) {
+            nxt = nxt2;
+            if (r + 2 < r1) nxt2 = load_row(p, lc, r + 2, true);  // two rows of operands in flight
+        } else {
+            if (r + 1 < r1) nxt = load_row(p, lc, r + 1, true);   // next row's operands in flight
+        }
         const bool emit = r >= r0;           // false on the prologue row of a segment
-        const bool row_next = r + 1 <= ny - 1;  // Vy_s / rho_vy have ny rows
         const bool cell_row = r <= ny - 2;   // P-node rows
         const int j = r - 1;                 // output (dVx/dVy) row of this step
         const bool has_out = emit && j >= 0;
         const bool dvx_ok = has_out && c < nx - 2;
         const bool dvy_ok = has_out && c < nx - 1 && j <= ny - 3;
 
-        const double vx_r1 = ldg0(p.vx_a, ix2(c, r + 1, lvx), c_vx);
-        const double vy_r1 = ldg0(p.vy_a, ix2(c, r + 1, lvy), c_vy && row_next);
-        const double rho_r1 = ldg0(p.rho, ix2(c, r + 1, lr), c_vy && row_next);
-        const double pold = ldg0(p.p_a, ix2(c, r, lp), c_p && cell_row);
-        const double mup = ldg0(p.mu_p, ix2(c, r, lp), c_p && cell_row);
-        const double mub = ldg0(p.mu_b, ix2(c, r, lvx), c_vx);
-        const double dvx_old = ldg0(p.dvx, ix2(c, j, nx - 2), dvx_ok);
-        const double dvy_old = ldg0(p.dvy, ix2(c, j, nx - 1), dvy_ok);
+        const double vx_r1 = in.vx1, vy_r1 = in.vy1, rho_r1 = in.rho1;
+        const double pold = in.pold, mup = in.mup, mub = in.mub;
+        const double dvx_old = in.dvx, dvy_old = in.dvy;
 
         const double vx_r1_p1 = lane_down(vx_r1, 1);
         const double vy_r1_p1 = lane_down(vy_r1, 1);
@@ -119,13 +159,16 @@ __global__ __launch_bounds__(64) void pt_fused_f64(PTParams p)
         if (emit && out_lane) {
             if (c_p && cell_row) {
                 p.p_b[ix2(c, r, lp)] = pn;
-                if (p.flags & PT_WRITE_TAU) {
-                    p.divv[ix2(c, r, lp)] = div;
-                    p.txx[ix2(c, r, lp)] = txx;
-                    p.tyy[ix2(c, r, lp)] = tyy;
+                if constexpr (CHECK) {
+                    if (p.flags & PT_WRITE_TAU) {
+                        p.divv[ix2(c, r, lp)] = div;
+                        p.txx[ix2(c, r, lp)] = txx;
+                        p.tyy[ix2(c, r, lp)] = tyy;
+                    }
                 }
             }
-            if ((p.flags & PT_WRITE_TAU) && c_vx) p.txy[ix2(c, r, lvx)] = txy;
+            if constexpr (CHECK)
+                if ((p.flags & PT_WRITE_TAU) && c_vx) p.txy[ix2(c, r, lvx)] = txy;
         }
 
         if (has_out && out_lane) {
@@ -151,7 +194,8 @@ __global__ __launch_bounds__(64) void pt_fused_f64(PTParams p)
                 p.vx_b[ix2(c + 1, j + 1, lvx)] = vx_new;
                 if (j == 0) p.vx_b[ix2(c + 1, 0, lvx)] = vx_new;        // bc_y_noflux!
                 if (j == ny - 2) p.vx_b[ix2(c + 1, ny, lvx)] = vx_new;
-                if (p.flags & PT_WRITE_RX) p.rx[ix2(c, j, nx - 2)] = rx;
+                if constexpr (CHECK)
+                    if (p.flags & PT_WRITE_RX) p.rx[ix2(c, j, nx - 2)] = rx;
             }
             if (c == 0 || c == nx - 2) {                                 // bc_x_zero!
                 const int col = (c == 0) ? 0 : nx - 1;
@@ -190,6 +234,21 @@ __global__ __launch_bounds__(64) void pt_fused_f64(PTParams p)
         rho_r = rho_r1;
         rho_r_p1 = rho_r1_p1;
         rho_r_p2 = rho_r1_p2;
+    }
+}
+
+// Launch: one wave per tile (max_waves < 0) or at most G waves grid-striding over the
+// tiles.  In each round the waves of one XCD take a contiguous band of tiles (strips
+// fastest), so the rows and columns two tiles share are read once through that XCD's L2.
+template <int DEPTH, bool CHECK>
+__global__ __launch_bounds__(64) void pt_fused_f64(PTParams p)
+{
+    const int G = gridDim.x;
+    for (int base = 0; base < p.ntiles; base += G) {
+        const int nround = min(G, p.ntiles - base);
+        if ((int)blockIdx.x >= nround) break;
+        const int logical = base + xcd_remap(blockIdx.x, nround);
+        pt_tile<DEPTH, CHECK>(p, logical % p.nstrips, logical / p.nstrips);
     }
 }
 
@@ -316,13 +375,53 @@ __global__ __launch_bounds__(256) void fill_timesteps(int nx, int ny, const doub
 
 int pt_num_strips(int nx) { return (nx + PT_OUT_LANES - 1) / PT_OUT_LANES; }
 
+// Waves of pt_fused_f64<depth> the current device holds at once (occupancy x CUs).
+int pt_resident_waves(int depth)
+{
+    static int cached[64][2] = {};
+    int dev = 0;
+    const int d = depth > 1 ? 1 : 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 4096;
+    if (cached[dev][d] == 0) {
+        int per_cu = 0, cus = 0;
+        hipError_t e = d ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pt_fused_f64<2, false>, 64, 0)
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pt_fused_f64<1, false>, 64, 0);
+        if (e != hipSuccess || per_cu < 1) per_cu = 4;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+        cached[dev][d] = per_cu * cus;
+    }
+    return cached[dev][d];
+}
+
+// Rows per tile.  Short tiles measured fastest on MI355X at every size from 512^2 to 4096^2
+// (one wave per tile, all strips sweeping the same few rows at once; the one-row prologue
+// a tile re-reads was just fetched into the same XCD's L2 by the tile above it).
+int pt_default_rows(int nx, int ny)
+{
+    (void)nx;
+    return ny >= 4 ? 4 : ny;
+}
+
 hipError_t launch_pt_fused(const PTParams& p0, hipStream_t s)
 {
     PTParams p = p0;
+    if (p.seg <= 0) p.seg = pt_default_rows(p.nx, p.ny);
+    const int depth = p.depth > 1 ? 2 : 1;
     p.nstrips = pt_num_strips(p.nx);
     const int nseg = (p.ny + p.seg - 1) / p.seg;
     p.ntiles = p.nstrips * nseg;
-    hipLaunchKernelGGL(pt_fused_f64, dim3(p.ntiles), dim3(64), 0, s, p);
+    // max_waves: 0 = one wave per resident slot (persistent), < 0 = one wave per tile
+    int grid = p.ntiles;
+    if (p.max_waves == 0) grid = std::min(p.ntiles, pt_resident_waves(depth));
+    else if (p.max_waves > 0) grid = std::min(p.ntiles, p.max_waves);
+    const bool check = p.flags != 0;
+    if (depth == 2) {
+        if (check) hipLaunchKernelGGL((pt_fused_f64<2, true>), dim3(grid), dim3(64), 0, s, p);
+        else hipLaunchKernelGGL((pt_fused_f64<2, false>), dim3(grid), dim3(64), 0, s, p);
+    } else {
+        if (check) hipLaunchKernelGGL((pt_fused_f64<1, true>), dim3(grid), dim3(64), 0, s, p);
+        else hipLaunchKernelGGL((pt_fused_f64<1, false>), dim3(grid), dim3(64), 0, s, p);
+    }
     return hipGetLastError();
 }
 
