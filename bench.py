@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--n", type=int, default=4096, help="nodes per direction")
     ap.add_argument("--cpu-iters", type=int, default=300, help="oracle iterations timed for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a 1 GiB device-to-device copy first (reference size for rocprofv3 byte counters)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
@@ -91,6 +93,12 @@ def main():
     nx = ny = a.n
     dx = dy = 1.0 / (nx - 1)
     md = 4
+    if a.calibrate_copy:  # 2^30 bytes read + 2^30 bytes written, profiled beside the PT kernel
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # set-up (untimed): markers on the device, material grids by marker -> grid
     x_m, y_m = sfhip.init_markers(nx, ny, md, dx, dy, 0.25, seed=3000)

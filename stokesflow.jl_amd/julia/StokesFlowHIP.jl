@@ -1,0 +1,147 @@
+"""
+    StokesFlowHIP
+
+Drop-in replacements for the StokesFlow.jl hot path on one AMD MI355X (gfx950), calling
+libsfhip through its C ABI (include/sf_hip.h).  The functions keep the reference's names,
+argument order and return values:
+
+  * `solveStokes!`          -- scripts/StokesSolver_multixpu.jl:14-96
+  * `bilinearMarkerToGrid!` -- scripts/MarkerToGrid.jl:18-38
+  * `moveMarkersRK4!`       -- scripts/MoveMarkers.jl:15-23
+
+Arrays are `AMDGPU.ROCArray{Float64}` (column-major, x fastest: the reference's
+`Data.Array` layout), passed to the library as device pointers.  One rank: `comm` and
+`grid` are accepted and ignored, as ImplicitGlobalGrid/MPI are identities at one rank.
+
+Julia is not available where this package is built and tested; the same ABI is exercised
+by the Python binding (stokesflow.jl_amd/sfhip).
+"""
+module StokesFlowHIP
+
+using AMDGPU
+
+export solveStokes!, bilinearMarkerToGrid!, moveMarkersRK4!
+
+const libsfhip = joinpath(@__DIR__, "..", "libsfhip.so")
+
+const SF_OK = Cint(0)
+
+struct SolveOut
+    dt_new::Cdouble
+    err::Cdouble
+    t_eff::Cdouble
+    runtime::Cdouble
+    niter::Int64
+    converged::Int32
+    reserved::Int32
+    dt_last::Cdouble
+    t_it::Cdouble
+    gbs_alg::Cdouble
+end
+SolveOut() = SolveOut(0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
+
+struct M2GOut
+    n_bad_markers::Int64
+    n_empty_nodes::Int64
+end
+
+struct RK4Out
+    n_clamped::Int64
+end
+
+function check(rc::Cint)
+    if rc != SF_OK
+        msg = unsafe_string(ccall((:sf_last_error, libsfhip), Cstring, ()))
+        error("libsfhip error $rc: $msg")
+    end
+    return nothing
+end
+
+# one context per (Nx, Ny, device); (0, 0) is the marker-only context
+const CONTEXTS = Dict{Tuple{Int,Int,Int},Ptr{Cvoid}}()
+
+function context(nx::Integer, ny::Integer)
+    key = (Int(nx), Int(ny), AMDGPU.device_id(AMDGPU.device()))
+    h = get(CONTEXTS, key, C_NULL)
+    if h == C_NULL
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:sf_ctx_create, libsfhip), Cint, (Ptr{Ptr{Cvoid}}, Cint, Cint, Ptr{Cvoid}),
+                    r, nx, ny, C_NULL))
+        h = r[]
+        CONTEXTS[key] = h
+    end
+    # enqueue on the task-local HIP stream AMDGPU.jl uses
+    check(ccall((:sf_ctx_set_stream, libsfhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), h,
+                AMDGPU.stream().stream))
+    return h
+end
+
+dptr(A) = A === nothing ? Ptr{Float64}(C_NULL) : reinterpret(Ptr{Float64}, pointer(A))
+
+"""
+    solveStokes!(P, Vx_pad, Vy_pad, ρ_vy, μ_b, μ_p, τxx, τyy, τxy, ∇V, dτPt, Rx, Ry, dVxdτ, dVydτ,
+                 dτVx, dτVy, Vx_s, Vy_s, g_y, dx, dy, Nx, Ny, dt, maxdisp, comm;
+                 use_free_surface_stabilization=true, ϵ=1e-8, print_info=true, iterMax=100000)
+        -> (dt, T_eff, runtime)
+"""
+function solveStokes!(P, Vx_pad, Vy_pad, ρ_vy, μ_b, μ_p, τxx, τyy, τxy, ∇V, dτPt, Rx, Ry, dVxdτ, dVydτ,
+                      dτVx, dτVy, Vx_s, Vy_s, g_y, dx, dy, Nx, Ny, dt, maxdisp, comm;
+                      use_free_surface_stabilization::Bool=true, ϵ=1e-8, print_info::Bool=true,
+                      iterMax=100000)
+    @assert size(P) == (Nx - 1, Ny - 1) && size(Vx_s) == (Nx, Ny + 1) && size(Vy_s) == (Nx + 1, Ny)
+    @assert size(Vx_pad) == (Nx + 2, Ny + 1) && size(Vy_pad) == (Nx + 1, Ny + 2)
+    h = context(Nx, Ny)
+    out = Ref(SolveOut())
+    check(ccall((:sf_solve_stokes, libsfhip), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                 Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                 Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                 Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Cdouble, Cint, Cint, Cdouble, Cdouble,
+                 Cint, Cdouble, Int64, Cint, Ptr{SolveOut}),
+                h, dptr(P), dptr(Vx_pad), dptr(Vy_pad), dptr(ρ_vy), dptr(μ_b), dptr(μ_p), dptr(τxx),
+                dptr(τyy), dptr(τxy), dptr(∇V), dptr(dτPt), dptr(Rx), dptr(Ry), dptr(dVxdτ), dptr(dVydτ),
+                dptr(dτVx), dptr(dτVy), dptr(Vx_s), dptr(Vy_s), g_y, dx, dy, Nx, Ny, dt, maxdisp,
+                use_free_surface_stabilization, ϵ, iterMax, print_info, out))
+    return out[].dt_new, out[].t_eff, out[].runtime
+end
+
+"""
+    bilinearMarkerToGrid!(x_grid_min, y_grid_min, val_grid, x_m, y_m, val_m, dx, dy, val_wt_sum, wt_sum, grid)
+"""
+function bilinearMarkerToGrid!(x_grid_min, y_grid_min, val_grid, x_m, y_m, val_m, dx, dy, val_wt_sum, wt_sum,
+                               grid=nothing)
+    Nx, Ny = size(val_grid)
+    Nm = size(x_m, 1)
+    @assert (size(y_m, 1) == Nm) && (size(val_m, 1) == Nm)
+    h = context(0, 0)
+    out = Ref(M2GOut(0, 0))
+    wnx, wny = wt_sum === nothing ? (0, 0) : size(wt_sum)
+    check(ccall((:sf_marker_to_grid, libsfhip), Cint,
+                (Ptr{Cvoid}, Cdouble, Cdouble, Ptr{Float64}, Cint, Cint, Ptr{Float64}, Ptr{Float64},
+                 Ptr{Float64}, Int64, Cdouble, Cdouble, Ptr{Float64}, Ptr{Float64}, Cint, Cint, Ptr{M2GOut}),
+                h, x_grid_min, y_grid_min, dptr(val_grid), Nx, Ny, dptr(x_m), dptr(y_m), dptr(val_m), Nm,
+                dx, dy, dptr(val_wt_sum), dptr(wt_sum), wnx, wny, out))
+    out[].n_bad_markers > 0 && println("ATTENTION: erroneous marker position ($(out[].n_bad_markers) markers)")
+    out[].n_empty_nodes > 0 &&
+        println("PROBLEM: no markers close enough for value interpolation at $(out[].n_empty_nodes) nodes")
+    return nothing
+end
+
+"""
+    moveMarkersRK4!(x_m, y_m, Vx, Vy, x_vx_min, y_vx_min, x_vy_min, y_vy_min, dt, lx, ly, dx, dy)
+"""
+function moveMarkersRK4!(x_m, y_m, Vx, Vy, x_vx_min, y_vx_min, x_vy_min, y_vy_min, dt, lx, ly, dx, dy)
+    Nm = length(x_m)
+    @assert (Nm == length(y_m)) "x and y Marker coords have not same length"
+    h = context(0, 0)
+    out = Ref(RK4Out(0))
+    check(ccall((:sf_move_markers_rk4, libsfhip), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64}, Cint, Cint, Ptr{Float64}, Cint,
+                 Cint, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble,
+                 Ptr{RK4Out}),
+                h, dptr(x_m), dptr(y_m), Nm, dptr(Vx), size(Vx, 1), size(Vx, 2), dptr(Vy), size(Vy, 1),
+                size(Vy, 2), x_vx_min, y_vx_min, x_vy_min, y_vy_min, dt, lx, ly, dx, dy, out))
+    return nothing
+end
+
+end # module
