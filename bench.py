@@ -68,6 +68,44 @@ def cpu_baseline(nx, ny, iters):
             "sample": f"{iters} PT iterations of the oracle on {nx}x{ny} nodes (FP64), {t:.1f} s"}
 
 
+def time_to_solution(n, seed=3000):
+    """Time-to-1e-8 of the FP64 solver and of FP32 sweeps in FP64 defect correction on the
+    plume set-up at n x n nodes (16 markers per cell), from the same initial state."""
+    import torch
+
+    import sfhip
+
+    dx = dy = 1.0 / (n - 1)
+    x, y = sfhip.init_markers(n, n, 4, dx, dy, 0.25, seed=seed)
+    rho, mu = sfhip.plume_properties(x, y, 0.1, 0.5, 0.8, 0.1, 0.001, 0.9, 1.0, 0.01, 0.1, 1.0)
+    res = {}
+    fields = {}
+    for mode in ("fp64", "fp32_refined"):
+        st = sfhip.FlowState(n, n)
+        o = sfhip.grid_origins(dx, dy)
+        sfhip.bilinearMarkerToGrid(*o["rho"], st.rho_vy, x, y, rho, dx, dy, print_info=False)
+        sfhip.bilinearMarkerToGrid(*o["mu_b"], st.mu_b, x, y, mu, dx, dy, print_info=False)
+        sfhip.bilinearMarkerToGrid(*o["mu_p"], st.mu_p, x, y, mu, dx, dy, print_info=False)
+        stats = {}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sfhip.solveStokes(st.P, st.Vx, st.Vy, st.rho_vy, st.mu_b, st.mu_p, None, None, None, None, None, None, None,
+                          st.dVxdtau, st.dVydtau, None, None, st.Vx_small, st.Vy_small, 1.0, dx, dy, n, n, 0.0, 0.5,
+                          eps=1e-8, print_info=False, iterMax=10_000_000, stats=stats,
+                          mixed_precision=(mode != "fp64"))
+        torch.cuda.synchronize()
+        res[mode] = {"seconds": time.perf_counter() - t0, "converged": bool(stats["converged"]),
+                     "err": stats["err"], "pt_iters": stats["niter"]}
+        if mode != "fp64":
+            res[mode]["rebases"] = stats["n_outer"]
+        fields[mode] = (st.Vx.clone(), st.Vy.clone(), st.P.clone())
+        del st
+    rel = {k: ((a - b).norm() / b.norm()).item()
+           for k, a, b in zip(("Vx", "Vy", "P"), fields["fp32_refined"], fields["fp64"])}
+    return {"n": n, "tolerance": 1e-8, **res, "speedup": res["fp64"]["seconds"] / res["fp32_refined"]["seconds"],
+            "refined_rel_l2_vs_fp64": rel}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +114,8 @@ def main():
     ap.add_argument("--n", type=int, default=4096, help="nodes per direction")
     ap.add_argument("--cpu-iters", type=int, default=300, help="oracle iterations timed for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tts-n", type=int, default=2048,
+                    help="nodes per direction of the time-to-1e-8 comparison (FP64 vs FP32-refined); 0 skips it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy first (reference size for rocprofv3 byte counters)")
     a = ap.parse_args()
@@ -161,6 +201,10 @@ def main():
                      "kernel": "pt_fused_f64", "bytes_alg_per_launch": b_iter, "launch_s": per_iter},
         "cpu_baseline": None if a.no_cpu else cpu_baseline(nx, ny, a.cpu_iters),
     }
+    if a.tts_n > 0:
+        del st
+        torch.cuda.empty_cache()
+        line["time_to_1e-8"] = time_to_solution(a.tts_n)
     print(json.dumps(line), flush=True)
 
 

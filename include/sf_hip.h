@@ -95,6 +95,41 @@ int sf_solve_stokes(sf_ctx *ctx, double *P, double *Vx_pad, double *Vy_pad, cons
                     double dy, int nx, int ny, double dt, double maxdisp, int use_fss, double eps,
                     int64_t iter_max, int print_info, sf_solve_out *out);
 
+/* FP32 PT sweeps inside FP64 defect correction.  Same arguments, shapes and outputs as
+ * sf_solve_stokes.  The PT operator is affine in (V, P) for a fixed stabilisation dt.
+ * Every rebase_interval iterations the FP64 residual of the current state is computed
+ * exactly; the next window of PT iterations runs in FP32 on the correction with that
+ * residual as forcing, and the correction is added to the state in FP64.  The PT momentum
+ * is carried from window to window (and starts from, and returns to, dVxdtau/dVydtau),
+ * so the windows continue one PT iteration; dt is refreshed every ndtupdate iterations as
+ * the reference does.  Convergence is declared by one FP64 PT iteration whose err
+ * (computed exactly as compute_err) is <= eps, so the returned state meets the
+ * reference's own test.  No reference counterpart: an added solver mode beside
+ * solveStokes!. */
+typedef struct {
+    int64_t rebase_interval; /* FP32 iterations between FP64 re-bases (<= 0: the reference's ncheck) */
+    int64_t iter_max;        /* cap on FP32 iterations (<= 0: 100000, the reference's default iterMax) */
+    int32_t verbose;         /* 1: one line per re-base on stderr */
+    int32_t reserved;
+} sf_refine_params;
+
+typedef struct {
+    int32_t n_outer;     /* re-bases (FP32 windows) */
+    int32_t reserved;
+    int64_t n_fp32_iter; /* FP32 PT iterations in total */
+    int64_t n_fp64_iter; /* FP64 PT (check) iterations */
+    double err_fp64;     /* err of the returned state, as compute_err computes it */
+    double seconds;      /* wall time of the solve */
+} sf_refine_out;
+
+int sf_solve_stokes_refined(sf_ctx *ctx, double *P, double *Vx_pad, double *Vy_pad, const double *rho_vy,
+                            const double *mu_b, const double *mu_p, double *txx, double *tyy, double *txy,
+                            double *divV, double *dtauP, double *Rx, double *Ry, double *dVxdtau,
+                            double *dVydtau, double *dtauVx, double *dtauVy, double *Vx_s, double *Vy_s,
+                            double g_y, double dx, double dy, int nx, int ny, double dt, double maxdisp,
+                            int use_fss, double eps, const sf_refine_params *prm, sf_solve_out *out,
+                            sf_refine_out *refine_out);
+
 /* n_iter iterations of the PT loop body (StokesSolver_multixpu.jl:48-58) at a fixed
  * stabilisation dt, with no dt refresh and no convergence check.  Building block for
  * fixed-iteration benchmarks and for bit-exact per-iteration parity. */

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -48,6 +49,11 @@ struct sf_ctx {
     unsigned long long *hcounters = nullptr;  // pinned host mirror
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     M2GScratch m2g;
+    // FP32 correction scratch (allocated on first refined solve)
+    float *r_vx[2] = {nullptr, nullptr}, *r_vy[2] = {nullptr, nullptr}, *r_p[2] = {nullptr, nullptr};
+    float *r_mx = nullptr, *r_my = nullptr, *r_rho = nullptr, *r_mub = nullptr, *r_mup = nullptr;
+    float *r_fx = nullptr, *r_fy = nullptr, *r_fc = nullptr, *r_rxo = nullptr, *r_ryo = nullptr, *r_rco = nullptr;
+    double* hpart = nullptr;  // pinned host copy of reduction partials
     // PT launch tuning (results do not depend on it)
     int seg = 0;        // rows per tile; 0 = size heuristic
     int depth = 1;      // rows of operands prefetched
@@ -71,6 +77,11 @@ static void free_ctx(sf_ctx* c)
                     c->m2g.rec_dx, c->m2g.rec_dy, c->m2g.rec_val, c->m2g.blocksums, c->m2g.counters};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    void* rbufs[] = {c->r_vx[0], c->r_vx[1], c->r_vy[0], c->r_vy[1], c->r_p[0], c->r_p[1], c->r_mx, c->r_my,
+                     c->r_rho, c->r_mub, c->r_mup, c->r_fx, c->r_fy, c->r_fc, c->r_rxo, c->r_ryo, c->r_rco};
+    for (void* b : rbufs)
+        if (b) (void)hipFree(b);
+    if (c->hpart) (void)hipHostFree(c->hpart);
     if (c->hscal) (void)hipHostFree(c->hscal);
     if (c->hcounters) (void)hipHostFree(c->hcounters);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -623,5 +634,315 @@ extern "C" int sf_rt_properties(sf_ctx* c, const double* x_m, const double* y_m,
     std::memcpy(m.v, mat, 8 * sizeof(double));
     HIPCHK(launch_rt_properties(x_m, y_m, nm, m, rho_m, mu_m, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return SF_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// FP32 PT sweeps inside FP64 defect correction
+
+static int refine_reserve(sf_ctx* c)
+{
+    if (c->r_vx[0]) return SF_OK;
+    const size_t nx = c->nx, ny = c->ny;
+    const size_t nvx = nx * (ny + 1), nvy = (nx + 1) * ny, np = (nx - 1) * (ny - 1);
+    const size_t nmx = (nx - 2) * (ny - 1), nmy = (nx - 1) * (ny - 2);
+    for (int b = 0; b < 2; ++b) {
+        HIPCHK(dalloc(&c->r_vx[b], nvx));
+        HIPCHK(dalloc(&c->r_vy[b], nvy));
+        HIPCHK(dalloc(&c->r_p[b], np));
+    }
+    HIPCHK(dalloc(&c->r_mx, nmx));
+    HIPCHK(dalloc(&c->r_my, nmy));
+    HIPCHK(dalloc(&c->r_rho, (nx + 1) * ny));
+    HIPCHK(dalloc(&c->r_mub, nx * ny));
+    HIPCHK(dalloc(&c->r_mup, np));
+    HIPCHK(dalloc(&c->r_fx, nmx));
+    HIPCHK(dalloc(&c->r_fy, nmy));
+    HIPCHK(dalloc(&c->r_fc, np));
+    HIPCHK(dalloc(&c->r_rxo, nmx));
+    HIPCHK(dalloc(&c->r_ryo, nmy));
+    HIPCHK(dalloc(&c->r_rco, np));
+    HIPCHK(hipHostMalloc((void**)&c->hpart, 3 * (size_t)RED_MAX_BLOCKS * sizeof(double), hipHostMallocDefault));
+    return SF_OK;
+}
+
+// One FP64 PT iteration in check mode from the caller's arrays (state copied back), giving
+// err exactly as compute_err does at a check iteration (StokesSolver_multixpu.jl:64-73).
+static int fp64_check_iteration(sf_ctx* c, PTParams pp, double* P, double* Vx_s, double* Vy_s, const double* rho_vy,
+                                double* r_y, double g_y, double dx, double dy, double maxdisp, int use_fss,
+                                double rdx, double rdy, double* err_out)
+{
+    const int nx = c->nx, ny = c->ny;
+    hipStream_t s = c->stream;
+    pp.flags = PT_WRITE_TAU | PT_WRITE_RX;
+    pp.vx_a = Vx_s;
+    pp.vy_a = Vy_s;
+    pp.p_a = P;
+    pp.vx_b = c->vx_b;
+    pp.vy_b = c->vy_b;
+    pp.p_b = c->p_b;
+    HIPCHK(launch_pt_fused(pp, s));
+    if (use_fss) {
+        int rc = enqueue_dt(c, c->vx_b, c->vy_b, dx, dy, maxdisp, 0);
+        if (rc) return rc;
+    } else {
+        HIPCHK(launch_set_scalar(c->scal + SC_DT_CHECK, 0.0, s));
+    }
+    ResYParams rq{};
+    rq.nx = nx;
+    rq.ny = ny;
+    rq.tyy = pp.tyy;
+    rq.txy = pp.txy;
+    rq.P = c->p_b;
+    rq.rho = rho_vy;
+    rq.vx = c->vx_b;
+    rq.vy = c->vy_b;
+    rq.dt = c->scal + SC_DT_CHECK;
+    rq.ry = r_y;
+    rq.g = g_y;
+    rq.rdx = rdx;
+    rq.rdy = rdy;
+    HIPCHK(launch_pt_resid_y(rq, s));
+    const double n_rx = (double)(nx - 2) * (ny - 1), n_ry = (double)(nx - 1) * (ny - 2), n_dv = (double)(nx - 1) * (ny - 1);
+    RedParams rp{};
+    rp.a[0] = pp.rx;
+    rp.n[0] = (int64_t)n_rx;
+    rp.a[1] = r_y;
+    rp.n[1] = (int64_t)n_ry;
+    rp.a[2] = pp.divv;
+    rp.n[2] = (int64_t)n_dv;
+    rp.part = c->part;
+    rp.mode = RED_SUM;
+    const int nb = red_num_blocks((int64_t)n_dv);
+    HIPCHK(launch_reduce(rp, 3, nb, s));
+    HIPCHK(launch_finalize_err(c->part, nb, n_rx, n_ry, n_dv, c->scal, s));
+    HIPCHK(hipMemcpyAsync(Vx_s, c->vx_b, sizeof(double) * (size_t)nx * (ny + 1), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(Vy_s, c->vy_b, sizeof(double) * (size_t)(nx + 1) * ny, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(P, c->p_b, sizeof(double) * (size_t)(nx - 1) * (ny - 1), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->hscal, c->scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *err_out = c->hscal[SC_ERR];
+    return SF_OK;
+}
+
+extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, double* Vy_pad, const double* rho_vy,
+                                       const double* mu_b, const double* mu_p, double* txx, double* tyy, double* txy,
+                                       double* divV, double* dtauP, double* Rx, double* Ry, double* dVx, double* dVy,
+                                       double* dtauVx, double* dtauVy, double* Vx_s, double* Vy_s, double g_y,
+                                       double dx, double dy, int nx, int ny, double dt, double maxdisp, int use_fss,
+                                       double eps, const sf_refine_params* prm, sf_solve_out* out,
+                                       sf_refine_out* rout)
+{
+    int rc = check_ctx(c, nx, ny, "sf_solve_stokes_refined");
+    if (rc) return rc;
+    if (!P || !Vx_pad || !Vy_pad || !rho_vy || !mu_b || !mu_p || !dVx || !dVy || !Vx_s || !Vy_s)
+        return set_err(SF_ERR_ARG, "sf_solve_stokes_refined: a state or material array is NULL");
+    if (!(dx > 0.0) || !(dy > 0.0)) return set_err(SF_ERR_ARG, "sf_solve_stokes_refined: dx, dy must be > 0");
+    const auto t_start = std::chrono::steady_clock::now();
+    hipStream_t s = c->stream;
+    const SolveConsts k = solve_consts(nx, ny, dx, dy);
+    if (!use_fss) dt = 0.0;
+    const int mn = std::min(nx, ny);
+    const int64_t ncheck = std::min<int64_t>(5 * (int64_t)mn, 2000);
+    const int64_t ndtupdate = 10 * (int64_t)mn;
+    const int64_t rebase = (prm && prm->rebase_interval > 0) ? prm->rebase_interval : ncheck;
+    const int64_t iter_max = (prm && prm->iter_max > 0) ? prm->iter_max : (int64_t)100000;
+    const int verbose = prm ? prm->verbose : 0;
+
+    rc = refine_reserve(c);
+    if (rc) return rc;
+    double* t_xx = txx ? txx : c->txx;
+    double* t_yy = tyy ? tyy : c->tyy;
+    double* t_xy = txy ? txy : c->txy;
+    double* d_v = divV ? divV : c->divv;
+    double* r_x = Rx ? Rx : c->rx;
+    double* r_y = Ry ? Ry : c->ry;
+    if (dtauP || dtauVx || dtauVy)
+        HIPCHK(launch_fill_timesteps(nx, ny, mu_p, dtauVx, dtauVy, dtauP, k.vsc_mdxy2, k.ptsc41, k.max_nxy, s));
+
+    const int64_t nvx = (int64_t)nx * (ny + 1), nvy = (int64_t)(nx + 1) * ny, np = (int64_t)(nx - 1) * (ny - 1);
+    const int64_t nmx = (int64_t)(nx - 2) * (ny - 1), nmy = (int64_t)(nx - 1) * (ny - 2);
+    HIPCHK(launch_cast_f32(c->r_rho, rho_vy, (int64_t)(nx + 1) * ny, s));
+    HIPCHK(launch_cast_f32(c->r_mub, mu_b, (int64_t)nx * ny, s));
+    HIPCHK(launch_cast_f32(c->r_mup, mu_p, np, s));
+    // the PT momentum continues from the caller's dVx/dVy (warm start across timesteps)
+    HIPCHK(launch_cast_f32(c->r_mx, dVx, nmx, s));
+    HIPCHK(launch_cast_f32(c->r_my, dVy, nmy, s));
+
+    PTParams pp{};
+    pp.nx = nx;
+    pp.ny = ny;
+    pp.seg = c->seg;
+    pp.depth = c->depth;
+    pp.max_waves = c->max_waves;
+    pp.dvx = dVx;
+    pp.dvy = dVy;
+    pp.rho = rho_vy;
+    pp.mu_b = mu_b;
+    pp.mu_p = mu_p;
+    pp.dt = c->scal + SC_DT;
+    pp.txx = t_xx;
+    pp.tyy = t_yy;
+    pp.txy = t_xy;
+    pp.divv = d_v;
+    pp.rx = r_x;
+    pp.g = g_y;
+    pp.dampx = k.dampx;
+    pp.dampy = k.dampy;
+    pp.rdx = k.rdx;
+    pp.rdy = k.rdy;
+    pp.vsc_mdxy2 = k.vsc_mdxy2;
+    pp.ptsc41 = k.ptsc41;
+    pp.max_nxy = k.max_nxy;
+
+    CorrParams cp{};
+    cp.nx = nx;
+    cp.ny = ny;
+    cp.seg = 4;
+    cp.mx = c->r_mx;
+    cp.my = c->r_my;
+    cp.rho = c->r_rho;
+    cp.mu_b = c->r_mub;
+    cp.mu_p = c->r_mup;
+    cp.fx = c->r_fx;
+    cp.fy = c->r_fy;
+    cp.fc = c->r_fc;
+    cp.rxo = c->r_rxo;
+    cp.ryo = c->r_ryo;
+    cp.rco = c->r_rco;
+    cp.g = (float)g_y;
+    cp.dampx = (float)k.dampx;
+    cp.dampy = (float)k.dampy;
+    cp.rdx = (float)k.rdx;
+    cp.rdy = (float)k.rdy;
+    cp.vsc_mdxy2 = (float)k.vsc_mdxy2;
+    cp.ptsc41 = (float)k.ptsc41;
+    cp.max_nxy = (float)k.max_nxy;
+
+    const double n_rx = (double)nmx, n_ry = (double)nmy, n_dv = (double)np;
+    const int nbr = red_num_blocks(np);
+    double dt_k = dt;
+    int64_t n32 = 0, n64 = 0;
+    int rebases = 0;
+    double err_ref = 2.0 * eps, err_k = 0.0;
+    bool converged = false;
+    while (n32 < iter_max) {
+        // FP64 residual of the current state: the exact forcing of the next FP32 window
+        ResidParams rp{};
+        rp.nx = nx;
+        rp.ny = ny;
+        rp.vx = Vx_s;
+        rp.vy = Vy_s;
+        rp.P = P;
+        rp.rho = rho_vy;
+        rp.mu_b = mu_b;
+        rp.mu_p = mu_p;
+        rp.fx = c->r_fx;
+        rp.fy = c->r_fy;
+        rp.fc = c->r_fc;
+        rp.part = c->part;
+        rp.dt = dt_k;
+        rp.g = g_y;
+        rp.rdx = k.rdx;
+        rp.rdy = k.rdy;
+        HIPCHK(launch_resid_f64(rp, nbr, s));
+        HIPCHK(hipMemcpyAsync(c->hpart, c->part, 3 * (size_t)nbr * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        double sx = 0.0, sy = 0.0, sc = 0.0;
+        for (int b = 0; b < nbr; ++b) {
+            sx += c->hpart[b];
+            sy += c->hpart[nbr + b];
+            sc += c->hpart[2 * nbr + b];
+        }
+        err_k = std::max(std::max(sx / n_rx, sy / n_ry), sc / n_dv);
+        if (verbose > 0)
+            fprintf(stderr, "FP32 iterations %lld: FP64 residual err %.3e, dt %.4e\n", (long long)n32, err_k, dt_k);
+        if (!std::isfinite(err_k)) break;
+        if (err_k <= eps) {
+            // confirm with the reference's own check: one FP64 PT iteration and compute_err
+            HIPCHK(launch_cast_f64(dVx, c->r_mx, nmx, s));
+            HIPCHK(launch_cast_f64(dVy, c->r_my, nmy, s));
+            HIPCHK(launch_set_scalar(c->scal + SC_DT, dt_k, s));
+            rc = fp64_check_iteration(c, pp, P, Vx_s, Vy_s, rho_vy, r_y, g_y, dx, dy, maxdisp, use_fss, k.rdx, k.rdy,
+                                      &err_ref);
+            if (rc) return rc;
+            n64 += 1;
+            HIPCHK(launch_cast_f32(c->r_mx, dVx, nmx, s));
+            HIPCHK(launch_cast_f32(c->r_my, dVy, nmy, s));
+            if (verbose > 0) fprintf(stderr, "  FP64 check iteration: err %.3e\n", err_ref);
+            if (err_ref <= eps) {
+                converged = true;
+                break;
+            }
+            continue;
+        }
+        // an FP32 window on the correction, the momentum carried over from the previous one
+        for (int b = 0; b < 2; ++b) {
+            HIPCHK(launch_fill_f32(c->r_vx[b], nvx, 0.f, s));
+            HIPCHK(launch_fill_f32(c->r_vy[b], nvy, 0.f, s));
+            HIPCHK(launch_fill_f32(c->r_p[b], np, 0.f, s));
+        }
+        cp.dt = (float)dt_k;
+        int cur = 0;
+        const int64_t steps = std::min(rebase, iter_max - n32);
+        for (int64_t it = 0; it < steps; ++it) {
+            cp.vx_a = c->r_vx[cur];
+            cp.vy_a = c->r_vy[cur];
+            cp.p_a = c->r_p[cur];
+            cp.vx_b = c->r_vx[1 - cur];
+            cp.vy_b = c->r_vy[1 - cur];
+            cp.p_b = c->r_p[1 - cur];
+            HIPCHK(launch_pt_corr(cp, false, s));
+            cur = 1 - cur;
+        }
+        const int64_t before = n32;
+        n32 += steps;
+        // fold the correction into the FP64 state
+        HIPCHK(launch_update_f64(Vx_s, c->r_vx[cur], nvx, s));
+        HIPCHK(launch_update_f64(Vy_s, c->r_vy[cur], nvy, s));
+        HIPCHK(launch_update_f64(P, c->r_p[cur], np, s));
+        rebases += 1;
+        // the reference refreshes the stabilisation dt every ndtupdate iterations (:60-62)
+        if (use_fss && (n32 / ndtupdate) > (before / ndtupdate)) {
+            rc = enqueue_dt(c, Vx_s, Vy_s, dx, dy, maxdisp, 0);
+            if (rc) return rc;
+            HIPCHK(hipMemcpyAsync(c->hscal, c->scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            dt_k = c->hscal[SC_DT_CHECK];
+        }
+    }
+    // hand the PT momentum back in FP64
+    if (!converged) {
+        HIPCHK(launch_cast_f64(dVx, c->r_mx, nmx, s));
+        HIPCHK(launch_cast_f64(dVy, c->r_my, nmy, s));
+    }
+
+    HIPCHK(launch_pad_mirror(nx, ny, Vx_s, Vy_s, Vx_pad, Vy_pad, s));
+    rc = enqueue_dt(c, Vx_s, Vy_s, dx, dy, maxdisp, 0);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->hscal, c->scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    if (!converged && std::isfinite(err_k)) err_ref = err_k;
+    if (out) {
+        out->dt_new = c->hscal[SC_DT_CHECK];
+        out->err = err_ref;
+        out->t_eff = 0.0;
+        out->runtime = secs;
+        out->niter = n32 + n64;
+        out->converged = converged;
+        out->reserved = 0;
+        out->dt_last = dt_k;
+        out->t_it = 0.0;
+        out->gbs_alg = 0.0;
+    }
+    if (rout) {
+        rout->n_outer = rebases;
+        rout->reserved = 0;
+        rout->n_fp32_iter = n32;
+        rout->n_fp64_iter = n64;
+        rout->err_fp64 = err_ref;
+        rout->seconds = secs;
+    }
     return SF_OK;
 }

@@ -77,6 +77,58 @@ hipError_t launch_pad_mirror(int nx, int ny, const double* vx, const double* vy,
 hipError_t launch_fill_timesteps(int nx, int ny, const double* mu_p, double* dtauvx, double* dtauvy, double* dtaup,
                                  double vsc_mdxy2, double ptsc41, double max_nxy, hipStream_t s);
 
+// ---- FP32 sweeps in FP64 defect correction ----
+struct CorrParams {
+    int nx, ny, seg, nstrips, ntiles;
+    const float* vx_a;  // correction state in (Vd_x, Vd_y, Pd)
+    const float* vy_a;
+    const float* p_a;
+    float* vx_b;  // correction state out
+    float* vy_b;
+    float* p_b;
+    float* mx;  // PT momentum of the correction (in place)
+    float* my;
+    const float* rho;  // FP32 materials
+    const float* mu_b;
+    const float* mu_p;
+    const float* fx;  // FP64 residual of the outer state, as forcing
+    const float* fy;
+    const float* fc;
+    float* rxo;  // check outputs: correction residuals
+    float* ryo;
+    float* rco;
+    float dt, g, dampx, dampy, rdx, rdy, vsc_mdxy2, ptsc41, max_nxy;
+};
+
+struct ResidParams {
+    int nx, ny;
+    const double* vx;
+    const double* vy;
+    const double* P;
+    const double* rho;
+    const double* mu_b;
+    const double* mu_p;
+    float* fx;
+    float* fy;
+    float* fc;
+    double* part;  // 3 x nblocks
+    double dt, g, rdx, rdy;
+};
+
+struct RedParamsF {
+    const float* a[3];
+    int64_t n[3];
+    double* part;
+};
+
+hipError_t launch_pt_corr(const CorrParams& p, bool check, hipStream_t s);
+hipError_t launch_resid_f64(const ResidParams& p, int nb, hipStream_t s);
+hipError_t launch_update_f64(double* a, const float* d, int64_t n, hipStream_t s);
+hipError_t launch_cast_f32(float* dst, const double* src, int64_t n, hipStream_t s);
+hipError_t launch_fill_f32(float* dst, int64_t n, float v, hipStream_t s);
+hipError_t launch_cast_f64(double* dst, const float* src, int64_t n, hipStream_t s);
+hipError_t launch_reduce_f32(const RedParamsF& p, int narr, int nb, hipStream_t s);
+
 // ---- marker -> grid ----
 struct M2GScratch {
     int64_t cap_markers = 0;

@@ -1,0 +1,365 @@
+// sf_refine.hip -- FP32 pseudo-transient sweeps inside FP64 defect correction (gfx950).
+//
+// The PT operator of solveStokes! is affine in (V, P) for a fixed stabilisation dt:
+//   Rx(u) = d(txx)/dx + d(txy)/dy - dP/dx,
+//   Ry(u) = d(tyy)/dy + d(txy)/dx - dP/dy + g (rho - dt (avVx drho/dx + Vy drho/dy)),
+//   C(u)  = div V,
+// with tau linear in V (StokesSolver_multixpu.jl:140-202).  Defect correction:
+//   1. FP64: r = (Rx, Ry, C)(u_k) at the current state (resid_f64), err_k = max mean |r|.
+//   2. FP32: a window of PT iterations on the correction d with the same stencils,
+//      homogeneous free-slip BCs and the FP64 residual as forcing (pt_corr_f32):
+//        Pd  <- Pd - dtauP (div Vd + C_k)
+//        Rdx  = Lx(d) + Rx_k,   Rdy = Ly(d) + Ry_k   (Ly holds the -g dt (...) term on Vd)
+//        mdx <- dampX mdx + Rdx, Vdx <- Vdx + dtauVx mdx  (same for y)
+//      The PT momentum md is carried from window to window, so the sequence of windows
+//      is the PT iteration itself; only the FP32 rounding is reset at every window.
+//   3. FP64: u_{k+1} = u_k + d (update_f64); dt refreshed every ndtupdate iterations.
+// The FP32 kernel is the FP64 fused iteration's structure (one launch per iteration,
+// register row-march, lane shifts for x neighbours) on 16 FP32 arrays per node: read
+// Vd, Pd, md (x, y), mu_p, mu_b, rho, forcing (x, y, C); write Vd, Pd, md.
+#include <algorithm>
+
+#include "sf_device.h"
+#include "sf_internal.h"
+
+namespace sf {
+
+#define CORR_OUT_LANES 62
+
+__device__ inline int xcd_remap_r(int k, int total)
+{
+    const int nxcd = 8;
+    int xcd = k % nxcd, idx = k / nxcd;
+    int bpx = total / nxcd, rem = total % nxcd;
+    return (xcd < rem) ? xcd * (bpx + 1) + idx : rem * (bpx + 1) + (xcd - rem) * bpx + idx;
+}
+
+__device__ inline int crow_r(int r, int nrows) { return r < 0 ? 0 : (r >= nrows ? nrows - 1 : r); }
+
+struct CorrRowIn {
+    float vx1, vy1, rho1, pold, mup, mub, fc, mx, my, fx, fy;
+};
+
+struct CorrCols {
+    unsigned vx, vy, p, dvx, dvy;
+};
+
+__device__ inline CorrRowIn corr_load_row(const CorrParams& p, const CorrCols& lc, int r, bool need_j)
+{
+    const int nx = p.nx, ny = p.ny;
+    CorrRowIn in;
+    in.vx1 = (p.vx_a + (int64_t)crow_r(r + 1, ny + 1) * nx)[lc.vx];
+    in.vy1 = (p.vy_a + (int64_t)crow_r(r + 1, ny) * (nx + 1))[lc.vy];
+    in.rho1 = (p.rho + (int64_t)crow_r(r + 1, ny) * (nx + 1))[lc.vy];
+    in.pold = (p.p_a + (int64_t)crow_r(r, ny - 1) * (nx - 1))[lc.p];
+    in.mup = (p.mu_p + (int64_t)crow_r(r, ny - 1) * (nx - 1))[lc.p];
+    in.fc = (p.fc + (int64_t)crow_r(r, ny - 1) * (nx - 1))[lc.p];
+    in.mub = (p.mu_b + (int64_t)crow_r(r, ny) * nx)[lc.vx];
+    if (need_j) {
+        in.mx = (p.mx + (int64_t)crow_r(r - 1, ny - 1) * (nx - 2))[lc.dvx];
+        in.fx = (p.fx + (int64_t)crow_r(r - 1, ny - 1) * (nx - 2))[lc.dvx];
+        in.my = (p.my + (int64_t)crow_r(r - 1, ny - 2) * (nx - 1))[lc.dvy];
+        in.fy = (p.fy + (int64_t)crow_r(r - 1, ny - 2) * (nx - 1))[lc.dvy];
+    } else {
+        in.mx = in.fx = in.my = in.fy = 0.f;
+    }
+    return in;
+}
+
+template <bool CHECK>
+__device__ inline void corr_tile(const CorrParams& p, int strip, int segid)
+{
+    const int lane = threadIdx.x;
+    const int c = strip * CORR_OUT_LANES + lane;
+    const int nx = p.nx, ny = p.ny;
+    const bool out_lane = lane < CORR_OUT_LANES;
+    const int r0 = segid * p.seg;
+    const int r1 = min(r0 + p.seg, ny);
+    const float dt = p.dt;
+    const int64_t lvx = nx, lvy = nx + 1, lp = nx - 1;
+    const bool c_p = c < nx - 1;
+    const float rdx = p.rdx, rdy = p.rdy;
+    CorrCols lc;
+    lc.vx = (unsigned)min(c, nx - 1);
+    lc.vy = (unsigned)min(c, nx);
+    lc.p = (unsigned)min(c, nx - 2);
+    lc.dvx = (unsigned)min(c, nx - 3);
+    lc.dvy = (unsigned)min(c, nx - 2);
+
+    int r = (r0 == 0) ? 0 : r0 - 1;
+    float vx_r = (p.vx_a + (int64_t)r * nx)[lc.vx];
+    float vy_r = (p.vy_a + (int64_t)r * (nx + 1))[lc.vy];
+    float rho_r = (p.rho + (int64_t)r * (nx + 1))[lc.vy];
+    CorrRowIn nxt = corr_load_row(p, lc, r, r >= r0);
+    float vx_r_p1 = lane_down(vx_r, 1);
+    float vy_r_p1 = lane_down(vy_r, 1);
+    float rho_r_p2 = lane_down(rho_r, 2);
+    float rho_rm1_p1 = 0.f, rho_r_ = rho_r;
+    float pn_prev = 0.f, tyy_prev = 0.f, txx_prev = 0.f;
+    float pn_prev_p1 = 0.f, txx_prev_p1 = 0.f, txy_prev_p1 = 0.f;
+    float mup_prev = 0.f, mup_prev_p1 = 0.f;
+    float rho_r_p1 = lane_down(rho_r, 1);
+
+    for (; r < r1; ++r) {
+        const CorrRowIn in = nxt;
+        if (r + 1 < r1) nxt = corr_load_row(p, lc, r + 1, true);
+        const bool emit = r >= r0;
+        const bool cell_row = r <= ny - 2;
+        const int j = r - 1;
+        const bool has_out = emit && j >= 0;
+        const bool dvx_ok = has_out && c < nx - 2;
+        const bool dvy_ok = has_out && c < nx - 1 && j <= ny - 3;
+
+        const float vx_r1_p1 = lane_down(in.vx1, 1);
+        const float vy_r1_p1 = lane_down(in.vy1, 1);
+        const float rho_r1_p1 = lane_down(in.rho1, 1);
+        const float rho_r1_p2 = lane_down(in.rho1, 2);
+
+        const float dvxdx = (vx_r1_p1 - in.vx1) * rdx;
+        const float dvydy = (vy_r1_p1 - vy_r_p1) * rdy;
+        const float div = dvxdx + dvydy;
+        const float dtaup = (p.ptsc41 * in.mup) / p.max_nxy;
+        const float cres = div + in.fc;                // continuity residual of the full state
+        const float pn = in.pold - dtaup * cres;
+        const float txx = (2.0f * in.mup) * (dvxdx - (1.0f / 3.0f) * div);
+        const float tyy = (2.0f * in.mup) * (dvydy - (1.0f / 3.0f) * div);
+        const float txy = in.mub * ((in.vx1 - vx_r) * rdy + (vy_r_p1 - vy_r) * rdx);
+        const float txy_p1 = lane_down(txy, 1);
+
+        if (emit && out_lane && c_p && cell_row) {
+            p.p_b[ix2(c, r, lp)] = pn;
+            if constexpr (CHECK) p.rco[ix2(c, r, lp)] = cres;
+        }
+        if (has_out && out_lane) {
+            const float rx = ((txx_prev_p1 - txx_prev) * rdx + (txy_p1 - txy_prev_p1) * rdy) -
+                             (pn_prev_p1 - pn_prev) * rdx + in.fx;
+            const float av = 0.25f * (((vx_r + vx_r_p1) + in.vx1) + vx_r1_p1);
+            const float d_xi = rho_r_p2 - rho_r_;
+            const float d_yi = rho_r1_p1 - rho_rm1_p1;
+            const float ry = (((tyy - tyy_prev) * rdy + (txy_p1 - txy) * rdx) - (pn - pn_prev) * rdy) -
+                             p.g * (dt * (((av * d_xi) * 0.5f) * rdx + ((vy_r_p1 * d_yi) * 0.5f) * rdy)) + in.fy;
+            const float mx_new = p.dampx * in.mx + rx;
+            const float my_new = p.dampy * in.my + ry;
+            const float dtauvx = (p.vsc_mdxy2 / ((mup_prev + mup_prev_p1) * 0.5f)) / 4.1f;
+            const float dtauvy = (p.vsc_mdxy2 / ((mup_prev + in.mup) * 0.5f)) / 4.1f;
+            const float vx_new = vx_r_p1 + dtauvx * mx_new;
+            const float vy_new = vy_r_p1 + dtauvy * my_new;
+            if (dvx_ok) {
+                p.mx[ix2(c, j, nx - 2)] = mx_new;
+                p.vx_b[ix2(c + 1, j + 1, lvx)] = vx_new;
+                if (j == 0) p.vx_b[ix2(c + 1, 0, lvx)] = vx_new;
+                if (j == ny - 2) p.vx_b[ix2(c + 1, ny, lvx)] = vx_new;
+                if constexpr (CHECK) p.rxo[ix2(c, j, nx - 2)] = rx;
+            }
+            if (c == 0 || c == nx - 2) {
+                const int col = (c == 0) ? 0 : nx - 1;
+                p.vx_b[ix2(col, j + 1, lvx)] = 0.f;
+                if (j == 0) p.vx_b[ix2(col, 0, lvx)] = 0.f;
+                if (j == ny - 2) p.vx_b[ix2(col, ny, lvx)] = 0.f;
+            }
+            if (dvy_ok) {
+                p.my[ix2(c, j, nx - 1)] = my_new;
+                p.vy_b[ix2(c + 1, j + 1, lvy)] = vy_new;
+                if (c == 0) p.vy_b[ix2(0, j + 1, lvy)] = vy_new;
+                if (c == nx - 2) p.vy_b[ix2(nx, j + 1, lvy)] = vy_new;
+                if constexpr (CHECK) p.ryo[ix2(c, j, nx - 1)] = ry;
+            }
+            if (c < nx - 1 && (j == 0 || j == ny - 2)) {
+                const int row = (j == 0) ? 0 : ny - 1;
+                p.vy_b[ix2(c + 1, row, lvy)] = 0.f;
+                if (c == 0) p.vy_b[ix2(0, row, lvy)] = 0.f;
+                if (c == nx - 2) p.vy_b[ix2(nx, row, lvy)] = 0.f;
+            }
+        }
+
+        pn_prev_p1 = lane_down(pn, 1);
+        txx_prev_p1 = lane_down(txx, 1);
+        mup_prev_p1 = lane_down(in.mup, 1);
+        pn_prev = pn;
+        txx_prev = txx;
+        tyy_prev = tyy;
+        txy_prev_p1 = txy_p1;
+        mup_prev = in.mup;
+        vx_r = in.vx1;
+        vx_r_p1 = vx_r1_p1;
+        vy_r = in.vy1;
+        vy_r_p1 = vy_r1_p1;
+        rho_rm1_p1 = rho_r_p1;
+        rho_r_ = in.rho1;
+        rho_r_p1 = rho_r1_p1;
+        rho_r_p2 = rho_r1_p2;
+    }
+}
+
+template <bool CHECK>
+__global__ __launch_bounds__(64) void pt_corr_f32(CorrParams p)
+{
+    const int G = gridDim.x;
+    for (int base = 0; base < p.ntiles; base += G) {
+        const int nround = min(G, p.ntiles - base);
+        if ((int)blockIdx.x >= nround) break;
+        const int logical = base + xcd_remap_r(blockIdx.x, nround);
+        corr_tile<CHECK>(p, logical % p.nstrips, logical / p.nstrips);
+    }
+}
+
+// FP64 residual of the state u = (V, P) without a pressure update, with stabilisation dt.
+// Writes the FP32 forcing of the correction solve and per-block sums of |Rx|, |Ry|, |C|.
+__global__ __launch_bounds__(256) void resid_f64(ResidParams p)
+{
+    __shared__ double sh[3][256];
+    const int nx = p.nx, ny = p.ny;
+    const int64_t lvx = nx, lvy = nx + 1, lp = nx - 1, lr = nx + 1;
+    const int t = threadIdx.x;
+    double s_rx = 0.0, s_ry = 0.0, s_c = 0.0;
+    const int64_t ncell = (int64_t)(nx - 1) * (ny - 1);
+    const double rdx = p.rdx, rdy = p.rdy;
+    const double* Vx = p.vx;
+    const double* Vy = p.vy;
+    auto txx_at = [&](int i, int j) {
+        const double dvxdx = (Vx[ix2(i + 1, j + 1, lvx)] - Vx[ix2(i, j + 1, lvx)]) * rdx;
+        const double dvydy = (Vy[ix2(i + 1, j + 1, lvy)] - Vy[ix2(i + 1, j, lvy)]) * rdy;
+        return (2.0 * p.mu_p[ix2(i, j, lp)]) * (dvxdx - (1.0 / 3.0) * (dvxdx + dvydy));
+    };
+    auto tyy_at = [&](int i, int j) {
+        const double dvxdx = (Vx[ix2(i + 1, j + 1, lvx)] - Vx[ix2(i, j + 1, lvx)]) * rdx;
+        const double dvydy = (Vy[ix2(i + 1, j + 1, lvy)] - Vy[ix2(i + 1, j, lvy)]) * rdy;
+        return (2.0 * p.mu_p[ix2(i, j, lp)]) * (dvydy - (1.0 / 3.0) * (dvxdx + dvydy));
+    };
+    auto txy_at = [&](int a, int b) {
+        return ((2.0 * p.mu_b[ix2(a, b, lvx)]) * 0.5) *
+               ((Vx[ix2(a, b + 1, lvx)] - Vx[ix2(a, b, lvx)]) * rdy + (Vy[ix2(a + 1, b, lvy)] - Vy[ix2(a, b, lvy)]) * rdx);
+    };
+    for (int64_t k = (int64_t)blockIdx.x * 256 + t; k < ncell; k += (int64_t)gridDim.x * 256) {
+        const int i = (int)(k % (nx - 1)), j = (int)(k / (nx - 1));
+        // continuity at P node (i, j)
+        const double cdiv = (Vx[ix2(i + 1, j + 1, lvx)] - Vx[ix2(i, j + 1, lvx)]) * rdx +
+                            (Vy[ix2(i + 1, j + 1, lvy)] - Vy[ix2(i + 1, j, lvy)]) * rdy;
+        p.fc[ix2(i, j, lp)] = (float)cdiv;
+        s_c += fabs(cdiv);
+        if (i < nx - 2) {  // x-momentum at dVx node (i, j)
+            const double rx = ((txx_at(i + 1, j) - txx_at(i, j)) * rdx + (txy_at(i + 1, j + 1) - txy_at(i + 1, j)) * rdy) -
+                              (p.P[ix2(i + 1, j, lp)] - p.P[ix2(i, j, lp)]) * rdx;
+            p.fx[ix2(i, j, nx - 2)] = (float)rx;
+            s_rx += fabs(rx);
+        }
+        if (j < ny - 2) {  // y-momentum at dVy node (i, j)
+            const double av = 0.25 * (((Vx[ix2(i, j + 1, lvx)] + Vx[ix2(i + 1, j + 1, lvx)]) + Vx[ix2(i, j + 2, lvx)]) +
+                                      Vx[ix2(i + 1, j + 2, lvx)]);
+            const double d_xi = p.rho[ix2(i + 2, j + 1, lr)] - p.rho[ix2(i, j + 1, lr)];
+            const double d_yi = p.rho[ix2(i + 1, j + 2, lr)] - p.rho[ix2(i + 1, j, lr)];
+            const double ry = (((tyy_at(i, j + 1) - tyy_at(i, j)) * rdy + (txy_at(i + 1, j + 1) - txy_at(i, j + 1)) * rdx) -
+                               (p.P[ix2(i, j + 1, lp)] - p.P[ix2(i, j, lp)]) * rdy) +
+                              p.g * (p.rho[ix2(i + 1, j + 1, lr)] -
+                                     p.dt * (((av * d_xi) * 0.5) * rdx + ((Vy[ix2(i + 1, j + 1, lvy)] * d_yi) * 0.5) * rdy));
+            p.fy[ix2(i, j, nx - 1)] = (float)ry;
+            s_ry += fabs(ry);
+        }
+    }
+    sh[0][t] = s_rx;
+    sh[1][t] = s_ry;
+    sh[2][t] = s_c;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s)
+            for (int q = 0; q < 3; ++q) sh[q][t] += sh[q][t + s];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int q = 0; q < 3; ++q) p.part[(int64_t)q * gridDim.x + blockIdx.x] = sh[q][0];
+}
+
+// u += d in FP64 (the corrections satisfy the homogeneous free-slip BCs, so u keeps its BCs)
+__global__ __launch_bounds__(256) void update_f64(double* __restrict__ a, const float* __restrict__ d, int64_t n)
+{
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256)
+        a[k] = a[k] + (double)d[k];
+}
+
+__global__ __launch_bounds__(256) void cast_f64(double* __restrict__ dst, const float* __restrict__ src, int64_t n)
+{
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256)
+        dst[k] = (double)src[k];
+}
+
+__global__ __launch_bounds__(256) void cast_f32(float* __restrict__ dst, const double* __restrict__ src, int64_t n)
+{
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256)
+        dst[k] = (float)src[k];
+}
+
+__global__ __launch_bounds__(256) void fill_f32(float* __restrict__ dst, int64_t n, float v)
+{
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) dst[k] = v;
+}
+
+// Per-block sums of |a| over up to three FP32 arrays (FP64 accumulation, fixed order).
+__global__ __launch_bounds__(256) void reduce_abs_f32(RedParamsF p)
+{
+    __shared__ double sh[256];
+    const int a = blockIdx.y;
+    const float* __restrict__ src = p.a[a];
+    const int64_t n = p.n[a];
+    const int t = threadIdx.x;
+    double acc = 0.0;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + t; k < n; k += (int64_t)gridDim.x * 256) acc += fabs((double)src[k]);
+    sh[t] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) sh[t] += sh[t + s];
+        __syncthreads();
+    }
+    if (t == 0) p.part[(int64_t)a * gridDim.x + blockIdx.x] = sh[0];
+}
+
+static int grid_for(int64_t n) { return (int)std::min<int64_t>(4096, std::max<int64_t>(1, (n + 255) / 256)); }
+
+hipError_t launch_pt_corr(const CorrParams& p0, bool check, hipStream_t s)
+{
+    CorrParams p = p0;
+    if (p.seg <= 0) p.seg = 4;
+    p.nstrips = (p.nx + CORR_OUT_LANES - 1) / CORR_OUT_LANES;
+    const int nseg = (p.ny + p.seg - 1) / p.seg;
+    p.ntiles = p.nstrips * nseg;
+    if (check) hipLaunchKernelGGL((pt_corr_f32<true>), dim3(p.ntiles), dim3(64), 0, s, p);
+    else hipLaunchKernelGGL((pt_corr_f32<false>), dim3(p.ntiles), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_resid_f64(const ResidParams& p, int nb, hipStream_t s)
+{
+    hipLaunchKernelGGL(resid_f64, dim3(nb), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_f64(double* a, const float* d, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(update_f64, dim3(grid_for(n)), dim3(256), 0, s, a, d, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_cast_f32(float* dst, const double* src, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(cast_f32, dim3(grid_for(n)), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_cast_f64(double* dst, const float* src, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(cast_f64, dim3(grid_for(n)), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float* dst, int64_t n, float v, hipStream_t s)
+{
+    hipLaunchKernelGGL(fill_f32, dim3(grid_for(n)), dim3(256), 0, s, dst, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_f32(const RedParamsF& p, int narr, int nb, hipStream_t s)
+{
+    hipLaunchKernelGGL(reduce_abs_f32, dim3(nb, narr), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+}  // namespace sf

@@ -22,7 +22,7 @@ EXPORTS = (
     "sf_version", "sf_last_error", "sf_ctx_create", "sf_ctx_destroy", "sf_ctx_set_stream", "sf_ctx_set_pt_tuning",
     "sf_solve_stokes",
     "sf_pt_iterations", "sf_marker_to_grid", "sf_move_markers_rk4", "sf_step", "sf_init_markers",
-    "sf_plume_properties", "sf_rt_properties",
+    "sf_plume_properties", "sf_rt_properties", "sf_solve_stokes_refined",
 )
 
 
@@ -39,6 +39,19 @@ class SolveOut(C.Structure):
         ("t_it", C.c_double),
         ("gbs_alg", C.c_double),
     ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+class RefineParams(C.Structure):
+    _fields_ = [("rebase_interval", C.c_int64), ("iter_max", C.c_int64), ("verbose", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class RefineOut(C.Structure):
+    _fields_ = [("n_outer", C.c_int32), ("reserved", C.c_int32), ("n_fp32_iter", C.c_int64),
+                ("n_fp64_iter", C.c_int64), ("err_fp64", C.c_double), ("seconds", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
@@ -95,6 +108,9 @@ def load():
     L.sf_ctx_set_pt_tuning.argtypes = [_vp, _i, _i, _i]
     L.sf_solve_stokes.argtypes = [_vp] + [_vp] * 19 + [_d, _d, _d, _i, _i, _d, _d, _i, _d, _i64, _i,
                                                         C.POINTER(SolveOut)]
+    L.sf_solve_stokes_refined.argtypes = [_vp] + [_vp] * 19 + [_d, _d, _d, _i, _i, _d, _d, _i, _d,
+                                                                C.POINTER(RefineParams), C.POINTER(SolveOut),
+                                                                C.POINTER(RefineOut)]
     L.sf_pt_iterations.argtypes = [_vp, _i64] + [_vp] * 8 + [_d, _d, _d, _i, _i, _d]
     L.sf_marker_to_grid.argtypes = [_vp, _d, _d, _vp, _i, _i, _vp, _vp, _vp, _i64, _d, _d, _vp, _vp, _i, _i,
                                     C.POINTER(M2GOut)]

@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import M2GOut, RK4Out, SolveOut, StepArgs, check
+from ._lib import M2GOut, RefineOut, RefineParams, RK4Out, SolveOut, StepArgs, check
 
 
 # --------------------------------------------------------------------------------------
@@ -128,11 +128,17 @@ def context(nx=0, ny=0):
 
 def solveStokes(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtauPt, Rx, Ry, dVxdtau, dVydtau,
                 dtauVx, dtauVy, Vx_s, Vy_s, g_y, dx, dy, Nx, Ny, dt, maxdisp, comm=None, *,
-                use_free_surface_stabilization=True, eps=1e-8, print_info=True, iterMax=100000, stats=None):
+                use_free_surface_stabilization=True, eps=1e-8, print_info=True, iterMax=100000, stats=None,
+                mixed_precision=False, refine=None):
     """Pseudo-transient Stokes solve (StokesSolver_multixpu.jl:14-96).  Returns (dt, T_eff, runtime).
 
     ``comm`` is accepted for signature compatibility and must be None (one rank).  The
     scratch arrays txx, tyy, txy, divV, dtauPt, Rx, Ry, dtauVx, dtauVy may be None.
+
+    ``mixed_precision=True`` solves with FP32 PT sweeps inside FP64 defect correction
+    (``sf_solve_stokes_refined``); ``refine`` may hold ``rebase_interval`` and ``verbose``;
+    ``iterMax`` caps the FP32 iterations.  Then T_eff is 0, runtime is the wall time and
+    ``stats`` also receives the refinement counters.
     """
     if comm is not None:
         raise ValueError("one-rank build: comm must be None")
@@ -150,6 +156,22 @@ def solveStokes(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtau
     ptrs = [_grid_ptr(args[k], sh[k], k) for k in sh]
     ctx = context(nx, ny)
     out = SolveOut()
+    if mixed_precision:
+        prm = RefineParams()
+        prm.iter_max = int(iterMax)
+        for key, val in (refine or {}).items():
+            setattr(prm, key, val)
+        rout = RefineOut()
+        check(ctx.lib.sf_solve_stokes_refined(ctx.handle, *ptrs, float(g_y), float(dx), float(dy), nx, ny, float(dt),
+                                              float(maxdisp), int(bool(use_free_surface_stabilization)), float(eps),
+                                              C.byref(prm), C.byref(out), C.byref(rout)))
+        if print_info:
+            print(f"Total steps = {out.niter} ({rout.n_fp32_iter} FP32 + {rout.n_fp64_iter} FP64, "
+                  f"{rout.n_outer} corrections), err = {out.err:1.3e}, time = {out.runtime:1.3e} sec")
+        if stats is not None:
+            stats.update(out.as_dict())
+            stats.update(rout.as_dict())
+        return out.dt_new, out.t_eff, out.runtime
     check(ctx.lib.sf_solve_stokes(ctx.handle, *ptrs, float(g_y), float(dx), float(dy), nx, ny, float(dt),
                                   float(maxdisp), int(bool(use_free_surface_stabilization)), float(eps),
                                   int(iterMax), int(bool(print_info)), C.byref(out)))

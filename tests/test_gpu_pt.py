@@ -152,3 +152,63 @@ def test_divergence_reproduced(gpu):
     assert st["niter"] == exp["niter"] == 960
     assert np.isnan(st["err"]) and np.isnan(exp["err"]) and not st["converged"]
     assert np.isnan(dt) and np.isnan(exp["dt_new"])
+
+
+# ------------------------------------------------------------------ FP32 sweeps in FP64 defect correction
+
+def test_refined_matches_fp64_solver_golden(gpu, golden):
+    """The refined solver takes the FP64 solver's iterations (the FP32 windows continue one PT
+    iteration), meets the reference err test in FP64, and its solution agrees with the FP64
+    oracle's to FP32 rounding levels."""
+    g = golden["stokes_solver"]
+    nx, ny = g["nx"], g["ny"]
+    S = cases.stokes_solver_inputs(g)
+    keys = list(S.keys())
+    G = to_gpu(S, keys)
+    st = {}
+    sfhip.solveStokes(G["P"], G["Vx_pad"], G["Vy_pad"], G["rho_vy"], G["mu_b"], G["mu_p"], None, None, None, None,
+                      None, None, None, G["dVx"], G["dVy"], None, None, G["Vx_s"], G["Vy_s"], g["g_y"], g["dx"],
+                      g["dy"], nx, ny, 0.0, g["maxdisp"], eps=g["eps"], print_info=False, stats=st,
+                      mixed_precision=True)
+    torch.cuda.synchronize()
+    exp = O.solve_stokes(S, g["g_y"], g["dx"], g["dy"], nx, ny, 0.0, g["maxdisp"], True, g["eps"])
+    assert st["converged"] and st["err_fp64"] <= g["eps"]
+    assert st["n_fp32_iter"] == exp["niter"] and st["n_fp64_iter"] == 1
+    R = {k: sfhip.to_numpy(G[k]) for k in ("Vx_pad", "Vy_pad", "P")}
+    for k in R:  # tolerance of the defect-corrected solution against the FP64 oracle
+        rel = np.linalg.norm(R[k] - S[k]) / np.linalg.norm(S[k])
+        assert rel <= 1e-6, (k, rel)
+    inds = np.asarray(g["test_inds"]) - 1
+    for a, ref in ((R["Vx_pad"], g["vx_ref"]), (R["Vy_pad"], g["vy_ref"])):
+        got = a.ravel(order="F")[inds]
+        assert np.linalg.norm(got - np.array(ref)) <= 1e-6 * np.linalg.norm(ref)
+    Vx, Vy = R["Vx_pad"], R["Vy_pad"]  # the corrections keep the free-slip identities exactly
+    assert np.all(Vx[1, :] == 0.0) and np.all(Vx[0, :] == -Vx[2, :]) and np.all(Vx[:, 0] == Vx[:, 1])
+    assert np.all(Vy[:, 1] == 0.0) and np.all(Vy[:, 0] == -Vy[:, 2]) and np.all(Vy[0, :] == Vy[1, :])
+
+
+@pytest.mark.parametrize("n,rebase", [(64, 0), (96, 97)])
+def test_refined_plume(gpu, n, rebase):
+    """Plume set-up (air mu 1e-2): same iteration count as the FP64 oracle (to the
+    re-base granularity), reference err <= 1e-8, solution close to the oracle's."""
+    dx, dy, x, y, rho, mu = cases.plume_markers(n, n, 4, seed=77)
+    S = O.solver_arrays(n, n)
+    vws, ws = O.zeros(n + 1, n), O.zeros(n + 1, n)
+    o = O.grid_origins(dx, dy)
+    O.marker_to_grid(*o["rho"], S["rho_vy"], x, y, rho, dx, dy, vws, ws)
+    O.marker_to_grid(*o["mu_b"], S["mu_b"], x, y, mu, dx, dy, vws, ws)
+    O.marker_to_grid(*o["mu_p"], S["mu_p"], x, y, mu, dx, dy, vws, ws)
+    G = to_gpu(S, list(S.keys()))
+    st = {}
+    sfhip.solveStokes(G["P"], G["Vx_pad"], G["Vy_pad"], G["rho_vy"], G["mu_b"], G["mu_p"], None, None, None, None,
+                      None, None, None, G["dVx"], G["dVy"], None, None, G["Vx_s"], G["Vy_s"], 1.0, dx, dy, n, n, 0.0,
+                      0.5, eps=1e-8, print_info=False, stats=st, mixed_precision=True,
+                      refine={"rebase_interval": rebase})
+    torch.cuda.synchronize()
+    exp = O.solve_stokes(S, 1.0, dx, dy, n, n, 0.0, 0.5, True, 1e-8)
+    assert st["converged"] and st["err_fp64"] <= 1e-8
+    window = rebase if rebase > 0 else min(5 * n, 2000)
+    assert abs(st["n_fp32_iter"] - exp["niter"]) <= window
+    for k in ("Vx_pad", "Vy_pad", "P"):
+        a = sfhip.to_numpy(G[k])
+        assert np.linalg.norm(a - S[k]) / np.linalg.norm(S[k]) <= 1e-4, k
