@@ -106,6 +106,96 @@ def time_to_solution(n, seed=3000):
             "refined_rel_l2_vs_fp64": rel}
 
 
+def ensemble_specs(n_inst, seed=4000, sizes=(64, 128, 256, 512), log10c=(1.0, 4.0)):
+    """The mixed-size plume ensemble: n uniform over ``sizes``, plume centre uniform over
+    [0.2,0.8]x[0.6,0.9], radius uniform over [0.05,0.15], plume viscosity 10^-c with c
+    uniform over ``log10c`` (background 1, air 1e-2)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    return [{"n": int(rng.choice(sizes)), "cx": float(rng.uniform(0.2, 0.8)), "cy": float(rng.uniform(0.6, 0.9)),
+             "r": float(rng.uniform(0.05, 0.15)), "c": float(rng.uniform(*log10c)), "seed": seed + 1 + k}
+            for k in range(n_inst)]
+
+
+def build_instance(spec):
+    """Markers (16 per cell, on the device) -> material grids of one ensemble instance."""
+    import sfhip
+
+    n = spec["n"]
+    dx = dy = 1.0 / (n - 1)
+    x, y = sfhip.init_markers(n, n, 4, dx, dy, 0.25, seed=spec["seed"])
+    rho, mu = sfhip.plume_properties(x, y, 0.1, spec["cx"], spec["cy"], spec["r"], 0.001, 0.9, 1.0, 0.01,
+                                     10.0 ** (-spec["c"]), 1.0)
+    st = sfhip.FlowState(n, n)
+    o = sfhip.grid_origins(dx, dy)
+    sfhip.bilinearMarkerToGrid(*o["rho"], st.rho_vy, x, y, rho, dx, dy, print_info=False)
+    sfhip.bilinearMarkerToGrid(*o["mu_b"], st.mu_b, x, y, mu, dx, dy, print_info=False)
+    sfhip.bilinearMarkerToGrid(*o["mu_p"], st.mu_p, x, y, mu, dx, dy, print_info=False)
+    return st, dx, x.numel()
+
+
+def run_ensemble(specs, loop=True, iter_max=100000, eps=1e-8):
+    """Solve every instance to err <= eps in one persistent launch, then (``loop``) again one
+    solveStokes call at a time from the same zero state; both must agree bit for bit."""
+    import torch
+
+    import sfhip
+
+    insts = [build_instance(s) for s in specs]
+    n_markers = sum(m for _, _, m in insts)
+
+    def reset():
+        for st, _, _ in insts:
+            for t in (st.P, st.Vx, st.Vy, st.dVxdtau, st.dVydtau, st.Vx_small, st.Vy_small):
+                t.zero_()
+
+    problems = [{"P": st.P, "Vx_pad": st.Vx, "Vy_pad": st.Vy, "dVxdtau": st.dVxdtau, "dVydtau": st.dVydtau,
+                 "Vx_s": st.Vx_small, "Vy_s": st.Vy_small, "rho_vy": st.rho_vy, "mu_b": st.mu_b, "mu_p": st.mu_p,
+                 "Nx": st.nx, "Ny": st.ny, "dx": dx, "dy": dx, "g_y": 1.0, "dt": 0.0, "maxdisp": 0.5, "eps": eps,
+                 "iterMax": iter_max} for st, dx, _ in insts]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    outs = sfhip.solveStokes_batched(problems)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    b_alg = sum(o["niter"] * bytes_per_iter(st.nx, st.ny) for o, (st, _, _) in zip(outs, insts))
+    res = {"instances": len(specs), "sizes": sorted({s["n"] for s in specs}),
+           "markers_total": n_markers,
+           "converged": sum(int(o["converged"]) for o in outs),
+           "pt_iters_total": sum(o["niter"] for o in outs),
+           "batched": {"seconds": t_batch, "solves_per_s": len(specs) / t_batch, "alg_gbs": b_alg / t_batch / 1e9}}
+    by_size = {}
+    for s, o in zip(specs, outs):
+        d = by_size.setdefault(s["n"], {"count": 0, "niter_max": 0, "niter_sum": 0, "unconverged": 0})
+        d["count"] += 1
+        d["niter_max"] = max(d["niter_max"], o["niter"])
+        d["niter_sum"] += o["niter"]
+        d["unconverged"] += 0 if o["converged"] else 1
+    res["by_size"] = {str(k): v for k, v in sorted(by_size.items())}
+    if loop:
+        sols = [(st.Vx.clone(), st.Vy.clone(), st.P.clone()) for st, _, _ in insts]
+        reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        louts = []
+        for st, dx, _ in insts:
+            stats = {}
+            sfhip.solveStokes(st.P, st.Vx, st.Vy, st.rho_vy, st.mu_b, st.mu_p, None, None, None, None, None, None,
+                              None, st.dVxdtau, st.dVydtau, None, None, st.Vx_small, st.Vy_small, 1.0, dx, dx,
+                              st.nx, st.ny, 0.0, 0.5, eps=eps, print_info=False, iterMax=iter_max, stats=stats)
+            louts.append(stats)
+        torch.cuda.synchronize()
+        t_loop = time.perf_counter() - t0
+        same = all(bool(torch.equal(a, st.Vx) and torch.equal(b, st.Vy) and torch.equal(p, st.P))
+                   for (a, b, p), (st, _, _) in zip(sols, insts))
+        same_iters = all(o["niter"] == l["niter"] for o, l in zip(outs, louts))
+        res["per_solve_loop"] = {"seconds": t_loop, "solves_per_s": len(specs) / t_loop}
+        res["speedup"] = t_loop / t_batch
+        res["bitwise_equal_to_loop"] = same and same_iters
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +208,10 @@ def main():
                     help="nodes per direction of the time-to-1e-8 comparison (FP64 vs FP32-refined); 0 skips it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy first (reference size for rocprofv3 byte counters)")
+    ap.add_argument("--ens", type=int, default=0,
+                    help="instances of the mixed-size plume ensemble (batched vs per-solve loop); 0 skips it")
+    ap.add_argument("--ens-c", type=float, nargs=2, default=(1.0, 4.0), metavar=("LO", "HI"),
+                    help="range of log10 viscosity contrast of the ensemble plumes")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
@@ -205,6 +299,11 @@ def main():
         del st
         torch.cuda.empty_cache()
         line["time_to_1e-8"] = time_to_solution(a.tts_n)
+    if a.ens > 0:
+        if "time_to_1e-8" not in line:
+            del st
+        torch.cuda.empty_cache()
+        line["ensemble"] = run_ensemble(ensemble_specs(a.ens, log10c=tuple(a.ens_c)))
     print(json.dumps(line), flush=True)
 
 

@@ -130,6 +130,33 @@ int sf_solve_stokes_refined(sf_ctx *ctx, double *P, double *Vx_pad, double *Vy_p
                             int use_fss, double eps, const sf_refine_params *prm, sf_solve_out *out,
                             sf_refine_out *refine_out);
 
+/* Batched ensemble: many independent solveStokes! problems (different sizes allowed) in one
+ * persistent launch.  Each workgroup takes an instance from a device work queue (largest
+ * first), runs its whole solve with the reference's ncheck/ndtupdate rules, exits it on its
+ * own convergence, and takes the next.  Every PT iteration is bit-identical to
+ * sf_solve_stokes; err is summed in a different fixed order.  outs[i] gets instance i's
+ * dt_new, err, niter, converged, dt_last (runtime = wall time of the batch).  No reference
+ * counterpart (the reference solves one problem per process). */
+typedef struct {
+    double *P, *Vx_pad, *Vy_pad, *dVxdtau, *dVydtau, *Vx_s, *Vy_s;
+    const double *rho_vy, *mu_b, *mu_p;
+    int nx, ny;
+    double dx, dy, g_y, dt, maxdisp, eps;
+    int64_t iter_max;
+    int use_fss;
+    int reserved;
+} sf_batch_instance;
+
+typedef struct {
+    int rows;       /* grid rows per tile (<= 0: 4) */
+    int threads;    /* threads per instance workgroup, multiple of 64, <= 768 (<= 0: 768) */
+    int max_blocks; /* cap on concurrently resident instances (<= 0: as many as fit) */
+    int reserved;
+} sf_batch_params;
+
+int sf_solve_stokes_batched(sf_ctx *ctx, const sf_batch_instance *inst, int n, const sf_batch_params *prm,
+                            sf_solve_out *outs);
+
 /* n_iter iterations of the PT loop body (StokesSolver_multixpu.jl:48-58) at a fixed
  * stabilisation dt, with no dt refresh and no convergence check.  Building block for
  * fixed-iteration benchmarks and for bit-exact per-iteration parity. */

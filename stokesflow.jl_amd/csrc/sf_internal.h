@@ -129,6 +129,29 @@ hipError_t launch_fill_f32(float* dst, int64_t n, float v, hipStream_t s);
 hipError_t launch_cast_f64(double* dst, const float* src, int64_t n, hipStream_t s);
 hipError_t launch_reduce_f32(const RedParamsF& p, int narr, int nb, hipStream_t s);
 
+// ---- batched ensemble ----
+// One ensemble instance.  pt[0] runs the caller's state into the scratch buffers, pt[1]
+// runs it back (tile parameters complete, flags unused); the rest is what the solve loop
+// and the epilogue need.
+struct BatchInstance {
+    PTParams pt[2];
+    double *P, *Vx_s, *Vy_s, *Vx_pad, *Vy_pad;  // caller state (in/out)
+    double *ry, *scal;                          // library scratch (scal[SC_DT] = stabilisation dt)
+    int nx, ny, use_fss, pad0;
+    double dx, dy, dt_in, maxdisp, eps;
+    int64_t iter_max;
+};
+
+struct BatchResult {
+    double err, dt_new, dt_last;
+    int64_t niter;
+    int converged, pad;
+};
+
+hipError_t launch_batch_solve(const BatchInstance* inst, const int* order, int n_inst, int* queue,
+                              BatchResult* res, int threads, int nblocks, hipStream_t s);
+int batch_blocks_per_cu(int threads);
+
 // ---- marker -> grid ----
 struct M2GScratch {
     int64_t cap_markers = 0;

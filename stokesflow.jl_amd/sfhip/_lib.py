@@ -22,7 +22,7 @@ EXPORTS = (
     "sf_version", "sf_last_error", "sf_ctx_create", "sf_ctx_destroy", "sf_ctx_set_stream", "sf_ctx_set_pt_tuning",
     "sf_solve_stokes",
     "sf_pt_iterations", "sf_marker_to_grid", "sf_move_markers_rk4", "sf_step", "sf_init_markers",
-    "sf_plume_properties", "sf_rt_properties", "sf_solve_stokes_refined",
+    "sf_plume_properties", "sf_rt_properties", "sf_solve_stokes_refined", "sf_solve_stokes_batched",
 )
 
 
@@ -81,6 +81,20 @@ class StepArgs(C.Structure):
     ]
 
 
+class BatchInstance(C.Structure):
+    """sf_batch_instance: one problem of a batched ensemble."""
+    _fields_ = [
+        ("P", _vp), ("Vx_pad", _vp), ("Vy_pad", _vp), ("dVxdtau", _vp), ("dVydtau", _vp), ("Vx_s", _vp),
+        ("Vy_s", _vp), ("rho_vy", _vp), ("mu_b", _vp), ("mu_p", _vp),
+        ("nx", _i), ("ny", _i), ("dx", _d), ("dy", _d), ("g_y", _d), ("dt", _d), ("maxdisp", _d), ("eps", _d),
+        ("iter_max", _i64), ("use_fss", _i), ("reserved", _i),
+    ]
+
+
+class BatchParams(C.Structure):
+    _fields_ = [("rows", _i), ("threads", _i), ("max_blocks", _i), ("reserved", _i)]
+
+
 _lib = None
 
 
@@ -111,6 +125,8 @@ def load():
     L.sf_solve_stokes_refined.argtypes = [_vp] + [_vp] * 19 + [_d, _d, _d, _i, _i, _d, _d, _i, _d,
                                                                 C.POINTER(RefineParams), C.POINTER(SolveOut),
                                                                 C.POINTER(RefineOut)]
+    L.sf_solve_stokes_batched.argtypes = [_vp, C.POINTER(BatchInstance), _i, C.POINTER(BatchParams),
+                                          C.POINTER(SolveOut)]
     L.sf_pt_iterations.argtypes = [_vp, _i64] + [_vp] * 8 + [_d, _d, _d, _i, _i, _d]
     L.sf_marker_to_grid.argtypes = [_vp, _d, _d, _vp, _i, _i, _vp, _vp, _vp, _i64, _d, _d, _vp, _vp, _i, _i,
                                     C.POINTER(M2GOut)]

@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import M2GOut, RefineOut, RefineParams, RK4Out, SolveOut, StepArgs, check
+from ._lib import BatchInstance, BatchParams, M2GOut, RefineOut, RefineParams, RK4Out, SolveOut, StepArgs, check
 
 
 # --------------------------------------------------------------------------------------
@@ -178,6 +178,43 @@ def solveStokes(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtau
     if stats is not None:
         stats.update(out.as_dict())
     return out.dt_new, out.t_eff, out.runtime
+
+
+_BATCH_ARRAYS = ("P", "Vx_pad", "Vy_pad", "dVxdtau", "dVydtau", "Vx_s", "Vy_s", "rho_vy", "mu_b", "mu_p")
+
+
+def solveStokes_batched(problems, *, rows=0, threads=0, max_blocks=0):
+    """Many independent solveStokes! problems in one persistent launch (sf_solve_stokes_batched).
+
+    ``problems`` is a sequence of dicts with the arrays P, Vx_pad, Vy_pad, dVxdtau, dVydtau,
+    Vx_s, Vy_s, rho_vy, mu_b, mu_p (shapes as for :func:`solveStokes`) and the scalars Nx, Ny,
+    dx, dy, g_y, dt, maxdisp, and optionally eps (1e-8), iterMax (100000) and
+    use_free_surface_stabilization (True).  Grids may differ in size.  Arrays are updated in
+    place; returns one stats dict per problem (dt_new, err, niter, converged, dt_last, and
+    runtime = wall time of the whole batch).
+    """
+    n = len(problems)
+    arr = (BatchInstance * max(n, 1))()
+    for i, pb in enumerate(problems):
+        nx, ny = int(pb["Nx"]), int(pb["Ny"])
+        sh = {"P": (nx - 1, ny - 1), "Vx_pad": (nx + 2, ny + 1), "Vy_pad": (nx + 1, ny + 2),
+              "dVxdtau": (nx - 2, ny - 1), "dVydtau": (nx - 1, ny - 2), "Vx_s": (nx, ny + 1), "Vy_s": (nx + 1, ny),
+              "rho_vy": (nx + 1, ny), "mu_b": (nx, ny), "mu_p": (nx - 1, ny - 1)}
+        b = arr[i]
+        for k in _BATCH_ARRAYS:
+            setattr(b, k, _grid_ptr(pb[k], sh[k], f"problem {i} {k}"))
+        b.nx, b.ny = nx, ny
+        b.dx, b.dy, b.g_y = float(pb["dx"]), float(pb["dy"]), float(pb["g_y"])
+        b.dt, b.maxdisp = float(pb["dt"]), float(pb["maxdisp"])
+        b.eps = float(pb.get("eps", 1e-8))
+        b.iter_max = int(pb.get("iterMax", 100000))
+        b.use_fss = int(bool(pb.get("use_free_surface_stabilization", True)))
+    prm = BatchParams(int(rows), int(threads), int(max_blocks), 0)
+    outs = (SolveOut * max(n, 1))()
+    ctx = context(0, 0)
+    check(ctx.lib.sf_solve_stokes_batched(ctx.handle, arr, n, C.byref(prm), outs))
+    keys = ("dt_new", "err", "niter", "converged", "dt_last", "runtime")
+    return [{k: getattr(outs[i], k) for k in keys} for i in range(n)]
 
 
 def pt_iterations(n_iter, P, Vx_s, Vy_s, dVxdtau, dVydtau, rho_vy, mu_b, mu_p, g_y, dx, dy, Nx, Ny, dt):

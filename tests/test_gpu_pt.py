@@ -207,8 +207,11 @@ def test_refined_plume(gpu, n, rebase):
     torch.cuda.synchronize()
     exp = O.solve_stokes(S, 1.0, dx, dy, n, n, 0.0, 0.5, True, 1e-8)
     assert st["converged"] and st["err_fp64"] <= 1e-8
-    window = rebase if rebase > 0 else min(5 * n, 2000)
-    assert abs(st["n_fp32_iter"] - exp["niter"]) <= window
+    # convergence is tested at multiples of ncheck, as in the reference; FP32 rounding may move
+    # the first passing test by one ncheck
+    ncheck = min(5 * n, 2000)
+    assert st["n_fp32_iter"] % ncheck == 0
+    assert abs(st["n_fp32_iter"] - exp["niter"]) <= ncheck
     for k in ("Vx_pad", "Vy_pad", "P"):
         a = sfhip.to_numpy(G[k])
         assert np.linalg.norm(a - S[k]) / np.linalg.norm(S[k]) <= 1e-4, k
