@@ -174,6 +174,16 @@ int sf_marker_to_grid(sf_ctx *ctx, double x_grid_min, double y_grid_min, double 
                       const double *x_m, const double *y_m, const double *val_m, int64_t nm, double dx,
                       double dy, double *val_wt_sum, double *wt_sum, int ws_nx, int ws_ny, sf_m2g_out *out);
 
+/* The integer cell keys of bilinearMarkerToGrid! (MarkerToGrid.jl:116-122 with
+ * topleftIndexRelDist, common.jl:22-33), for checking them bit for bit: keys[m] =
+ * iy*(gnx+1) + ix for marker m's 1-based top-left node (ix, iy) when 0 <= ix <= gnx and
+ * 0 <= iy <= gny, else -1 (a marker the reference reports as erroneous; counted in
+ * out->n_bad_markers).  counts[(gnx+1)*(gny+1)] gets the markers per key.  These are the keys
+ * sf_marker_to_grid sorts by.  No reference entry point: the reference computes them inline. */
+int sf_marker_cell_keys(sf_ctx *ctx, double x_grid_min, double y_grid_min, int gnx, int gny, const double *x_m,
+                        const double *y_m, int64_t nm, double dx, double dy, int32_t *keys, int32_t *counts,
+                        sf_m2g_out *out);
+
 /* moveMarkersRK4! -- replaces scripts/MoveMarkers.jl:15-23
  * (moveMarkersRK4!(x_m, y_m, Vx, Vy, x_vx_min, y_vx_min, x_vy_min, y_vy_min, dt, lx, ly,
  *  dx, dy)).  Vx is (vx_n1, vx_n2) = size(Vx), normally the padded (Nx+2, Ny+1) array;

@@ -381,6 +381,32 @@ static int key_representable(double v)
     return isfinite(v) && v > -9.2e18 && v < 9.2e18;
 }
 
+int sfo_cell_keys(double x_grid_min, double y_grid_min, int gnx, int gny, const double *x_m,
+                  const double *y_m, int64_t nm, double dx, double dy, int32_t *keys, int32_t *counts,
+                  int64_t *n_bad)
+{
+    const int64_t ncells = (int64_t)(gnx + 1) * (gny + 1);
+    if (counts) memset(counts, 0, sizeof(int32_t) * (size_t)ncells);
+    int64_t bad = 0;
+    for (int64_t m = 0; m < nm; ++m) {
+        if (!key_representable((x_m[m] - x_grid_min) / dx) || !key_representable((y_m[m] - y_grid_min) / dy))
+            return -1;
+        int64_t ix, iy;
+        double dxm, dym;
+        sfo_topleft_index(x_grid_min, y_grid_min, x_m[m], y_m[m], dx, dy, &ix, &iy, &dxm, &dym);
+        if (!(ix >= 0 && ix <= gnx && iy >= 0 && iy <= gny)) {
+            keys[m] = -1;
+            ++bad;
+            continue;
+        }
+        const int32_t k = (int32_t)(iy * (int64_t)(gnx + 1) + ix);
+        keys[m] = k;
+        if (counts) counts[k] += 1;
+    }
+    if (n_bad) *n_bad = bad;
+    return 0;
+}
+
 int sfo_marker_to_grid(double x_grid_min, double y_grid_min, double *val_grid, int gnx, int gny,
                        const double *x_m, const double *y_m, const double *val_m, int64_t nm,
                        double dx, double dy, double *val_wt_sum, double *wt_sum, int ws_nx,

@@ -109,6 +109,15 @@ int sfo_marker_to_grid(double x_grid_min, double y_grid_min, double *val_grid, i
                        double dx, double dy, double *val_wt_sum, double *wt_sum, int ws_nx,
                        int ws_ny, int64_t *n_bad, int64_t *n_empty);
 
+/* Cell keys of bilinearMarkerToGrid!'s CPU path (MarkerToGrid.jl:116-122 with
+ * topleftIndexRelDist, common.jl:22-33): for the 1-based top-left node (ix, iy) of marker m,
+ * keys[m] = iy*(gnx+1) + ix when 0 <= ix <= gnx and 0 <= iy <= gny, else -1 (the markers the
+ * reference reports as erroneous).  counts[(gnx+1)*(gny+1)] (may be NULL) gets the number of
+ * markers per key.  Returns -1 for a non-finite coordinate. */
+int sfo_cell_keys(double x_grid_min, double y_grid_min, int gnx, int gny, const double *x_m,
+                  const double *y_m, int64_t nm, double dx, double dy, int32_t *keys, int32_t *counts,
+                  int64_t *n_bad);
+
 /* MoveMarkers.jl:76-174 (with bilinearInterp :181-187).  n_clamp (may be NULL) is incremented
  * once per failsafe clamp taken.  Returns -1 for a non-finite position. */
 int sfo_interpolate_v(double x, double y, const double *Vx, int vx_n1, int vx_n2, const double *Vy,

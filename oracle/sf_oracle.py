@@ -63,6 +63,8 @@ def lib():
         L.sfo_topleft_index.argtypes = [d, d, d, d, d, d, _i64p, _i64p, _dp, _dp]
         L.sfo_marker_to_grid.argtypes = [d, d, _dp, i, i, _dp, _dp, _dp, i64, d, d, _dp, _dp, i, i, _i64p, _i64p]
         L.sfo_marker_to_grid.restype = i
+        L.sfo_cell_keys.argtypes = [d, d, i, i, _dp, _dp, i64, d, d, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                    _i64p]
         L.sfo_interpolate_v.argtypes = [d, d, _dp, i, i, _dp, i, i, d, d, d, d, d, d, _dp, _dp, _i64p]
         L.sfo_interpolate_v.restype = i
         L.sfo_move_markers_rk4.argtypes = [_dp, _dp, i64, _dp, i, i, _dp, i, i, d, d, d, d, d, d, d, d, d, _i64p]
@@ -159,6 +161,20 @@ def marker_to_grid(xmin, ymin, val_grid, x_m, y_m, val_m, dx, dy, val_wt_sum, wt
     if rc != 0:
         raise ValueError(f"sfo_marker_to_grid failed ({rc})")
     return nb.value, ne.value
+
+
+def cell_keys(xmin, ymin, gnx, gny, x_m, y_m, dx, dy):
+    """Per-marker cell keys and per-cell counts of the M2G CPU path; returns (keys, counts, n_bad)."""
+    assert x_m.shape == y_m.shape
+    keys = np.empty(x_m.size, dtype=np.int32)
+    counts = np.empty((gnx + 1) * (gny + 1), dtype=np.int32)
+    nb = C.c_int64()
+    rc = lib().sfo_cell_keys(xmin, ymin, gnx, gny, _p(x_m), _p(y_m), x_m.size, dx, dy,
+                             keys.ctypes.data_as(C.POINTER(C.c_int32)), counts.ctypes.data_as(C.POINTER(C.c_int32)),
+                             C.byref(nb))
+    if rc != 0:
+        raise ValueError("non-finite marker coordinate")
+    return keys, counts, nb.value
 
 
 def interpolate_v(x, y, Vx, Vy, x_vx_min, y_vx_min, x_vy_min, y_vy_min, dx, dy):

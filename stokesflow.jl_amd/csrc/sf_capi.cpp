@@ -543,6 +543,33 @@ extern "C" int sf_marker_to_grid(sf_ctx* c, double xmin, double ymin, double* va
                     out);
 }
 
+extern "C" int sf_marker_cell_keys(sf_ctx* c, double xmin, double ymin, int gnx, int gny, const double* x_m,
+                                   const double* y_m, int64_t nm, double dx, double dy, int32_t* keys,
+                                   int32_t* counts, sf_m2g_out* out)
+{
+    if (!c) return set_err(SF_ERR_ARG, "sf_marker_cell_keys: NULL ctx");
+    if (nm < 0) return set_err(SF_ERR_ARG, "sf_marker_cell_keys: nm < 0");
+    if (!counts || (nm > 0 && (!x_m || !y_m || !keys))) return set_err(SF_ERR_ARG, "sf_marker_cell_keys: NULL array");
+    if (gnx < 1 || gny < 1) return set_err(SF_ERR_ARG, "sf_marker_cell_keys: empty grid %d x %d", gnx, gny);
+    if (!(dx > 0.0) || !(dy > 0.0)) return set_err(SF_ERR_ARG, "sf_marker_cell_keys: dx, dy must be > 0");
+    const int64_t ncells = (int64_t)(gnx + 1) * (gny + 1);
+    if (nm >= ((int64_t)1 << 31) - 1 || ncells >= ((int64_t)1 << 31) - 1)
+        return set_err(SF_ERR_SIZE, "sf_marker_cell_keys: %lld markers / %lld cells exceed the 32-bit index limit",
+                       (long long)nm, (long long)ncells);
+    HIPCHK(m2g_cell_keys(c->m2g, xmin, ymin, gnx, gny, x_m, y_m, nm, dx, dy, keys, counts, c->stream));
+    HIPCHK(hipMemcpyAsync(c->hcounters, c->m2g.counters, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->hcounters[2] > 0)
+        return set_err(SF_ERR_NONFINITE, "sf_marker_cell_keys: %llu markers have non-finite coordinates",
+                       (unsigned long long)c->hcounters[2]);
+    if (out) {
+        out->n_bad_markers = (int64_t)c->hcounters[0];
+        out->n_empty_nodes = 0;
+    }
+    return SF_OK;
+}
+
 // ---------------------------------------------------------------------------------------
 // RK4
 

@@ -173,6 +173,9 @@ hipError_t m2g_run(M2GScratch& s, double xmin, double ymin, double* val_grid, in
                    const double* y_m, const double* val_m, int64_t nm, double dx, double dy, double* val_wt_sum,
                    double* wt_sum, int ws_nx, int ws_ny, hipStream_t st);
 
+hipError_t m2g_cell_keys(M2GScratch& s, double xmin, double ymin, int gnx, int gny, const double* x_m,
+                         const double* y_m, int64_t nm, double dx, double dy, int* keys, int* counts, hipStream_t st);
+
 // ---- RK4 / markers ----
 hipError_t launch_rk4(double* x_m, double* y_m, int64_t nm, const double* vx, int vx_n1, int vx_n2, const double* vy,
                       int vy_n1, int vy_n2, double x_vx_min, double y_vx_min, double x_vy_min, double y_vy_min,

@@ -245,6 +245,18 @@ __global__ __launch_bounds__(256) void m2g_nodes(const int* __restrict__ off, co
 
 static int grid1d(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
+hipError_t m2g_cell_keys(M2GScratch& S, double xmin, double ymin, int gnx, int gny, const double* x_m,
+                         const double* y_m, int64_t nm, double dx, double dy, int* keys, int* counts, hipStream_t st)
+{
+    const int64_t ncells = (int64_t)(gnx + 1) * (gny + 1);
+    hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, st, S.counters);
+    hipLaunchKernelGGL(fill_int, dim3(1024), dim3(256), 0, st, counts, ncells, 0);
+    if (nm > 0)
+        hipLaunchKernelGGL(m2g_keys, dim3(grid1d(nm, 256)), dim3(256), 0, st, x_m, y_m, nm, xmin, ymin, dx, dy, gnx,
+                           gny, keys, counts, S.counters);
+    return hipGetLastError();
+}
+
 hipError_t m2g_run(M2GScratch& S, double xmin, double ymin, double* val_grid, int gnx, int gny, const double* x_m,
                    const double* y_m, const double* val_m, int64_t nm, double dx, double dy, double* val_wt_sum,
                    double* wt_sum, int ws_nx, int ws_ny, hipStream_t st)

@@ -23,6 +23,7 @@ EXPORTS = (
     "sf_solve_stokes",
     "sf_pt_iterations", "sf_marker_to_grid", "sf_move_markers_rk4", "sf_step", "sf_init_markers",
     "sf_plume_properties", "sf_rt_properties", "sf_solve_stokes_refined", "sf_solve_stokes_batched",
+    "sf_marker_cell_keys",
 )
 
 
@@ -130,6 +131,7 @@ def load():
     L.sf_pt_iterations.argtypes = [_vp, _i64] + [_vp] * 8 + [_d, _d, _d, _i, _i, _d]
     L.sf_marker_to_grid.argtypes = [_vp, _d, _d, _vp, _i, _i, _vp, _vp, _vp, _i64, _d, _d, _vp, _vp, _i, _i,
                                     C.POINTER(M2GOut)]
+    L.sf_marker_cell_keys.argtypes = [_vp, _d, _d, _i, _i, _vp, _vp, _i64, _d, _d, _vp, _vp, C.POINTER(M2GOut)]
     L.sf_move_markers_rk4.argtypes = [_vp, _vp, _vp, _i64, _vp, _i, _i, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d,
                                       _d, _d, C.POINTER(RK4Out)]
     L.sf_step.argtypes = [_vp, C.POINTER(StepArgs), C.POINTER(SolveOut), C.POINTER(M2GOut), C.POINTER(RK4Out)]

@@ -270,6 +270,22 @@ def bilinearMarkerToGrid(x_grid_min, y_grid_min, val_grid, x_m, y_m, val_m, dx, 
 # --------------------------------------------------------------------------------------
 # moveMarkersRK4!
 
+def marker_cell_keys(x_grid_min, y_grid_min, gnx, gny, x_m, y_m, dx, dy):
+    """Integer cell keys of the marker -> grid pass (sf_marker_cell_keys): returns (keys, counts,
+    n_bad) as int32 CUDA tensors (keys[m] = iy*(gnx+1)+ix of the 1-based top-left node, -1 if
+    outside) and the number of markers outside the grid range."""
+    px, py = _marker_ptr(x_m, "x_m"), _marker_ptr(y_m, "y_m")
+    assert x_m.numel() == y_m.numel()
+    keys = torch.empty(x_m.numel(), dtype=torch.int32, device=x_m.device)
+    counts = torch.empty((int(gnx) + 1) * (int(gny) + 1), dtype=torch.int32, device=x_m.device)
+    out = M2GOut()
+    ctx = context(0, 0)
+    check(ctx.lib.sf_marker_cell_keys(ctx.handle, float(x_grid_min), float(y_grid_min), int(gnx), int(gny), px, py,
+                                      x_m.numel(), float(dx), float(dy), keys.data_ptr(), counts.data_ptr(),
+                                      C.byref(out)))
+    return keys, counts, out.n_bad_markers
+
+
 def moveMarkersRK4(x_m, y_m, Vx, Vy, x_vx_min, y_vx_min, x_vy_min, y_vy_min, dt, lx, ly, dx, dy, *, stats=None,
                    print_info=True):
     """RK4 marker advection (MoveMarkers.jl:15-69), positions updated in place."""

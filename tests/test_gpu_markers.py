@@ -178,3 +178,27 @@ def test_rt_properties(gpu):
     up = y < yi
     assert np.array_equal(sfhip.to_numpy(rg)[far], np.where(up, 1.1, 1.0)[far])
     assert np.array_equal(sfhip.to_numpy(mg)[far], np.where(up, 1e3, 1.0)[far])
+
+
+@pytest.mark.parametrize("origin", ["rho", "mu_b", "mu_p"])
+@pytest.mark.parametrize("nx,ny,md,seed", [(9, 7, 4, 1), (130, 67, 3, 3)])
+def test_cell_keys_bitwise_vs_oracle(gpu, origin, nx, ny, md, seed):
+    """Per-marker integer cell keys and per-cell counts equal the oracle's, including markers
+    exactly on grid lines and outside the grid."""
+    dx, dy, x, y, _, _ = cases.plume_markers(nx, ny, md, seed)
+    o = O.grid_origins(dx, dy)[origin]
+    k = np.arange(5, dtype=np.float64)
+    x = np.concatenate([x, o[0] + k * dx, [o[0] - 1.5 * dx, 0.5, 1e3], o[0] + 0.5 * dx + 0 * k])
+    y = np.concatenate([y, o[1] + 0.5 * dy + 0 * k, [0.5, o[1] - 2.5 * dy, 0.5], o[1] + k * dy])
+    gnx, gny = {"rho": (nx + 1, ny), "mu_b": (nx, ny), "mu_p": (nx - 1, ny - 1)}[origin]
+    keys, counts, nb = sfhip.marker_cell_keys(o[0], o[1], gnx, gny, sfhip.from_numpy(x), sfhip.from_numpy(y), dx, dy)
+    ek, ec, enb = O.cell_keys(o[0], o[1], gnx, gny, x, y, dx, dy)
+    assert np.array_equal(keys.cpu().numpy(), ek)
+    assert np.array_equal(counts.cpu().numpy(), ec)
+    assert nb == enb >= 2
+
+
+def test_cell_keys_empty(gpu):
+    e = torch.empty(0, dtype=torch.float64, device="cuda")
+    keys, counts, nb = sfhip.marker_cell_keys(0.0, 0.0, 4, 3, e, e, 0.25, 0.5)
+    assert keys.numel() == 0 and nb == 0 and int(counts.sum()) == 0

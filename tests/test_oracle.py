@@ -293,3 +293,27 @@ def test_flow_golden_vectors(golden):
     ex = np.linalg.norm(linear(S["Vx_pad"], f["inds"]) - np.array(f["vx_ref"]))
     ey = np.linalg.norm(linear(S["Vy_pad"], f["inds"]) - np.array(f["vy_ref"]))
     assert ex < 1e-3 and ey < 1e-3, (ex, ey)
+
+
+def test_cell_keys_lattice_closed_form():
+    """An unjittered md x md lattice puts md^2 markers in every cell of the mu_b grid (origin
+    (0,0)) and md^2/2 in the half cells at the x ends of the rho grid (origin (-dx/2, 0));
+    markers left of the first node by more than a cell are counted as erroneous."""
+    n, md = 12, 4
+    dx = dy = 1.0 / (n - 1)
+    x, y = O.lattice_markers(n, n, md, (n - 1) * dx, (n - 1) * dy, dx, dy)
+    o = O.grid_origins(dx, dy)
+    keys, counts, nb = O.cell_keys(*o["mu_b"], n, n, x, y, dx, dy)
+    c = counts.reshape(n + 1, n + 1)  # [iy, ix]
+    assert nb == 0 and np.all(c[1:n, 1:n] == md * md) and c.sum() == x.size
+    assert np.array_equal(np.bincount(keys, minlength=c.size), counts)
+    keys, counts, nb = O.cell_keys(*o["rho"], n + 1, n, x, y, dx, dy)
+    c = counts.reshape(n + 1, n + 2)
+    assert nb == 0 and np.all(c[1:n, 2:n] == md * md)
+    assert np.all(c[1:n, 1] == md * md // 2) and np.all(c[1:n, n] == md * md // 2)
+    xb = np.concatenate([x, [-1.5 * dx, -0.5 * dx]])
+    yb = np.concatenate([y, [0.5, 0.5]])
+    keys, counts, nb = O.cell_keys(*o["mu_b"], n, n, xb, yb, dx, dy)
+    assert nb == 1 and keys[-2] == -1 and keys[-1] % (n + 1) == 0  # ix = 0 is inside the accepted range
+    g, vws, ws = O.zeros(n, n), O.zeros(n + 1, n), O.zeros(n + 1, n)
+    assert O.marker_to_grid(*o["mu_b"], g, xb, yb, np.ones_like(xb), dx, dy, vws, ws)[0] == nb
