@@ -213,6 +213,24 @@ typedef struct {
 int sf_step(sf_ctx *ctx, sf_step_args *args, sf_solve_out *solve_out, sf_m2g_out m2g_out[3],
             sf_rk4_out *rk4_out);
 
+/* Ensemble timestep captured into a hipGraph: n independent problems (grid sizes may differ),
+ * each described like sf_step (its dt field is the initial dt, 0 on a first step).  One step
+ * of every member = the three marker -> grid passes per member, ONE batched solve
+ * (sf_solve_stokes_batched's persistent kernel) and RK4 per member with the dt that the solve
+ * produced, read on the device -- the loop body of scripts/StokesFlow2D_multixpu.jl:164-181 for
+ * each member, with no host synchronisation inside the step.  With use_graph the step is
+ * captured once at creation and replayed by sf_ensemble_step; without it the same kernels are
+ * enqueued every step (identical results).  The ensemble keeps the members' array pointers:
+ * the arrays must outlive it.  It runs on a stream of its own, ordered after the context's
+ * stream at each call.  outs[i] / dt[i] (either may be NULL) receive member i's solve results
+ * and dt after the last step.  No reference counterpart (the reference runs one problem per
+ * process). */
+typedef struct sf_ensemble sf_ensemble;
+int sf_ensemble_create(sf_ctx *ctx, const sf_step_args *members, int n, const sf_batch_params *prm, int use_graph,
+                       sf_ensemble **ens);
+int sf_ensemble_step(sf_ensemble *ens, int n_steps, sf_solve_out *outs, double *dt);
+int sf_ensemble_destroy(sf_ensemble *ens);
+
 /* Seeded marker set-up: md x md markers per cell, x outer / y inner (the order of
  * common.jl:76-83), uniform jitter of +-jitter_frac cells from splitmix64(seed, index),
  * clamped to the domain.  x_m, y_m hold md^2 (nx-1)(ny-1) values.  No reference

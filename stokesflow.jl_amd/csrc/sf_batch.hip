@@ -99,7 +99,10 @@ __device__ void solve_instance(CBatch& I, BatchResult* __restrict__ R, double* s
     const int64_t iter_max = I.iter_max;
     const bool fss = I.use_fss != 0;
 
-    if (threadIdx.x == 0) I.scal[SC_DT] = fss ? I.dt_in : 0.0;
+    if (threadIdx.x == 0) {
+        const double* dd = I.dt_dev;
+        I.scal[SC_DT] = fss ? (dd ? *dd : I.dt_in) : 0.0;
+    }
     __syncthreads();
 
     double err = 2.0 * eps;

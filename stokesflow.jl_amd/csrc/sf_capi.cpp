@@ -468,9 +468,16 @@ extern "C" int sf_pt_iterations(sf_ctx* c, int64_t n_iter, double* P, double* Vx
 // ---------------------------------------------------------------------------------------
 // marker -> grid
 
-static int m2g_reserve(sf_ctx* c, int64_t nm, int64_t ncells)
+static void m2g_free(M2GScratch& S)
 {
-    M2GScratch& S = c->m2g;
+    for (void* p : {(void*)S.key, (void*)S.rec_idx, (void*)S.rec_dx, (void*)S.rec_dy, (void*)S.rec_val, (void*)S.count,
+                    (void*)S.offset, (void*)S.fill, (void*)S.blocksums, (void*)S.counters})
+        if (p) (void)hipFree(p);
+    S = M2GScratch();
+}
+
+static int m2g_reserve(M2GScratch& S, int64_t nm, int64_t ncells)
+{
     if (nm > S.cap_markers) {
         for (void* p : {(void*)S.key, (void*)S.rec_idx, (void*)S.rec_dx, (void*)S.rec_dy, (void*)S.rec_val})
             if (p) HIPCHK(hipFree(p));
@@ -517,7 +524,7 @@ static int m2g_call(sf_ctx* c, double xmin, double ymin, double* val_grid, int g
     if (nm >= ((int64_t)1 << 31) - 1 || ncells >= ((int64_t)1 << 31) - 1)
         return set_err(SF_ERR_SIZE, "sf_marker_to_grid: %lld markers / %lld cells exceed the 32-bit index limit",
                        (long long)nm, (long long)ncells);
-    int rc = m2g_reserve(c, nm, ncells);
+    int rc = m2g_reserve(c->m2g, nm, ncells);
     if (rc) return rc;
     HIPCHK(m2g_run(c->m2g, xmin, ymin, val_grid, gnx, gny, x_m, y_m, val_m, nm, dx, dy, vws, ws, ws_nx, ws_ny,
                    c->stream));
@@ -995,33 +1002,153 @@ extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, dou
 // ---------------------------------------------------------------------------------------
 // batched ensemble: many independent solves in one persistent launch
 
+// Per-instance scratch (doubles) of the batched solver: ping-pong state, tau/divV, Rx/Ry, scalars.
+static size_t batch_scratch_doubles(int nx_, int ny_)
+{
+    const size_t nx = nx_, ny = ny_, np = (nx - 1) * (ny - 1);
+    return nx * (ny + 1) + (nx + 1) * ny + np + 3 * np + nx * ny + (nx - 2) * (ny - 1) + (nx - 1) * (ny - 2) + SC_COUNT;
+}
+
+static int batch_check(const sf_batch_instance& a, int i, const char* who)
+{
+    if (a.nx < 3 || a.ny < 3) return set_err(SF_ERR_ARG, "%s: instance %d: need Nx, Ny >= 3", who, i);
+    if ((int64_t)(a.nx + 2) * (a.ny + 2) >= (int64_t)1 << 31)
+        return set_err(SF_ERR_SIZE, "%s: instance %d: grid too large", who, i);
+    if (!a.P || !a.Vx_pad || !a.Vy_pad || !a.rho_vy || !a.mu_b || !a.mu_p || !a.dVxdtau || !a.dVydtau || !a.Vx_s ||
+        !a.Vy_s)
+        return set_err(SF_ERR_ARG, "%s: instance %d: a state or material array is NULL", who, i);
+    if (!(a.dx > 0.0) || !(a.dy > 0.0)) return set_err(SF_ERR_ARG, "%s: instance %d: dx, dy must be > 0", who, i);
+    return SF_OK;
+}
+
+// The device descriptor of one instance; scratch carved from q (batch_scratch_doubles long).
+static BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, const double* dt_dev)
+{
+    BatchInstance b;
+    std::memset(&b, 0, sizeof(b));
+    const int nx = a.nx, ny = a.ny;
+    const size_t np = (size_t)(nx - 1) * (ny - 1);
+    double* vx_b = q; q += (size_t)nx * (ny + 1);
+    double* vy_b = q; q += (size_t)(nx + 1) * ny;
+    double* p_b = q; q += np;
+    double* t_xx = q; q += np;
+    double* t_yy = q; q += np;
+    double* d_v = q; q += np;
+    double* t_xy = q; q += (size_t)nx * ny;
+    double* r_x = q; q += (size_t)(nx - 2) * (ny - 1);
+    double* r_y = q; q += (size_t)(nx - 1) * (ny - 2);
+    double* scal = q;
+    const SolveConsts k = solve_consts(nx, ny, a.dx, a.dy);
+    PTParams pp{};
+    pp.nx = nx;
+    pp.ny = ny;
+    pp.seg = rows;
+    pp.depth = 1;
+    pp.nstrips = pt_num_strips(nx);
+    pp.ntiles = pp.nstrips * ((ny + rows - 1) / rows);
+    pp.dvx = a.dVxdtau;
+    pp.dvy = a.dVydtau;
+    pp.rho = a.rho_vy;
+    pp.mu_b = a.mu_b;
+    pp.mu_p = a.mu_p;
+    pp.dt = scal + SC_DT;
+    pp.txx = t_xx;
+    pp.tyy = t_yy;
+    pp.txy = t_xy;
+    pp.divv = d_v;
+    pp.rx = r_x;
+    pp.g = a.g_y;
+    pp.dampx = k.dampx;
+    pp.dampy = k.dampy;
+    pp.rdx = k.rdx;
+    pp.rdy = k.rdy;
+    pp.vsc_mdxy2 = k.vsc_mdxy2;
+    pp.ptsc41 = k.ptsc41;
+    pp.max_nxy = k.max_nxy;
+    b.pt[0] = pp;
+    b.pt[0].vx_a = a.Vx_s;
+    b.pt[0].vy_a = a.Vy_s;
+    b.pt[0].p_a = a.P;
+    b.pt[0].vx_b = vx_b;
+    b.pt[0].vy_b = vy_b;
+    b.pt[0].p_b = p_b;
+    b.pt[1] = pp;
+    b.pt[1].vx_a = vx_b;
+    b.pt[1].vy_a = vy_b;
+    b.pt[1].p_a = p_b;
+    b.pt[1].vx_b = a.Vx_s;
+    b.pt[1].vy_b = a.Vy_s;
+    b.pt[1].p_b = a.P;
+    b.P = a.P;
+    b.Vx_s = a.Vx_s;
+    b.Vy_s = a.Vy_s;
+    b.Vx_pad = a.Vx_pad;
+    b.Vy_pad = a.Vy_pad;
+    b.ry = r_y;
+    b.scal = scal;
+    b.dt_dev = dt_dev;
+    b.nx = nx;
+    b.ny = ny;
+    b.use_fss = a.use_fss;
+    b.dx = a.dx;
+    b.dy = a.dy;
+    b.dt_in = a.dt;
+    b.maxdisp = a.maxdisp;
+    b.eps = a.eps;
+    b.iter_max = a.iter_max;
+    return b;
+}
+
+// Largest expected work first: nodes x an iteration count that grows with the grid size.
+static std::vector<int> batch_order(const std::vector<std::pair<int, int>>& dims)
+{
+    const int n = (int)dims.size();
+    std::vector<int> order((size_t)n);
+    for (int i = 0; i < n; ++i) order[(size_t)i] = i;
+    auto work = [&](int i) {
+        const double a = dims[(size_t)i].first, b = dims[(size_t)i].second;
+        return a * b * std::max(a, b);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+    return order;
+}
+
+static int batch_grid(sf_ctx* c, int n, int threads, int max_blocks)
+{
+    int cus = 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1) cus = 256;
+    int resident = cus * batch_blocks_per_cu(threads);
+    if (max_blocks > 0) resident = std::min(resident, max_blocks);
+    return std::min(n, resident);
+}
+
+static int batch_threads(const sf_batch_params* prm, int* threads, int* rows, const char* who)
+{
+    *threads = (prm && prm->threads > 0) ? prm->threads : 768;
+    *rows = (prm && prm->rows > 0) ? prm->rows : 4;
+    if (*threads % 64 != 0 || *threads > 768) return set_err(SF_ERR_ARG, "%s: threads %d", who, *threads);
+    return SF_OK;
+}
+
 extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, int n, const sf_batch_params* prm,
                                        sf_solve_out* outs)
 {
-    if (!c || (n > 0 && (!in || !outs))) return set_err(SF_ERR_ARG, "sf_solve_stokes_batched: NULL argument");
-    if (n < 0) return set_err(SF_ERR_ARG, "sf_solve_stokes_batched: n < 0");
+    const char* who = "sf_solve_stokes_batched";
+    if (!c || (n > 0 && (!in || !outs))) return set_err(SF_ERR_ARG, "%s: NULL argument", who);
+    if (n < 0) return set_err(SF_ERR_ARG, "%s: n < 0", who);
     if (n == 0) return SF_OK;
     const auto t_start = std::chrono::steady_clock::now();
     hipStream_t s = c->stream;
-    const int threads = (prm && prm->threads > 0) ? prm->threads : 768;
-    const int rows = (prm && prm->rows > 0) ? prm->rows : 4;
-    if (threads % 64 != 0 || threads > 768) return set_err(SF_ERR_ARG, "sf_solve_stokes_batched: threads %d", threads);
+    int threads, rows;
+    int rc = batch_threads(prm, &threads, &rows, who);
+    if (rc) return rc;
     std::vector<size_t> off((size_t)n);
     size_t need = 0;
     for (int i = 0; i < n; ++i) {
-        const sf_batch_instance& a = in[i];
-        if (a.nx < 3 || a.ny < 3) return set_err(SF_ERR_ARG, "instance %d: need Nx, Ny >= 3", i);
-        if ((int64_t)(a.nx + 2) * (a.ny + 2) >= (int64_t)1 << 31) return set_err(SF_ERR_SIZE, "instance %d: grid too large", i);
-        if (!a.P || !a.Vx_pad || !a.Vy_pad || !a.rho_vy || !a.mu_b || !a.mu_p || !a.dVxdtau || !a.dVydtau || !a.Vx_s ||
-            !a.Vy_s)
-            return set_err(SF_ERR_ARG, "instance %d: a state or material array is NULL", i);
-        if (!(a.dx > 0.0) || !(a.dy > 0.0)) return set_err(SF_ERR_ARG, "instance %d: dx, dy must be > 0", i);
-        const size_t nx = a.nx, ny = a.ny, np = (nx - 1) * (ny - 1);
+        rc = batch_check(in[i], i, who);
+        if (rc) return rc;
         off[(size_t)i] = need;
-        need += nx * (ny + 1) + (nx + 1) * ny + np         // ping-pong state
-                + 3 * np + nx * ny                         // txx, tyy, divV, txy
-                + (nx - 2) * (ny - 1) + (nx - 1) * (ny - 2)  // Rx, Ry
-                + SC_COUNT;
+        need += batch_scratch_doubles(in[i].nx, in[i].ny);
     }
     if (need > c->b_pool_cap) {
         if (c->b_pool) HIPCHK(hipFree(c->b_pool));
@@ -1045,93 +1172,13 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
         c->b_cap = n;
     }
     std::vector<BatchInstance> host((size_t)n);
-    for (int i = 0; i < n; ++i) {
-        const sf_batch_instance& a = in[i];
-        BatchInstance& b = host[(size_t)i];
-        std::memset(&b, 0, sizeof(b));
-        const int nx = a.nx, ny = a.ny;
-        const size_t np = (size_t)(nx - 1) * (ny - 1);
-        double* q = c->b_pool + off[(size_t)i];
-        double* vx_b = q; q += (size_t)nx * (ny + 1);
-        double* vy_b = q; q += (size_t)(nx + 1) * ny;
-        double* p_b = q; q += np;
-        double* t_xx = q; q += np;
-        double* t_yy = q; q += np;
-        double* d_v = q; q += np;
-        double* t_xy = q; q += (size_t)nx * ny;
-        double* r_x = q; q += (size_t)(nx - 2) * (ny - 1);
-        double* r_y = q; q += (size_t)(nx - 1) * (ny - 2);
-        double* scal = q;
-        const SolveConsts k = solve_consts(nx, ny, a.dx, a.dy);
-        PTParams pp{};
-        pp.nx = nx;
-        pp.ny = ny;
-        pp.seg = rows;
-        pp.depth = 1;
-        pp.nstrips = pt_num_strips(nx);
-        pp.ntiles = pp.nstrips * ((ny + rows - 1) / rows);
-        pp.dvx = a.dVxdtau;
-        pp.dvy = a.dVydtau;
-        pp.rho = a.rho_vy;
-        pp.mu_b = a.mu_b;
-        pp.mu_p = a.mu_p;
-        pp.dt = scal + SC_DT;
-        pp.txx = t_xx;
-        pp.tyy = t_yy;
-        pp.txy = t_xy;
-        pp.divv = d_v;
-        pp.rx = r_x;
-        pp.g = a.g_y;
-        pp.dampx = k.dampx;
-        pp.dampy = k.dampy;
-        pp.rdx = k.rdx;
-        pp.rdy = k.rdy;
-        pp.vsc_mdxy2 = k.vsc_mdxy2;
-        pp.ptsc41 = k.ptsc41;
-        pp.max_nxy = k.max_nxy;
-        b.pt[0] = pp;
-        b.pt[0].vx_a = a.Vx_s;
-        b.pt[0].vy_a = a.Vy_s;
-        b.pt[0].p_a = a.P;
-        b.pt[0].vx_b = vx_b;
-        b.pt[0].vy_b = vy_b;
-        b.pt[0].p_b = p_b;
-        b.pt[1] = pp;
-        b.pt[1].vx_a = vx_b;
-        b.pt[1].vy_a = vy_b;
-        b.pt[1].p_a = p_b;
-        b.pt[1].vx_b = a.Vx_s;
-        b.pt[1].vy_b = a.Vy_s;
-        b.pt[1].p_b = a.P;
-        b.P = a.P;
-        b.Vx_s = a.Vx_s;
-        b.Vy_s = a.Vy_s;
-        b.Vx_pad = a.Vx_pad;
-        b.Vy_pad = a.Vy_pad;
-        b.ry = r_y;
-        b.scal = scal;
-        b.nx = nx;
-        b.ny = ny;
-        b.use_fss = a.use_fss;
-        b.dx = a.dx;
-        b.dy = a.dy;
-        b.dt_in = a.dt;
-        b.maxdisp = a.maxdisp;
-        b.eps = a.eps;
-        b.iter_max = a.iter_max;
-    }
-    // largest expected work first: nodes x an iteration count that grows with the grid size
-    std::vector<int> order((size_t)n);
-    for (int i = 0; i < n; ++i) order[(size_t)i] = i;
-    auto work = [&](int i) { return (double)in[i].nx * in[i].ny * std::max(in[i].nx, in[i].ny); };
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+    for (int i = 0; i < n; ++i) host[(size_t)i] = batch_descriptor(in[i], rows, c->b_pool + off[(size_t)i], nullptr);
+    std::vector<std::pair<int, int>> dims((size_t)n);
+    for (int i = 0; i < n; ++i) dims[(size_t)i] = {in[i].nx, in[i].ny};
+    const std::vector<int> order = batch_order(dims);
     HIPCHK(hipMemcpyAsync(c->b_inst, host.data(), sizeof(BatchInstance) * (size_t)n, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->b_order, order.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, s));
-    int cus = 256;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1) cus = 256;
-    int resident = cus * batch_blocks_per_cu(threads);
-    if (prm && prm->max_blocks > 0) resident = std::min(resident, prm->max_blocks);
-    const int nblocks = std::min(n, resident);
+    const int nblocks = batch_grid(c, n, threads, prm ? prm->max_blocks : 0);
     HIPCHK(launch_batch_solve(c->b_inst, c->b_order, n, c->b_queue, c->b_res, threads, nblocks, s));
     std::vector<BatchResult> res((size_t)n);
     HIPCHK(hipMemcpyAsync(res.data(), c->b_res, sizeof(BatchResult) * (size_t)n, hipMemcpyDeviceToHost, s));
@@ -1148,5 +1195,249 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
         o.dt_last = r.dt_last;
         o.runtime = secs;  // the whole batch
     }
+    return SF_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// ensemble timestep (StokesFlow2D_multixpu.jl:164-181 for many independent problems),
+// captured once into a hipGraph and replayed per step
+
+struct sf_ensemble {
+    sf_ctx* c = nullptr;
+    hipStream_t stream = nullptr;  // own non-blocking stream: capture needs one
+    hipEvent_t ev = nullptr;
+    int n = 0, threads = 768, nblocks = 0, use_graph = 1;
+    std::vector<sf_step_args> mem;
+    double* pool = nullptr;
+    BatchInstance* inst = nullptr;
+    int* order = nullptr;
+    int* queue = nullptr;
+    BatchResult* res = nullptr;    // res[i].dt_new is member i's dt between steps
+    unsigned long long* cnt = nullptr;   // per member: 3 x 4 marker -> grid counters, 4 RK4 counters
+    unsigned long long* hcnt = nullptr;  // pinned
+    BatchResult* hres = nullptr;         // pinned
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    M2GScratch m2g;  // own sort scratch: the captured graph holds its addresses
+};
+
+enum { ENS_CNT = 16 };
+
+static void free_ensemble(sf_ensemble* e)
+{
+    if (!e) return;
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (void* p : {(void*)e->pool, (void*)e->inst, (void*)e->order, (void*)e->queue, (void*)e->res, (void*)e->cnt})
+        if (p) (void)hipFree(p);
+    if (e->hcnt) (void)hipHostFree(e->hcnt);
+    if (e->hres) (void)hipHostFree(e->hres);
+    if (e->ev) (void)hipEventDestroy(e->ev);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    m2g_free(e->m2g);
+    delete e;
+}
+
+// Enqueue one timestep of every member on e->stream: marker -> grid (rho_vy, mu_b, mu_p) per
+// member, one batched solve, RK4 per member with the solve's new dt (read on the device).
+static hipError_t ensemble_enqueue(sf_ensemble* e)
+{
+    hipStream_t s = e->stream;
+    for (int i = 0; i < e->n; ++i) {
+        const sf_step_args& a = e->mem[(size_t)i];
+        const double dx = a.dx, dy = a.dy;
+        // grid origins, StokesFlow2D_multixpu.jl:117-126
+        const double x_rho = 0.0 * dx - 0.5 * dx, y_rho = 0.0 * dy;
+        const double x_p = 0.0 * dx + 0.5 * dx, y_p = 0.0 * dy + 0.5 * dy;
+        M2GScratch S = e->m2g;
+        S.counters = e->cnt + (size_t)ENS_CNT * i;
+        hipError_t r = m2g_run(S, x_rho, y_rho, a.rho_vy, a.nx + 1, a.ny, a.x_m, a.y_m, a.rho_m, a.nm, dx, dy, nullptr,
+                               nullptr, 0, 0, s);
+        if (r != hipSuccess) return r;
+        S.counters = e->cnt + (size_t)ENS_CNT * i + 4;
+        r = m2g_run(S, 0.0, 0.0, a.mu_b, a.nx, a.ny, a.x_m, a.y_m, a.mu_m, a.nm, dx, dy, nullptr, nullptr, 0, 0, s);
+        if (r != hipSuccess) return r;
+        S.counters = e->cnt + (size_t)ENS_CNT * i + 8;
+        r = m2g_run(S, x_p, y_p, a.mu_p, a.nx - 1, a.ny - 1, a.x_m, a.y_m, a.mu_m, a.nm, dx, dy, nullptr, nullptr, 0,
+                    0, s);
+        if (r != hipSuccess) return r;
+    }
+    hipError_t r = launch_batch_solve(e->inst, e->order, e->n, e->queue, e->res, e->threads, e->nblocks, s);
+    if (r != hipSuccess) return r;
+    for (int i = 0; i < e->n; ++i) {
+        const sf_step_args& a = e->mem[(size_t)i];
+        const double dx = a.dx, dy = a.dy;
+        const double lx = (double)(a.nx - 1) * dx, ly = (double)(a.ny - 1) * dy;
+        const double x_vx = -1.0 * dx, y_vx = 0.0 * dy - 0.5 * dy;
+        const double x_vy = 0.0 * dx - 0.5 * dx, y_vy = -1.0 * dy;
+        unsigned long long* rc = e->cnt + (size_t)ENS_CNT * i + 12;
+        r = launch_zero_u64(rc, 4, s);
+        if (r != hipSuccess) return r;
+        r = launch_rk4(a.x_m, a.y_m, a.nm, a.Vx_pad, a.nx + 2, a.ny + 1, a.Vy_pad, a.nx + 1, a.ny + 2, x_vx, y_vx, x_vy,
+                       y_vy, 0.0, lx, ly, dx, dy, rc, s, &e->res[i].dt_new);
+        if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+}
+
+extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n, const sf_batch_params* prm,
+                                  int use_graph, sf_ensemble** out)
+{
+    const char* who = "sf_ensemble_create";
+    if (!c || !out || (n > 0 && !members)) return set_err(SF_ERR_ARG, "%s: NULL argument", who);
+    *out = nullptr;
+    if (n < 1) return set_err(SF_ERR_ARG, "%s: n < 1", who);
+    int threads, rows;
+    int rc = batch_threads(prm, &threads, &rows, who);
+    if (rc) return rc;
+    std::vector<sf_batch_instance> bi((size_t)n);
+    int64_t max_nm = 0, max_cells = 0;
+    size_t need = 0;
+    std::vector<size_t> off((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const sf_step_args& a = members[i];
+        sf_batch_instance& b = bi[(size_t)i];
+        std::memset(&b, 0, sizeof(b));
+        b.P = a.P;
+        b.Vx_pad = a.Vx_pad;
+        b.Vy_pad = a.Vy_pad;
+        b.dVxdtau = a.dVxdtau;
+        b.dVydtau = a.dVydtau;
+        b.Vx_s = a.Vx_s;
+        b.Vy_s = a.Vy_s;
+        b.rho_vy = a.rho_vy;
+        b.mu_b = a.mu_b;
+        b.mu_p = a.mu_p;
+        b.nx = a.nx;
+        b.ny = a.ny;
+        b.dx = a.dx;
+        b.dy = a.dy;
+        b.g_y = a.g_y;
+        b.dt = a.dt;
+        b.maxdisp = a.maxdisp;
+        b.eps = a.eps;
+        b.iter_max = a.iter_max;
+        b.use_fss = a.use_fss;
+        rc = batch_check(b, i, who);
+        if (rc) return rc;
+        if (a.nm < 0 || (a.nm > 0 && (!a.x_m || !a.y_m || !a.rho_m || !a.mu_m)))
+            return set_err(SF_ERR_ARG, "%s: member %d: marker arrays", who, i);
+        const int64_t cells = (int64_t)(a.nx + 2) * (a.ny + 1);  // the largest of the three grids' cell counts
+        if (a.nm >= ((int64_t)1 << 31) - 1) return set_err(SF_ERR_SIZE, "%s: member %d: too many markers", who, i);
+        max_nm = std::max(max_nm, a.nm);
+        max_cells = std::max(max_cells, cells);
+        off[(size_t)i] = need;
+        need += batch_scratch_doubles(a.nx, a.ny);
+    }
+    sf_ensemble* e = new sf_ensemble();
+    rc = m2g_reserve(e->m2g, max_nm, max_cells);
+    if (rc == SF_OK && dalloc(&e->m2g.counters, 4) != hipSuccess) rc = set_err(SF_ERR_HIP, "%s: hipMalloc failed", who);
+    if (rc) {
+        free_ensemble(e);
+        return rc;
+    }
+    e->c = c;
+    e->n = n;
+    e->threads = threads;
+    e->use_graph = use_graph ? 1 : 0;
+    e->mem.assign(members, members + n);
+    hipError_t er = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (er == hipSuccess) er = hipEventCreateWithFlags(&e->ev, hipEventDisableTiming);
+    if (er == hipSuccess) er = dalloc(&e->pool, need);
+    if (er == hipSuccess) er = dalloc(&e->inst, (size_t)n);
+    if (er == hipSuccess) er = dalloc(&e->order, (size_t)n);
+    if (er == hipSuccess) er = dalloc(&e->queue, 1);
+    if (er == hipSuccess) er = dalloc(&e->res, (size_t)n);
+    if (er == hipSuccess) er = dalloc(&e->cnt, (size_t)ENS_CNT * n);
+    if (er == hipSuccess) er = hipHostMalloc((void**)&e->hcnt, sizeof(unsigned long long) * ENS_CNT * n, hipHostMallocDefault);
+    if (er == hipSuccess) er = hipHostMalloc((void**)&e->hres, sizeof(BatchResult) * n, hipHostMallocDefault);
+    if (er != hipSuccess) {
+        free_ensemble(e);
+        return set_err(SF_ERR_HIP, "%s: %s", who, hipGetErrorString(er));
+    }
+    std::vector<BatchInstance> host((size_t)n);
+    std::vector<std::pair<int, int>> dims((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        host[(size_t)i] = batch_descriptor(bi[(size_t)i], rows, e->pool + off[(size_t)i], &e->res[i].dt_new);
+        dims[(size_t)i] = {members[i].nx, members[i].ny};
+        std::memset(&e->hres[i], 0, sizeof(BatchResult));
+        e->hres[i].dt_new = members[i].use_fss ? members[i].dt : 0.0;
+    }
+    const std::vector<int> order = batch_order(dims);
+    e->nblocks = batch_grid(c, n, threads, prm ? prm->max_blocks : 0);
+    // make the caller's earlier work on its stream visible, then upload on our stream
+    er = hipEventRecord(e->ev, c->stream);
+    if (er == hipSuccess) er = hipStreamWaitEvent(e->stream, e->ev, 0);
+    if (er == hipSuccess)
+        er = hipMemcpyAsync(e->inst, host.data(), sizeof(BatchInstance) * (size_t)n, hipMemcpyHostToDevice, e->stream);
+    if (er == hipSuccess)
+        er = hipMemcpyAsync(e->order, order.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, e->stream);
+    if (er == hipSuccess)
+        er = hipMemcpyAsync(e->res, e->hres, sizeof(BatchResult) * (size_t)n, hipMemcpyHostToDevice, e->stream);
+    if (er == hipSuccess) er = hipStreamSynchronize(e->stream);
+    if (er == hipSuccess && e->use_graph) {
+        er = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+        if (er == hipSuccess) {
+            const hipError_t eq = ensemble_enqueue(e);
+            er = hipStreamEndCapture(e->stream, &e->graph);
+            if (eq != hipSuccess) er = eq;
+        }
+        if (er == hipSuccess) er = hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0);
+    }
+    if (er != hipSuccess) {
+        free_ensemble(e);
+        return set_err(SF_ERR_HIP, "%s: %s", who, hipGetErrorString(er));
+    }
+    *out = e;
+    return SF_OK;
+}
+
+extern "C" int sf_ensemble_step(sf_ensemble* e, int n_steps, sf_solve_out* outs, double* dt)
+{
+    const char* who = "sf_ensemble_step";
+    if (!e) return set_err(SF_ERR_ARG, "%s: NULL ensemble", who);
+    if (n_steps < 0) return set_err(SF_ERR_ARG, "%s: n_steps < 0", who);
+    const auto t_start = std::chrono::steady_clock::now();
+    HIPCHK(hipEventRecord(e->ev, e->c->stream));
+    HIPCHK(hipStreamWaitEvent(e->stream, e->ev, 0));
+    for (int k = 0; k < n_steps; ++k) {
+        if (e->use_graph)
+            HIPCHK(hipGraphLaunch(e->exec, e->stream));
+        else
+            HIPCHK(ensemble_enqueue(e));
+    }
+    HIPCHK(hipMemcpyAsync(e->hres, e->res, sizeof(BatchResult) * (size_t)e->n, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(e->hcnt, e->cnt, sizeof(unsigned long long) * ENS_CNT * (size_t)e->n, hipMemcpyDeviceToHost,
+                          e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    if (n_steps > 0)
+        for (int i = 0; i < e->n; ++i) {
+            const unsigned long long* q = e->hcnt + (size_t)ENS_CNT * i;
+            if (q[2] + q[6] + q[10] > 0 || q[13] > 0)
+                return set_err(SF_ERR_NONFINITE, "%s: member %d has markers with non-finite coordinates", who, i);
+        }
+    for (int i = 0; i < e->n; ++i) {
+        const BatchResult& r = e->hres[i];
+        if (outs) {
+            sf_solve_out& o = outs[i];
+            std::memset(&o, 0, sizeof(o));
+            o.dt_new = r.dt_new;
+            o.err = r.err;
+            o.niter = r.niter;
+            o.converged = r.converged;
+            o.dt_last = r.dt_last;
+            o.runtime = secs;
+        }
+        if (dt) dt[i] = r.dt_new;
+    }
+    return SF_OK;
+}
+
+extern "C" int sf_ensemble_destroy(sf_ensemble* e)
+{
+    if (!e) return SF_OK;
+    (void)hipStreamSynchronize(e->stream);
+    free_ensemble(e);
     return SF_OK;
 }

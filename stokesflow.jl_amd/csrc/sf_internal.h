@@ -137,6 +137,7 @@ struct BatchInstance {
     PTParams pt[2];
     double *P, *Vx_s, *Vy_s, *Vx_pad, *Vy_pad;  // caller state (in/out)
     double *ry, *scal;                          // library scratch (scal[SC_DT] = stabilisation dt)
+    const double* dt_dev;                       // if set, the input dt is read here (device) instead of dt_in
     int nx, ny, use_fss, pad0;
     double dx, dy, dt_in, maxdisp, eps;
     int64_t iter_max;
@@ -180,7 +181,7 @@ hipError_t m2g_cell_keys(M2GScratch& s, double xmin, double ymin, int gnx, int g
 hipError_t launch_rk4(double* x_m, double* y_m, int64_t nm, const double* vx, int vx_n1, int vx_n2, const double* vy,
                       int vy_n1, int vy_n2, double x_vx_min, double y_vx_min, double x_vy_min, double y_vy_min,
                       double dt, double lx, double ly, double dx, double dy, unsigned long long* counters,
-                      hipStream_t s);
+                      hipStream_t s, const double* dt_ptr = nullptr);
 hipError_t launch_init_markers(double* x_m, double* y_m, int nx, int ny, int md, double dx, double dy,
                                double jitter_frac, uint64_t seed, hipStream_t s);
 struct Mat10 {

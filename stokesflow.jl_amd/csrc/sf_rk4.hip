@@ -88,12 +88,13 @@ __device__ inline bool interp_v(double x, double y, const double* __restrict__ V
 __global__ __launch_bounds__(256) void rk4_kernel(double* __restrict__ x_m, double* __restrict__ y_m, int64_t nm,
                                                   const double* __restrict__ Vx, int n1x, int n2x,
                                                   const double* __restrict__ Vy, int n1y, int n2y, double x_vx_min,
-                                                  double y_vx_min, double x_vy_min, double y_vy_min, double dt,
-                                                  double lx, double ly, double dx, double dy,
-                                                  unsigned long long* __restrict__ counters)
+                                                  double y_vx_min, double x_vy_min, double y_vy_min, double dt_val,
+                                                  const double* __restrict__ dt_ptr, double lx, double ly, double dx,
+                                                  double dy, unsigned long long* __restrict__ counters)
 {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nm) return;
+    const double dt = dt_ptr ? *dt_ptr : dt_val;  // a device dt: set by the solve earlier in a captured step
     const double rk_dt[4] = {0.0, 0.5 * dt, 0.5 * dt, dt};
     const double rk_wt[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
     const double x_old = x_m[m], y_old = y_m[m];
@@ -186,11 +187,12 @@ static int g1(int64_t n) { return (int)((n + 255) / 256); }
 
 hipError_t launch_rk4(double* x_m, double* y_m, int64_t nm, const double* vx, int vx_n1, int vx_n2, const double* vy,
                       int vy_n1, int vy_n2, double x_vx_min, double y_vx_min, double x_vy_min, double y_vy_min,
-                      double dt, double lx, double ly, double dx, double dy, unsigned long long* counters, hipStream_t s)
+                      double dt, double lx, double ly, double dx, double dy, unsigned long long* counters, hipStream_t s,
+                      const double* dt_ptr)
 {
     if (nm <= 0) return hipSuccess;
     hipLaunchKernelGGL(rk4_kernel, dim3(g1(nm)), dim3(256), 0, s, x_m, y_m, nm, vx, vx_n1, vx_n2, vy, vy_n1, vy_n2,
-                       x_vx_min, y_vx_min, x_vy_min, y_vy_min, dt, lx, ly, dx, dy, counters);
+                       x_vx_min, y_vx_min, x_vy_min, y_vy_min, dt, dt_ptr, lx, ly, dx, dy, counters);
     return hipGetLastError();
 }
 

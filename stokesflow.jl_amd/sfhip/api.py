@@ -370,17 +370,14 @@ class FlowState:
         self.Vy_small = z(nx + 1, ny)
 
 
-def step(state, x_m, y_m, rho_m, mu_m, dx, dy, dt, *, g_y=9.81, maxdisp=0.5, eps=1e-8, iterMax=100000,
-         use_free_surface_stabilization=True):
-    """One timestep (StokesFlow2D_multixpu.jl:164-181): M2G x3 -> solve -> RK4.  Returns (dt_new, stats)."""
+def _step_args(state, x_m, y_m, rho_m, mu_m, dx, dy, dt, g_y, maxdisp, eps, iterMax, use_fss):
     nm = x_m.numel()
     assert y_m.numel() == nm and rho_m.numel() == nm and mu_m.numel() == nm
     s = state
-    ctx = context(s.nx, s.ny)
     a = StepArgs()
     a.nx, a.ny, a.dx, a.dy = s.nx, s.ny, float(dx), float(dy)
     a.g_y, a.maxdisp, a.eps, a.iter_max = float(g_y), float(maxdisp), float(eps), int(iterMax)
-    a.use_fss, a.dt = int(bool(use_free_surface_stabilization)), float(dt)
+    a.use_fss, a.dt = int(bool(use_fss)), float(dt)
     nx, ny = s.nx, s.ny
     a.P = _grid_ptr(s.P, (nx - 1, ny - 1), "P")
     a.Vx_pad = _grid_ptr(s.Vx, (nx + 2, ny + 1), "Vx")
@@ -395,6 +392,15 @@ def step(state, x_m, y_m, rho_m, mu_m, dx, dy, dt, *, g_y=9.81, maxdisp=0.5, eps
     a.x_m, a.y_m = _marker_ptr(x_m, "x_m"), _marker_ptr(y_m, "y_m")
     a.rho_m, a.mu_m = _marker_ptr(rho_m, "rho_m"), _marker_ptr(mu_m, "mu_m")
     a.nm = nm
+    return a
+
+
+def step(state, x_m, y_m, rho_m, mu_m, dx, dy, dt, *, g_y=9.81, maxdisp=0.5, eps=1e-8, iterMax=100000,
+         use_free_surface_stabilization=True):
+    """One timestep (StokesFlow2D_multixpu.jl:164-181): M2G x3 -> solve -> RK4.  Returns (dt_new, stats)."""
+    a = _step_args(state, x_m, y_m, rho_m, mu_m, dx, dy, dt, g_y, maxdisp, eps, iterMax,
+                   use_free_surface_stabilization)
+    ctx = context(state.nx, state.ny)
     so = SolveOut()
     mo = (M2GOut * 3)()
     ro = RK4Out()
@@ -403,6 +409,54 @@ def step(state, x_m, y_m, rho_m, mu_m, dx, dy, dt, *, g_y=9.81, maxdisp=0.5, eps
     stats["m2g"] = [(m.n_bad_markers, m.n_empty_nodes) for m in mo]
     stats["n_clamped"] = ro.n_clamped
     return a.dt, stats
+
+
+class Ensemble:
+    """Many independent StokesFlow2D problems stepped together (sf_ensemble_*).
+
+    ``members`` is a sequence of dicts with keys state (a :class:`FlowState`), x_m, y_m, rho_m,
+    mu_m, dx, dy and optionally dt (0), g_y (9.81), maxdisp (0.5), eps (1e-8), iterMax (100000),
+    use_free_surface_stabilization (True).  One :meth:`step` runs, for every member, the three
+    marker -> grid passes, then one batched solve of all members, then RK4 with each member's
+    new dt -- with ``use_graph`` as one replay of a hipGraph captured at construction.  The
+    members' tensors must stay alive while the ensemble exists.
+    """
+
+    def __init__(self, members, *, use_graph=True, rows=0, threads=0, max_blocks=0):
+        self.lib = _lib.load()
+        self.n = len(members)
+        self._keep = list(members)
+        arr = (StepArgs * max(self.n, 1))()
+        for i, m in enumerate(members):
+            arr[i] = _step_args(m["state"], m["x_m"], m["y_m"], m["rho_m"], m["mu_m"], m["dx"], m["dy"],
+                                m.get("dt", 0.0), m.get("g_y", 9.81), m.get("maxdisp", 0.5), m.get("eps", 1e-8),
+                                m.get("iterMax", 100000), m.get("use_free_surface_stabilization", True))
+        ctx = context(0, 0)
+        self._ctx = ctx
+        self.handle = C.c_void_p()
+        prm = BatchParams(int(rows), int(threads), int(max_blocks), 0)
+        check(self.lib.sf_ensemble_create(ctx.handle, arr, self.n, C.byref(prm), int(bool(use_graph)),
+                                          C.byref(self.handle)))
+
+    def step(self, n_steps=1):
+        """Advance every member n_steps timesteps; returns per-member stats of the last step."""
+        self._ctx.use_current_stream()
+        outs = (SolveOut * max(self.n, 1))()
+        dts = (C.c_double * max(self.n, 1))()
+        check(self.lib.sf_ensemble_step(self.handle, int(n_steps), outs, dts))
+        keys = ("dt_new", "err", "niter", "converged", "dt_last", "runtime")
+        return [{k: getattr(outs[i], k) for k in keys} for i in range(self.n)]
+
+    def close(self):
+        if self.handle:
+            self.lib.sf_ensemble_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def StokesFlow2D(Nt, Nx, Ny, Lx_glob, Ly_glob, x_m, y_m, rho_m, mu_m, *, stokes_eps=1e-5, g_y=9.81, maxdisp=0.5,
