@@ -466,6 +466,104 @@ extern "C" int sf_pt_iterations(sf_ctx* c, int64_t n_iter, double* P, double* Vx
 }
 
 // ---------------------------------------------------------------------------------------
+// autotuner of the PT kernel's launch shape: times each candidate on a synthetic state of
+// the context's size (results do not depend on the shape, so any data will do)
+
+extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, sf_pt_tuning* best, sf_pt_tuning* table, int table_cap,
+                                  int* n_table)
+{
+    const char* who = "sf_ctx_autotune_pt";
+    if (!c) return set_err(SF_ERR_ARG, "%s: NULL ctx", who);
+    if (c->nx < 3 || c->ny < 3) return set_err(SF_ERR_ARG, "%s: marker-only context", who);
+    if (iters < 1) iters = 20;
+    const int nx = c->nx, ny = c->ny;
+    const size_t nvx = (size_t)nx * (ny + 1), nvy = (size_t)(nx + 1) * ny, np = (size_t)(nx - 1) * (ny - 1);
+    const size_t ndx = (size_t)(nx - 2) * (ny - 1), ndy = (size_t)(nx - 1) * (ny - 2);
+    double *P = nullptr, *vx = nullptr, *vy = nullptr, *dvx = nullptr, *dvy = nullptr, *rho = nullptr, *mub = nullptr,
+           *mup = nullptr;
+    hipError_t e = dalloc(&P, np);
+    if (e == hipSuccess) e = dalloc(&vx, nvx);
+    if (e == hipSuccess) e = dalloc(&vy, nvy);
+    if (e == hipSuccess) e = dalloc(&dvx, ndx);
+    if (e == hipSuccess) e = dalloc(&dvy, ndy);
+    if (e == hipSuccess) e = dalloc(&rho, nvy);
+    if (e == hipSuccess) e = dalloc(&mub, (size_t)nx * ny);
+    if (e == hipSuccess) e = dalloc(&mup, np);
+    auto release = [&]() {
+        for (double* p : {P, vx, vy, dvx, dvy, rho, mub, mup})
+            if (p) (void)hipFree(p);
+    };
+    if (e != hipSuccess) {
+        release();
+        return set_err(SF_ERR_HIP, "%s: %s", who, hipGetErrorString(e));
+    }
+    hipStream_t s = c->stream;
+    const double one = 1.0;
+    // a smooth, finite synthetic state: materials 1, state 0
+    e = launch_fill_f64(rho, (int64_t)nvy, one, s);
+    if (e == hipSuccess) e = launch_fill_f64(mub, (int64_t)nx * ny, one, s);
+    if (e == hipSuccess) e = launch_fill_f64(mup, (int64_t)np, one, s);
+    for (double* p : {P, vx, vy, dvx, dvy}) {
+        const size_t n = p == P ? np : p == vx ? nvx : p == vy ? nvy : p == dvx ? ndx : ndy;
+        if (e == hipSuccess) e = launch_fill_f64(p, (int64_t)n, 0.0, s);
+    }
+    if (e != hipSuccess) {
+        release();
+        return set_err(SF_ERR_HIP, "%s: %s", who, hipGetErrorString(e));
+    }
+    const int saved_seg = c->seg, saved_depth = c->depth, saved_waves = c->max_waves;
+    const int rows_c[] = {1, 2, 4, 8, 16};
+    const int depth_c[] = {1, 2};
+    const int waves_c[] = {-1, 0};
+    sf_pt_tuning bestv{};
+    bestv.us_per_iter = 1e300;
+    int nt = 0;
+    int rc = SF_OK;
+    for (int rows : rows_c)
+        for (int depth : depth_c)
+            for (int waves : waves_c) {
+                c->seg = rows;
+                c->depth = depth;
+                c->max_waves = waves;
+                rc = sf_pt_iterations(c, 3, P, vx, vy, dvx, dvy, rho, mub, mup, 1.0, 1.0 / (nx - 1), 1.0 / (ny - 1), nx,
+                                      ny, 0.0);
+                if (rc) break;
+                if (hipEventRecord(c->ev0, s) != hipSuccess) { rc = set_err(SF_ERR_HIP, "%s: event", who); break; }
+                rc = sf_pt_iterations(c, iters, P, vx, vy, dvx, dvy, rho, mub, mup, 1.0, 1.0 / (nx - 1),
+                                      1.0 / (ny - 1), nx, ny, 0.0);
+                if (rc) break;
+                float ms = 0.f;
+                if (hipEventRecord(c->ev1, s) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) {
+                    rc = set_err(SF_ERR_HIP, "%s: event timing failed", who);
+                    break;
+                }
+                sf_pt_tuning t{};
+                t.rows = rows;
+                t.depth = depth;
+                t.max_waves = waves;
+                t.us_per_iter = 1e3 * (double)ms / iters;
+                if (table && nt < table_cap) table[nt] = t;
+                ++nt;
+                if (t.us_per_iter < bestv.us_per_iter) bestv = t;
+            }
+    (void)hipStreamSynchronize(s);
+    release();
+    if (rc) {
+        c->seg = saved_seg;
+        c->depth = saved_depth;
+        c->max_waves = saved_waves;
+        return rc;
+    }
+    c->seg = bestv.rows;
+    c->depth = bestv.depth;
+    c->max_waves = bestv.max_waves;
+    if (best) *best = bestv;
+    if (n_table) *n_table = nt;
+    return SF_OK;
+}
+
+// ---------------------------------------------------------------------------------------
 // marker -> grid
 
 static void m2g_free(M2GScratch& S)

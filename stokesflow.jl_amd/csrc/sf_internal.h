@@ -126,6 +126,7 @@ hipError_t launch_resid_f64(const ResidParams& p, int nb, hipStream_t s);
 hipError_t launch_update_f64(double* a, const float* d, int64_t n, hipStream_t s);
 hipError_t launch_cast_f32(float* dst, const double* src, int64_t n, hipStream_t s);
 hipError_t launch_fill_f32(float* dst, int64_t n, float v, hipStream_t s);
+hipError_t launch_fill_f64(double* dst, int64_t n, double v, hipStream_t s);
 hipError_t launch_cast_f64(double* dst, const float* src, int64_t n, hipStream_t s);
 hipError_t launch_reduce_f32(const RedParamsF& p, int narr, int nb, hipStream_t s);
 

@@ -293,6 +293,11 @@ __global__ __launch_bounds__(256) void fill_f32(float* __restrict__ dst, int64_t
     for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) dst[k] = v;
 }
 
+__global__ __launch_bounds__(256) void fill_f64(double* __restrict__ dst, int64_t n, double v)
+{
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) dst[k] = v;
+}
+
 // Per-block sums of |a| over up to three FP32 arrays (FP64 accumulation, fixed order).
 __global__ __launch_bounds__(256) void reduce_abs_f32(RedParamsF p)
 {
@@ -353,6 +358,12 @@ hipError_t launch_cast_f64(double* dst, const float* src, int64_t n, hipStream_t
 hipError_t launch_fill_f32(float* dst, int64_t n, float v, hipStream_t s)
 {
     hipLaunchKernelGGL(fill_f32, dim3(grid_for(n)), dim3(256), 0, s, dst, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double* dst, int64_t n, double v, hipStream_t s)
+{
+    hipLaunchKernelGGL(fill_f64, dim3(grid_for(n)), dim3(256), 0, s, dst, n, v);
     return hipGetLastError();
 }
 

@@ -113,6 +113,17 @@ class Context:
 _contexts = {}
 
 
+def autotune_pt(nx, ny, iters=20):
+    """Time every PT launch shape on an nx x ny grid (sf_ctx_autotune_pt) and keep the fastest
+    for later solves of that size.  Returns (best, table) as dicts."""
+    ctx = context(int(nx), int(ny))
+    best = _lib.PTTuning()
+    table = (_lib.PTTuning * 64)()
+    n = C.c_int(0)
+    check(ctx.lib.sf_ctx_autotune_pt(ctx.handle, int(iters), C.byref(best), table, 64, C.byref(n)))
+    return best.as_dict(), [table[i].as_dict() for i in range(n.value)]
+
+
 def context(nx=0, ny=0):
     key = (nx, ny, torch.cuda.current_device())
     ctx = _contexts.get(key)
