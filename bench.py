@@ -196,6 +196,70 @@ def run_ensemble(specs, loop=True, iter_max=100000, eps=1e-8):
     return res
 
 
+def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 2.0)):
+    """Timesteps of a Rayleigh-Taylor ensemble (n_inst instances at n x n nodes, 16 markers per
+    cell): upper (small-y) layer rho 1.1 and viscosity 10^c, lower layer rho 1 and viscosity 1,
+    interface y = 0.5 + 0.02 sin(2 pi k x), k uniform over 1..8, c uniform over ``log10c``.
+    The same steps run three ways from the same initial state: the hipGraph-captured ensemble
+    step, the same ensemble step enqueued eagerly, and sf_step per instance in a loop.  All
+    three must agree bit for bit."""
+    import numpy as np
+    import torch
+
+    import sfhip
+
+    rng = np.random.default_rng(seed)
+    specs = [{"k": int(rng.integers(1, 9)), "c": float(rng.uniform(*log10c)), "seed": seed + 1 + i}
+             for i in range(n_inst)]
+    dx = dy = 1.0 / (n - 1)
+
+    def members():
+        out = []
+        for s in specs:
+            x, y = sfhip.init_markers(n, n, 4, dx, dy, 0.25, seed=s["seed"])
+            rho, mu = sfhip.rt_properties(x, y, 0.5, 0.02, s["k"], 1.0, 1.1, 10.0 ** s["c"], 1.0, 1.0)
+            out.append({"state": sfhip.FlowState(n, n), "x_m": x, "y_m": y, "rho_m": rho, "mu_m": mu, "dx": dx,
+                        "dy": dy, "g_y": 1.0, "eps": 1e-8})
+        return out
+
+    res = {"instances": n_inst, "n": n, "steps": steps, "markers_per_instance": 16 * (n - 1) ** 2,
+           "log10_contrast": list(log10c)}
+    finals = {}
+    for mode in ("graph", "eager", "loop"):
+        ms = members()
+        torch.cuda.synchronize()
+        if mode == "loop":
+            dts = [0.0] * n_inst
+            t0 = time.perf_counter()
+            iters = 0
+            for _ in range(steps):
+                for i, m in enumerate(ms):
+                    dts[i], st = sfhip.step(m["state"], m["x_m"], m["y_m"], m["rho_m"], m["mu_m"], dx, dy, dts[i],
+                                            g_y=1.0, eps=1e-8)
+                    iters += st["niter"]
+            torch.cuda.synchronize()
+            secs = time.perf_counter() - t0
+        else:
+            ens = sfhip.Ensemble(ms, use_graph=(mode == "graph"))
+            t0 = time.perf_counter()
+            iters = 0
+            for _ in range(steps):
+                outs = ens.step(1)
+                iters += sum(o["niter"] for o in outs)
+            torch.cuda.synchronize()
+            secs = time.perf_counter() - t0
+            ens.close()
+        res[mode] = {"seconds": secs, "steps_per_s": steps / secs, "pt_iters": iters}
+        finals[mode] = [(m["state"].Vx.clone(), m["x_m"].clone()) for m in ms]
+        del ms
+        torch.cuda.empty_cache()
+    res["graph_speedup_vs_loop"] = res["loop"]["seconds"] / res["graph"]["seconds"]
+    res["graph_speedup_vs_eager"] = res["eager"]["seconds"] / res["graph"]["seconds"]
+    res["bitwise_equal"] = all(torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+                               for mode in ("eager", "loop") for a, b in zip(finals["graph"], finals[mode]))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,6 +276,8 @@ def main():
                     help="instances of the mixed-size plume ensemble (batched vs per-solve loop); 0 skips it")
     ap.add_argument("--ens-c", type=float, nargs=2, default=(1.0, 4.0), metavar=("LO", "HI"),
                     help="range of log10 viscosity contrast of the ensemble plumes")
+    ap.add_argument("--ens-steps", type=int, nargs=3, default=None, metavar=("INSTANCES", "N", "STEPS"),
+                    help="Rayleigh-Taylor ensemble timesteps: graph-captured vs eager vs per-instance loop")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
@@ -304,6 +370,9 @@ def main():
             del st
         torch.cuda.empty_cache()
         line["ensemble"] = run_ensemble(ensemble_specs(a.ens, log10c=tuple(a.ens_c)))
+    if a.ens_steps:
+        torch.cuda.empty_cache()
+        line["ensemble_steps"] = run_ensemble_steps(*a.ens_steps)
     print(json.dumps(line), flush=True)
 
 

@@ -109,9 +109,9 @@ def test_batched_no_stabilisation_and_mixed_eps(gpu):
     pb, Gb = problem(S, nx, ny, dx, dy, eps=1e-5)
     oa, ob = sfhip.solveStokes_batched([pa, pb])
     torch.cuda.synchronize()
-    Sa = {k: v.copy() for k, v in S.items()}
+    Sa = {k: v.copy(order="F") for k, v in S.items()}
     xa = O.solve_stokes(Sa, 1.0, dx, dy, nx, ny, 0.0, 0.5, False, 1e-8)
-    Sb = {k: v.copy() for k, v in S.items()}
+    Sb = {k: v.copy(order="F") for k, v in S.items()}
     xb = O.solve_stokes(Sb, 1.0, dx, dy, nx, ny, 0.0, 0.5, True, 1e-5)
     assert oa["niter"] == xa["niter"] and ob["niter"] == xb["niter"] and ob["niter"] < xa["niter"]
     for k in OUT:

@@ -222,7 +222,7 @@ def test_autotune_keeps_results(gpu, golden):
     best, table = sfhip.autotune_pt(63, 47, iters=5)
     assert len(table) == 20 and best["us_per_iter"] == min(t["us_per_iter"] for t in table)
     S = cases.smooth_state(63, 47, seed=4)
-    exp = {k: v.copy() for k, v in S.items()}
+    exp = {k: v.copy(order="F") for k, v in S.items()}
     got = run_gpu_iterations(S, 63, 47, 9, 1.0, 1.0 / 62, 1.0 / 46, 0.01)
     ref = oracle_iterations(exp, 63, 47, 9, 1.0, 1.0 / 62, 1.0 / 46, 0.01)
     for k in STATE:
