@@ -135,9 +135,10 @@ def build_instance(spec):
     return st, dx, x.numel()
 
 
-def run_ensemble(specs, loop=True, iter_max=100000, eps=1e-8):
-    """Solve every instance to err <= eps in one persistent launch, then (``loop``) again one
-    solveStokes call at a time from the same zero state; both must agree bit for bit."""
+def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, budget_mb=0.0):
+    """Solve every instance to err <= eps with the batched solver in each of ``modes``
+    ("wide", "persistent"), then (``loop``) one solveStokes call at a time, each from the same
+    zero state; all runs must agree bit for bit."""
     import torch
 
     import sfhip
@@ -150,49 +151,56 @@ def run_ensemble(specs, loop=True, iter_max=100000, eps=1e-8):
             for t in (st.P, st.Vx, st.Vy, st.dVxdtau, st.dVydtau, st.Vx_small, st.Vy_small):
                 t.zero_()
 
+    def snapshot():
+        return [(st.Vx.clone(), st.Vy.clone(), st.P.clone()) for st, _, _ in insts]
+
     problems = [{"P": st.P, "Vx_pad": st.Vx, "Vy_pad": st.Vy, "dVxdtau": st.dVxdtau, "dVydtau": st.dVydtau,
                  "Vx_s": st.Vx_small, "Vy_s": st.Vy_small, "rho_vy": st.rho_vy, "mu_b": st.mu_b, "mu_p": st.mu_p,
                  "Nx": st.nx, "Ny": st.ny, "dx": dx, "dy": dx, "g_y": 1.0, "dt": 0.0, "maxdisp": 0.5, "eps": eps,
                  "iterMax": iter_max} for st, dx, _ in insts]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    outs = sfhip.solveStokes_batched(problems)
-    torch.cuda.synchronize()
-    t_batch = time.perf_counter() - t0
-    b_alg = sum(o["niter"] * bytes_per_iter(st.nx, st.ny) for o, (st, _, _) in zip(outs, insts))
-    res = {"instances": len(specs), "sizes": sorted({s["n"] for s in specs}),
-           "markers_total": n_markers,
-           "converged": sum(int(o["converged"]) for o in outs),
-           "pt_iters_total": sum(o["niter"] for o in outs),
-           "batched": {"seconds": t_batch, "solves_per_s": len(specs) / t_batch, "alg_gbs": b_alg / t_batch / 1e9}}
-    by_size = {}
-    for s, o in zip(specs, outs):
-        d = by_size.setdefault(s["n"], {"count": 0, "niter_max": 0, "niter_sum": 0, "unconverged": 0})
-        d["count"] += 1
-        d["niter_max"] = max(d["niter_max"], o["niter"])
-        d["niter_sum"] += o["niter"]
-        d["unconverged"] += 0 if o["converged"] else 1
-    res["by_size"] = {str(k): v for k, v in sorted(by_size.items())}
-    if loop:
-        sols = [(st.Vx.clone(), st.Vy.clone(), st.P.clone()) for st, _, _ in insts]
+    res = {"instances": len(specs), "sizes": sorted({s["n"] for s in specs}), "markers_total": n_markers}
+    sols, iters = {}, {}
+    for mode in modes:
         reset()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        louts = []
+        outs = sfhip.solveStokes_batched(problems, mode=mode, budget_mb=budget_mb)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        b_alg = sum(o["niter"] * bytes_per_iter(st.nx, st.ny) for o, (st, _, _) in zip(outs, insts))
+        res[mode] = {"seconds": t, "solves_per_s": len(specs) / t, "alg_gbs": b_alg / t / 1e9}
+        sols[mode], iters[mode] = snapshot(), [o["niter"] for o in outs]
+        res["converged"] = sum(int(o["converged"]) for o in outs)
+        res["pt_iters_total"] = sum(o["niter"] for o in outs)
+        by_size = {}
+        for s, o in zip(specs, outs):
+            d = by_size.setdefault(s["n"], {"count": 0, "niter_max": 0, "niter_sum": 0, "unconverged": 0})
+            d["count"] += 1
+            d["niter_max"] = max(d["niter_max"], o["niter"])
+            d["niter_sum"] += o["niter"]
+            d["unconverged"] += 0 if o["converged"] else 1
+        res["by_size"] = {str(k): v for k, v in sorted(by_size.items())}
+    if loop:
+        reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lit = []
         for st, dx, _ in insts:
             stats = {}
             sfhip.solveStokes(st.P, st.Vx, st.Vy, st.rho_vy, st.mu_b, st.mu_p, None, None, None, None, None, None,
                               None, st.dVxdtau, st.dVydtau, None, None, st.Vx_small, st.Vy_small, 1.0, dx, dx,
                               st.nx, st.ny, 0.0, 0.5, eps=eps, print_info=False, iterMax=iter_max, stats=stats)
-            louts.append(stats)
+            lit.append(stats["niter"])
         torch.cuda.synchronize()
         t_loop = time.perf_counter() - t0
-        same = all(bool(torch.equal(a, st.Vx) and torch.equal(b, st.Vy) and torch.equal(p, st.P))
-                   for (a, b, p), (st, _, _) in zip(sols, insts))
-        same_iters = all(o["niter"] == l["niter"] for o, l in zip(outs, louts))
         res["per_solve_loop"] = {"seconds": t_loop, "solves_per_s": len(specs) / t_loop}
-        res["speedup"] = t_loop / t_batch
-        res["bitwise_equal_to_loop"] = same and same_iters
+        sols["loop"], iters["loop"] = snapshot(), lit
+        for mode in modes:
+            res[mode]["speedup_vs_loop"] = t_loop / res[mode]["seconds"]
+    names = list(sols)
+    res["bitwise_equal"] = all(
+        iters[m] == iters[names[0]] and all(torch.equal(a, b) for x, y in zip(sols[m], sols[names[0]])
+                                            for a, b in zip(x, y)) for m in names[1:])
     return res
 
 
@@ -274,8 +282,9 @@ def main():
                     help="run a 1 GiB device-to-device copy first (reference size for rocprofv3 byte counters)")
     ap.add_argument("--ens", type=int, default=0,
                     help="instances of the mixed-size plume ensemble (batched vs per-solve loop); 0 skips it")
-    ap.add_argument("--ens-c", type=float, nargs=2, default=(1.0, 4.0), metavar=("LO", "HI"),
+    ap.add_argument("--ens-c", type=float, nargs=2, default=(0.5, 1.25), metavar=("LO", "HI"),
                     help="range of log10 viscosity contrast of the ensemble plumes")
+    ap.add_argument("--ens-modes", nargs="+", default=["wide"], help="batched modes to time (wide, persistent)")
     ap.add_argument("--ens-steps", type=int, nargs=3, default=None, metavar=("INSTANCES", "N", "STEPS"),
                     help="Rayleigh-Taylor ensemble timesteps: graph-captured vs eager vs per-instance loop")
     a = ap.parse_args()
@@ -369,7 +378,7 @@ def main():
         if "time_to_1e-8" not in line:
             del st
         torch.cuda.empty_cache()
-        line["ensemble"] = run_ensemble(ensemble_specs(a.ens, log10c=tuple(a.ens_c)))
+        line["ensemble"] = run_ensemble(ensemble_specs(a.ens, log10c=tuple(a.ens_c)), modes=tuple(a.ens_modes))
     if a.ens_steps:
         torch.cuda.empty_cache()
         line["ensemble_steps"] = run_ensemble_steps(*a.ens_steps)

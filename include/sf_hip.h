@@ -141,12 +141,15 @@ int sf_solve_stokes_refined(sf_ctx *ctx, double *P, double *Vx_pad, double *Vy_p
                             int use_fss, double eps, const sf_refine_params *prm, sf_solve_out *out,
                             sf_refine_out *refine_out);
 
-/* Batched ensemble: many independent solveStokes! problems (different sizes allowed) in one
- * persistent launch.  Each workgroup takes an instance from a device work queue (largest
+/* Batched ensemble: many independent solveStokes! problems (different sizes allowed).
+ * PERSISTENT: one launch; each workgroup takes an instance from a device work queue (largest
  * first), runs its whole solve with the reference's ncheck/ndtupdate rules, exits it on its
- * own convergence, and takes the next.  Every PT iteration is bit-identical to
- * sf_solve_stokes; err is summed in a different fixed order.  outs[i] gets instance i's
- * dt_new, err, niter, converged, dt_last (runtime = wall time of the batch).  No reference
+ * own convergence, and takes the next (no host synchronisation; capturable).
+ * WIDE: one launch per PT iteration covers every instance of an active set whose working
+ * sets fit budget_mb; each instance is checked at its own ncheck iterations and retired on
+ * convergence, and the set is refilled from the largest-first queue.
+ * Every PT iteration is bit-identical to sf_solve_stokes.  outs[i] gets instance i's dt_new,
+ * err, niter, converged, dt_last (runtime = wall time of the batch).  No reference
  * counterpart (the reference solves one problem per process). */
 typedef struct {
     double *P, *Vx_pad, *Vy_pad, *dVxdtau, *dVydtau, *Vx_s, *Vy_s;
@@ -158,11 +161,13 @@ typedef struct {
     int reserved;
 } sf_batch_instance;
 
+enum { SF_BATCH_AUTO = 0, SF_BATCH_PERSISTENT = 1, SF_BATCH_WIDE = 2 };
 typedef struct {
-    int rows;       /* grid rows per tile (<= 0: 4) */
-    int threads;    /* threads per instance workgroup, multiple of 64, <= 768 (<= 0: 768) */
-    int max_blocks; /* cap on concurrently resident instances (<= 0: as many as fit) */
-    int reserved;
+    int rows;         /* grid rows per tile (<= 0: 4) */
+    int threads;      /* persistent: threads per instance workgroup, multiple of 64, <= 768 (<= 0: 768) */
+    int max_blocks;   /* persistent: cap on concurrently resident instances (<= 0: as many as fit) */
+    int mode;         /* SF_BATCH_*; AUTO = WIDE */
+    double budget_mb; /* wide: working-set budget of the active set in MiB (<= 0: 192, the Infinity Cache) */
 } sf_batch_params;
 
 int sf_solve_stokes_batched(sf_ctx *ctx, const sf_batch_instance *inst, int n, const sf_batch_params *prm,

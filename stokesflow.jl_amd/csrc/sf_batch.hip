@@ -201,6 +201,41 @@ __global__ __launch_bounds__(BATCH_MAX_THREADS) void batch_solve_kernel(const Ba
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Wide mode: one launch runs one PT iteration of every instance in the active set.  Each
+// 64-lane workgroup is one tile of one instance; the slot is found by a search over the
+// tile prefix table in the kernel arguments.  The blockIdx -> tile map is XCD-banded, so an
+// XCD works on a contiguous range of instances and their tiles share its L2.
+
+__global__ __launch_bounds__(64) void wide_pt_kernel(WideArgs a)
+{
+    const int total = a.tile_start[a.n_active];
+    const int t_glob = xcd_remap(blockIdx.x, gridDim.x);
+    if (t_glob >= total) return;
+    int lo = 0, hi = a.n_active - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.tile_start[mid] <= t_glob) lo = mid;
+        else hi = mid - 1;
+    }
+    const int t = t_glob - a.tile_start[lo];
+    CBatch* cinst = (CBatch*)a.desc;
+    const int pf = a.slot_pf[lo];
+    const PTParams SF_CONST& pp = cinst[a.slot_inst[lo]].pt[pf & 1];
+    const int flags = pf >> 1;
+    const int nstrips = pp.nstrips;
+    if (flags) pt_tile<1, true>(pp, t % nstrips, t / nstrips, flags);
+    else pt_tile<1, false>(pp, t % nstrips, t / nstrips, 0);
+}
+
+hipError_t launch_wide_pt(const WideArgs& a, hipStream_t s)
+{
+    const int total = a.tile_start[a.n_active];
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(wide_pt_kernel, dim3(total), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
 __global__ void zero_int(int* p) { *p = 0; }
 
 int batch_blocks_per_cu(int threads)

@@ -63,6 +63,8 @@ struct sf_ctx {
     int* b_queue = nullptr;
     BatchResult* b_res = nullptr;
     int b_cap = 0;
+    double* b_hscal = nullptr;  // pinned per-instance scalars (wide mode)
+    int b_hscal_cap = 0;
     // PT launch tuning (results do not depend on it)
     int seg = 0;        // rows per tile; 0 = size heuristic
     int depth = 1;      // rows of operands prefetched
@@ -91,6 +93,7 @@ static void free_ctx(sf_ctx* c)
     for (void* b : rbufs)
         if (b) (void)hipFree(b);
     if (c->hpart) (void)hipHostFree(c->hpart);
+    if (c->b_hscal) (void)hipHostFree(c->b_hscal);
     for (void* b : {(void*)c->b_pool, (void*)c->b_inst, (void*)c->b_order, (void*)c->b_queue, (void*)c->b_res})
         if (b) (void)hipFree(b);
     if (c->hscal) (void)hipHostFree(c->hscal);
@@ -215,20 +218,26 @@ static int check_ctx(sf_ctx* c, int nx, int ny, const char* fn)
 }
 
 // max|Vx|, max|Vy| -> dt_check (and dt if update) on the device
-static int enqueue_dt(sf_ctx* c, const double* vx, const double* vy, double dx, double dy, double maxdisp, int update)
+// compute_dt (StokesSolver_multixpu.jl:118-121) into scal[SC_DT_CHECK] (and scal[SC_DT] if update).
+static int enqueue_dt_raw(hipStream_t s, int nx, int ny, const double* vx, const double* vy, double dx, double dy,
+                          double maxdisp, int update, double* part, double* scal)
 {
-    const int nx = c->nx, ny = c->ny;
     RedParams rp{};
     rp.a[0] = vx;
     rp.n[0] = (int64_t)nx * (ny + 1);
     rp.a[1] = vy;
     rp.n[1] = (int64_t)(nx + 1) * ny;
-    rp.part = c->part;
+    rp.part = part;
     rp.mode = RED_MAX;
     const int nb = red_num_blocks(std::max(rp.n[0], rp.n[1]));
-    HIPCHK(launch_reduce(rp, 2, nb, c->stream));
-    HIPCHK(launch_finalize_dt(c->part, nb, dx, dy, maxdisp, c->scal, update, c->stream));
+    HIPCHK(launch_reduce(rp, 2, nb, s));
+    HIPCHK(launch_finalize_dt(part, nb, dx, dy, maxdisp, scal, update, s));
     return SF_OK;
+}
+
+static int enqueue_dt(sf_ctx* c, const double* vx, const double* vy, double dx, double dy, double maxdisp, int update)
+{
+    return enqueue_dt_raw(c->stream, c->nx, c->ny, vx, vy, dx, dy, maxdisp, update, c->part, c->scal);
 }
 
 extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_pad, const double* rho_vy,
@@ -1228,6 +1237,178 @@ static int batch_threads(const sf_batch_params* prm, int* threads, int* rows, co
     return SF_OK;
 }
 
+// Bytes an instance keeps live while it iterates: both ping-pong copies of Vx, Vy, P, plus
+// dVx, dVy and the three material grids.
+static double batch_working_set(int nx_, int ny_)
+{
+    const double nx = nx_, ny = ny_, np = (nx - 1) * (ny - 1);
+    return 8.0 * (2.0 * (nx * (ny + 1) + (nx + 1) * ny + np) + (nx - 2) * (ny - 1) + (nx - 1) * (ny - 2) +
+                  (nx + 1) * ny + nx * ny + np);
+}
+
+// Wide mode: the host keeps an active set of instances whose working sets fit a byte budget
+// (the Infinity Cache), launches one PT iteration of all of them per kernel launch, runs each
+// instance's dt refresh and convergence check at its own iteration numbers (every check reads
+// err back, as the reference does), retires converged instances and refills the set from the
+// largest-first queue.  Every instance runs exactly the iterations of solveStokes!.
+static int batch_run_wide(sf_ctx* c, const sf_batch_instance* in, int n, const std::vector<BatchInstance>& desc,
+                          const std::vector<int>& order, double budget, std::vector<BatchResult>& res)
+{
+    hipStream_t s = c->stream;
+    struct Run {
+        int id;
+        int64_t iter, ncheck, ndtupdate;
+        double err, ws;
+        bool checking;
+    };
+    if (c->b_hscal_cap < n) {
+        if (c->b_hscal) HIPCHK(hipHostFree(c->b_hscal));
+        c->b_hscal = nullptr;
+        c->b_hscal_cap = 0;
+        HIPCHK(hipHostMalloc((void**)&c->b_hscal, sizeof(double) * SC_COUNT * (size_t)n, hipHostMallocDefault));
+        c->b_hscal_cap = n;
+    }
+    std::vector<Run> act;
+    size_t q = 0;
+    double ws_active = 0.0;
+    res.assign((size_t)n, BatchResult{});
+    auto start = [&](int id) -> int {
+        const sf_batch_instance& a = in[id];
+        const int mn = std::min(a.nx, a.ny);
+        Run r{id, 1, std::min<int64_t>(5 * (int64_t)mn, 2000), 10 * (int64_t)mn, 2.0 * a.eps,
+              batch_working_set(a.nx, a.ny), false};
+        HIPCHK(launch_set_scalar(desc[(size_t)id].scal + SC_DT, a.use_fss ? a.dt : 0.0, s));
+        act.push_back(r);
+        ws_active += r.ws;
+        return SF_OK;
+    };
+    auto finish = [&](const Run& r) -> int {
+        const sf_batch_instance& a = in[r.id];
+        const BatchInstance& d = desc[(size_t)r.id];
+        const int64_t niter = r.iter - 1;
+        if (niter & 1) {  // the last iterate is in the scratch buffers
+            const PTParams& p0 = d.pt[0];
+            HIPCHK(hipMemcpyAsync(a.Vx_s, p0.vx_b, sizeof(double) * (size_t)a.nx * (a.ny + 1), hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(a.Vy_s, p0.vy_b, sizeof(double) * (size_t)(a.nx + 1) * a.ny, hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(a.P, p0.p_b, sizeof(double) * (size_t)(a.nx - 1) * (a.ny - 1), hipMemcpyDeviceToDevice,
+                                  s));
+        }
+        HIPCHK(launch_pad_mirror(a.nx, a.ny, a.Vx_s, a.Vy_s, a.Vx_pad, a.Vy_pad, s));
+        int rc = enqueue_dt_raw(s, a.nx, a.ny, a.Vx_s, a.Vy_s, a.dx, a.dy, a.maxdisp, 0, c->part, d.scal);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(c->b_hscal + (size_t)SC_COUNT * r.id, d.scal, SC_COUNT * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+        BatchResult& o = res[(size_t)r.id];
+        o.err = r.err;
+        o.niter = niter;
+        o.converged = r.err <= a.eps;
+        return SF_OK;
+    };
+    int rc = SF_OK;
+    WideArgs wa;
+    std::memset(&wa, 0, sizeof(wa));
+    wa.desc = c->b_inst;
+    while (!act.empty() || q < order.size()) {
+        while (q < order.size() && (int)act.size() < WIDE_MAX &&
+               (act.empty() || ws_active + batch_working_set(in[order[q]].nx, in[order[q]].ny) <= budget)) {
+            rc = start(order[q++]);
+            if (rc) return rc;
+        }
+        // one PT iteration of every active instance
+        wa.n_active = (int)act.size();
+        int tiles = 0;
+        for (size_t k = 0; k < act.size(); ++k) {
+            Run& r = act[k];
+            const bool check = (r.iter % r.ncheck) == 0;
+            const bool last = r.iter == in[r.id].iter_max;
+            const int flags = ((check || last) ? PT_WRITE_TAU : 0) | (check ? PT_WRITE_RX : 0);
+            wa.tile_start[k] = tiles;
+            wa.slot_inst[k] = r.id;
+            wa.slot_pf[k] = (int)((r.iter - 1) & 1) | (flags << 1);
+            tiles += desc[(size_t)r.id].pt[0].ntiles;
+            r.checking = check;
+        }
+        wa.tile_start[act.size()] = tiles;
+        HIPCHK(launch_wide_pt(wa, s));
+        bool any_check = false;
+        for (Run& r : act) {
+            const sf_batch_instance& a = in[r.id];
+            const BatchInstance& d = desc[(size_t)r.id];
+            const PTParams& p = d.pt[(r.iter - 1) & 1];  // its outputs are now the current state
+            const bool dt_upd = a.use_fss && (r.iter % r.ndtupdate) == 0;
+            if (dt_upd) {
+                rc = enqueue_dt_raw(s, a.nx, a.ny, p.vx_b, p.vy_b, a.dx, a.dy, a.maxdisp, 1, c->part, d.scal);
+                if (rc) return rc;
+            }
+            if (!r.checking) continue;
+            any_check = true;
+            if (a.use_fss) {
+                if (!dt_upd) {
+                    rc = enqueue_dt_raw(s, a.nx, a.ny, p.vx_b, p.vy_b, a.dx, a.dy, a.maxdisp, 0, c->part, d.scal);
+                    if (rc) return rc;
+                }
+            } else {
+                HIPCHK(launch_set_scalar(d.scal + SC_DT_CHECK, 0.0, s));
+            }
+            const SolveConsts k = solve_consts(a.nx, a.ny, a.dx, a.dy);
+            ResYParams rq{};
+            rq.nx = a.nx;
+            rq.ny = a.ny;
+            rq.tyy = p.tyy;
+            rq.txy = p.txy;
+            rq.P = p.p_b;
+            rq.rho = a.rho_vy;
+            rq.vx = p.vx_b;
+            rq.vy = p.vy_b;
+            rq.dt = d.scal + SC_DT_CHECK;
+            rq.ry = d.ry;
+            rq.g = a.g_y;
+            rq.rdx = k.rdx;
+            rq.rdy = k.rdy;
+            HIPCHK(launch_pt_resid_y(rq, s));
+            const double n_rx = (double)(a.nx - 2) * (a.ny - 1), n_ry = (double)(a.nx - 1) * (a.ny - 2),
+                         n_dv = (double)(a.nx - 1) * (a.ny - 1);
+            RedParams rp{};
+            rp.a[0] = p.rx;
+            rp.n[0] = (int64_t)n_rx;
+            rp.a[1] = d.ry;
+            rp.n[1] = (int64_t)n_ry;
+            rp.a[2] = p.divv;
+            rp.n[2] = (int64_t)n_dv;
+            rp.part = c->part;
+            rp.mode = RED_SUM;
+            const int nb = red_num_blocks((int64_t)n_dv);
+            HIPCHK(launch_reduce(rp, 3, nb, s));
+            HIPCHK(launch_finalize_err(c->part, nb, n_rx, n_ry, n_dv, d.scal, s));
+            HIPCHK(hipMemcpyAsync(c->b_hscal + (size_t)SC_COUNT * r.id, d.scal, SC_COUNT * sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
+        }
+        if (any_check) HIPCHK(hipStreamSynchronize(s));
+        // advance, retire converged / capped instances
+        std::vector<Run> keep;
+        keep.reserve(act.size());
+        for (Run& r : act) {
+            if (r.checking) r.err = c->b_hscal[(size_t)SC_COUNT * r.id + SC_ERR];
+            r.iter += 1;
+            const sf_batch_instance& a = in[r.id];
+            if (r.err > a.eps && r.iter <= a.iter_max) {
+                keep.push_back(r);
+            } else {
+                rc = finish(r);
+                if (rc) return rc;
+                ws_active -= r.ws;
+            }
+        }
+        act.swap(keep);
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    for (int i = 0; i < n; ++i) {
+        res[(size_t)i].dt_new = c->b_hscal[(size_t)SC_COUNT * i + SC_DT_CHECK];
+        res[(size_t)i].dt_last = c->b_hscal[(size_t)SC_COUNT * i + SC_DT];
+    }
+    return SF_OK;
+}
+
 extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, int n, const sf_batch_params* prm,
                                        sf_solve_out* outs)
 {
@@ -1276,11 +1457,20 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
     const std::vector<int> order = batch_order(dims);
     HIPCHK(hipMemcpyAsync(c->b_inst, host.data(), sizeof(BatchInstance) * (size_t)n, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->b_order, order.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, s));
-    const int nblocks = batch_grid(c, n, threads, prm ? prm->max_blocks : 0);
-    HIPCHK(launch_batch_solve(c->b_inst, c->b_order, n, c->b_queue, c->b_res, threads, nblocks, s));
     std::vector<BatchResult> res((size_t)n);
-    HIPCHK(hipMemcpyAsync(res.data(), c->b_res, sizeof(BatchResult) * (size_t)n, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    const int mode = prm ? prm->mode : SF_BATCH_AUTO;
+    if (mode == SF_BATCH_PERSISTENT) {
+        const int nblocks = batch_grid(c, n, threads, prm ? prm->max_blocks : 0);
+        HIPCHK(launch_batch_solve(c->b_inst, c->b_order, n, c->b_queue, c->b_res, threads, nblocks, s));
+        HIPCHK(hipMemcpyAsync(res.data(), c->b_res, sizeof(BatchResult) * (size_t)n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    } else if (mode == SF_BATCH_WIDE || mode == SF_BATCH_AUTO) {
+        const double budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 192.0 * 1048576.0;
+        rc = batch_run_wide(c, in, n, host, order, budget, res);
+        if (rc) return rc;
+    } else {
+        return set_err(SF_ERR_ARG, "%s: unknown mode %d", who, mode);
+    }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     for (int i = 0; i < n; ++i) {
         const BatchResult& r = res[(size_t)i];

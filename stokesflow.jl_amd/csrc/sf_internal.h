@@ -150,6 +150,17 @@ struct BatchResult {
     int converged, pad;
 };
 
+// Wide mode of the batched solver: the active set of one PT-iteration launch.
+#define WIDE_MAX 64
+struct WideArgs {
+    const BatchInstance* desc;
+    int n_active;
+    int tile_start[WIDE_MAX + 1];  // prefix of the slots' tile counts
+    int slot_inst[WIDE_MAX];       // descriptor index
+    int slot_pf[WIDE_MAX];         // ping-pong parity | PT_WRITE_* flags << 1
+};
+hipError_t launch_wide_pt(const WideArgs& a, hipStream_t s);
+
 hipError_t launch_batch_solve(const BatchInstance* inst, const int* order, int n_inst, int* queue,
                               BatchResult* res, int threads, int nblocks, hipStream_t s);
 int batch_blocks_per_cu(int threads);

@@ -100,8 +100,11 @@ class PTTuning(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
 
 
+SF_BATCH_AUTO, SF_BATCH_PERSISTENT, SF_BATCH_WIDE = 0, 1, 2
+
+
 class BatchParams(C.Structure):
-    _fields_ = [("rows", _i), ("threads", _i), ("max_blocks", _i), ("reserved", _i)]
+    _fields_ = [("rows", _i), ("threads", _i), ("max_blocks", _i), ("mode", _i), ("budget_mb", _d)]
 
 
 _lib = None

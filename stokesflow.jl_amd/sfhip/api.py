@@ -194,7 +194,7 @@ def solveStokes(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtau
 _BATCH_ARRAYS = ("P", "Vx_pad", "Vy_pad", "dVxdtau", "dVydtau", "Vx_s", "Vy_s", "rho_vy", "mu_b", "mu_p")
 
 
-def solveStokes_batched(problems, *, rows=0, threads=0, max_blocks=0):
+def solveStokes_batched(problems, *, mode="auto", rows=0, threads=0, max_blocks=0, budget_mb=0.0):
     """Many independent solveStokes! problems in one persistent launch (sf_solve_stokes_batched).
 
     ``problems`` is a sequence of dicts with the arrays P, Vx_pad, Vy_pad, dVxdtau, dVydtau,
@@ -203,6 +203,10 @@ def solveStokes_batched(problems, *, rows=0, threads=0, max_blocks=0):
     use_free_surface_stabilization (True).  Grids may differ in size.  Arrays are updated in
     place; returns one stats dict per problem (dt_new, err, niter, converged, dt_last, and
     runtime = wall time of the whole batch).
+
+    ``mode`` "persistent": one launch, one workgroup per instance from a device work queue;
+    "wide" (= "auto"): one launch per PT iteration over an active set of instances whose
+    working sets fit ``budget_mb`` (default 192 MiB), refilled as instances converge.
     """
     n = len(problems)
     arr = (BatchInstance * max(n, 1))()
@@ -220,7 +224,8 @@ def solveStokes_batched(problems, *, rows=0, threads=0, max_blocks=0):
         b.eps = float(pb.get("eps", 1e-8))
         b.iter_max = int(pb.get("iterMax", 100000))
         b.use_fss = int(bool(pb.get("use_free_surface_stabilization", True)))
-    prm = BatchParams(int(rows), int(threads), int(max_blocks), 0)
+    modes = {"auto": _lib.SF_BATCH_AUTO, "persistent": _lib.SF_BATCH_PERSISTENT, "wide": _lib.SF_BATCH_WIDE}
+    prm = BatchParams(int(rows), int(threads), int(max_blocks), modes[mode], float(budget_mb))
     outs = (SolveOut * max(n, 1))()
     ctx = context(0, 0)
     check(ctx.lib.sf_solve_stokes_batched(ctx.handle, arr, n, C.byref(prm), outs))

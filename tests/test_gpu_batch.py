@@ -52,7 +52,10 @@ def single(S, nx, ny, dx, dy, eps=1e-8, iterMax=100000):
 SIZES = [(24, 24), (33, 47), (40, 29), (64, 64), (17, 90)]
 
 
-@pytest.mark.parametrize("prm", [{}, {"max_blocks": 1}, {"threads": 64}, {"threads": 320, "rows": 7}])
+@pytest.mark.parametrize("prm", [{"mode": "persistent"}, {"mode": "persistent", "max_blocks": 1},
+                                 {"mode": "persistent", "threads": 64},
+                                 {"mode": "persistent", "threads": 320, "rows": 7},
+                                 {"mode": "wide"}, {"mode": "wide", "budget_mb": 1e-3}, {"mode": "wide", "rows": 7}])
 def test_batched_matches_single_solves(gpu, prm):
     insts = [(nx, ny, *plume_instance(nx, ny, seed=10 + k)) for k, (nx, ny) in enumerate(SIZES)]
     problems, gs = [], []
@@ -72,11 +75,12 @@ def test_batched_matches_single_solves(gpu, prm):
             assert np.array_equal(sfhip.to_numpy(G[k]), exp[k]), (nx, ny, k)
 
 
-def test_batched_against_oracle(gpu):
+@pytest.mark.parametrize("mode", ["persistent", "wide"])
+def test_batched_against_oracle(gpu, mode):
     nx, ny = 40, 29
     S, dx, dy = plume_instance(nx, ny, seed=5)
     pb, G = problem(S, nx, ny, dx, dy)
-    (o,) = sfhip.solveStokes_batched([pb])
+    (o,) = sfhip.solveStokes_batched([pb], mode=mode)
     torch.cuda.synchronize()
     exp = O.solve_stokes(S, 1.0, dx, dy, nx, ny, 0.0, 0.5, True, 1e-8)
     assert o["niter"] == exp["niter"] and o["dt_new"] == exp["dt_new"]
@@ -84,8 +88,9 @@ def test_batched_against_oracle(gpu):
         assert np.array_equal(sfhip.to_numpy(G[k]), S[k]), k
 
 
+@pytest.mark.parametrize("mode", ["persistent", "wide"])
 @pytest.mark.parametrize("cap", [37, 250])
-def test_batched_iteration_cap(gpu, cap):
+def test_batched_iteration_cap(gpu, cap, mode):
     """An odd cap leaves the last iterate in the scratch buffers: it must be copied back."""
     insts = [(nx, ny, *plume_instance(nx, ny, seed=3 + k)) for k, (nx, ny) in enumerate([(24, 24), (33, 47)])]
     problems, gs = [], []
@@ -93,7 +98,7 @@ def test_batched_iteration_cap(gpu, cap):
         pb, G = problem(S, nx, ny, dx, dy, iterMax=cap)
         problems.append(pb)
         gs.append(G)
-    outs = sfhip.solveStokes_batched(problems)
+    outs = sfhip.solveStokes_batched(problems, mode=mode)
     torch.cuda.synchronize()
     for (nx, ny, S, dx, dy), o, G in zip(insts, outs, gs):
         st, exp = single(S, nx, ny, dx, dy, iterMax=cap)
@@ -102,12 +107,13 @@ def test_batched_iteration_cap(gpu, cap):
             assert np.array_equal(sfhip.to_numpy(G[k]), exp[k]), (nx, ny, k)
 
 
-def test_batched_no_stabilisation_and_mixed_eps(gpu):
+@pytest.mark.parametrize("mode", ["persistent", "wide"])
+def test_batched_no_stabilisation_and_mixed_eps(gpu, mode):
     nx, ny = 33, 47
     S, dx, dy = plume_instance(nx, ny, seed=8)
     pa, Ga = problem(S, nx, ny, dx, dy, use_free_surface_stabilization=False)
     pb, Gb = problem(S, nx, ny, dx, dy, eps=1e-5)
-    oa, ob = sfhip.solveStokes_batched([pa, pb])
+    oa, ob = sfhip.solveStokes_batched([pa, pb], mode=mode)
     torch.cuda.synchronize()
     Sa = {k: v.copy(order="F") for k, v in S.items()}
     xa = O.solve_stokes(Sa, 1.0, dx, dy, nx, ny, 0.0, 0.5, False, 1e-8)
@@ -126,4 +132,4 @@ def test_batched_errors(gpu):
     with pytest.raises(ValueError):
         sfhip.solveStokes_batched([dict(pb, Nx=25)])  # shape check on the host
     with pytest.raises(sfhip.SFError):
-        sfhip.solveStokes_batched([pb], threads=100)
+        sfhip.solveStokes_batched([pb], mode="persistent", threads=100)
