@@ -287,6 +287,8 @@ def main():
     ap.add_argument("--ens-modes", nargs="+", default=["wide"], help="batched modes to time (wide, persistent)")
     ap.add_argument("--ens-steps", type=int, nargs=3, default=None, metavar=("INSTANCES", "N", "STEPS"),
                     help="Rayleigh-Taylor ensemble timesteps: graph-captured vs eager vs per-instance loop")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the PT launch-shape autotuner first (default: the size default shape)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
@@ -320,6 +322,8 @@ def main():
     n_markers = x_m.numel()
     del x_m, y_m, rho_m, mu_m
     torch.cuda.empty_cache()
+
+    tuned = sfhip.autotune_pt(nx, ny, iters=30)[0] if a.tune else None
 
     def solve(iters):
         stats = {}
@@ -365,6 +369,7 @@ def main():
         "t_eff_gbs": 6 * nx * ny * 8 / (wall / a.steps) / 1e9,
         "alg_gbs": b_iter / (wall / a.steps) / 1e9,
         "state_finite": finite,
+        "pt_launch_shape": tuned or "size default (4 rows, 2 rows of operands in flight above 2048^2)",
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "pt_fused_f64", "bytes_alg_per_launch": b_iter, "launch_s": per_iter},

@@ -74,9 +74,9 @@ int sf_ctx_destroy(sf_ctx *ctx);
 /* Replace the stream work is enqueued on. */
 int sf_ctx_set_stream(sf_ctx *ctx, void *hip_stream);
 /* Launch tuning of the PT iteration kernel; results do not depend on it.  rows: grid rows
- * each tile marches through (0 = size heuristic); depth: rows of operands prefetched (1 or
- * 2); max_waves: 0 = one wave per resident slot (persistent), > 0 = at most that many
- * waves, < 0 = one wave per tile.  No reference counterpart (ParallelStencil picks launch
+ * each tile marches through (0 = size default); depth: rows of operands prefetched (1 or 2;
+ * 0 = size default); max_waves: 0 = one wave per resident slot (persistent), > 0 = at most
+ * that many waves, < 0 = one wave per tile.  No reference counterpart (ParallelStencil picks launch
  * shapes itself). */
 int sf_ctx_set_pt_tuning(sf_ctx *ctx, int rows, int depth, int max_waves);
 
@@ -163,10 +163,12 @@ typedef struct {
 
 enum { SF_BATCH_AUTO = 0, SF_BATCH_PERSISTENT = 1, SF_BATCH_WIDE = 2 };
 typedef struct {
-    int rows;         /* grid rows per tile (<= 0: 4) */
+    int rows;         /* grid rows per tile (<= 0: per instance, as the single solver's default) */
+    int depth;        /* wide: rows of operands prefetched, 1 or 2 (<= 0: 1) */
     int threads;      /* persistent: threads per instance workgroup, multiple of 64, <= 768 (<= 0: 768) */
     int max_blocks;   /* persistent: cap on concurrently resident instances (<= 0: as many as fit) */
     int mode;         /* SF_BATCH_*; AUTO = WIDE */
+    int reserved;
     double budget_mb; /* wide: working-set budget of the active set in MiB (<= 0: 192, the Infinity Cache) */
 } sf_batch_params;
 

@@ -207,6 +207,7 @@ __global__ __launch_bounds__(BATCH_MAX_THREADS) void batch_solve_kernel(const Ba
 // tile prefix table in the kernel arguments.  The blockIdx -> tile map is XCD-banded, so an
 // XCD works on a contiguous range of instances and their tiles share its L2.
 
+template <int DEPTH>
 __global__ __launch_bounds__(64) void wide_pt_kernel(WideArgs a)
 {
     const int total = a.tile_start[a.n_active];
@@ -224,15 +225,16 @@ __global__ __launch_bounds__(64) void wide_pt_kernel(WideArgs a)
     const PTParams SF_CONST& pp = cinst[a.slot_inst[lo]].pt[pf & 1];
     const int flags = pf >> 1;
     const int nstrips = pp.nstrips;
-    if (flags) pt_tile<1, true>(pp, t % nstrips, t / nstrips, flags);
-    else pt_tile<1, false>(pp, t % nstrips, t / nstrips, 0);
+    if (flags) pt_tile<DEPTH, true>(pp, t % nstrips, t / nstrips, flags);
+    else pt_tile<DEPTH, false>(pp, t % nstrips, t / nstrips, 0);
 }
 
-hipError_t launch_wide_pt(const WideArgs& a, hipStream_t s)
+hipError_t launch_wide_pt(const WideArgs& a, int depth, hipStream_t s)
 {
     const int total = a.tile_start[a.n_active];
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(wide_pt_kernel, dim3(total), dim3(64), 0, s, a);
+    if (depth > 1) hipLaunchKernelGGL(wide_pt_kernel<2>, dim3(total), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL(wide_pt_kernel<1>, dim3(total), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

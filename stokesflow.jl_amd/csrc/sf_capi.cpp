@@ -67,7 +67,7 @@ struct sf_ctx {
     int b_hscal_cap = 0;
     // PT launch tuning (results do not depend on it)
     int seg = 0;        // rows per tile; 0 = size heuristic
-    int depth = 1;      // rows of operands prefetched
+    int depth = 0;      // rows of operands prefetched; 0 = size default
     int max_waves = -1; // 0 persistent (resident slots), > 0 cap, < 0 one wave per tile
 };
 
@@ -159,7 +159,7 @@ extern "C" int sf_ctx_set_pt_tuning(sf_ctx* c, int rows, int depth, int max_wave
 {
     if (!c) return set_err(SF_ERR_ARG, "sf_ctx_set_pt_tuning: NULL ctx");
     if (rows < 0 || rows > 4096) return set_err(SF_ERR_ARG, "sf_ctx_set_pt_tuning: rows %d outside [0, 4096]", rows);
-    if (depth < 1 || depth > 2) return set_err(SF_ERR_ARG, "sf_ctx_set_pt_tuning: depth %d not 1 or 2", depth);
+    if (depth < 0 || depth > 2) return set_err(SF_ERR_ARG, "sf_ctx_set_pt_tuning: depth %d not 0, 1 or 2", depth);
     c->seg = rows;
     c->depth = depth;
     c->max_waves = max_waves;
@@ -1146,6 +1146,8 @@ static BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, doub
     double* r_y = q; q += (size_t)(nx - 1) * (ny - 2);
     double* scal = q;
     const SolveConsts k = solve_consts(nx, ny, a.dx, a.dy);
+    if (rows <= 0) rows = pt_default_rows(nx, ny);
+    rows = std::min(rows, ny);
     PTParams pp{};
     pp.nx = nx;
     pp.ny = ny;
@@ -1232,7 +1234,7 @@ static int batch_grid(sf_ctx* c, int n, int threads, int max_blocks)
 static int batch_threads(const sf_batch_params* prm, int* threads, int* rows, const char* who)
 {
     *threads = (prm && prm->threads > 0) ? prm->threads : 768;
-    *rows = (prm && prm->rows > 0) ? prm->rows : 4;
+    *rows = (prm && prm->rows > 0) ? prm->rows : 0;  // 0: per instance, pt_default_rows
     if (*threads % 64 != 0 || *threads > 768) return set_err(SF_ERR_ARG, "%s: threads %d", who, *threads);
     return SF_OK;
 }
@@ -1252,7 +1254,7 @@ static double batch_working_set(int nx_, int ny_)
 // err back, as the reference does), retires converged instances and refills the set from the
 // largest-first queue.  Every instance runs exactly the iterations of solveStokes!.
 static int batch_run_wide(sf_ctx* c, const sf_batch_instance* in, int n, const std::vector<BatchInstance>& desc,
-                          const std::vector<int>& order, double budget, std::vector<BatchResult>& res)
+                          const std::vector<int>& order, double budget, int depth, std::vector<BatchResult>& res)
 {
     hipStream_t s = c->stream;
     struct Run {
@@ -1329,7 +1331,7 @@ static int batch_run_wide(sf_ctx* c, const sf_batch_instance* in, int n, const s
             r.checking = check;
         }
         wa.tile_start[act.size()] = tiles;
-        HIPCHK(launch_wide_pt(wa, s));
+        HIPCHK(launch_wide_pt(wa, depth, s));
         bool any_check = false;
         for (Run& r : act) {
             const sf_batch_instance& a = in[r.id];
@@ -1466,7 +1468,7 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
         HIPCHK(hipStreamSynchronize(s));
     } else if (mode == SF_BATCH_WIDE || mode == SF_BATCH_AUTO) {
         const double budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 192.0 * 1048576.0;
-        rc = batch_run_wide(c, in, n, host, order, budget, res);
+        rc = batch_run_wide(c, in, n, host, order, budget, (prm && prm->depth > 1) ? 2 : 1, res);
         if (rc) return rc;
     } else {
         return set_err(SF_ERR_ARG, "%s: unknown mode %d", who, mode);

@@ -61,6 +61,7 @@ struct RedParams {
 
 int pt_num_strips(int nx);
 int pt_default_rows(int nx, int ny);
+int pt_default_depth(int nx, int ny);
 int pt_resident_waves(int depth);
 int red_num_blocks(int64_t nmax);
 hipError_t launch_pt_fused(const PTParams& p, hipStream_t s);
@@ -159,7 +160,7 @@ struct WideArgs {
     int slot_inst[WIDE_MAX];       // descriptor index
     int slot_pf[WIDE_MAX];         // ping-pong parity | PT_WRITE_* flags << 1
 };
-hipError_t launch_wide_pt(const WideArgs& a, hipStream_t s);
+hipError_t launch_wide_pt(const WideArgs& a, int depth, hipStream_t s);
 
 hipError_t launch_batch_solve(const BatchInstance* inst, const int* order, int n_inst, int* queue,
                               BatchResult* res, int threads, int nblocks, hipStream_t s);

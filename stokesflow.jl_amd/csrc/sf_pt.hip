@@ -45,6 +45,14 @@ __global__ __launch_bounds__(64) void pt_fused_f64(PTParams p)
     }
 }
 
+// One wave per tile, no grid-stride loop (fewer live registers: 4 waves per SIMD).
+template <int DEPTH, bool CHECK>
+__global__ __launch_bounds__(64) void pt_tile_f64(PTParams p)
+{
+    const int logical = xcd_remap(blockIdx.x, gridDim.x);
+    pt_tile<DEPTH, CHECK>(p, logical % p.nstrips, logical / p.nstrips, p.flags);
+}
+
 // compute_ResY with post-update V and dt_check (compute_Residuals!, :241-249).
 __global__ __launch_bounds__(256) void pt_resid_y(ResYParams p)
 {
@@ -189,17 +197,22 @@ int pt_resident_waves(int depth)
 // Rows per tile.  Short tiles measured fastest on MI355X at every size from 512^2 to 4096^2
 // (one wave per tile, all strips sweeping the same few rows at once; the one-row prologue
 // a tile re-reads was just fetched into the same XCD's L2 by the tile above it).
+// Default launch shape (measured on MI355X with the autotuner, sf_ctx_autotune_pt): short
+// tiles win on grids up to 1024^2 (more waves), 4-row tiles with two rows of operands in
+// flight on larger ones.
 int pt_default_rows(int nx, int ny)
 {
-    (void)nx;
-    return ny >= 4 ? 4 : ny;
+    const int r = (std::max(nx, ny) <= 1024) ? 2 : 4;
+    return ny >= r ? r : ny;
 }
+
+int pt_default_depth(int nx, int ny) { return ((int64_t)nx * ny > (int64_t)2048 * 2048) ? 2 : 1; }
 
 hipError_t launch_pt_fused(const PTParams& p0, hipStream_t s)
 {
     PTParams p = p0;
     if (p.seg <= 0) p.seg = pt_default_rows(p.nx, p.ny);
-    const int depth = p.depth > 1 ? 2 : 1;
+    const int depth = p.depth <= 0 ? pt_default_depth(p.nx, p.ny) : (p.depth > 1 ? 2 : 1);
     p.nstrips = pt_num_strips(p.nx);
     const int nseg = (p.ny + p.seg - 1) / p.seg;
     p.ntiles = p.nstrips * nseg;
@@ -208,7 +221,15 @@ hipError_t launch_pt_fused(const PTParams& p0, hipStream_t s)
     if (p.max_waves == 0) grid = std::min(p.ntiles, pt_resident_waves(depth));
     else if (p.max_waves > 0) grid = std::min(p.ntiles, p.max_waves);
     const bool check = p.flags != 0;
-    if (depth == 2) {
+    if (grid == p.ntiles) {  // one wave per tile
+        if (depth == 2) {
+            if (check) hipLaunchKernelGGL((pt_tile_f64<2, true>), dim3(grid), dim3(64), 0, s, p);
+            else hipLaunchKernelGGL((pt_tile_f64<2, false>), dim3(grid), dim3(64), 0, s, p);
+        } else {
+            if (check) hipLaunchKernelGGL((pt_tile_f64<1, true>), dim3(grid), dim3(64), 0, s, p);
+            else hipLaunchKernelGGL((pt_tile_f64<1, false>), dim3(grid), dim3(64), 0, s, p);
+        }
+    } else if (depth == 2) {
         if (check) hipLaunchKernelGGL((pt_fused_f64<2, true>), dim3(grid), dim3(64), 0, s, p);
         else hipLaunchKernelGGL((pt_fused_f64<2, false>), dim3(grid), dim3(64), 0, s, p);
     } else {

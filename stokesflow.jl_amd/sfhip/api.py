@@ -194,7 +194,7 @@ def solveStokes(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtau
 _BATCH_ARRAYS = ("P", "Vx_pad", "Vy_pad", "dVxdtau", "dVydtau", "Vx_s", "Vy_s", "rho_vy", "mu_b", "mu_p")
 
 
-def solveStokes_batched(problems, *, mode="auto", rows=0, threads=0, max_blocks=0, budget_mb=0.0):
+def solveStokes_batched(problems, *, mode="auto", rows=0, depth=0, threads=0, max_blocks=0, budget_mb=0.0):
     """Many independent solveStokes! problems in one persistent launch (sf_solve_stokes_batched).
 
     ``problems`` is a sequence of dicts with the arrays P, Vx_pad, Vy_pad, dVxdtau, dVydtau,
@@ -225,7 +225,7 @@ def solveStokes_batched(problems, *, mode="auto", rows=0, threads=0, max_blocks=
         b.iter_max = int(pb.get("iterMax", 100000))
         b.use_fss = int(bool(pb.get("use_free_surface_stabilization", True)))
     modes = {"auto": _lib.SF_BATCH_AUTO, "persistent": _lib.SF_BATCH_PERSISTENT, "wide": _lib.SF_BATCH_WIDE}
-    prm = BatchParams(int(rows), int(threads), int(max_blocks), modes[mode], float(budget_mb))
+    prm = BatchParams(int(rows), int(depth), int(threads), int(max_blocks), modes[mode], 0, float(budget_mb))
     outs = (SolveOut * max(n, 1))()
     ctx = context(0, 0)
     check(ctx.lib.sf_solve_stokes_batched(ctx.handle, arr, n, C.byref(prm), outs))
@@ -450,7 +450,7 @@ class Ensemble:
         ctx = context(0, 0)
         self._ctx = ctx
         self.handle = C.c_void_p()
-        prm = BatchParams(int(rows), int(threads), int(max_blocks), 0)
+        prm = BatchParams(int(rows), 0, int(threads), int(max_blocks), _lib.SF_BATCH_PERSISTENT, 0, 0.0)
         check(self.lib.sf_ensemble_create(ctx.handle, arr, self.n, C.byref(prm), int(bool(use_graph)),
                                           C.byref(self.handle)))
 
