@@ -38,7 +38,7 @@ def bytes_per_iter(nx, ny):
 def profiled_traffic(nx):
     """HBM bytes per launch of the fused kernel from the committed rocprofv3 summary of this
     workload (FETCH_SIZE and WRITE_SIZE, each scaled by a copy kernel of known size)."""
-    path = os.path.join(ROOT, "profiles", f"pt_fused_f64_n{nx}.txt")
+    path = os.path.join(ROOT, "profiles", f"pt_tile_f64_n{nx}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -323,7 +323,7 @@ def main():
     del x_m, y_m, rho_m, mu_m
     torch.cuda.empty_cache()
 
-    tuned = sfhip.autotune_pt(nx, ny, iters=30)[0] if a.tune else None
+    tuned = sfhip.autotune_pt(nx, ny, iters=30, state=st, g_y=1.0, dx=dx, dy=dy)[0] if a.tune else None
 
     def solve(iters):
         stats = {}
@@ -369,10 +369,10 @@ def main():
         "t_eff_gbs": 6 * nx * ny * 8 / (wall / a.steps) / 1e9,
         "alg_gbs": b_iter / (wall / a.steps) / 1e9,
         "state_finite": finite,
-        "pt_launch_shape": tuned or "size default (4 rows, 2 rows of operands in flight above 2048^2)",
+        "pt_launch_shape": tuned or "size default (4-row tiles above 1024^2, one row of operands in flight)",
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "pt_fused_f64", "bytes_alg_per_launch": b_iter, "launch_s": per_iter},
+                     "kernel": "pt_tile_f64", "bytes_alg_per_launch": b_iter, "launch_s": per_iter},
         "cpu_baseline": None if a.no_cpu else cpu_baseline(nx, ny, a.cpu_iters),
     }
     if a.tts_n > 0:

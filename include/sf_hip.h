@@ -81,15 +81,20 @@ int sf_ctx_set_stream(sf_ctx *ctx, void *hip_stream);
 int sf_ctx_set_pt_tuning(sf_ctx *ctx, int rows, int depth, int max_waves);
 
 /* Autotuner of that launch shape: times `iters` PT iterations of every candidate (rows 1, 2,
- * 4, 8, 16 x depth 1, 2 x one wave per tile or persistent) on a synthetic state of the
- * context's grid size, in scratch it allocates and frees, and sets the fastest on the
- * context.  best (may be NULL) receives it; table (may be NULL, table_cap entries) every
+ * 4, 8, 16 x depth 1, 2 x one wave per tile or persistent) on a copy of the caller's problem
+ * (P ... mu_p as for sf_pt_iterations; the caller's arrays are not modified) or, when any of
+ * them is NULL, on a synthetic one (materials 1, state 0), in scratch it allocates and frees;
+ * the fastest is set on the context.  Timing can depend on the data, so tune on the real
+ * problem.  best (may be NULL) receives it; table (may be NULL, table_cap entries) every
  * candidate's time, n_table how many were timed.  No reference counterpart. */
 typedef struct {
     int rows, depth, max_waves, reserved;
     double us_per_iter;
 } sf_pt_tuning;
-int sf_ctx_autotune_pt(sf_ctx *ctx, int iters, sf_pt_tuning *best, sf_pt_tuning *table, int table_cap, int *n_table);
+int sf_ctx_autotune_pt(sf_ctx *ctx, int iters, const double *P, const double *Vx_s, const double *Vy_s,
+                       const double *dVxdtau, const double *dVydtau, const double *rho_vy, const double *mu_b,
+                       const double *mu_p, double g_y, double dx, double dy, sf_pt_tuning *best, sf_pt_tuning *table,
+                       int table_cap, int *n_table);
 
 /* solveStokes! -- replaces scripts/StokesSolver_multixpu.jl:14-96
  * (solveStokes!(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtauPt, Rx, Ry,

@@ -478,8 +478,10 @@ extern "C" int sf_pt_iterations(sf_ctx* c, int64_t n_iter, double* P, double* Vx
 // autotuner of the PT kernel's launch shape: times each candidate on a synthetic state of
 // the context's size (results do not depend on the shape, so any data will do)
 
-extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, sf_pt_tuning* best, sf_pt_tuning* table, int table_cap,
-                                  int* n_table)
+extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, const double* P_in, const double* Vx_in, const double* Vy_in,
+                                  const double* dVx_in, const double* dVy_in, const double* rho_in,
+                                  const double* mub_in, const double* mup_in, double g_y, double dx_in, double dy_in,
+                                  sf_pt_tuning* best, sf_pt_tuning* table, int table_cap, int* n_table)
 {
     const char* who = "sf_ctx_autotune_pt";
     if (!c) return set_err(SF_ERR_ARG, "%s: NULL ctx", who);
@@ -508,13 +510,24 @@ extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, sf_pt_tuning* best, sf_p
     }
     hipStream_t s = c->stream;
     const double one = 1.0;
-    // a smooth, finite synthetic state: materials 1, state 0
-    e = launch_fill_f64(rho, (int64_t)nvy, one, s);
-    if (e == hipSuccess) e = launch_fill_f64(mub, (int64_t)nx * ny, one, s);
-    if (e == hipSuccess) e = launch_fill_f64(mup, (int64_t)np, one, s);
-    for (double* p : {P, vx, vy, dvx, dvy}) {
-        const size_t n = p == P ? np : p == vx ? nvx : p == vy ? nvy : p == dvx ? ndx : ndy;
-        if (e == hipSuccess) e = launch_fill_f64(p, (int64_t)n, 0.0, s);
+    // the caller's problem (copied: its arrays are not touched), or a synthetic one
+    const bool given = P_in && Vx_in && Vy_in && dVx_in && dVy_in && rho_in && mub_in && mup_in;
+    const double gy = given ? g_y : 1.0;
+    const double dx = given ? dx_in : 1.0 / (nx - 1), dy = given ? dy_in : 1.0 / (ny - 1);
+    if (given) {
+        const std::pair<double*, const double*> cp[] = {{P, P_in},   {vx, Vx_in},   {vy, Vy_in},   {dvx, dVx_in},
+                                                        {dvy, dVy_in}, {rho, rho_in}, {mub, mub_in}, {mup, mup_in}};
+        const size_t len[] = {np, nvx, nvy, ndx, ndy, nvy, (size_t)nx * ny, np};
+        for (int q = 0; q < 8 && e == hipSuccess; ++q)
+            e = hipMemcpyAsync(cp[q].first, cp[q].second, len[q] * sizeof(double), hipMemcpyDeviceToDevice, s);
+    } else {
+        e = launch_fill_f64(rho, (int64_t)nvy, one, s);
+        if (e == hipSuccess) e = launch_fill_f64(mub, (int64_t)nx * ny, one, s);
+        if (e == hipSuccess) e = launch_fill_f64(mup, (int64_t)np, one, s);
+        for (double* p : {P, vx, vy, dvx, dvy}) {
+            const size_t n = p == P ? np : p == vx ? nvx : p == vy ? nvy : p == dvx ? ndx : ndy;
+            if (e == hipSuccess) e = launch_fill_f64(p, (int64_t)n, 0.0, s);
+        }
     }
     if (e != hipSuccess) {
         release();
@@ -534,12 +547,10 @@ extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, sf_pt_tuning* best, sf_p
                 c->seg = rows;
                 c->depth = depth;
                 c->max_waves = waves;
-                rc = sf_pt_iterations(c, 3, P, vx, vy, dvx, dvy, rho, mub, mup, 1.0, 1.0 / (nx - 1), 1.0 / (ny - 1), nx,
-                                      ny, 0.0);
+                rc = sf_pt_iterations(c, 3, P, vx, vy, dvx, dvy, rho, mub, mup, gy, dx, dy, nx, ny, 0.0);
                 if (rc) break;
                 if (hipEventRecord(c->ev0, s) != hipSuccess) { rc = set_err(SF_ERR_HIP, "%s: event", who); break; }
-                rc = sf_pt_iterations(c, iters, P, vx, vy, dvx, dvy, rho, mub, mup, 1.0, 1.0 / (nx - 1),
-                                      1.0 / (ny - 1), nx, ny, 0.0);
+                rc = sf_pt_iterations(c, iters, P, vx, vy, dvx, dvy, rho, mub, mup, gy, dx, dy, nx, ny, 0.0);
                 if (rc) break;
                 float ms = 0.f;
                 if (hipEventRecord(c->ev1, s) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess ||

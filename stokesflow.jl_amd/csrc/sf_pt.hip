@@ -197,16 +197,21 @@ int pt_resident_waves(int depth)
 // Rows per tile.  Short tiles measured fastest on MI355X at every size from 512^2 to 4096^2
 // (one wave per tile, all strips sweeping the same few rows at once; the one-row prologue
 // a tile re-reads was just fetched into the same XCD's L2 by the tile above it).
-// Default launch shape (measured on MI355X with the autotuner, sf_ctx_autotune_pt): short
-// tiles win on grids up to 1024^2 (more waves), 4-row tiles with two rows of operands in
-// flight on larger ones.
+// Default launch shape (measured on MI355X with plume data): 2-row tiles on grids up to
+// 1024^2 (more waves), 4-row tiles above; one row of operands in flight (124 VGPRs, four
+// waves per SIMD, beat two rows in flight at three waves per SIMD).
 int pt_default_rows(int nx, int ny)
 {
     const int r = (std::max(nx, ny) <= 1024) ? 2 : 4;
     return ny >= r ? r : ny;
 }
 
-int pt_default_depth(int nx, int ny) { return ((int64_t)nx * ny > (int64_t)2048 * 2048) ? 2 : 1; }
+int pt_default_depth(int nx, int ny)
+{
+    (void)nx;
+    (void)ny;
+    return 1;
+}
 
 hipError_t launch_pt_fused(const PTParams& p0, hipStream_t s)
 {

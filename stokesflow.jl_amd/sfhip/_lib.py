@@ -143,7 +143,8 @@ def load():
     L.sf_pt_iterations.argtypes = [_vp, _i64] + [_vp] * 8 + [_d, _d, _d, _i, _i, _d]
     L.sf_marker_to_grid.argtypes = [_vp, _d, _d, _vp, _i, _i, _vp, _vp, _vp, _i64, _d, _d, _vp, _vp, _i, _i,
                                     C.POINTER(M2GOut)]
-    L.sf_ctx_autotune_pt.argtypes = [_vp, _i, C.POINTER(PTTuning), C.POINTER(PTTuning), _i, C.POINTER(_i)]
+    L.sf_ctx_autotune_pt.argtypes = [_vp, _i] + [_vp] * 8 + [_d, _d, _d, C.POINTER(PTTuning), C.POINTER(PTTuning), _i,
+                                                             C.POINTER(_i)]
     L.sf_ensemble_create.argtypes = [_vp, C.POINTER(StepArgs), _i, C.POINTER(BatchParams), _i, C.POINTER(_vp)]
     L.sf_ensemble_step.argtypes = [_vp, _i, C.POINTER(SolveOut), C.POINTER(_d)]
     L.sf_ensemble_destroy.argtypes = [_vp]

@@ -113,14 +113,27 @@ class Context:
 _contexts = {}
 
 
-def autotune_pt(nx, ny, iters=20):
+def autotune_pt(nx, ny, iters=20, state=None, g_y=1.0, dx=None, dy=None):
     """Time every PT launch shape on an nx x ny grid (sf_ctx_autotune_pt) and keep the fastest
-    for later solves of that size.  Returns (best, table) as dicts."""
-    ctx = context(int(nx), int(ny))
+    for later solves of that size.  ``state`` (a :class:`FlowState`) gives the problem to time
+    on (copied, not modified); without it a synthetic problem is used.  Returns (best, table)."""
+    nx, ny = int(nx), int(ny)
+    ctx = context(nx, ny)
     best = _lib.PTTuning()
     table = (_lib.PTTuning * 64)()
     n = C.c_int(0)
-    check(ctx.lib.sf_ctx_autotune_pt(ctx.handle, int(iters), C.byref(best), table, 64, C.byref(n)))
+    if state is not None:
+        s = state
+        ptrs = [_grid_ptr(s.P, (nx - 1, ny - 1), "P"), _grid_ptr(s.Vx_small, (nx, ny + 1), "Vx_small"),
+                _grid_ptr(s.Vy_small, (nx + 1, ny), "Vy_small"), _grid_ptr(s.dVxdtau, (nx - 2, ny - 1), "dVxdtau"),
+                _grid_ptr(s.dVydtau, (nx - 1, ny - 2), "dVydtau"), _grid_ptr(s.rho_vy, (nx + 1, ny), "rho_vy"),
+                _grid_ptr(s.mu_b, (nx, ny), "mu_b"), _grid_ptr(s.mu_p, (nx - 1, ny - 1), "mu_p")]
+    else:
+        ptrs = [None] * 8
+    ddx = float(dx) if dx is not None else 1.0 / (nx - 1)
+    ddy = float(dy) if dy is not None else 1.0 / (ny - 1)
+    check(ctx.lib.sf_ctx_autotune_pt(ctx.handle, int(iters), *ptrs, float(g_y), ddx, ddy, C.byref(best), table, 64,
+                                     C.byref(n)))
     return best.as_dict(), [table[i].as_dict() for i in range(n.value)]
 
 
