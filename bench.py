@@ -10,7 +10,14 @@ iteration 11, T_eff = 6 Nx Ny 8 B / t_it).
 
 Printed: one JSON line with PT iterations/s (value), the reference's T_eff, the roofline of
 the fused iteration kernel (algorithmic bytes / average launch time against 8 TB/s), and the
-CPU oracle timed on a bounded sample of iterations on the host cores.
+CPU oracle timed on a bounded sample of iterations on the host cores.  Two further records
+ride on the same line (both skippable):
+
+* time_to_1e-8 (--tts-n, default 2048): the FP64 solver against FP32 PT sweeps inside FP64
+  defect correction, from the same state to the same FP64 tolerance;
+* ensemble (--ens, default 64 instances): a mixed-size plume ensemble (64..512 nodes per
+  side) solved by the batched solver and by one solveStokes call per instance, bit for bit
+  the same results; solves/s for both.
 """
 import argparse
 import json
@@ -68,7 +75,7 @@ def cpu_baseline(nx, ny, iters):
             "sample": f"{iters} PT iterations of the oracle on {nx}x{ny} nodes (FP64), {t:.1f} s"}
 
 
-def time_to_solution(n, seed=3000):
+def time_to_solution(n, seed=3000, progress=False):
     """Time-to-1e-8 of the FP64 solver and of FP32 sweeps in FP64 defect correction on the
     plume set-up at n x n nodes (16 markers per cell), from the same initial state."""
     import torch
@@ -91,8 +98,8 @@ def time_to_solution(n, seed=3000):
         t0 = time.perf_counter()
         sfhip.solveStokes(st.P, st.Vx, st.Vy, st.rho_vy, st.mu_b, st.mu_p, None, None, None, None, None, None, None,
                           st.dVxdtau, st.dVydtau, None, None, st.Vx_small, st.Vy_small, 1.0, dx, dy, n, n, 0.0, 0.5,
-                          eps=1e-8, print_info=False, iterMax=10_000_000, stats=stats,
-                          mixed_precision=(mode != "fp64"))
+                          eps=1e-8, print_info=2 if progress else False, iterMax=10_000_000, stats=stats,
+                          mixed_precision=(mode != "fp64"), refine={"verbose": 1} if progress else None)
         torch.cuda.synchronize()
         res[mode] = {"seconds": time.perf_counter() - t0, "converged": bool(stats["converged"]),
                      "err": stats["err"], "pt_iters": stats["niter"]}
@@ -280,13 +287,14 @@ def main():
                     help="nodes per direction of the time-to-1e-8 comparison (FP64 vs FP32-refined); 0 skips it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy first (reference size for rocprofv3 byte counters)")
-    ap.add_argument("--ens", type=int, default=0,
+    ap.add_argument("--ens", type=int, default=64,
                     help="instances of the mixed-size plume ensemble (batched vs per-solve loop); 0 skips it")
     ap.add_argument("--ens-c", type=float, nargs=2, default=(0.5, 1.25), metavar=("LO", "HI"),
                     help="range of log10 viscosity contrast of the ensemble plumes")
     ap.add_argument("--ens-modes", nargs="+", default=["wide"], help="batched modes to time (wide, persistent)")
     ap.add_argument("--ens-steps", type=int, nargs=3, default=None, metavar=("INSTANCES", "N", "STEPS"),
                     help="Rayleigh-Taylor ensemble timesteps: graph-captured vs eager vs per-instance loop")
+    ap.add_argument("--progress", action="store_true", help="print the solvers' convergence checks (long runs)")
     ap.add_argument("--tune", action="store_true",
                     help="run the PT launch-shape autotuner first (default: the size default shape)")
     a = ap.parse_args()
@@ -378,7 +386,7 @@ def main():
     if a.tts_n > 0:
         del st
         torch.cuda.empty_cache()
-        line["time_to_1e-8"] = time_to_solution(a.tts_n)
+        line["time_to_1e-8"] = time_to_solution(a.tts_n, progress=a.progress)
     if a.ens > 0:
         if "time_to_1e-8" not in line:
             del st

@@ -103,7 +103,8 @@ int sf_ctx_autotune_pt(sf_ctx *ctx, int iters, const double *P, const double *Vx
  * In/out: P, Vx_s, Vy_s, dVxdtau, dVydtau (state), Vx_pad, Vy_pad (output).
  * Scratch arguments txx, tyy, txy, divV, dtauP, Rx, Ry, dtauVx, dtauVy may be NULL; when
  * given they are left holding what the reference leaves in them.  comm is absent: one
- * rank.  The caller's rho_vy, mu_b, mu_p are read only. */
+ * rank.  The caller's rho_vy, mu_b, mu_p are read only.  print_info: 0 silent, 1 the
+ * reference's summary line (:82-83), 2 also one line per convergence check. */
 int sf_solve_stokes(sf_ctx *ctx, double *P, double *Vx_pad, double *Vy_pad, const double *rho_vy,
                     const double *mu_b, const double *mu_p, double *txx, double *tyy, double *txy,
                     double *divV, double *dtauP, double *Rx, double *Ry, double *dVxdtau, double *dVydtau,

@@ -371,6 +371,10 @@ extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_
             HIPCHK(hipMemcpyAsync(c->hscal, c->scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
             err = c->hscal[SC_ERR];
+            if (print_info >= 2) {
+                printf("  iter = %lld, err = %1.3e\n", (long long)iter, err);
+                fflush(stdout);
+            }
         }
         iter += 1;
         niter += 1;
@@ -961,7 +965,8 @@ extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, dou
     CorrParams cp{};
     cp.nx = nx;
     cp.ny = ny;
-    cp.seg = 4;
+    cp.seg = c->seg > 0 ? c->seg : 4;  // the context's tile rows, when set
+    cp.depth = c->depth > 1 ? 2 : 1;
     cp.mx = c->r_mx;
     cp.my = c->r_my;
     cp.rho = c->r_rho;

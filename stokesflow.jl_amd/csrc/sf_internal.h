@@ -80,7 +80,7 @@ hipError_t launch_fill_timesteps(int nx, int ny, const double* mu_p, double* dta
 
 // ---- FP32 sweeps in FP64 defect correction ----
 struct CorrParams {
-    int nx, ny, seg, nstrips, ntiles;
+    int nx, ny, seg, nstrips, ntiles, depth;
     const float* vx_a;  // correction state in (Vd_x, Vd_y, Pd)
     const float* vy_a;
     const float* p_a;

@@ -66,7 +66,7 @@ __device__ inline CorrRowIn corr_load_row(const CorrParams& p, const CorrCols& l
     return in;
 }
 
-template <bool CHECK>
+template <int DEPTH, bool CHECK>
 __device__ inline void corr_tile(const CorrParams& p, int strip, int segid)
 {
     const int lane = threadIdx.x;
@@ -91,6 +91,9 @@ __device__ inline void corr_tile(const CorrParams& p, int strip, int segid)
     float vy_r = (p.vy_a + (int64_t)r * (nx + 1))[lc.vy];
     float rho_r = (p.rho + (int64_t)r * (nx + 1))[lc.vy];
     CorrRowIn nxt = corr_load_row(p, lc, r, r >= r0);
+    CorrRowIn nxt2;
+    if constexpr (DEPTH > 1)
+        if (r + 1 < r1) nxt2 = corr_load_row(p, lc, r + 1, r + 1 >= r0);
     float vx_r_p1 = lane_down(vx_r, 1);
     float vy_r_p1 = lane_down(vy_r, 1);
     float rho_r_p2 = lane_down(rho_r, 2);
@@ -102,7 +105,12 @@ __device__ inline void corr_tile(const CorrParams& p, int strip, int segid)
 
     for (; r < r1; ++r) {
         const CorrRowIn in = nxt;
-        if (r + 1 < r1) nxt = corr_load_row(p, lc, r + 1, true);
+        if constexpr (DEPTH > 1) {
+            nxt = nxt2;
+            if (r + 2 < r1) nxt2 = corr_load_row(p, lc, r + 2, true);  // two rows of operands in flight
+        } else {
+            if (r + 1 < r1) nxt = corr_load_row(p, lc, r + 1, true);
+        }
         const bool emit = r >= r0;
         const bool cell_row = r <= ny - 2;
         const int j = r - 1;
@@ -191,16 +199,12 @@ __device__ inline void corr_tile(const CorrParams& p, int strip, int segid)
     }
 }
 
-template <bool CHECK>
+// One wave per tile, XCD-banded tile order (as the FP64 kernel).
+template <int DEPTH, bool CHECK>
 __global__ __launch_bounds__(64) void pt_corr_f32(CorrParams p)
 {
-    const int G = gridDim.x;
-    for (int base = 0; base < p.ntiles; base += G) {
-        const int nround = min(G, p.ntiles - base);
-        if ((int)blockIdx.x >= nround) break;
-        const int logical = base + xcd_remap_r(blockIdx.x, nround);
-        corr_tile<CHECK>(p, logical % p.nstrips, logical / p.nstrips);
-    }
+    const int logical = xcd_remap_r(blockIdx.x, gridDim.x);
+    corr_tile<DEPTH, CHECK>(p, logical % p.nstrips, logical / p.nstrips);
 }
 
 // FP64 residual of the state u = (V, P) without a pressure update, with stabilisation dt.
@@ -326,8 +330,13 @@ hipError_t launch_pt_corr(const CorrParams& p0, bool check, hipStream_t s)
     p.nstrips = (p.nx + CORR_OUT_LANES - 1) / CORR_OUT_LANES;
     const int nseg = (p.ny + p.seg - 1) / p.seg;
     p.ntiles = p.nstrips * nseg;
-    if (check) hipLaunchKernelGGL((pt_corr_f32<true>), dim3(p.ntiles), dim3(64), 0, s, p);
-    else hipLaunchKernelGGL((pt_corr_f32<false>), dim3(p.ntiles), dim3(64), 0, s, p);
+    if (p.depth > 1) {
+        if (check) hipLaunchKernelGGL((pt_corr_f32<2, true>), dim3(p.ntiles), dim3(64), 0, s, p);
+        else hipLaunchKernelGGL((pt_corr_f32<2, false>), dim3(p.ntiles), dim3(64), 0, s, p);
+    } else {
+        if (check) hipLaunchKernelGGL((pt_corr_f32<1, true>), dim3(p.ntiles), dim3(64), 0, s, p);
+        else hipLaunchKernelGGL((pt_corr_f32<1, false>), dim3(p.ntiles), dim3(64), 0, s, p);
+    }
     return hipGetLastError();
 }
 

@@ -198,7 +198,7 @@ def solveStokes(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtau
         return out.dt_new, out.t_eff, out.runtime
     check(ctx.lib.sf_solve_stokes(ctx.handle, *ptrs, float(g_y), float(dx), float(dy), nx, ny, float(dt),
                                   float(maxdisp), int(bool(use_free_surface_stabilization)), float(eps),
-                                  int(iterMax), int(bool(print_info)), C.byref(out)))
+                                  int(iterMax), int(print_info), C.byref(out)))
     if stats is not None:
         stats.update(out.as_dict())
     return out.dt_new, out.t_eff, out.runtime
