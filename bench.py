@@ -211,13 +211,14 @@ def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, b
     return res
 
 
-def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 2.0)):
+def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 1.0), eps=1e-5):
     """Timesteps of a Rayleigh-Taylor ensemble (n_inst instances at n x n nodes, 16 markers per
     cell): upper (small-y) layer rho 1.1 and viscosity 10^c, lower layer rho 1 and viscosity 1,
     interface y = 0.5 + 0.02 sin(2 pi k x), k uniform over 1..8, c uniform over ``log10c``.
-    The same steps run three ways from the same initial state: the hipGraph-captured ensemble
-    step, the same ensemble step enqueued eagerly, and sf_step per instance in a loop.  All
-    three must agree bit for bit."""
+    The same steps run several ways from the same initial state: the ensemble step in its
+    wide and persistent forms, each with hipGraphs and enqueued eagerly, and sf_step per
+    instance in a loop.  All must agree bit for bit.  eps defaults to the reference's timestep
+    loop tolerance (stokes_eps = 1e-5, StokesFlow2D_multixpu.jl:56)."""
     import numpy as np
     import torch
 
@@ -234,44 +235,45 @@ def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 2.0)):
             x, y = sfhip.init_markers(n, n, 4, dx, dy, 0.25, seed=s["seed"])
             rho, mu = sfhip.rt_properties(x, y, 0.5, 0.02, s["k"], 1.0, 1.1, 10.0 ** s["c"], 1.0, 1.0)
             out.append({"state": sfhip.FlowState(n, n), "x_m": x, "y_m": y, "rho_m": rho, "mu_m": mu, "dx": dx,
-                        "dy": dy, "g_y": 1.0, "eps": 1e-8})
+                        "dy": dy, "g_y": 1.0, "eps": eps})
         return out
 
     res = {"instances": n_inst, "n": n, "steps": steps, "markers_per_instance": 16 * (n - 1) ** 2,
-           "log10_contrast": list(log10c)}
+           "log10_contrast": list(log10c), "eps": eps}
     finals = {}
-    for mode in ("graph", "eager", "loop"):
+    runs = [("wide_graph", "wide", True), ("wide_eager", "wide", False), ("persistent_graph", "persistent", True),
+            ("persistent_eager", "persistent", False), ("loop", None, None)]
+    for name, mode, graph in runs:
         ms = members()
         torch.cuda.synchronize()
-        if mode == "loop":
+        iters = 0
+        if mode is None:
             dts = [0.0] * n_inst
             t0 = time.perf_counter()
-            iters = 0
             for _ in range(steps):
                 for i, m in enumerate(ms):
                     dts[i], st = sfhip.step(m["state"], m["x_m"], m["y_m"], m["rho_m"], m["mu_m"], dx, dy, dts[i],
-                                            g_y=1.0, eps=1e-8)
+                                            g_y=1.0, eps=eps)
                     iters += st["niter"]
             torch.cuda.synchronize()
             secs = time.perf_counter() - t0
         else:
-            ens = sfhip.Ensemble(ms, use_graph=(mode == "graph"))
+            ens = sfhip.Ensemble(ms, use_graph=graph, mode=mode)
             t0 = time.perf_counter()
-            iters = 0
             for _ in range(steps):
                 outs = ens.step(1)
                 iters += sum(o["niter"] for o in outs)
             torch.cuda.synchronize()
             secs = time.perf_counter() - t0
             ens.close()
-        res[mode] = {"seconds": secs, "steps_per_s": steps / secs, "pt_iters": iters}
-        finals[mode] = [(m["state"].Vx.clone(), m["x_m"].clone()) for m in ms]
+        res[name] = {"seconds": secs, "steps_per_s": steps / secs, "pt_iters": iters}
+        finals[name] = [(m["state"].Vx.clone(), m["x_m"].clone()) for m in ms]
         del ms
         torch.cuda.empty_cache()
-    res["graph_speedup_vs_loop"] = res["loop"]["seconds"] / res["graph"]["seconds"]
-    res["graph_speedup_vs_eager"] = res["eager"]["seconds"] / res["graph"]["seconds"]
+    for name, _, _ in runs[:-1]:
+        res[name]["speedup_vs_loop"] = res["loop"]["seconds"] / res[name]["seconds"]
     res["bitwise_equal"] = all(torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
-                               for mode in ("eager", "loop") for a, b in zip(finals["graph"], finals[mode]))
+                               for name in finals for a, b in zip(finals["loop"], finals[name]))
     return res
 
 

@@ -237,18 +237,20 @@ typedef struct {
 int sf_step(sf_ctx *ctx, sf_step_args *args, sf_solve_out *solve_out, sf_m2g_out m2g_out[3],
             sf_rk4_out *rk4_out);
 
-/* Ensemble timestep captured into a hipGraph: n independent problems (grid sizes may differ),
- * each described like sf_step (its dt field is the initial dt, 0 on a first step).  One step
- * of every member = the three marker -> grid passes per member, ONE batched solve
- * (sf_solve_stokes_batched's persistent kernel) and RK4 per member with the dt that the solve
- * produced, read on the device -- the loop body of scripts/StokesFlow2D_multixpu.jl:164-181 for
- * each member, with no host synchronisation inside the step.  With use_graph the step is
- * captured once at creation and replayed by sf_ensemble_step; without it the same kernels are
- * enqueued every step (identical results).  The ensemble keeps the members' array pointers:
- * the arrays must outlive it.  It runs on a stream of its own, ordered after the context's
- * stream at each call.  outs[i] / dt[i] (either may be NULL) receive member i's solve results
- * and dt after the last step.  No reference counterpart (the reference runs one problem per
- * process). */
+/* Ensemble timestep with hipGraphs: n independent problems (grid sizes may differ), each
+ * described like sf_step (its dt field is the initial dt, 0 on a first step).  One step of
+ * every member = the three marker -> grid passes per member, ONE batched solve of all members
+ * and RK4 per member with the dt that the solve produced, read on the device -- the loop body
+ * of scripts/StokesFlow2D_multixpu.jl:164-181 for each member.
+ * prm->mode PERSISTENT: the whole step is one hipGraph (the persistent batched kernel in the
+ * middle), replayed with no host synchronisation; best for small members.  WIDE (= AUTO):
+ * the marker -> grid passes and the RK4 passes of all members are two hipGraphs, and the
+ * wide batched solve runs between them; best when members reach 256^2 and beyond.
+ * Without use_graph the same kernels are enqueued every step (identical results).  The
+ * ensemble keeps the members' array pointers: the arrays must outlive it.  It runs on a
+ * stream of its own, ordered after the context's stream at each call.  outs[i] / dt[i]
+ * (either may be NULL) receive member i's solve results and dt after the last step.  No
+ * reference counterpart (the reference runs one problem per process). */
 typedef struct sf_ensemble sf_ensemble;
 int sf_ensemble_create(sf_ctx *ctx, const sf_step_args *members, int n, const sf_batch_params *prm, int use_graph,
                        sf_ensemble **ens);

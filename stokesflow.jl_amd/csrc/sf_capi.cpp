@@ -1269,23 +1269,26 @@ static double batch_working_set(int nx_, int ny_)
 // instance's dt refresh and convergence check at its own iteration numbers (every check reads
 // err back, as the reference does), retires converged instances and refills the set from the
 // largest-first queue.  Every instance runs exactly the iterations of solveStokes!.
-static int batch_run_wide(sf_ctx* c, const sf_batch_instance* in, int n, const std::vector<BatchInstance>& desc,
-                          const std::vector<int>& order, double budget, int depth, std::vector<BatchResult>& res)
+// Buffers a wide-mode run uses: its stream, device descriptors, reduction partials
+// (3 * RED_MAX_BLOCKS) and pinned per-instance scalars (SC_COUNT per instance).
+struct WideEnv {
+    hipStream_t s;
+    const BatchInstance* desc_dev;
+    double* part;
+    double* hscal;
+};
+
+static int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
+                          const std::vector<BatchInstance>& desc, const std::vector<int>& order, double budget,
+                          int depth, std::vector<BatchResult>& res)
 {
-    hipStream_t s = c->stream;
+    hipStream_t s = env.s;
     struct Run {
         int id;
         int64_t iter, ncheck, ndtupdate;
         double err, ws;
         bool checking;
     };
-    if (c->b_hscal_cap < n) {
-        if (c->b_hscal) HIPCHK(hipHostFree(c->b_hscal));
-        c->b_hscal = nullptr;
-        c->b_hscal_cap = 0;
-        HIPCHK(hipHostMalloc((void**)&c->b_hscal, sizeof(double) * SC_COUNT * (size_t)n, hipHostMallocDefault));
-        c->b_hscal_cap = n;
-    }
     std::vector<Run> act;
     size_t q = 0;
     double ws_active = 0.0;
@@ -1312,9 +1315,9 @@ static int batch_run_wide(sf_ctx* c, const sf_batch_instance* in, int n, const s
                                   s));
         }
         HIPCHK(launch_pad_mirror(a.nx, a.ny, a.Vx_s, a.Vy_s, a.Vx_pad, a.Vy_pad, s));
-        int rc = enqueue_dt_raw(s, a.nx, a.ny, a.Vx_s, a.Vy_s, a.dx, a.dy, a.maxdisp, 0, c->part, d.scal);
+        int rc = enqueue_dt_raw(s, a.nx, a.ny, a.Vx_s, a.Vy_s, a.dx, a.dy, a.maxdisp, 0, env.part, d.scal);
         if (rc) return rc;
-        HIPCHK(hipMemcpyAsync(c->b_hscal + (size_t)SC_COUNT * r.id, d.scal, SC_COUNT * sizeof(double),
+        HIPCHK(hipMemcpyAsync(env.hscal + (size_t)SC_COUNT * r.id, d.scal, SC_COUNT * sizeof(double),
                               hipMemcpyDeviceToHost, s));
         BatchResult& o = res[(size_t)r.id];
         o.err = r.err;
@@ -1325,7 +1328,7 @@ static int batch_run_wide(sf_ctx* c, const sf_batch_instance* in, int n, const s
     int rc = SF_OK;
     WideArgs wa;
     std::memset(&wa, 0, sizeof(wa));
-    wa.desc = c->b_inst;
+    wa.desc = env.desc_dev;
     while (!act.empty() || q < order.size()) {
         while (q < order.size() && (int)act.size() < WIDE_MAX &&
                (act.empty() || ws_active + batch_working_set(in[order[q]].nx, in[order[q]].ny) <= budget)) {
@@ -1355,14 +1358,14 @@ static int batch_run_wide(sf_ctx* c, const sf_batch_instance* in, int n, const s
             const PTParams& p = d.pt[(r.iter - 1) & 1];  // its outputs are now the current state
             const bool dt_upd = a.use_fss && (r.iter % r.ndtupdate) == 0;
             if (dt_upd) {
-                rc = enqueue_dt_raw(s, a.nx, a.ny, p.vx_b, p.vy_b, a.dx, a.dy, a.maxdisp, 1, c->part, d.scal);
+                rc = enqueue_dt_raw(s, a.nx, a.ny, p.vx_b, p.vy_b, a.dx, a.dy, a.maxdisp, 1, env.part, d.scal);
                 if (rc) return rc;
             }
             if (!r.checking) continue;
             any_check = true;
             if (a.use_fss) {
                 if (!dt_upd) {
-                    rc = enqueue_dt_raw(s, a.nx, a.ny, p.vx_b, p.vy_b, a.dx, a.dy, a.maxdisp, 0, c->part, d.scal);
+                    rc = enqueue_dt_raw(s, a.nx, a.ny, p.vx_b, p.vy_b, a.dx, a.dy, a.maxdisp, 0, env.part, d.scal);
                     if (rc) return rc;
                 }
             } else {
@@ -1393,12 +1396,12 @@ static int batch_run_wide(sf_ctx* c, const sf_batch_instance* in, int n, const s
             rp.n[1] = (int64_t)n_ry;
             rp.a[2] = p.divv;
             rp.n[2] = (int64_t)n_dv;
-            rp.part = c->part;
+            rp.part = env.part;
             rp.mode = RED_SUM;
             const int nb = red_num_blocks((int64_t)n_dv);
             HIPCHK(launch_reduce(rp, 3, nb, s));
-            HIPCHK(launch_finalize_err(c->part, nb, n_rx, n_ry, n_dv, d.scal, s));
-            HIPCHK(hipMemcpyAsync(c->b_hscal + (size_t)SC_COUNT * r.id, d.scal, SC_COUNT * sizeof(double),
+            HIPCHK(launch_finalize_err(env.part, nb, n_rx, n_ry, n_dv, d.scal, s));
+            HIPCHK(hipMemcpyAsync(env.hscal + (size_t)SC_COUNT * r.id, d.scal, SC_COUNT * sizeof(double),
                                   hipMemcpyDeviceToHost, s));
         }
         if (any_check) HIPCHK(hipStreamSynchronize(s));
@@ -1406,7 +1409,7 @@ static int batch_run_wide(sf_ctx* c, const sf_batch_instance* in, int n, const s
         std::vector<Run> keep;
         keep.reserve(act.size());
         for (Run& r : act) {
-            if (r.checking) r.err = c->b_hscal[(size_t)SC_COUNT * r.id + SC_ERR];
+            if (r.checking) r.err = env.hscal[(size_t)SC_COUNT * r.id + SC_ERR];
             r.iter += 1;
             const sf_batch_instance& a = in[r.id];
             if (r.err > a.eps && r.iter <= a.iter_max) {
@@ -1421,8 +1424,8 @@ static int batch_run_wide(sf_ctx* c, const sf_batch_instance* in, int n, const s
     }
     HIPCHK(hipStreamSynchronize(s));
     for (int i = 0; i < n; ++i) {
-        res[(size_t)i].dt_new = c->b_hscal[(size_t)SC_COUNT * i + SC_DT_CHECK];
-        res[(size_t)i].dt_last = c->b_hscal[(size_t)SC_COUNT * i + SC_DT];
+        res[(size_t)i].dt_new = env.hscal[(size_t)SC_COUNT * i + SC_DT_CHECK];
+        res[(size_t)i].dt_last = env.hscal[(size_t)SC_COUNT * i + SC_DT];
     }
     return SF_OK;
 }
@@ -1484,7 +1487,15 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
         HIPCHK(hipStreamSynchronize(s));
     } else if (mode == SF_BATCH_WIDE || mode == SF_BATCH_AUTO) {
         const double budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 192.0 * 1048576.0;
-        rc = batch_run_wide(c, in, n, host, order, budget, (prm && prm->depth > 1) ? 2 : 1, res);
+        if (c->b_hscal_cap < n) {
+            if (c->b_hscal) HIPCHK(hipHostFree(c->b_hscal));
+            c->b_hscal = nullptr;
+            c->b_hscal_cap = 0;
+            HIPCHK(hipHostMalloc((void**)&c->b_hscal, sizeof(double) * SC_COUNT * (size_t)n, hipHostMallocDefault));
+            c->b_hscal_cap = n;
+        }
+        const WideEnv env{s, c->b_inst, c->part, c->b_hscal};
+        rc = batch_run_wide(env, in, n, host, order, budget, (prm && prm->depth > 1) ? 2 : 1, res);
         if (rc) return rc;
     } else {
         return set_err(SF_ERR_ARG, "%s: unknown mode %d", who, mode);
@@ -1505,26 +1516,32 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
 }
 
 // ---------------------------------------------------------------------------------------
-// ensemble timestep (StokesFlow2D_multixpu.jl:164-181 for many independent problems),
-// captured once into a hipGraph and replayed per step
+// ensemble timestep (StokesFlow2D_multixpu.jl:164-181 for many independent problems)
 
 struct sf_ensemble {
     sf_ctx* c = nullptr;
     hipStream_t stream = nullptr;  // own non-blocking stream: capture needs one
     hipEvent_t ev = nullptr;
-    int n = 0, threads = 768, nblocks = 0, use_graph = 1;
+    int n = 0, threads = 768, nblocks = 0, use_graph = 1, wide = 1, depth = 1;
+    double budget = 0.0;
     std::vector<sf_step_args> mem;
+    std::vector<sf_batch_instance> bi;     // the members as batched-solver instances (dt updated per step)
+    std::vector<BatchInstance> desc;       // host copies of the device descriptors
+    std::vector<int> order;
+    std::vector<double> dt;                // members' dt between steps (wide form)
     double* pool = nullptr;
     BatchInstance* inst = nullptr;
-    int* order = nullptr;
+    int* order_dev = nullptr;
     int* queue = nullptr;
-    BatchResult* res = nullptr;    // res[i].dt_new is member i's dt between steps
+    BatchResult* res = nullptr;          // persistent form: res[i].dt_new is member i's dt between steps
+    double* part = nullptr;              // wide form: reduction partials
+    double* hscal = nullptr;             // wide form: pinned per-member scalars
     unsigned long long* cnt = nullptr;   // per member: 3 x 4 marker -> grid counters, 4 RK4 counters
     unsigned long long* hcnt = nullptr;  // pinned
     BatchResult* hres = nullptr;         // pinned
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    M2GScratch m2g;  // own sort scratch: the captured graph holds its addresses
+    hipGraph_t graph[2] = {nullptr, nullptr};  // persistent: [0] whole step; wide: [0] marker -> grid, [1] RK4
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    M2GScratch m2g;  // own sort scratch: the captured graphs hold its addresses
 };
 
 enum { ENS_CNT = 16 };
@@ -1532,21 +1549,23 @@ enum { ENS_CNT = 16 };
 static void free_ensemble(sf_ensemble* e)
 {
     if (!e) return;
-    if (e->exec) (void)hipGraphExecDestroy(e->exec);
-    if (e->graph) (void)hipGraphDestroy(e->graph);
-    for (void* p : {(void*)e->pool, (void*)e->inst, (void*)e->order, (void*)e->queue, (void*)e->res, (void*)e->cnt})
+    for (int k = 0; k < 2; ++k) {
+        if (e->exec[k]) (void)hipGraphExecDestroy(e->exec[k]);
+        if (e->graph[k]) (void)hipGraphDestroy(e->graph[k]);
+    }
+    for (void* p : {(void*)e->pool, (void*)e->inst, (void*)e->order_dev, (void*)e->queue, (void*)e->res,
+                    (void*)e->cnt, (void*)e->part})
         if (p) (void)hipFree(p);
-    if (e->hcnt) (void)hipHostFree(e->hcnt);
-    if (e->hres) (void)hipHostFree(e->hres);
+    for (void* p : {(void*)e->hcnt, (void*)e->hres, (void*)e->hscal})
+        if (p) (void)hipHostFree(p);
     if (e->ev) (void)hipEventDestroy(e->ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     m2g_free(e->m2g);
     delete e;
 }
 
-// Enqueue one timestep of every member on e->stream: marker -> grid (rho_vy, mu_b, mu_p) per
-// member, one batched solve, RK4 per member with the solve's new dt (read on the device).
-static hipError_t ensemble_enqueue(sf_ensemble* e)
+// The three marker -> grid passes of every member.
+static hipError_t ensemble_enqueue_m2g(sf_ensemble* e)
 {
     hipStream_t s = e->stream;
     for (int i = 0; i < e->n; ++i) {
@@ -1568,8 +1587,13 @@ static hipError_t ensemble_enqueue(sf_ensemble* e)
                     0, s);
         if (r != hipSuccess) return r;
     }
-    hipError_t r = launch_batch_solve(e->inst, e->order, e->n, e->queue, e->res, e->threads, e->nblocks, s);
-    if (r != hipSuccess) return r;
+    return hipSuccess;
+}
+
+// RK4 of every member with the dt the solve produced, read on the device.
+static hipError_t ensemble_enqueue_rk4(sf_ensemble* e)
+{
+    hipStream_t s = e->stream;
     for (int i = 0; i < e->n; ++i) {
         const sf_step_args& a = e->mem[(size_t)i];
         const double dx = a.dx, dy = a.dy;
@@ -1577,13 +1601,35 @@ static hipError_t ensemble_enqueue(sf_ensemble* e)
         const double x_vx = -1.0 * dx, y_vx = 0.0 * dy - 0.5 * dy;
         const double x_vy = 0.0 * dx - 0.5 * dx, y_vy = -1.0 * dy;
         unsigned long long* rc = e->cnt + (size_t)ENS_CNT * i + 12;
-        r = launch_zero_u64(rc, 4, s);
+        const double* dt_dev = e->wide ? e->desc[(size_t)i].scal + SC_DT_CHECK : &e->res[i].dt_new;
+        hipError_t r = launch_zero_u64(rc, 4, s);
         if (r != hipSuccess) return r;
         r = launch_rk4(a.x_m, a.y_m, a.nm, a.Vx_pad, a.nx + 2, a.ny + 1, a.Vy_pad, a.nx + 1, a.ny + 2, x_vx, y_vx, x_vy,
-                       y_vy, 0.0, lx, ly, dx, dy, rc, s, &e->res[i].dt_new);
+                       y_vy, 0.0, lx, ly, dx, dy, rc, s, dt_dev);
         if (r != hipSuccess) return r;
     }
     return hipSuccess;
+}
+
+// Persistent form: the whole step, with the persistent batched kernel in the middle.
+static hipError_t ensemble_enqueue_persistent(sf_ensemble* e)
+{
+    hipError_t r = ensemble_enqueue_m2g(e);
+    if (r != hipSuccess) return r;
+    r = launch_batch_solve(e->inst, e->order_dev, e->n, e->queue, e->res, e->threads, e->nblocks, e->stream);
+    if (r != hipSuccess) return r;
+    return ensemble_enqueue_rk4(e);
+}
+
+static hipError_t ensemble_capture(sf_ensemble* e, int k, hipError_t (*fn)(sf_ensemble*))
+{
+    hipError_t er = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+    if (er != hipSuccess) return er;
+    const hipError_t eq = fn(e);
+    er = hipStreamEndCapture(e->stream, &e->graph[k]);
+    if (eq != hipSuccess) return eq;
+    if (er != hipSuccess) return er;
+    return hipGraphInstantiate(&e->exec[k], e->graph[k], nullptr, nullptr, 0);
 }
 
 extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n, const sf_batch_params* prm,
@@ -1596,6 +1642,9 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
     int threads, rows;
     int rc = batch_threads(prm, &threads, &rows, who);
     if (rc) return rc;
+    const int mode = prm ? prm->mode : SF_BATCH_AUTO;
+    if (mode != SF_BATCH_AUTO && mode != SF_BATCH_WIDE && mode != SF_BATCH_PERSISTENT)
+        return set_err(SF_ERR_ARG, "%s: unknown mode %d", who, mode);
     std::vector<sf_batch_instance> bi((size_t)n);
     int64_t max_nm = 0, max_cells = 0;
     size_t need = 0;
@@ -1646,49 +1695,56 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
     e->n = n;
     e->threads = threads;
     e->use_graph = use_graph ? 1 : 0;
+    e->wide = (mode != SF_BATCH_PERSISTENT);
+    e->depth = (prm && prm->depth > 1) ? 2 : 1;
+    e->budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 192.0 * 1048576.0;
     e->mem.assign(members, members + n);
+    e->bi = bi;
     hipError_t er = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (er == hipSuccess) er = hipEventCreateWithFlags(&e->ev, hipEventDisableTiming);
     if (er == hipSuccess) er = dalloc(&e->pool, need);
     if (er == hipSuccess) er = dalloc(&e->inst, (size_t)n);
-    if (er == hipSuccess) er = dalloc(&e->order, (size_t)n);
+    if (er == hipSuccess) er = dalloc(&e->order_dev, (size_t)n);
     if (er == hipSuccess) er = dalloc(&e->queue, 1);
     if (er == hipSuccess) er = dalloc(&e->res, (size_t)n);
     if (er == hipSuccess) er = dalloc(&e->cnt, (size_t)ENS_CNT * n);
+    if (er == hipSuccess) er = dalloc(&e->part, 3 * (size_t)RED_MAX_BLOCKS);
     if (er == hipSuccess) er = hipHostMalloc((void**)&e->hcnt, sizeof(unsigned long long) * ENS_CNT * n, hipHostMallocDefault);
     if (er == hipSuccess) er = hipHostMalloc((void**)&e->hres, sizeof(BatchResult) * n, hipHostMallocDefault);
+    if (er == hipSuccess) er = hipHostMalloc((void**)&e->hscal, sizeof(double) * SC_COUNT * n, hipHostMallocDefault);
     if (er != hipSuccess) {
         free_ensemble(e);
         return set_err(SF_ERR_HIP, "%s: %s", who, hipGetErrorString(er));
     }
-    std::vector<BatchInstance> host((size_t)n);
+    e->desc.resize((size_t)n);
+    e->dt.resize((size_t)n);
     std::vector<std::pair<int, int>> dims((size_t)n);
     for (int i = 0; i < n; ++i) {
-        host[(size_t)i] = batch_descriptor(bi[(size_t)i], rows, e->pool + off[(size_t)i], &e->res[i].dt_new);
+        e->desc[(size_t)i] = batch_descriptor(bi[(size_t)i], rows, e->pool + off[(size_t)i], e->wide ? nullptr : &e->res[i].dt_new);
         dims[(size_t)i] = {members[i].nx, members[i].ny};
         std::memset(&e->hres[i], 0, sizeof(BatchResult));
         e->hres[i].dt_new = members[i].use_fss ? members[i].dt : 0.0;
+        e->dt[(size_t)i] = members[i].dt;
     }
-    const std::vector<int> order = batch_order(dims);
+    e->order = batch_order(dims);
     e->nblocks = batch_grid(c, n, threads, prm ? prm->max_blocks : 0);
     // make the caller's earlier work on its stream visible, then upload on our stream
     er = hipEventRecord(e->ev, c->stream);
     if (er == hipSuccess) er = hipStreamWaitEvent(e->stream, e->ev, 0);
     if (er == hipSuccess)
-        er = hipMemcpyAsync(e->inst, host.data(), sizeof(BatchInstance) * (size_t)n, hipMemcpyHostToDevice, e->stream);
+        er = hipMemcpyAsync(e->inst, e->desc.data(), sizeof(BatchInstance) * (size_t)n, hipMemcpyHostToDevice, e->stream);
     if (er == hipSuccess)
-        er = hipMemcpyAsync(e->order, order.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, e->stream);
+        er = hipMemcpyAsync(e->order_dev, e->order.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, e->stream);
     if (er == hipSuccess)
         er = hipMemcpyAsync(e->res, e->hres, sizeof(BatchResult) * (size_t)n, hipMemcpyHostToDevice, e->stream);
     if (er == hipSuccess) er = hipStreamSynchronize(e->stream);
     if (er == hipSuccess && e->use_graph) {
-        er = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
-        if (er == hipSuccess) {
-            const hipError_t eq = ensemble_enqueue(e);
-            er = hipStreamEndCapture(e->stream, &e->graph);
-            if (eq != hipSuccess) er = eq;
+        if (e->wide) {
+            er = ensemble_capture(e, 0, ensemble_enqueue_m2g);
+            if (er == hipSuccess) er = ensemble_capture(e, 1, ensemble_enqueue_rk4);
+        } else {
+            er = ensemble_capture(e, 0, ensemble_enqueue_persistent);
         }
-        if (er == hipSuccess) er = hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0);
     }
     if (er != hipSuccess) {
         free_ensemble(e);
@@ -1706,13 +1762,31 @@ extern "C" int sf_ensemble_step(sf_ensemble* e, int n_steps, sf_solve_out* outs,
     const auto t_start = std::chrono::steady_clock::now();
     HIPCHK(hipEventRecord(e->ev, e->c->stream));
     HIPCHK(hipStreamWaitEvent(e->stream, e->ev, 0));
+    std::vector<BatchResult> res;
     for (int k = 0; k < n_steps; ++k) {
+        if (!e->wide) {
+            if (e->use_graph)
+                HIPCHK(hipGraphLaunch(e->exec[0], e->stream));
+            else
+                HIPCHK(ensemble_enqueue_persistent(e));
+            continue;
+        }
         if (e->use_graph)
-            HIPCHK(hipGraphLaunch(e->exec, e->stream));
+            HIPCHK(hipGraphLaunch(e->exec[0], e->stream));
         else
-            HIPCHK(ensemble_enqueue(e));
+            HIPCHK(ensemble_enqueue_m2g(e));
+        for (int i = 0; i < e->n; ++i) e->bi[(size_t)i].dt = e->dt[(size_t)i];
+        const WideEnv env{e->stream, e->inst, e->part, e->hscal};
+        int rc = batch_run_wide(env, e->bi.data(), e->n, e->desc, e->order, e->budget, e->depth, res);
+        if (rc) return rc;
+        for (int i = 0; i < e->n; ++i) e->dt[(size_t)i] = res[(size_t)i].dt_new;
+        if (e->use_graph)
+            HIPCHK(hipGraphLaunch(e->exec[1], e->stream));
+        else
+            HIPCHK(ensemble_enqueue_rk4(e));
     }
-    HIPCHK(hipMemcpyAsync(e->hres, e->res, sizeof(BatchResult) * (size_t)e->n, hipMemcpyDeviceToHost, e->stream));
+    if (!e->wide)
+        HIPCHK(hipMemcpyAsync(e->hres, e->res, sizeof(BatchResult) * (size_t)e->n, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipMemcpyAsync(e->hcnt, e->cnt, sizeof(unsigned long long) * ENS_CNT * (size_t)e->n, hipMemcpyDeviceToHost,
                           e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
@@ -1724,6 +1798,7 @@ extern "C" int sf_ensemble_step(sf_ensemble* e, int n_steps, sf_solve_out* outs,
                 return set_err(SF_ERR_NONFINITE, "%s: member %d has markers with non-finite coordinates", who, i);
         }
     for (int i = 0; i < e->n; ++i) {
+        if (e->wide && n_steps > 0) e->hres[i] = res[(size_t)i];
         const BatchResult& r = e->hres[i];
         if (outs) {
             sf_solve_out& o = outs[i];
@@ -1735,7 +1810,7 @@ extern "C" int sf_ensemble_step(sf_ensemble* e, int n_steps, sf_solve_out* outs,
             o.dt_last = r.dt_last;
             o.runtime = secs;
         }
-        if (dt) dt[i] = r.dt_new;
+        if (dt) dt[i] = e->wide ? e->dt[(size_t)i] : r.dt_new;
     }
     return SF_OK;
 }

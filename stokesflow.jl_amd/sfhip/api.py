@@ -447,11 +447,12 @@ class Ensemble:
     mu_m, dx, dy and optionally dt (0), g_y (9.81), maxdisp (0.5), eps (1e-8), iterMax (100000),
     use_free_surface_stabilization (True).  One :meth:`step` runs, for every member, the three
     marker -> grid passes, then one batched solve of all members, then RK4 with each member's
-    new dt -- with ``use_graph`` as one replay of a hipGraph captured at construction.  The
-    members' tensors must stay alive while the ensemble exists.
+    new dt.  ``mode`` "persistent": the whole step is one hipGraph replay (small members);
+    "wide" (= "auto"): marker -> grid and RK4 are hipGraph replays around the wide batched
+    solve.  The members' tensors must stay alive while the ensemble exists.
     """
 
-    def __init__(self, members, *, use_graph=True, rows=0, threads=0, max_blocks=0):
+    def __init__(self, members, *, use_graph=True, mode="auto", rows=0, threads=0, max_blocks=0, budget_mb=0.0):
         self.lib = _lib.load()
         self.n = len(members)
         self._keep = list(members)
@@ -463,7 +464,8 @@ class Ensemble:
         ctx = context(0, 0)
         self._ctx = ctx
         self.handle = C.c_void_p()
-        prm = BatchParams(int(rows), 0, int(threads), int(max_blocks), _lib.SF_BATCH_PERSISTENT, 0, 0.0)
+        modes = {"auto": _lib.SF_BATCH_AUTO, "persistent": _lib.SF_BATCH_PERSISTENT, "wide": _lib.SF_BATCH_WIDE}
+        prm = BatchParams(int(rows), 0, int(threads), int(max_blocks), modes[mode], 0, float(budget_mb))
         check(self.lib.sf_ensemble_create(ctx.handle, arr, self.n, C.byref(prm), int(bool(use_graph)),
                                           C.byref(self.handle)))
 

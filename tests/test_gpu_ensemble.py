@@ -32,11 +32,12 @@ def same(a, b):
     assert torch.equal(a["x_m"], b["x_m"]) and torch.equal(a["y_m"], b["y_m"])
 
 
+@pytest.mark.parametrize("mode", ["persistent", "wide"])
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_ensemble_matches_single_steps(gpu, use_graph):
+def test_ensemble_matches_single_steps(gpu, use_graph, mode):
     ens_m = [member(nx, ny, 20 + k) for k, (nx, ny) in enumerate(SIZES)]
     ref_m = [member(nx, ny, 20 + k) for k, (nx, ny) in enumerate(SIZES)]
-    ens = sfhip.Ensemble(ens_m, use_graph=use_graph)
+    ens = sfhip.Ensemble(ens_m, use_graph=use_graph, mode=mode)
     dts = [0.0] * len(SIZES)
     for t in range(3):
         outs = ens.step(1)
@@ -52,11 +53,12 @@ def test_ensemble_matches_single_steps(gpu, use_graph):
     ens.close()
 
 
-def test_ensemble_multi_step_replay(gpu):
+@pytest.mark.parametrize("mode", ["persistent", "wide"])
+def test_ensemble_multi_step_replay(gpu, mode):
     """n_steps replays in one call equal n single-step calls."""
     a = [member(nx, ny, 40 + k) for k, (nx, ny) in enumerate(SIZES)]
     b = [member(nx, ny, 40 + k) for k, (nx, ny) in enumerate(SIZES)]
-    ea, eb = sfhip.Ensemble(a), sfhip.Ensemble(b)
+    ea, eb = sfhip.Ensemble(a, mode=mode), sfhip.Ensemble(b, mode=mode)
     oa = ea.step(3)
     for _ in range(3):
         ob = eb.step(1)
@@ -68,10 +70,11 @@ def test_ensemble_multi_step_replay(gpu):
     eb.close()
 
 
-def test_ensemble_nonfinite_marker(gpu):
+@pytest.mark.parametrize("mode", ["persistent", "wide"])
+def test_ensemble_nonfinite_marker(gpu, mode):
     m = member(24, 24, 3)
     m["x_m"][5] = float("nan")
-    ens = sfhip.Ensemble([m])
+    ens = sfhip.Ensemble([m], mode=mode)
     with pytest.raises(sfhip.SFError):
         ens.step(1)
     ens.close()
