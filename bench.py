@@ -54,6 +54,14 @@ def profiled_traffic(nx):
     return None
 
 
+def bytes_per_iter_fp32(nx, ny):
+    """Algorithmic bytes of one FP32 correction sweep: read+write dVx, dVy, dP and the two
+    momenta; read rho_vy, mu_b, mu_p and the three forcing arrays (FP32)."""
+    n_vx, n_vy, n_p = nx * (ny + 1), (nx + 1) * ny, (nx - 1) * (ny - 1)
+    n_dvx, n_dvy = (nx - 2) * (ny - 1), (nx - 1) * (ny - 2)
+    return 4 * (2 * (n_vx + n_vy + n_p + n_dvx + n_dvy) + n_p + nx * ny + (nx + 1) * ny + n_dvx + n_dvy + n_p)
+
+
 def cpu_baseline(nx, ny, iters):
     """The CPU oracle (C, OpenMP) timed on `iters` PT iterations of the same grid size."""
     import numpy as np
@@ -171,14 +179,17 @@ def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, b
         reset()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        outs = sfhip.solveStokes_batched(problems, mode=mode, budget_mb=budget_mb)
+        outs = sfhip.solveStokes_batched(problems, mode=mode.replace("_refined", ""), budget_mb=budget_mb,
+                                         mixed_precision=mode.endswith("_refined"))
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
-        b_alg = sum(o["niter"] * bytes_per_iter(st.nx, st.ny) for o, (st, _, _) in zip(outs, insts))
-        res[mode] = {"seconds": t, "solves_per_s": len(specs) / t, "alg_gbs": b_alg / t / 1e9}
+        bpi = bytes_per_iter_fp32 if mode.endswith("_refined") else bytes_per_iter
+        b_alg = sum(o["niter"] * bpi(st.nx, st.ny) for o, (st, _, _) in zip(outs, insts))
+        res[mode] = {"seconds": t, "solves_per_s": len(specs) / t, "alg_gbs": b_alg / t / 1e9,
+                     "converged": sum(int(o["converged"]) for o in outs), "pt_iters": sum(o["niter"] for o in outs)}
         sols[mode], iters[mode] = snapshot(), [o["niter"] for o in outs]
-        res["converged"] = sum(int(o["converged"]) for o in outs)
-        res["pt_iters_total"] = sum(o["niter"] for o in outs)
+        if mode.endswith("_refined"):
+            continue
         by_size = {}
         for s, o in zip(specs, outs):
             d = by_size.setdefault(s["n"], {"count": 0, "niter_max": 0, "niter_sum": 0, "unconverged": 0})
@@ -204,10 +215,15 @@ def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, b
         sols["loop"], iters["loop"] = snapshot(), lit
         for mode in modes:
             res[mode]["speedup_vs_loop"] = t_loop / res[mode]["seconds"]
-    names = list(sols)
+    names = [m for m in sols if not m.endswith("_refined")]
     res["bitwise_equal"] = all(
         iters[m] == iters[names[0]] and all(torch.equal(a, b) for x, y in zip(sols[m], sols[names[0]])
                                             for a, b in zip(x, y)) for m in names[1:])
+    for m in sols:
+        if m.endswith("_refined") and names:
+            res[m]["max_rel_l2_vs_fp64"] = max(
+                ((a - b).norm() / b.norm().clamp_min(1e-300)).item() for x, y in zip(sols[m], sols[names[0]])
+                for a, b in zip(x, y))
     return res
 
 
@@ -293,7 +309,8 @@ def main():
                     help="instances of the mixed-size plume ensemble (batched vs per-solve loop); 0 skips it")
     ap.add_argument("--ens-c", type=float, nargs=2, default=(0.5, 1.25), metavar=("LO", "HI"),
                     help="range of log10 viscosity contrast of the ensemble plumes")
-    ap.add_argument("--ens-modes", nargs="+", default=["wide"], help="batched modes to time (wide, persistent)")
+    ap.add_argument("--ens-modes", nargs="+", default=["wide"],
+                    help="batched modes to time (wide, persistent, wide_refined)")
     ap.add_argument("--ens-steps", type=int, nargs=3, default=None, metavar=("INSTANCES", "N", "STEPS"),
                     help="Rayleigh-Taylor ensemble timesteps: graph-captured vs eager vs per-instance loop")
     ap.add_argument("--progress", action="store_true", help="print the solvers' convergence checks (long runs)")

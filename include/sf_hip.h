@@ -154,7 +154,9 @@ int sf_solve_stokes_refined(sf_ctx *ctx, double *P, double *Vx_pad, double *Vy_p
  * WIDE: one launch per PT iteration covers every instance of an active set whose working
  * sets fit budget_mb; each instance is checked at its own ncheck iterations and retired on
  * convergence, and the set is refilled from the largest-first queue.
- * Every PT iteration is bit-identical to sf_solve_stokes.  outs[i] gets instance i's dt_new,
+ * With precision 1 (WIDE only) every instance is solved as sf_solve_stokes_refined solves it,
+ * with one launch per FP32 sweep of all active instances.
+ * In FP64 every PT iteration is bit-identical to sf_solve_stokes.  outs[i] gets instance i's dt_new,
  * err, niter, converged, dt_last (runtime = wall time of the batch).  No reference
  * counterpart (the reference solves one problem per process). */
 typedef struct {
@@ -174,7 +176,8 @@ typedef struct {
     int threads;      /* persistent: threads per instance workgroup, multiple of 64, <= 768 (<= 0: 768) */
     int max_blocks;   /* persistent: cap on concurrently resident instances (<= 0: as many as fit) */
     int mode;         /* SF_BATCH_*; AUTO = WIDE */
-    int reserved;
+    int precision;    /* wide: 0 FP64 PT iterations; 1 FP32 PT sweeps inside FP64 defect correction (the
+                         algorithm of sf_solve_stokes_refined, FP64 convergence test) */
     double budget_mb; /* wide: working-set budget of the active set in MiB (<= 0: 192, the Infinity Cache) */
 } sf_batch_params;
 

@@ -162,6 +162,17 @@ struct WideArgs {
 };
 hipError_t launch_wide_pt(const WideArgs& a, int depth, hipStream_t s);
 
+// Wide mode of the FP32 correction sweeps: slot k runs descriptor slot_desc[k] (two
+// descriptors per instance, one per ping-pong direction).
+struct WideCorrArgs {
+    const CorrParams* desc;
+    int n_active;
+    int tile_start[WIDE_MAX + 1];
+    int slot_desc[WIDE_MAX];
+};
+hipError_t launch_wide_corr(const WideCorrArgs& a, hipStream_t s);
+int corr_num_strips(int nx);
+
 hipError_t launch_batch_solve(const BatchInstance* inst, const int* order, int n_inst, int* queue,
                               BatchResult* res, int threads, int nblocks, hipStream_t s);
 int batch_blocks_per_cu(int threads);

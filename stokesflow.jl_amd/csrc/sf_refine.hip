@@ -44,7 +44,8 @@ struct CorrCols {
     unsigned vx, vy, p, dvx, dvy;
 };
 
-__device__ inline CorrRowIn corr_load_row(const CorrParams& p, const CorrCols& lc, int r, bool need_j)
+template <class PP>
+__device__ __forceinline__ CorrRowIn corr_load_row(const PP& p, const CorrCols& lc, int r, bool need_j)
 {
     const int nx = p.nx, ny = p.ny;
     CorrRowIn in;
@@ -66,8 +67,10 @@ __device__ inline CorrRowIn corr_load_row(const CorrParams& p, const CorrCols& l
     return in;
 }
 
-template <int DEPTH, bool CHECK>
-__device__ inline void corr_tile(const CorrParams& p, int strip, int segid)
+// PP is CorrParams in any address space (a kernel argument, or constant memory in the wide
+// batched kernel).
+template <int DEPTH, bool CHECK, class PP>
+__device__ __forceinline__ void corr_tile(const PP& p, int strip, int segid)
 {
     const int lane = threadIdx.x;
     const int c = strip * CORR_OUT_LANES + lane;
@@ -205,6 +208,36 @@ __global__ __launch_bounds__(64) void pt_corr_f32(CorrParams p)
 {
     const int logical = xcd_remap_r(blockIdx.x, gridDim.x);
     corr_tile<DEPTH, CHECK>(p, logical % p.nstrips, logical / p.nstrips);
+}
+
+// Wide batched mode: one launch runs one FP32 correction iteration of every instance in the
+// active set (see wide_pt_kernel in sf_batch.hip for the slot search and the tile order).
+__global__ __launch_bounds__(64) void wide_corr_kernel(WideCorrArgs a)
+{
+    const int total = a.tile_start[a.n_active];
+    const int t_glob = xcd_remap_r(blockIdx.x, gridDim.x);
+    if (t_glob >= total) return;
+    int lo = 0, hi = a.n_active - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.tile_start[mid] <= t_glob) lo = mid;
+        else hi = mid - 1;
+    }
+    const int t = t_glob - a.tile_start[lo];
+    typedef const CorrParams __attribute__((address_space(4))) CCorr;
+    CCorr& cp = ((CCorr*)a.desc)[a.slot_desc[lo]];
+    const int nstrips = cp.nstrips;
+    corr_tile<1, false>(cp, t % nstrips, t / nstrips);
+}
+
+int corr_num_strips(int nx) { return (nx + CORR_OUT_LANES - 1) / CORR_OUT_LANES; }
+
+hipError_t launch_wide_corr(const WideCorrArgs& a, hipStream_t s)
+{
+    const int total = a.tile_start[a.n_active];
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(wide_corr_kernel, dim3(total), dim3(64), 0, s, a);
+    return hipGetLastError();
 }
 
 // FP64 residual of the state u = (V, P) without a pressure update, with stabilisation dt.

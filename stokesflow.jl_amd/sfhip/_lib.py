@@ -104,7 +104,7 @@ SF_BATCH_AUTO, SF_BATCH_PERSISTENT, SF_BATCH_WIDE = 0, 1, 2
 
 
 class BatchParams(C.Structure):
-    _fields_ = [("rows", _i), ("depth", _i), ("threads", _i), ("max_blocks", _i), ("mode", _i), ("reserved", _i),
+    _fields_ = [("rows", _i), ("depth", _i), ("threads", _i), ("max_blocks", _i), ("mode", _i), ("precision", _i),
                 ("budget_mb", _d)]
 
 

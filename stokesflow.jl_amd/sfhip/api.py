@@ -207,7 +207,8 @@ def solveStokes(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtau
 _BATCH_ARRAYS = ("P", "Vx_pad", "Vy_pad", "dVxdtau", "dVydtau", "Vx_s", "Vy_s", "rho_vy", "mu_b", "mu_p")
 
 
-def solveStokes_batched(problems, *, mode="auto", rows=0, depth=0, threads=0, max_blocks=0, budget_mb=0.0):
+def solveStokes_batched(problems, *, mode="auto", rows=0, depth=0, threads=0, max_blocks=0, budget_mb=0.0,
+                        mixed_precision=False):
     """Many independent solveStokes! problems in one persistent launch (sf_solve_stokes_batched).
 
     ``problems`` is a sequence of dicts with the arrays P, Vx_pad, Vy_pad, dVxdtau, dVydtau,
@@ -220,6 +221,9 @@ def solveStokes_batched(problems, *, mode="auto", rows=0, depth=0, threads=0, ma
     ``mode`` "persistent": one launch, one workgroup per instance from a device work queue;
     "wide" (= "auto"): one launch per PT iteration over an active set of instances whose
     working sets fit ``budget_mb`` (default 192 MiB), refilled as instances converge.
+    ``mixed_precision=True`` (wide mode): every instance is solved as
+    ``solveStokes(..., mixed_precision=True)`` solves it (FP32 sweeps inside FP64 defect
+    correction), one launch per FP32 sweep of all active instances.
     """
     n = len(problems)
     arr = (BatchInstance * max(n, 1))()
@@ -238,7 +242,8 @@ def solveStokes_batched(problems, *, mode="auto", rows=0, depth=0, threads=0, ma
         b.iter_max = int(pb.get("iterMax", 100000))
         b.use_fss = int(bool(pb.get("use_free_surface_stabilization", True)))
     modes = {"auto": _lib.SF_BATCH_AUTO, "persistent": _lib.SF_BATCH_PERSISTENT, "wide": _lib.SF_BATCH_WIDE}
-    prm = BatchParams(int(rows), int(depth), int(threads), int(max_blocks), modes[mode], 0, float(budget_mb))
+    prm = BatchParams(int(rows), int(depth), int(threads), int(max_blocks), modes[mode], int(bool(mixed_precision)),
+                      float(budget_mb))
     outs = (SolveOut * max(n, 1))()
     ctx = context(0, 0)
     check(ctx.lib.sf_solve_stokes_batched(ctx.handle, arr, n, C.byref(prm), outs))

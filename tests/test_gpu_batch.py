@@ -133,3 +133,66 @@ def test_batched_errors(gpu):
         sfhip.solveStokes_batched([dict(pb, Nx=25)])  # shape check on the host
     with pytest.raises(sfhip.SFError):
         sfhip.solveStokes_batched([pb], mode="persistent", threads=100)
+
+
+def test_wide_refills_beyond_active_set_limit(gpu):
+    """More instances than one wide launch can hold (64 slots): the active set must retire and
+    refill many times, and every instance still matches its single solve."""
+    sizes = [(12 + (k % 5), 10 + (k % 7)) for k in range(70)]
+    insts = [(nx, ny, *plume_instance(nx, ny, seed=100 + k)) for k, (nx, ny) in enumerate(sizes)]
+    problems, gs = [], []
+    for nx, ny, S, dx, dy in insts:
+        pb, G = problem(S, nx, ny, dx, dy)
+        problems.append(pb)
+        gs.append(G)
+    outs = sfhip.solveStokes_batched(problems, mode="wide")
+    torch.cuda.synchronize()
+    for (nx, ny, S, dx, dy), o, G in zip(insts, outs, gs):
+        st, exp = single(S, nx, ny, dx, dy)
+        assert o["niter"] == st["niter"] and o["dt_new"] == st["dt_new"], (nx, ny)
+        for k in OUT:
+            assert np.array_equal(sfhip.to_numpy(G[k]), exp[k]), (nx, ny, k)
+
+
+def single_refined(S, nx, ny, dx, dy, eps=1e-8, iterMax=100000):
+    G = {k: sfhip.from_numpy(S[k]) for k in ARRAYS}
+    st = {}
+    sfhip.solveStokes(G["P"], G["Vx_pad"], G["Vy_pad"], G["rho_vy"], G["mu_b"], G["mu_p"], None, None, None, None,
+                      None, None, None, G["dVx"], G["dVy"], None, None, G["Vx_s"], G["Vy_s"], 1.0, dx, dy, nx, ny,
+                      0.0, 0.5, eps=eps, iterMax=iterMax, print_info=False, stats=st, mixed_precision=True)
+    torch.cuda.synchronize()
+    return st, {k: sfhip.to_numpy(G[k]) for k in OUT}
+
+
+@pytest.mark.parametrize("prm", [{}, {"budget_mb": 1e-3}, {"rows": 7}])
+def test_wide_refined_matches_single_refined(gpu, prm):
+    """FP32 sweeps in FP64 defect correction, batched: every instance runs the algorithm of the
+    single refined solver with the same kernels' arithmetic, so results are bitwise equal."""
+    insts = [(nx, ny, *plume_instance(nx, ny, seed=60 + k)) for k, (nx, ny) in enumerate(SIZES)]
+    problems, gs = [], []
+    for nx, ny, S, dx, dy in insts:
+        pb, G = problem(S, nx, ny, dx, dy)
+        problems.append(pb)
+        gs.append(G)
+    outs = sfhip.solveStokes_batched(problems, mode="wide", mixed_precision=True, **prm)
+    torch.cuda.synchronize()
+    for (nx, ny, S, dx, dy), o, G in zip(insts, outs, gs):
+        st, exp = single_refined(S, nx, ny, dx, dy)
+        assert o["converged"] and st["converged"] and o["err"] <= 1e-8, (nx, ny)
+        assert o["niter"] == st["niter"] and o["dt_new"] == st["dt_new"] and o["err"] == st["err"], (nx, ny)
+        for k in OUT:
+            assert np.array_equal(sfhip.to_numpy(G[k]), exp[k]), (nx, ny, k)
+
+
+def test_wide_refined_iteration_cap_and_errors(gpu):
+    nx, ny = 33, 47
+    S, dx, dy = plume_instance(nx, ny, seed=9)
+    pb, G = problem(S, nx, ny, dx, dy, iterMax=777)
+    (o,) = sfhip.solveStokes_batched([pb], mode="wide", mixed_precision=True)
+    torch.cuda.synchronize()
+    st, exp = single_refined(S, nx, ny, dx, dy, iterMax=777)
+    assert not o["converged"] and o["niter"] == st["niter"]
+    for k in OUT:
+        assert np.array_equal(sfhip.to_numpy(G[k]), exp[k]), k
+    with pytest.raises(sfhip.SFError):
+        sfhip.solveStokes_batched([pb], mode="persistent", mixed_precision=True)
