@@ -1979,6 +1979,15 @@ struct sf_ensemble {
     hipGraph_t graph[2] = {nullptr, nullptr};  // persistent: [0] whole step; wide: [0] marker -> grid, [1] RK4
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     M2GScratch m2g;  // own sort scratch: the captured graphs hold its addresses
+    // wide form with FP32 sweeps inside FP64 defect correction
+    int refined = 0;
+    float* pool32 = nullptr;
+    double* rpart = nullptr;
+    CorrParams* cdesc_dev = nullptr;
+    double* hpart = nullptr;
+    float* hdt = nullptr;
+    std::vector<RefineBufs> rb;
+    std::vector<CorrParams> cdesc;
 };
 
 enum { ENS_CNT = 16 };
@@ -1991,9 +2000,9 @@ static void free_ensemble(sf_ensemble* e)
         if (e->graph[k]) (void)hipGraphDestroy(e->graph[k]);
     }
     for (void* p : {(void*)e->pool, (void*)e->inst, (void*)e->order_dev, (void*)e->queue, (void*)e->res,
-                    (void*)e->cnt, (void*)e->part})
+                    (void*)e->cnt, (void*)e->part, (void*)e->pool32, (void*)e->rpart, (void*)e->cdesc_dev})
         if (p) (void)hipFree(p);
-    for (void* p : {(void*)e->hcnt, (void*)e->hres, (void*)e->hscal})
+    for (void* p : {(void*)e->hcnt, (void*)e->hres, (void*)e->hscal, (void*)e->hpart, (void*)e->hdt})
         if (p) (void)hipHostFree(p);
     if (e->ev) (void)hipEventDestroy(e->ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -2133,6 +2142,11 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
     e->threads = threads;
     e->use_graph = use_graph ? 1 : 0;
     e->wide = (mode != SF_BATCH_PERSISTENT);
+    e->refined = prm ? prm->precision : 0;
+    if (e->refined != 0 && (e->refined != 1 || !e->wide)) {
+        free_ensemble(e);
+        return set_err(SF_ERR_ARG, "%s: FP32 refinement needs the wide form (precision %d)", who, prm->precision);
+    }
     e->depth = (prm && prm->depth > 1) ? 2 : 1;
     e->budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 192.0 * 1048576.0;
     e->mem.assign(members, members + n);
@@ -2165,6 +2179,37 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
     }
     e->order = batch_order(dims);
     e->nblocks = batch_grid(c, n, threads, prm ? prm->max_blocks : 0);
+    if (e->refined) {
+        size_t nf = 0;
+        std::vector<size_t> foff((size_t)n);
+        for (int i = 0; i < n; ++i) {
+            foff[(size_t)i] = nf;
+            nf += refine_floats(members[i].nx, members[i].ny);
+        }
+        er = dalloc(&e->pool32, nf);
+        if (er == hipSuccess) er = dalloc(&e->rpart, (size_t)3 * RED_MAX_BLOCKS * n);
+        if (er == hipSuccess) er = dalloc(&e->cdesc_dev, 2 * (size_t)n);
+        if (er == hipSuccess)
+            er = hipHostMalloc((void**)&e->hpart, sizeof(double) * 3 * RED_MAX_BLOCKS * (size_t)n, hipHostMallocDefault);
+        if (er == hipSuccess) er = hipHostMalloc((void**)&e->hdt, sizeof(float) * (size_t)n, hipHostMallocDefault);
+        if (er != hipSuccess) {
+            free_ensemble(e);
+            return set_err(SF_ERR_HIP, "%s: %s", who, hipGetErrorString(er));
+        }
+        e->rb.resize((size_t)n);
+        e->cdesc.resize(2 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            e->rb[(size_t)i] = refine_carve(members[i].nx, members[i].ny, e->pool32 + foff[(size_t)i],
+                                            e->rpart + (size_t)3 * RED_MAX_BLOCKS * i);
+            refine_descriptors(bi[(size_t)i], e->rb[(size_t)i], prm ? prm->rows : 0, &e->cdesc[2 * (size_t)i]);
+        }
+        er = hipMemcpyAsync(e->cdesc_dev, e->cdesc.data(), sizeof(CorrParams) * 2 * (size_t)n, hipMemcpyHostToDevice,
+                            e->stream);
+        if (er != hipSuccess) {
+            free_ensemble(e);
+            return set_err(SF_ERR_HIP, "%s: %s", who, hipGetErrorString(er));
+        }
+    }
     // make the caller's earlier work on its stream visible, then upload on our stream
     er = hipEventRecord(e->ev, c->stream);
     if (er == hipSuccess) er = hipStreamWaitEvent(e->stream, e->ev, 0);
@@ -2214,7 +2259,13 @@ extern "C" int sf_ensemble_step(sf_ensemble* e, int n_steps, sf_solve_out* outs,
             HIPCHK(ensemble_enqueue_m2g(e));
         for (int i = 0; i < e->n; ++i) e->bi[(size_t)i].dt = e->dt[(size_t)i];
         const WideEnv env{e->stream, e->inst, e->part, e->hscal};
-        int rc = batch_run_wide(env, e->bi.data(), e->n, e->desc, e->order, e->budget, e->depth, res);
+        int rc;
+        if (e->refined) {
+            const RefineEnv renv{env, &e->rb, e->cdesc_dev, e->hpart, e->hdt, 0};
+            rc = batch_run_wide_refined(renv, e->bi.data(), e->n, e->desc, e->cdesc, e->order, e->budget, res);
+        } else {
+            rc = batch_run_wide(env, e->bi.data(), e->n, e->desc, e->order, e->budget, e->depth, res);
+        }
         if (rc) return rc;
         for (int i = 0; i < e->n; ++i) e->dt[(size_t)i] = res[(size_t)i].dt_new;
         if (e->use_graph)

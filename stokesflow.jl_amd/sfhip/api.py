@@ -454,10 +454,12 @@ class Ensemble:
     marker -> grid passes, then one batched solve of all members, then RK4 with each member's
     new dt.  ``mode`` "persistent": the whole step is one hipGraph replay (small members);
     "wide" (= "auto"): marker -> grid and RK4 are hipGraph replays around the wide batched
-    solve.  The members' tensors must stay alive while the ensemble exists.
+    solve; ``mixed_precision=True`` (wide form) solves with FP32 sweeps inside FP64 defect
+    correction.  The members' tensors must stay alive while the ensemble exists.
     """
 
-    def __init__(self, members, *, use_graph=True, mode="auto", rows=0, threads=0, max_blocks=0, budget_mb=0.0):
+    def __init__(self, members, *, use_graph=True, mode="auto", rows=0, threads=0, max_blocks=0, budget_mb=0.0,
+                 mixed_precision=False):
         self.lib = _lib.load()
         self.n = len(members)
         self._keep = list(members)
@@ -470,7 +472,8 @@ class Ensemble:
         self._ctx = ctx
         self.handle = C.c_void_p()
         modes = {"auto": _lib.SF_BATCH_AUTO, "persistent": _lib.SF_BATCH_PERSISTENT, "wide": _lib.SF_BATCH_WIDE}
-        prm = BatchParams(int(rows), 0, int(threads), int(max_blocks), modes[mode], 0, float(budget_mb))
+        prm = BatchParams(int(rows), 0, int(threads), int(max_blocks), modes[mode], int(bool(mixed_precision)),
+                          float(budget_mb))
         check(self.lib.sf_ensemble_create(ctx.handle, arr, self.n, C.byref(prm), int(bool(use_graph)),
                                           C.byref(self.handle)))
 

@@ -78,3 +78,38 @@ def test_ensemble_nonfinite_marker(gpu, mode):
     with pytest.raises(sfhip.SFError):
         ens.step(1)
     ens.close()
+
+
+def step_refined(m, dt):
+    """One StokesFlow2D step with the refined solver, from the reference-named calls."""
+    st, dx, dy = m["state"], m["dx"], m["dy"]
+    nx, ny = st.nx, st.ny
+    o = sfhip.grid_origins(dx, dy)
+    sfhip.bilinearMarkerToGrid(*o["rho"], st.rho_vy, m["x_m"], m["y_m"], m["rho_m"], dx, dy, print_info=False)
+    sfhip.bilinearMarkerToGrid(*o["mu_b"], st.mu_b, m["x_m"], m["y_m"], m["mu_m"], dx, dy, print_info=False)
+    sfhip.bilinearMarkerToGrid(*o["mu_p"], st.mu_p, m["x_m"], m["y_m"], m["mu_m"], dx, dy, print_info=False)
+    stats = {}
+    dt, _, _ = sfhip.solveStokes(st.P, st.Vx, st.Vy, st.rho_vy, st.mu_b, st.mu_p, None, None, None, None, None, None,
+                                 None, st.dVxdtau, st.dVydtau, None, None, st.Vx_small, st.Vy_small, 1.0, dx, dy, nx,
+                                 ny, dt, 0.5, eps=1e-8, print_info=False, stats=stats, mixed_precision=True)
+    sfhip.moveMarkersRK4(m["x_m"], m["y_m"], st.Vx, st.Vy, *o["vx"], *o["vy"], dt, (nx - 1) * dx, (ny - 1) * dy, dx,
+                         dy, print_info=False)
+    return dt, stats
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_ensemble_refined_matches_single_refined_steps(gpu, use_graph):
+    ens_m = [member(nx, ny, 70 + k) for k, (nx, ny) in enumerate(SIZES)]
+    ref_m = [member(nx, ny, 70 + k) for k, (nx, ny) in enumerate(SIZES)]
+    ens = sfhip.Ensemble(ens_m, use_graph=use_graph, mixed_precision=True)
+    dts = [0.0] * len(SIZES)
+    for t in range(2):
+        outs = ens.step(1)
+        torch.cuda.synchronize()
+        for k, m in enumerate(ref_m):
+            dts[k], st = step_refined(m, dts[k])
+            assert outs[k]["niter"] == st["niter"] and outs[k]["dt_new"] == dts[k], (t, k)
+        torch.cuda.synchronize()
+        for a, b in zip(ens_m, ref_m):
+            same(a, b)
+    ens.close()
