@@ -257,9 +257,12 @@ def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 1.0), eps=1e-5)
     res = {"instances": n_inst, "n": n, "steps": steps, "markers_per_instance": 16 * (n - 1) ** 2,
            "log10_contrast": list(log10c), "eps": eps}
     finals = {}
-    runs = [("wide_graph", "wide", True), ("wide_eager", "wide", False), ("persistent_graph", "persistent", True),
-            ("persistent_eager", "persistent", False), ("loop", None, None)]
-    for name, mode, graph in runs:
+    runs = [("wide_graph", "wide", True, False), ("wide_eager", "wide", False, False),
+            ("wide_refined_graph", "wide", True, True)]
+    if n <= 128:  # one CU per member: only for small members
+        runs += [("persistent_graph", "persistent", True, False), ("persistent_eager", "persistent", False, False)]
+    runs += [("loop", None, None, False)]
+    for name, mode, graph, refined in runs:
         ms = members()
         torch.cuda.synchronize()
         iters = 0
@@ -274,7 +277,7 @@ def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 1.0), eps=1e-5)
             torch.cuda.synchronize()
             secs = time.perf_counter() - t0
         else:
-            ens = sfhip.Ensemble(ms, use_graph=graph, mode=mode)
+            ens = sfhip.Ensemble(ms, use_graph=graph, mode=mode, mixed_precision=refined)
             t0 = time.perf_counter()
             for _ in range(steps):
                 outs = ens.step(1)
@@ -286,10 +289,14 @@ def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 1.0), eps=1e-5)
         finals[name] = [(m["state"].Vx.clone(), m["x_m"].clone()) for m in ms]
         del ms
         torch.cuda.empty_cache()
-    for name, _, _ in runs[:-1]:
+    for name, _, _, _ in runs[:-1]:
         res[name]["speedup_vs_loop"] = res["loop"]["seconds"] / res[name]["seconds"]
+    fp64 = [name for name, _, _, refined in runs if not refined]
     res["bitwise_equal"] = all(torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
-                               for name in finals for a, b in zip(finals["loop"], finals[name]))
+                               for name in fp64 for a, b in zip(finals["loop"], finals[name]))
+    res["wide_refined_graph"]["max_rel_l2_vs_fp64"] = max(
+        ((a[0] - b[0]).norm() / b[0].norm().clamp_min(1e-300)).item()
+        for a, b in zip(finals["wide_refined_graph"], finals["loop"]))
     return res
 
 
