@@ -321,6 +321,8 @@ def main():
     ap.add_argument("--ens-steps", type=int, nargs=3, default=None, metavar=("INSTANCES", "N", "STEPS"),
                     help="Rayleigh-Taylor ensemble timesteps: graph-captured vs eager vs per-instance loop")
     ap.add_argument("--progress", action="store_true", help="print the solvers' convergence checks (long runs)")
+    ap.add_argument("--refined-iters", type=int, default=0,
+                    help="also time this many FP32 correction sweeps (defect-correction solver) on the same state")
     ap.add_argument("--tune", action="store_true",
                     help="run the PT launch-shape autotuner first (default: the size default shape)")
     a = ap.parse_args()
@@ -409,6 +411,19 @@ def main():
                      "kernel": "pt_tile_f64", "bytes_alg_per_launch": b_iter, "launch_s": per_iter},
         "cpu_baseline": None if a.no_cpu else cpu_baseline(nx, ny, a.cpu_iters),
     }
+    if a.refined_iters > 0:
+        for t in (st.P, st.Vx, st.Vy, st.dVxdtau, st.dVydtau, st.Vx_small, st.Vy_small):
+            t.zero_()
+        rstats = {}
+        sfhip.solveStokes(st.P, st.Vx, st.Vy, st.rho_vy, st.mu_b, st.mu_p, None, None, None, None, None, None, None,
+                          st.dVxdtau, st.dVydtau, None, None, st.Vx_small, st.Vy_small, 1.0, dx, dy, nx, ny, 0.0, 0.5,
+                          eps=1e-300, print_info=False, iterMax=a.refined_iters, stats=rstats, mixed_precision=True)
+        torch.cuda.synchronize()
+        t_r = rstats["seconds"] / max(1, rstats["n_fp32_iter"])
+        line["refined_sweeps"] = {"fp32_iters": rstats["n_fp32_iter"], "windows": rstats["n_outer"],
+                                  "s_per_iter": t_r, "iters_per_s": 1.0 / t_r,
+                                  "alg_gbs_fp32_kernel_bytes": bytes_per_iter_fp32(nx, ny) / t_r / 1e9,
+                                  "speedup_per_iter_vs_fp64": (wall / a.steps) / t_r}
     if a.tts_n > 0:
         del st
         torch.cuda.empty_cache()
