@@ -316,7 +316,7 @@ def main():
                     help="instances of the mixed-size plume ensemble (batched vs per-solve loop); 0 skips it")
     ap.add_argument("--ens-c", type=float, nargs=2, default=(0.5, 1.25), metavar=("LO", "HI"),
                     help="range of log10 viscosity contrast of the ensemble plumes")
-    ap.add_argument("--ens-modes", nargs="+", default=["wide"],
+    ap.add_argument("--ens-modes", nargs="+", default=["wide", "wide_refined"],
                     help="batched modes to time (wide, persistent, wide_refined)")
     ap.add_argument("--ens-steps", type=int, nargs=3, default=None, metavar=("INSTANCES", "N", "STEPS"),
                     help="Rayleigh-Taylor ensemble timesteps: graph-captured vs eager vs per-instance loop")
