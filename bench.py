@@ -215,6 +215,21 @@ def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, b
         sols["loop"], iters["loop"] = snapshot(), lit
         for mode in modes:
             res[mode]["speedup_vs_loop"] = t_loop / res[mode]["seconds"]
+        if any(m.endswith("_refined") for m in modes):  # the same loop with the refined single solver
+            reset()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for st, dx, _ in insts:
+                sfhip.solveStokes(st.P, st.Vx, st.Vy, st.rho_vy, st.mu_b, st.mu_p, None, None, None, None, None,
+                                  None, None, st.dVxdtau, st.dVydtau, None, None, st.Vx_small, st.Vy_small, 1.0, dx,
+                                  dx, st.nx, st.ny, 0.0, 0.5, eps=eps, print_info=False, iterMax=iter_max,
+                                  mixed_precision=True)
+            torch.cuda.synchronize()
+            t_lr = time.perf_counter() - t0
+            res["per_solve_loop_refined"] = {"seconds": t_lr, "solves_per_s": len(specs) / t_lr}
+            for mode in modes:
+                if mode.endswith("_refined"):
+                    res[mode]["speedup_vs_refined_loop"] = t_lr / res[mode]["seconds"]
     names = [m for m in sols if not m.endswith("_refined")]
     res["bitwise_equal"] = all(
         iters[m] == iters[names[0]] and all(torch.equal(a, b) for x, y in zip(sols[m], sols[names[0]])

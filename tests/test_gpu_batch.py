@@ -196,3 +196,42 @@ def test_wide_refined_iteration_cap_and_errors(gpu):
         assert np.array_equal(sfhip.to_numpy(G[k]), exp[k]), k
     with pytest.raises(sfhip.SFError):
         sfhip.solveStokes_batched([pb], mode="persistent", mixed_precision=True)
+
+
+@pytest.mark.parametrize("mode", ["persistent", "wide"])
+def test_batched_minimal_grids(gpu, mode):
+    """The smallest grids the solver accepts, batched with a larger one (tile edge cases)."""
+    sizes = [(3, 3), (4, 7), (5, 3), (63, 65)]
+    insts = []
+    for k, (nx, ny) in enumerate(sizes):
+        S = cases.smooth_state(nx, ny, seed=200 + k)
+        insts.append((nx, ny, S, 1.0 / (nx - 1), 1.0 / (ny - 1)))
+    problems, gs = [], []
+    for nx, ny, S, dx, dy in insts:
+        pb, G = problem(S, nx, ny, dx, dy, iterMax=500)
+        problems.append(pb)
+        gs.append(G)
+    outs = sfhip.solveStokes_batched(problems, mode=mode)
+    torch.cuda.synchronize()
+    for (nx, ny, S, dx, dy), o, G in zip(insts, outs, gs):
+        st, exp = single(S, nx, ny, dx, dy, iterMax=500)
+        assert o["niter"] == st["niter"], (nx, ny)
+        for k in OUT:
+            assert np.array_equal(sfhip.to_numpy(G[k]), exp[k]), (nx, ny, k)
+
+
+def test_wide_refined_refills_beyond_active_set_limit(gpu):
+    sizes = [(12 + (k % 5), 10 + (k % 7)) for k in range(70)]
+    insts = [(nx, ny, *plume_instance(nx, ny, seed=400 + k)) for k, (nx, ny) in enumerate(sizes)]
+    problems, gs = [], []
+    for nx, ny, S, dx, dy in insts:
+        pb, G = problem(S, nx, ny, dx, dy)
+        problems.append(pb)
+        gs.append(G)
+    outs = sfhip.solveStokes_batched(problems, mode="wide", mixed_precision=True)
+    torch.cuda.synchronize()
+    for (nx, ny, S, dx, dy), o, G in zip(insts, outs, gs):
+        st, exp = single_refined(S, nx, ny, dx, dy)
+        assert o["niter"] == st["niter"] and o["dt_new"] == st["dt_new"], (nx, ny)
+        for k in OUT:
+            assert np.array_equal(sfhip.to_numpy(G[k]), exp[k]), (nx, ny, k)
