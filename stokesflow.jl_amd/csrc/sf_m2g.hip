@@ -7,7 +7,7 @@
 //
 //   1. key:     per marker the top-left node (ix, iy) of common.jl:22-33 with the same
 //               IEEE operations; markers outside [0,Nx]x[0,Ny] are counted ("erroneous
-//               marker position") and dropped.  cell = iy*(Nx+1) + ix.
+//               marker position") and dropped.  cell = ix*(Ny+1) + iy (column-major: see m2g_keys).
 //   2. count + exclusive scan of markers per cell (integer atomics: exact).
 //   3. scatter marker indices into their cell segment, then sort each segment by marker
 //      index, and store per slot the marker's (dxij, dyij, val).
@@ -58,7 +58,7 @@ __global__ void zero_counters(unsigned long long* c)
 
 __global__ __launch_bounds__(256) void m2g_keys(const double* __restrict__ x_m, const double* __restrict__ y_m, int64_t nm,
                                                 double xmin, double ymin, double dx, double dy, int gnx, int gny,
-                                                int* __restrict__ key, int* __restrict__ count,
+                                                int colmajor, int* __restrict__ key, int* __restrict__ count,
                                                 unsigned long long* __restrict__ counters)
 {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -77,7 +77,10 @@ __global__ __launch_bounds__(256) void m2g_keys(const double* __restrict__ x_m, 
         atomicAdd(&counters[CNT_BAD], 1ull);
         return;
     }
-    const int k = (int)(iy * (int64_t)(gnx + 1) + ix);
+    // the sort numbers cells column-major (ix slowest), the order in which the x-outer /
+    // y-inner marker set-up visits them, so the per-cell gathers below read nearly in order;
+    // the exported keys are row-major (sf_marker_cell_keys)
+    const int k = colmajor ? (int)(ix * (int64_t)(gny + 1) + iy) : (int)(iy * (int64_t)(gnx + 1) + ix);
     key[m] = k;
     atomicAdd(&count[k], 1);
 }
@@ -185,6 +188,7 @@ __global__ __launch_bounds__(256) void m2g_sort_cells(const int* __restrict__ of
     }
 }
 
+
 __device__ inline double corner_weight(int corner, double dxij, double dyij)
 {
     // MarkerToGrid.jl:127,133,141,147 -- 0 TL, 1 TR, 2 BL, 3 BR
@@ -203,13 +207,14 @@ __global__ __launch_bounds__(256) void m2g_nodes(const int* __restrict__ off, co
                                                  double* __restrict__ wt_sum, int ws_nx,
                                                  unsigned long long* __restrict__ counters)
 {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;  // y fastest: cells are numbered column-major
+    const int a = blockIdx.y;
     if (a >= gnx || b >= gny) return;
     const int A = a + 1, B = b + 1;  // 1-based node
-    const int64_t ld = gnx + 1;
+    const int64_t ld = gny + 1;
     // cells: TL (A,B), TR (A-1,B), BL (A,B-1), BR (A-1,B-1)
-    const int64_t kc[4] = {B * ld + A, B * ld + (A - 1), (int64_t)(B - 1) * ld + A, (int64_t)(B - 1) * ld + (A - 1)};
+    const int64_t kc[4] = {(int64_t)A * ld + B, (int64_t)(A - 1) * ld + B, (int64_t)A * ld + (B - 1),
+                           (int64_t)(A - 1) * ld + (B - 1)};
     int h[4], e[4], head[4];
     for (int q = 0; q < 4; ++q) {
         h[q] = off[kc[q]];
@@ -253,7 +258,7 @@ hipError_t m2g_cell_keys(M2GScratch& S, double xmin, double ymin, int gnx, int g
     hipLaunchKernelGGL(fill_int, dim3(1024), dim3(256), 0, st, counts, ncells, 0);
     if (nm > 0)
         hipLaunchKernelGGL(m2g_keys, dim3(grid1d(nm, 256)), dim3(256), 0, st, x_m, y_m, nm, xmin, ymin, dx, dy, gnx,
-                           gny, keys, counts, S.counters);
+                           gny, 0, keys, counts, S.counters);
     return hipGetLastError();
 }
 
@@ -273,7 +278,7 @@ hipError_t m2g_run(M2GScratch& S, double xmin, double ymin, double* val_grid, in
     }
     if (nm > 0)
         hipLaunchKernelGGL(m2g_keys, dim3(grid1d(nm, 256)), dim3(256), 0, st, x_m, y_m, nm, xmin, ymin, dx, dy, gnx,
-                           gny, S.key, S.count, S.counters);
+                           gny, 1, S.key, S.count, S.counters);
     hipLaunchKernelGGL(scan_block_sums, dim3(nblk), dim3(SCAN_BLOCK), 0, st, S.count, ncells, S.blocksums);
     hipLaunchKernelGGL(scan_top, dim3(1), dim3(64), 0, st, S.blocksums, nblk);
     hipLaunchKernelGGL(scan_apply, dim3(nblk), dim3(SCAN_BLOCK), 0, st, S.count, ncells, S.blocksums, S.offset);
@@ -282,7 +287,7 @@ hipError_t m2g_run(M2GScratch& S, double xmin, double ymin, double* val_grid, in
                            S.rec_idx);
     hipLaunchKernelGGL(m2g_sort_cells, dim3(grid1d(ncells, 256)), dim3(256), 0, st, S.offset, ncells, S.rec_idx, x_m,
                        y_m, val_m, xmin, ymin, dx, dy, S.rec_dx, S.rec_dy, S.rec_val);
-    hipLaunchKernelGGL(m2g_nodes, dim3(grid1d(gnx, 256), gny), dim3(256), 0, st, S.offset, S.rec_idx, S.rec_dx,
+    hipLaunchKernelGGL(m2g_nodes, dim3(grid1d(gny, 256), gnx), dim3(256), 0, st, S.offset, S.rec_idx, S.rec_dx,
                        S.rec_dy, S.rec_val, gnx, gny, val_grid, val_wt_sum, wt_sum, ws_nx, S.counters);
     return hipGetLastError();
 }
