@@ -1,0 +1,798 @@
+// sf_batch_host.cpp -- host side of the batched solver (sf_solve_stokes_batched): instance
+// descriptors and scratch, the wide-mode schedulers (FP64, and FP32 sweeps inside FP64
+// defect correction) and the persistent launch.  The kernels are in sf_batch.hip and
+// sf_refine.hip.
+#include "sf_host.h"
+
+using namespace sf;
+
+namespace sf {
+
+// Per-instance scratch (doubles) of the batched solver: ping-pong state, tau/divV, Rx/Ry, scalars.
+size_t batch_scratch_doubles(int nx_, int ny_)
+{
+    const size_t nx = nx_, ny = ny_, np = (nx - 1) * (ny - 1);
+    return nx * (ny + 1) + (nx + 1) * ny + np + 3 * np + nx * ny + (nx - 2) * (ny - 1) + (nx - 1) * (ny - 2) + SC_COUNT;
+}
+
+int batch_check(const sf_batch_instance& a, int i, const char* who)
+{
+    if (a.nx < 3 || a.ny < 3) return set_err(SF_ERR_ARG, "%s: instance %d: need Nx, Ny >= 3", who, i);
+    if ((int64_t)(a.nx + 2) * (a.ny + 2) >= (int64_t)1 << 31)
+        return set_err(SF_ERR_SIZE, "%s: instance %d: grid too large", who, i);
+    if (!a.P || !a.Vx_pad || !a.Vy_pad || !a.rho_vy || !a.mu_b || !a.mu_p || !a.dVxdtau || !a.dVydtau || !a.Vx_s ||
+        !a.Vy_s)
+        return set_err(SF_ERR_ARG, "%s: instance %d: a state or material array is NULL", who, i);
+    if (!(a.dx > 0.0) || !(a.dy > 0.0)) return set_err(SF_ERR_ARG, "%s: instance %d: dx, dy must be > 0", who, i);
+    return SF_OK;
+}
+
+// The device descriptor of one instance; scratch carved from q (batch_scratch_doubles long).
+BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, const double* dt_dev)
+{
+    BatchInstance b;
+    std::memset(&b, 0, sizeof(b));
+    const int nx = a.nx, ny = a.ny;
+    const size_t np = (size_t)(nx - 1) * (ny - 1);
+    double* vx_b = q; q += (size_t)nx * (ny + 1);
+    double* vy_b = q; q += (size_t)(nx + 1) * ny;
+    double* p_b = q; q += np;
+    double* t_xx = q; q += np;
+    double* t_yy = q; q += np;
+    double* d_v = q; q += np;
+    double* t_xy = q; q += (size_t)nx * ny;
+    double* r_x = q; q += (size_t)(nx - 2) * (ny - 1);
+    double* r_y = q; q += (size_t)(nx - 1) * (ny - 2);
+    double* scal = q;
+    const SolveConsts k = solve_consts(nx, ny, a.dx, a.dy);
+    if (rows <= 0) rows = pt_default_rows(nx, ny);
+    rows = std::min(rows, ny);
+    PTParams pp{};
+    pp.nx = nx;
+    pp.ny = ny;
+    pp.seg = rows;
+    pp.depth = 1;
+    pp.nstrips = pt_num_strips(nx);
+    pp.ntiles = pp.nstrips * ((ny + rows - 1) / rows);
+    pp.dvx = a.dVxdtau;
+    pp.dvy = a.dVydtau;
+    pp.rho = a.rho_vy;
+    pp.mu_b = a.mu_b;
+    pp.mu_p = a.mu_p;
+    pp.dt = scal + SC_DT;
+    pp.txx = t_xx;
+    pp.tyy = t_yy;
+    pp.txy = t_xy;
+    pp.divv = d_v;
+    pp.rx = r_x;
+    pp.g = a.g_y;
+    pp.dampx = k.dampx;
+    pp.dampy = k.dampy;
+    pp.rdx = k.rdx;
+    pp.rdy = k.rdy;
+    pp.vsc_mdxy2 = k.vsc_mdxy2;
+    pp.ptsc41 = k.ptsc41;
+    pp.max_nxy = k.max_nxy;
+    b.pt[0] = pp;
+    b.pt[0].vx_a = a.Vx_s;
+    b.pt[0].vy_a = a.Vy_s;
+    b.pt[0].p_a = a.P;
+    b.pt[0].vx_b = vx_b;
+    b.pt[0].vy_b = vy_b;
+    b.pt[0].p_b = p_b;
+    b.pt[1] = pp;
+    b.pt[1].vx_a = vx_b;
+    b.pt[1].vy_a = vy_b;
+    b.pt[1].p_a = p_b;
+    b.pt[1].vx_b = a.Vx_s;
+    b.pt[1].vy_b = a.Vy_s;
+    b.pt[1].p_b = a.P;
+    b.P = a.P;
+    b.Vx_s = a.Vx_s;
+    b.Vy_s = a.Vy_s;
+    b.Vx_pad = a.Vx_pad;
+    b.Vy_pad = a.Vy_pad;
+    b.ry = r_y;
+    b.scal = scal;
+    b.dt_dev = dt_dev;
+    b.nx = nx;
+    b.ny = ny;
+    b.use_fss = a.use_fss;
+    b.dx = a.dx;
+    b.dy = a.dy;
+    b.dt_in = a.dt;
+    b.maxdisp = a.maxdisp;
+    b.eps = a.eps;
+    b.iter_max = a.iter_max;
+    return b;
+}
+
+// Largest expected work first: nodes x an iteration count that grows with the grid size.
+std::vector<int> batch_order(const std::vector<std::pair<int, int>>& dims)
+{
+    const int n = (int)dims.size();
+    std::vector<int> order((size_t)n);
+    for (int i = 0; i < n; ++i) order[(size_t)i] = i;
+    auto work = [&](int i) {
+        const double a = dims[(size_t)i].first, b = dims[(size_t)i].second;
+        return a * b * std::max(a, b);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+    return order;
+}
+
+int batch_grid(sf_ctx* c, int n, int threads, int max_blocks)
+{
+    int cus = 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1) cus = 256;
+    int resident = cus * batch_blocks_per_cu(threads);
+    if (max_blocks > 0) resident = std::min(resident, max_blocks);
+    return std::min(n, resident);
+}
+
+int batch_threads(const sf_batch_params* prm, int* threads, int* rows, const char* who)
+{
+    *threads = (prm && prm->threads > 0) ? prm->threads : 768;
+    *rows = (prm && prm->rows > 0) ? prm->rows : 0;  // 0: per instance, pt_default_rows
+    if (*threads % 64 != 0 || *threads > 768) return set_err(SF_ERR_ARG, "%s: threads %d", who, *threads);
+    return SF_OK;
+}
+
+// Bytes an instance keeps live while it iterates: both ping-pong copies of Vx, Vy, P, plus
+// dVx, dVy and the three material grids.
+static double batch_working_set(int nx_, int ny_)
+{
+    const double nx = nx_, ny = ny_, np = (nx - 1) * (ny - 1);
+    return 8.0 * (2.0 * (nx * (ny + 1) + (nx + 1) * ny + np) + (nx - 2) * (ny - 1) + (nx - 1) * (ny - 2) +
+                  (nx + 1) * ny + nx * ny + np);
+}
+
+int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
+                          const std::vector<BatchInstance>& desc, const std::vector<int>& order, double budget,
+                          int depth, std::vector<BatchResult>& res)
+{
+    hipStream_t s = env.s;
+    struct Run {
+        int id;
+        int64_t iter, ncheck, ndtupdate;
+        double err, ws;
+        bool checking;
+    };
+    std::vector<Run> act;
+    size_t q = 0;
+    double ws_active = 0.0;
+    res.assign((size_t)n, BatchResult{});
+    auto start = [&](int id) -> int {
+        const sf_batch_instance& a = in[id];
+        const int mn = std::min(a.nx, a.ny);
+        Run r{id, 1, std::min<int64_t>(5 * (int64_t)mn, 2000), 10 * (int64_t)mn, 2.0 * a.eps,
+              batch_working_set(a.nx, a.ny), false};
+        HIPCHK(launch_set_scalar(desc[(size_t)id].scal + SC_DT, a.use_fss ? a.dt : 0.0, s));
+        act.push_back(r);
+        ws_active += r.ws;
+        return SF_OK;
+    };
+    auto finish = [&](const Run& r) -> int {
+        const sf_batch_instance& a = in[r.id];
+        const BatchInstance& d = desc[(size_t)r.id];
+        const int64_t niter = r.iter - 1;
+        if (niter & 1) {  // the last iterate is in the scratch buffers
+            const PTParams& p0 = d.pt[0];
+            HIPCHK(hipMemcpyAsync(a.Vx_s, p0.vx_b, sizeof(double) * (size_t)a.nx * (a.ny + 1), hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(a.Vy_s, p0.vy_b, sizeof(double) * (size_t)(a.nx + 1) * a.ny, hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(a.P, p0.p_b, sizeof(double) * (size_t)(a.nx - 1) * (a.ny - 1), hipMemcpyDeviceToDevice,
+                                  s));
+        }
+        HIPCHK(launch_pad_mirror(a.nx, a.ny, a.Vx_s, a.Vy_s, a.Vx_pad, a.Vy_pad, s));
+        int rc = enqueue_dt_raw(s, a.nx, a.ny, a.Vx_s, a.Vy_s, a.dx, a.dy, a.maxdisp, 0, env.part, d.scal);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(env.hscal + (size_t)SC_COUNT * r.id, d.scal, SC_COUNT * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+        BatchResult& o = res[(size_t)r.id];
+        o.err = r.err;
+        o.niter = niter;
+        o.converged = r.err <= a.eps;
+        return SF_OK;
+    };
+    int rc = SF_OK;
+    WideArgs wa;
+    std::memset(&wa, 0, sizeof(wa));
+    wa.desc = env.desc_dev;
+    while (!act.empty() || q < order.size()) {
+        while (q < order.size() && (int)act.size() < WIDE_MAX &&
+               (act.empty() || ws_active + batch_working_set(in[order[q]].nx, in[order[q]].ny) <= budget)) {
+            rc = start(order[q++]);
+            if (rc) return rc;
+        }
+        // one PT iteration of every active instance
+        wa.n_active = (int)act.size();
+        int tiles = 0;
+        for (size_t k = 0; k < act.size(); ++k) {
+            Run& r = act[k];
+            const bool check = (r.iter % r.ncheck) == 0;
+            const bool last = r.iter == in[r.id].iter_max;
+            const int flags = ((check || last) ? PT_WRITE_TAU : 0) | (check ? PT_WRITE_RX : 0);
+            wa.tile_start[k] = tiles;
+            wa.slot_inst[k] = r.id;
+            wa.slot_pf[k] = (int)((r.iter - 1) & 1) | (flags << 1);
+            tiles += desc[(size_t)r.id].pt[0].ntiles;
+            r.checking = check;
+        }
+        wa.tile_start[act.size()] = tiles;
+        HIPCHK(launch_wide_pt(wa, depth, s));
+        bool any_check = false;
+        for (Run& r : act) {
+            const sf_batch_instance& a = in[r.id];
+            const BatchInstance& d = desc[(size_t)r.id];
+            const PTParams& p = d.pt[(r.iter - 1) & 1];  // its outputs are now the current state
+            const bool dt_upd = a.use_fss && (r.iter % r.ndtupdate) == 0;
+            if (dt_upd) {
+                rc = enqueue_dt_raw(s, a.nx, a.ny, p.vx_b, p.vy_b, a.dx, a.dy, a.maxdisp, 1, env.part, d.scal);
+                if (rc) return rc;
+            }
+            if (!r.checking) continue;
+            any_check = true;
+            if (a.use_fss) {
+                if (!dt_upd) {
+                    rc = enqueue_dt_raw(s, a.nx, a.ny, p.vx_b, p.vy_b, a.dx, a.dy, a.maxdisp, 0, env.part, d.scal);
+                    if (rc) return rc;
+                }
+            } else {
+                HIPCHK(launch_set_scalar(d.scal + SC_DT_CHECK, 0.0, s));
+            }
+            const SolveConsts k = solve_consts(a.nx, a.ny, a.dx, a.dy);
+            ResYParams rq{};
+            rq.nx = a.nx;
+            rq.ny = a.ny;
+            rq.tyy = p.tyy;
+            rq.txy = p.txy;
+            rq.P = p.p_b;
+            rq.rho = a.rho_vy;
+            rq.vx = p.vx_b;
+            rq.vy = p.vy_b;
+            rq.dt = d.scal + SC_DT_CHECK;
+            rq.ry = d.ry;
+            rq.g = a.g_y;
+            rq.rdx = k.rdx;
+            rq.rdy = k.rdy;
+            HIPCHK(launch_pt_resid_y(rq, s));
+            const double n_rx = (double)(a.nx - 2) * (a.ny - 1), n_ry = (double)(a.nx - 1) * (a.ny - 2),
+                         n_dv = (double)(a.nx - 1) * (a.ny - 1);
+            RedParams rp{};
+            rp.a[0] = p.rx;
+            rp.n[0] = (int64_t)n_rx;
+            rp.a[1] = d.ry;
+            rp.n[1] = (int64_t)n_ry;
+            rp.a[2] = p.divv;
+            rp.n[2] = (int64_t)n_dv;
+            rp.part = env.part;
+            rp.mode = RED_SUM;
+            const int nb = red_num_blocks((int64_t)n_dv);
+            HIPCHK(launch_reduce(rp, 3, nb, s));
+            HIPCHK(launch_finalize_err(env.part, nb, n_rx, n_ry, n_dv, d.scal, s));
+            HIPCHK(hipMemcpyAsync(env.hscal + (size_t)SC_COUNT * r.id, d.scal, SC_COUNT * sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
+        }
+        if (any_check) HIPCHK(hipStreamSynchronize(s));
+        // advance, retire converged / capped instances
+        std::vector<Run> keep;
+        keep.reserve(act.size());
+        for (Run& r : act) {
+            if (r.checking) r.err = env.hscal[(size_t)SC_COUNT * r.id + SC_ERR];
+            r.iter += 1;
+            const sf_batch_instance& a = in[r.id];
+            if (r.err > a.eps && r.iter <= a.iter_max) {
+                keep.push_back(r);
+            } else {
+                rc = finish(r);
+                if (rc) return rc;
+                ws_active -= r.ws;
+            }
+        }
+        act.swap(keep);
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    for (int i = 0; i < n; ++i) {
+        res[(size_t)i].dt_new = env.hscal[(size_t)SC_COUNT * i + SC_DT_CHECK];
+        res[(size_t)i].dt_last = env.hscal[(size_t)SC_COUNT * i + SC_DT];
+    }
+    return SF_OK;
+}
+
+// ---- wide mode with FP32 sweeps inside FP64 defect correction ----
+
+size_t refine_floats(int nx_, int ny_)
+{
+    const size_t nx = nx_, ny = ny_, nvx = nx * (ny + 1), nvy = (nx + 1) * ny, np = (nx - 1) * (ny - 1);
+    const size_t nmx = (nx - 2) * (ny - 1), nmy = (nx - 1) * (ny - 2);
+    return 2 * (nvx + nvy + np) + 2 * (nmx + nmy) + nvy + nx * ny + 2 * np + nmx + nmy;
+}
+
+RefineBufs refine_carve(int nx_, int ny_, float* q, double* part)
+{
+    const size_t nx = nx_, ny = ny_, nvx = nx * (ny + 1), nvy = (nx + 1) * ny, np = (nx - 1) * (ny - 1);
+    const size_t nmx = (nx - 2) * (ny - 1), nmy = (nx - 1) * (ny - 2);
+    RefineBufs b;
+    for (int k = 0; k < 2; ++k) {
+        b.vx[k] = q; q += nvx;
+        b.vy[k] = q; q += nvy;
+        b.p[k] = q; q += np;
+    }
+    b.mx = q; q += nmx;
+    b.my = q; q += nmy;
+    b.rho = q; q += nvy;
+    b.mub = q; q += nx * ny;
+    b.mup = q; q += np;
+    b.fx = q; q += nmx;
+    b.fy = q; q += nmy;
+    b.fc = q; q += np;
+    b.part = part;
+    return b;
+}
+
+// FP32 working set of a refined instance while it iterates.
+static double refine_working_set(int nx, int ny) { return 4.0 * (double)refine_floats(nx, ny); }
+
+// The FP32 correction descriptors of one instance: [0] runs vx[0] -> vx[1], [1] back.
+void refine_descriptors(const sf_batch_instance& a, const RefineBufs& b, int rows, CorrParams out[2])
+{
+    const SolveConsts k = solve_consts(a.nx, a.ny, a.dx, a.dy);
+    CorrParams cp{};
+    cp.nx = a.nx;
+    cp.ny = a.ny;
+    cp.seg = std::min(rows > 0 ? rows : 4, a.ny);
+    cp.nstrips = corr_num_strips(a.nx);
+    cp.ntiles = cp.nstrips * ((a.ny + cp.seg - 1) / cp.seg);
+    cp.depth = 1;
+    cp.mx = b.mx;
+    cp.my = b.my;
+    cp.rho = b.rho;
+    cp.mu_b = b.mub;
+    cp.mu_p = b.mup;
+    cp.fx = b.fx;
+    cp.fy = b.fy;
+    cp.fc = b.fc;
+    cp.dt = (float)(a.use_fss ? a.dt : 0.0);
+    cp.g = (float)a.g_y;
+    cp.dampx = (float)k.dampx;
+    cp.dampy = (float)k.dampy;
+    cp.rdx = (float)k.rdx;
+    cp.rdy = (float)k.rdy;
+    cp.vsc_mdxy2 = (float)k.vsc_mdxy2;
+    cp.ptsc41 = (float)k.ptsc41;
+    cp.max_nxy = (float)k.max_nxy;
+    for (int d = 0; d < 2; ++d) {
+        out[d] = cp;
+        out[d].vx_a = b.vx[d];
+        out[d].vy_a = b.vy[d];
+        out[d].p_a = b.p[d];
+        out[d].vx_b = b.vx[1 - d];
+        out[d].vy_b = b.vy[1 - d];
+        out[d].p_b = b.p[1 - d];
+    }
+}
+
+static int batch_refine_reserve(sf_ctx* c, int n, size_t nfloats)
+{
+    if (nfloats > c->b_pool32_cap) {
+        if (c->b_pool32) HIPCHK(hipFree(c->b_pool32));
+        c->b_pool32 = nullptr;
+        c->b_pool32_cap = 0;
+        HIPCHK(dalloc(&c->b_pool32, nfloats));
+        c->b_pool32_cap = nfloats;
+    }
+    if (n > c->b_rcap) {
+        for (void* p : {(void*)c->b_part, (void*)c->b_cdesc})
+            if (p) HIPCHK(hipFree(p));
+        for (void* p : {(void*)c->b_hpart, (void*)c->b_hdt})
+            if (p) HIPCHK(hipHostFree(p));
+        c->b_part = nullptr;
+        c->b_cdesc = nullptr;
+        c->b_hpart = nullptr;
+        c->b_hdt = nullptr;
+        c->b_rcap = 0;
+        HIPCHK(dalloc(&c->b_part, (size_t)3 * RED_MAX_BLOCKS * n));
+        HIPCHK(dalloc(&c->b_cdesc, 2 * (size_t)n));
+        HIPCHK(hipHostMalloc((void**)&c->b_hpart, sizeof(double) * 3 * RED_MAX_BLOCKS * (size_t)n, hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void**)&c->b_hdt, sizeof(float) * (size_t)n, hipHostMallocDefault));
+        c->b_rcap = n;
+    }
+    return SF_OK;
+}
+
+// One FP64 PT iteration of instance `a` with the reference's check (compute_Residuals! and
+// compute_err, StokesSolver_multixpu.jl:63-75), as the single refined solver's final check.
+static int instance_check_iteration(hipStream_t s, const sf_batch_instance& a, const BatchInstance& d, double* part,
+                                    double* hscal, double* err_out)
+{
+    PTParams pp = d.pt[0];
+    pp.flags = PT_WRITE_TAU | PT_WRITE_RX;
+    pp.max_waves = -1;  // one wave per tile
+    HIPCHK(launch_pt_fused(pp, s));
+    if (a.use_fss) {
+        int rc = enqueue_dt_raw(s, a.nx, a.ny, pp.vx_b, pp.vy_b, a.dx, a.dy, a.maxdisp, 0, part, d.scal);
+        if (rc) return rc;
+    } else {
+        HIPCHK(launch_set_scalar(d.scal + SC_DT_CHECK, 0.0, s));
+    }
+    const SolveConsts k = solve_consts(a.nx, a.ny, a.dx, a.dy);
+    ResYParams rq{};
+    rq.nx = a.nx;
+    rq.ny = a.ny;
+    rq.tyy = pp.tyy;
+    rq.txy = pp.txy;
+    rq.P = pp.p_b;
+    rq.rho = a.rho_vy;
+    rq.vx = pp.vx_b;
+    rq.vy = pp.vy_b;
+    rq.dt = d.scal + SC_DT_CHECK;
+    rq.ry = d.ry;
+    rq.g = a.g_y;
+    rq.rdx = k.rdx;
+    rq.rdy = k.rdy;
+    HIPCHK(launch_pt_resid_y(rq, s));
+    const double n_rx = (double)(a.nx - 2) * (a.ny - 1), n_ry = (double)(a.nx - 1) * (a.ny - 2),
+                 n_dv = (double)(a.nx - 1) * (a.ny - 1);
+    RedParams rp{};
+    rp.a[0] = pp.rx;
+    rp.n[0] = (int64_t)n_rx;
+    rp.a[1] = d.ry;
+    rp.n[1] = (int64_t)n_ry;
+    rp.a[2] = pp.divv;
+    rp.n[2] = (int64_t)n_dv;
+    rp.part = part;
+    rp.mode = RED_SUM;
+    const int nb = red_num_blocks((int64_t)n_dv);
+    HIPCHK(launch_reduce(rp, 3, nb, s));
+    HIPCHK(launch_finalize_err(part, nb, n_rx, n_ry, n_dv, d.scal, s));
+    HIPCHK(hipMemcpyAsync(a.Vx_s, pp.vx_b, sizeof(double) * (size_t)a.nx * (a.ny + 1), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(a.Vy_s, pp.vy_b, sizeof(double) * (size_t)(a.nx + 1) * a.ny, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(a.P, pp.p_b, sizeof(double) * (size_t)(a.nx - 1) * (a.ny - 1), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(hscal, d.scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *err_out = hscal[SC_ERR];
+    return SF_OK;
+}
+
+// The refined wide mode: every instance runs the algorithm of sf_solve_stokes_refined
+// (FP32 windows of the correction iteration driven by the FP64 residual, FP64 update,
+// dt refresh and convergence test at the reference's iteration numbers, a final FP64 check
+// iteration), and one launch runs one FP32 correction iteration of every active instance.
+int batch_run_wide_refined(const RefineEnv& env, const sf_batch_instance* in, int n,
+                                  const std::vector<BatchInstance>& desc, std::vector<CorrParams>& cdesc,
+                                  const std::vector<int>& order, double budget, std::vector<BatchResult>& res)
+{
+    hipStream_t s = env.w.s;
+    const std::vector<RefineBufs>& rb = *env.rb;
+    struct Run {
+        int id, cur, rebases;
+        int64_t n32, n64, ncheck, ndtupdate, checked_at, window, left, wstart;
+        double err_k, err_ref, dt_k, ws;
+        bool converged;
+    };
+    res.assign((size_t)n, BatchResult{});
+    auto resid_err = [&](Run& r, double* err) -> int {
+        const sf_batch_instance& a = in[r.id];
+        const RefineBufs& b = rb[(size_t)r.id];
+        const SolveConsts k = solve_consts(a.nx, a.ny, a.dx, a.dy);
+        const int64_t np = (int64_t)(a.nx - 1) * (a.ny - 1);
+        const int nbr = red_num_blocks(np);
+        ResidParams rp{};
+        rp.nx = a.nx;
+        rp.ny = a.ny;
+        rp.vx = a.Vx_s;
+        rp.vy = a.Vy_s;
+        rp.P = a.P;
+        rp.rho = a.rho_vy;
+        rp.mu_b = a.mu_b;
+        rp.mu_p = a.mu_p;
+        rp.fx = b.fx;
+        rp.fy = b.fy;
+        rp.fc = b.fc;
+        rp.part = b.part;
+        rp.dt = r.dt_k;
+        rp.g = a.g_y;
+        rp.rdx = k.rdx;
+        rp.rdy = k.rdy;
+        HIPCHK(launch_resid_f64(rp, nbr, s));
+        double* h = env.hpart + (size_t)3 * RED_MAX_BLOCKS * r.id;
+        HIPCHK(hipMemcpyAsync(h, b.part, 3 * (size_t)nbr * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        double sx = 0.0, sy = 0.0, sc = 0.0;
+        for (int q = 0; q < nbr; ++q) {
+            sx += h[q];
+            sy += h[nbr + q];
+            sc += h[2 * nbr + q];
+        }
+        const double n_rx = (double)(a.nx - 2) * (a.ny - 1), n_ry = (double)(a.nx - 1) * (a.ny - 2);
+        *err = std::max(std::max(sx / n_rx, sy / n_ry), sc / (double)np);
+        return SF_OK;
+    };
+    // Runs checks / starts the next window; sets done when the instance has finished.
+    auto begin_window = [&](Run& r, bool* done) -> int {
+        const sf_batch_instance& a = in[r.id];
+        const BatchInstance& d = desc[(size_t)r.id];
+        const RefineBufs& b = rb[(size_t)r.id];
+        const int64_t nmx = (int64_t)(a.nx - 2) * (a.ny - 1), nmy = (int64_t)(a.nx - 1) * (a.ny - 2);
+        *done = false;
+        for (;;) {
+            if (r.n32 >= a.iter_max) {
+                *done = true;
+                return SF_OK;
+            }
+            int rc = resid_err(r, &r.err_k);
+            if (rc) return rc;
+            if (!std::isfinite(r.err_k)) {
+                *done = true;
+                return SF_OK;
+            }
+            if (r.n32 > 0 && r.n32 % r.ncheck == 0 && r.n32 != r.checked_at && r.err_k <= a.eps) {
+                r.checked_at = r.n32;
+                HIPCHK(launch_cast_f64(a.dVxdtau, b.mx, nmx, s));
+                HIPCHK(launch_cast_f64(a.dVydtau, b.my, nmy, s));
+                HIPCHK(launch_set_scalar(d.scal + SC_DT, r.dt_k, s));
+                rc = instance_check_iteration(s, a, d, b.part, env.w.hscal + (size_t)SC_COUNT * r.id, &r.err_ref);
+                if (rc) return rc;
+                r.n64 += 1;
+                HIPCHK(launch_cast_f32(b.mx, a.dVxdtau, nmx, s));
+                HIPCHK(launch_cast_f32(b.my, a.dVydtau, nmy, s));
+                if (r.err_ref <= a.eps) {
+                    r.converged = true;
+                    *done = true;
+                    return SF_OK;
+                }
+                continue;
+            }
+            const int64_t nvx = (int64_t)a.nx * (a.ny + 1), nvy = (int64_t)(a.nx + 1) * a.ny,
+                          np = (int64_t)(a.nx - 1) * (a.ny - 1);
+            for (int q = 0; q < 2; ++q) {
+                HIPCHK(launch_fill_f32(b.vx[q], nvx, 0.f, s));
+                HIPCHK(launch_fill_f32(b.vy[q], nvy, 0.f, s));
+                HIPCHK(launch_fill_f32(b.p[q], np, 0.f, s));
+            }
+            const float dtf = (float)r.dt_k;
+            if (cdesc[2 * (size_t)r.id].dt != dtf) {
+                cdesc[2 * (size_t)r.id].dt = dtf;
+                cdesc[2 * (size_t)r.id + 1].dt = dtf;
+                env.hdt[r.id] = dtf;
+                for (int q = 0; q < 2; ++q)
+                    HIPCHK(hipMemcpyAsync((char*)(env.cdesc_dev + 2 * (size_t)r.id + q) + offsetof(CorrParams, dt),
+                                          env.hdt + r.id, sizeof(float), hipMemcpyHostToDevice, s));
+            }
+            int64_t steps = std::min(r.window, a.iter_max - r.n32);
+            steps = std::min(steps, r.ncheck - r.n32 % r.ncheck);
+            if (a.use_fss) steps = std::min(steps, r.ndtupdate - r.n32 % r.ndtupdate);
+            r.left = steps;
+            r.wstart = r.n32;
+            r.cur = 0;
+            return SF_OK;
+        }
+    };
+    auto end_window = [&](Run& r) -> int {
+        const sf_batch_instance& a = in[r.id];
+        const BatchInstance& d = desc[(size_t)r.id];
+        const RefineBufs& b = rb[(size_t)r.id];
+        HIPCHK(launch_update_f64(a.Vx_s, b.vx[r.cur], (int64_t)a.nx * (a.ny + 1), s));
+        HIPCHK(launch_update_f64(a.Vy_s, b.vy[r.cur], (int64_t)(a.nx + 1) * a.ny, s));
+        HIPCHK(launch_update_f64(a.P, b.p[r.cur], (int64_t)(a.nx - 1) * (a.ny - 1), s));
+        r.rebases += 1;
+        // the reference refreshes the stabilisation dt every ndtupdate iterations (:60-62)
+        if (a.use_fss && (r.n32 / r.ndtupdate) > (r.wstart / r.ndtupdate)) {
+            int rc = enqueue_dt_raw(s, a.nx, a.ny, a.Vx_s, a.Vy_s, a.dx, a.dy, a.maxdisp, 0, b.part, d.scal);
+            if (rc) return rc;
+            double* hs = env.w.hscal + (size_t)SC_COUNT * r.id;
+            HIPCHK(hipMemcpyAsync(hs, d.scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            r.dt_k = hs[SC_DT_CHECK];
+        }
+        return SF_OK;
+    };
+    auto finish = [&](const Run& r) -> int {
+        const sf_batch_instance& a = in[r.id];
+        const BatchInstance& d = desc[(size_t)r.id];
+        const RefineBufs& b = rb[(size_t)r.id];
+        if (!r.converged) {
+            HIPCHK(launch_cast_f64(a.dVxdtau, b.mx, (int64_t)(a.nx - 2) * (a.ny - 1), s));
+            HIPCHK(launch_cast_f64(a.dVydtau, b.my, (int64_t)(a.nx - 1) * (a.ny - 2), s));
+        }
+        HIPCHK(launch_pad_mirror(a.nx, a.ny, a.Vx_s, a.Vy_s, a.Vx_pad, a.Vy_pad, s));
+        int rc = enqueue_dt_raw(s, a.nx, a.ny, a.Vx_s, a.Vy_s, a.dx, a.dy, a.maxdisp, 0, b.part, d.scal);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(env.w.hscal + (size_t)SC_COUNT * r.id, d.scal, SC_COUNT * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+        BatchResult& o = res[(size_t)r.id];
+        o.err = (!r.converged && std::isfinite(r.err_k)) ? r.err_k : r.err_ref;
+        o.niter = r.n32 + r.n64;
+        o.converged = r.converged;
+        o.dt_last = r.dt_k;
+        return SF_OK;
+    };
+    std::vector<Run> act;
+    size_t q = 0;
+    double ws_active = 0.0;
+    WideCorrArgs wa;
+    std::memset(&wa, 0, sizeof(wa));
+    wa.desc = env.cdesc_dev;
+    int rc = SF_OK;
+    while (!act.empty() || q < order.size()) {
+        while (q < order.size() && (int)act.size() < WIDE_MAX &&
+               (act.empty() || ws_active + refine_working_set(in[order[q]].nx, in[order[q]].ny) <= budget)) {
+            const int id = order[q++];
+            const sf_batch_instance& a = in[id];
+            const RefineBufs& b = rb[(size_t)id];
+            const int mn = std::min(a.nx, a.ny);
+            Run r{};
+            r.id = id;
+            r.ncheck = std::min<int64_t>(5 * (int64_t)mn, 2000);
+            r.ndtupdate = 10 * (int64_t)mn;
+            r.window = env.rebase > 0 ? env.rebase : r.ncheck;
+            r.checked_at = -1;
+            r.err_ref = 2.0 * a.eps;
+            r.dt_k = a.use_fss ? a.dt : 0.0;
+            r.ws = refine_working_set(a.nx, a.ny);
+            HIPCHK(launch_cast_f32(b.rho, a.rho_vy, (int64_t)(a.nx + 1) * a.ny, s));
+            HIPCHK(launch_cast_f32(b.mub, a.mu_b, (int64_t)a.nx * a.ny, s));
+            HIPCHK(launch_cast_f32(b.mup, a.mu_p, (int64_t)(a.nx - 1) * (a.ny - 1), s));
+            HIPCHK(launch_cast_f32(b.mx, a.dVxdtau, (int64_t)(a.nx - 2) * (a.ny - 1), s));
+            HIPCHK(launch_cast_f32(b.my, a.dVydtau, (int64_t)(a.nx - 1) * (a.ny - 2), s));
+            bool done = false;
+            rc = begin_window(r, &done);
+            if (rc) return rc;
+            if (done) {
+                rc = finish(r);
+                if (rc) return rc;
+            } else {
+                act.push_back(r);
+                ws_active += r.ws;
+            }
+        }
+        if (act.empty()) continue;
+        wa.n_active = (int)act.size();
+        int tiles = 0;
+        for (size_t k = 0; k < act.size(); ++k) {
+            wa.tile_start[k] = tiles;
+            wa.slot_desc[k] = 2 * act[k].id + act[k].cur;
+            tiles += cdesc[2 * (size_t)act[k].id].ntiles;
+        }
+        wa.tile_start[act.size()] = tiles;
+        HIPCHK(launch_wide_corr(wa, s));
+        std::vector<Run> keep;
+        keep.reserve(act.size());
+        for (Run& r : act) {
+            r.cur ^= 1;
+            r.n32 += 1;
+            r.left -= 1;
+            bool done = false;
+            if (r.left == 0) {
+                rc = end_window(r);
+                if (rc) return rc;
+                rc = begin_window(r, &done);
+                if (rc) return rc;
+            }
+            if (done) {
+                rc = finish(r);
+                if (rc) return rc;
+                ws_active -= r.ws;
+            } else {
+                keep.push_back(r);
+            }
+        }
+        act.swap(keep);
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    for (int i = 0; i < n; ++i) res[(size_t)i].dt_new = env.w.hscal[(size_t)SC_COUNT * i + SC_DT_CHECK];
+    return SF_OK;
+}
+
+}  // namespace sf
+
+extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, int n, const sf_batch_params* prm,
+                                       sf_solve_out* outs)
+{
+    const char* who = "sf_solve_stokes_batched";
+    if (!c || (n > 0 && (!in || !outs))) return set_err(SF_ERR_ARG, "%s: NULL argument", who);
+    if (n < 0) return set_err(SF_ERR_ARG, "%s: n < 0", who);
+    if (n == 0) return SF_OK;
+    const auto t_start = std::chrono::steady_clock::now();
+    hipStream_t s = c->stream;
+    int threads, rows;
+    int rc = batch_threads(prm, &threads, &rows, who);
+    if (rc) return rc;
+    std::vector<size_t> off((size_t)n);
+    size_t need = 0;
+    for (int i = 0; i < n; ++i) {
+        rc = batch_check(in[i], i, who);
+        if (rc) return rc;
+        off[(size_t)i] = need;
+        need += batch_scratch_doubles(in[i].nx, in[i].ny);
+    }
+    if (need > c->b_pool_cap) {
+        if (c->b_pool) HIPCHK(hipFree(c->b_pool));
+        c->b_pool = nullptr;
+        c->b_pool_cap = 0;
+        HIPCHK(dalloc(&c->b_pool, need));
+        c->b_pool_cap = need;
+    }
+    if (n > c->b_cap) {
+        for (void* p : {(void*)c->b_inst, (void*)c->b_order, (void*)c->b_queue, (void*)c->b_res})
+            if (p) HIPCHK(hipFree(p));
+        c->b_inst = nullptr;
+        c->b_order = nullptr;
+        c->b_queue = nullptr;
+        c->b_res = nullptr;
+        c->b_cap = 0;
+        HIPCHK(dalloc(&c->b_inst, (size_t)n));
+        HIPCHK(dalloc(&c->b_order, (size_t)n));
+        HIPCHK(dalloc(&c->b_queue, 1));
+        HIPCHK(dalloc(&c->b_res, (size_t)n));
+        c->b_cap = n;
+    }
+    std::vector<BatchInstance> host((size_t)n);
+    for (int i = 0; i < n; ++i) host[(size_t)i] = batch_descriptor(in[i], rows, c->b_pool + off[(size_t)i], nullptr);
+    std::vector<std::pair<int, int>> dims((size_t)n);
+    for (int i = 0; i < n; ++i) dims[(size_t)i] = {in[i].nx, in[i].ny};
+    const std::vector<int> order = batch_order(dims);
+    HIPCHK(hipMemcpyAsync(c->b_inst, host.data(), sizeof(BatchInstance) * (size_t)n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->b_order, order.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, s));
+    std::vector<BatchResult> res((size_t)n);
+    if (c->b_hscal_cap < n) {
+        if (c->b_hscal) HIPCHK(hipHostFree(c->b_hscal));
+        c->b_hscal = nullptr;
+        c->b_hscal_cap = 0;
+        HIPCHK(hipHostMalloc((void**)&c->b_hscal, sizeof(double) * SC_COUNT * (size_t)n, hipHostMallocDefault));
+        c->b_hscal_cap = n;
+    }
+    const int mode = prm ? prm->mode : SF_BATCH_AUTO;
+    const int precision = prm ? prm->precision : 0;
+    if (precision != 0 && precision != 1) return set_err(SF_ERR_ARG, "%s: precision %d not 0 or 1", who, precision);
+    if (precision == 1 && mode == SF_BATCH_PERSISTENT)
+        return set_err(SF_ERR_ARG, "%s: FP32 refinement runs in the wide mode only", who);
+    if (precision == 1) {
+        const double budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 192.0 * 1048576.0;
+        size_t nf = 0;
+        std::vector<size_t> foff((size_t)n);
+        for (int i = 0; i < n; ++i) {
+            foff[(size_t)i] = nf;
+            nf += refine_floats(in[i].nx, in[i].ny);
+        }
+        rc = batch_refine_reserve(c, n, nf);
+        if (rc) return rc;
+        std::vector<RefineBufs> rb((size_t)n);
+        std::vector<CorrParams> cdesc(2 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            rb[(size_t)i] = refine_carve(in[i].nx, in[i].ny, c->b_pool32 + foff[(size_t)i],
+                                         c->b_part + (size_t)3 * RED_MAX_BLOCKS * i);
+            refine_descriptors(in[i], rb[(size_t)i], prm ? prm->rows : 0, &cdesc[2 * (size_t)i]);
+        }
+        HIPCHK(hipMemcpyAsync(c->b_cdesc, cdesc.data(), sizeof(CorrParams) * 2 * (size_t)n, hipMemcpyHostToDevice, s));
+        const RefineEnv env{{s, c->b_inst, c->part, c->b_hscal}, &rb, c->b_cdesc, c->b_hpart, c->b_hdt, 0};
+        rc = batch_run_wide_refined(env, in, n, host, cdesc, order, budget, res);
+        if (rc) return rc;
+    } else if (mode == SF_BATCH_PERSISTENT) {
+        const int nblocks = batch_grid(c, n, threads, prm ? prm->max_blocks : 0);
+        HIPCHK(launch_batch_solve(c->b_inst, c->b_order, n, c->b_queue, c->b_res, threads, nblocks, s));
+        HIPCHK(hipMemcpyAsync(res.data(), c->b_res, sizeof(BatchResult) * (size_t)n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    } else if (mode == SF_BATCH_WIDE || mode == SF_BATCH_AUTO) {
+        const double budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 192.0 * 1048576.0;
+        const WideEnv env{s, c->b_inst, c->part, c->b_hscal};
+        rc = batch_run_wide(env, in, n, host, order, budget, (prm && prm->depth > 1) ? 2 : 1, res);
+        if (rc) return rc;
+    } else {
+        return set_err(SF_ERR_ARG, "%s: unknown mode %d", who, mode);
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    for (int i = 0; i < n; ++i) {
+        const BatchResult& r = res[(size_t)i];
+        sf_solve_out& o = outs[i];
+        std::memset(&o, 0, sizeof(o));
+        o.dt_new = r.dt_new;
+        o.err = r.err;
+        o.niter = r.niter;
+        o.converged = r.converged;
+        o.dt_last = r.dt_last;
+        o.runtime = secs;  // the whole batch
+    }
+    return SF_OK;
+}
+

@@ -1,0 +1,139 @@
+// sf_host.h -- private declarations shared by the host files of libsfhip (sf_capi.cpp,
+// sf_batch_host.cpp, sf_ensemble.cpp).  Not part of the C ABI.
+#pragma once
+
+#include "../../include/sf_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <utility>
+#include <vector>
+
+#include "sf_internal.h"
+
+namespace sf {
+
+// Records the message for sf_last_error() and returns code.
+int set_err(int code, const char* fmt, ...);
+
+template <typename T>
+inline hipError_t dalloc(T** p, size_t n)
+{
+    return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+struct SolveConsts {
+    double vsc_mdxy2, ptsc41, max_nxy, dampx, dampy, rdx, rdy;
+};
+SolveConsts solve_consts(int nx, int ny, double dx, double dy);
+
+// compute_dt (StokesSolver_multixpu.jl:118-121) into scal[SC_DT_CHECK] (and scal[SC_DT] if update).
+int enqueue_dt_raw(hipStream_t s, int nx, int ny, const double* vx, const double* vy, double dx, double dy,
+                   double maxdisp, int update, double* part, double* scal);
+
+void m2g_free(M2GScratch& S);
+int m2g_reserve(M2GScratch& S, int64_t nm, int64_t ncells);
+
+// ---- batched solver (sf_batch_host.cpp), also used by the ensembles ----
+
+// Wide mode: the host keeps an active set of instances whose working sets fit a byte budget
+// (the Infinity Cache), launches one PT iteration of all of them per kernel launch, runs each
+// instance's dt refresh and convergence check at its own iteration numbers (every check reads
+// err back, as the reference does), retires converged instances and refills the set from the
+// largest-first queue.  Every instance runs exactly the iterations of solveStokes!.
+// Buffers a wide-mode run uses: its stream, device descriptors, reduction partials
+// (3 * RED_MAX_BLOCKS) and pinned per-instance scalars (SC_COUNT per instance).
+struct WideEnv {
+    hipStream_t s;
+    const BatchInstance* desc_dev;
+    double* part;
+    double* hscal;
+};
+
+// Per-instance FP32 buffers of the refined wide mode (the layout of sf_solve_stokes_refined's).
+struct RefineBufs {
+    float *vx[2], *vy[2], *p[2], *mx, *my, *rho, *mub, *mup, *fx, *fy, *fc;
+    double* part;  // 3 * RED_MAX_BLOCKS residual partial sums
+};
+
+struct RefineEnv {
+    WideEnv w;
+    const std::vector<RefineBufs>* rb;
+    CorrParams* cdesc_dev;   // 2 per instance
+    double* hpart;           // pinned, 3 * RED_MAX_BLOCKS per instance
+    float* hdt;              // pinned, 1 per instance
+    int64_t rebase;          // FP32 window length (<= 0: ncheck)
+};
+
+size_t batch_scratch_doubles(int nx, int ny);
+int batch_check(const sf_batch_instance& a, int i, const char* who);
+BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, const double* dt_dev);
+std::vector<int> batch_order(const std::vector<std::pair<int, int>>& dims);
+int batch_grid(sf_ctx* c, int n, int threads, int max_blocks);
+int batch_threads(const sf_batch_params* prm, int* threads, int* rows, const char* who);
+int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n, const std::vector<BatchInstance>& desc,
+                   const std::vector<int>& order, double budget, int depth, std::vector<BatchResult>& res);
+size_t refine_floats(int nx, int ny);
+RefineBufs refine_carve(int nx, int ny, float* q, double* part);
+void refine_descriptors(const sf_batch_instance& a, const RefineBufs& b, int rows, CorrParams out[2]);
+int batch_run_wide_refined(const RefineEnv& env, const sf_batch_instance* in, int n,
+                           const std::vector<BatchInstance>& desc, std::vector<CorrParams>& cdesc,
+                           const std::vector<int>& order, double budget, std::vector<BatchResult>& res);
+
+}  // namespace sf
+
+#define HIPCHK(expr)                                                                                      \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) return sf::set_err(SF_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                                 __FILE__, __LINE__);                                     \
+    } while (0)
+
+// The context: device scratch for one grid size and the stream work is enqueued on.
+struct sf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nx = 0, ny = 0;
+    // PT ping-pong state and check-iteration scratch
+    double *vx_b = nullptr, *vy_b = nullptr, *p_b = nullptr;
+    double *txx = nullptr, *tyy = nullptr, *txy = nullptr, *divv = nullptr, *rx = nullptr, *ry = nullptr;
+    double *part = nullptr;   // 3 * RED_MAX_BLOCKS
+    double *scal = nullptr;   // SC_COUNT device scalars
+    double *hscal = nullptr;  // pinned host mirror
+    unsigned long long *counters = nullptr;   // device counters (RK4)
+    unsigned long long *hcounters = nullptr;  // pinned host mirror
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    sf::M2GScratch m2g;
+    // FP32 correction scratch (allocated on first refined solve)
+    float *r_vx[2] = {nullptr, nullptr}, *r_vy[2] = {nullptr, nullptr}, *r_p[2] = {nullptr, nullptr};
+    float *r_mx = nullptr, *r_my = nullptr, *r_rho = nullptr, *r_mub = nullptr, *r_mup = nullptr;
+    float *r_fx = nullptr, *r_fy = nullptr, *r_fc = nullptr, *r_rxo = nullptr, *r_ryo = nullptr, *r_rco = nullptr;
+    double* hpart = nullptr;  // pinned host copy of reduction partials
+    // batched ensemble scratch
+    double* b_pool = nullptr;
+    size_t b_pool_cap = 0;
+    sf::BatchInstance* b_inst = nullptr;
+    int* b_order = nullptr;
+    int* b_queue = nullptr;
+    sf::BatchResult* b_res = nullptr;
+    int b_cap = 0;
+    double* b_hscal = nullptr;  // pinned per-instance scalars (wide mode)
+    int b_hscal_cap = 0;
+    // refined wide mode
+    float* b_pool32 = nullptr;
+    size_t b_pool32_cap = 0;
+    double* b_part = nullptr;      // 3 * RED_MAX_BLOCKS per instance
+    sf::CorrParams* b_cdesc = nullptr;  // 2 per instance
+    double* b_hpart = nullptr;     // pinned
+    float* b_hdt = nullptr;        // pinned
+    int b_rcap = 0;
+    // PT launch tuning (results do not depend on it)
+    int seg = 0;        // rows per tile; 0 = size heuristic
+    int depth = 0;      // rows of operands prefetched; 0 = size default
+    int max_waves = -1; // 0 persistent (resident slots), > 0 cap, < 0 one wave per tile
+};
