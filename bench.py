@@ -397,6 +397,7 @@ def main():
     gpu_s = ev0.elapsed_time(ev1) * 1e-3
     finite = bool(torch.isfinite(st.Vx_small).all().item())
 
+    probe = sfhip.probe_dram(1 << 30)  # the box's DRAM ceiling for simple streams
     per_iter = gpu_s / a.steps  # HIP-event time per launch of the fused kernel (plus launch gaps)
     b_iter = bytes_per_iter(nx, ny)
     achieved = b_iter / per_iter / 1e9
@@ -423,7 +424,8 @@ def main():
         "pt_launch_shape": tuned or "size default (4-row tiles above 1024^2, one row of operands in flight)",
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "pt_tile_f64", "bytes_alg_per_launch": b_iter, "launch_s": per_iter},
+                     "kernel": "pt_tile_f64", "bytes_alg_per_launch": b_iter, "launch_s": per_iter,
+                     "dram_probe_gbs": probe, "frac_of_probe_copy": achieved / probe["copy_gbs"]},
         "cpu_baseline": None if a.no_cpu else cpu_baseline(nx, ny, a.cpu_iters),
     }
     if a.refined_iters > 0:

@@ -96,6 +96,11 @@ int sf_ctx_autotune_pt(sf_ctx *ctx, int iters, const double *P, const double *Vx
                        const double *mu_p, double g_y, double dx, double dy, sf_pt_tuning *best, sf_pt_tuning *table,
                        int table_cap, int *n_table);
 
+/* DRAM probe: best-of-8 bandwidth of a copy (1 read + 1 write) and of a 3-read + 1-write
+ * kernel over four `bytes`-sized scratch arrays (allocated and freed here), in GB/s.  The
+ * ceiling the stencil kernels' bandwidth is read against.  No reference counterpart. */
+int sf_probe_dram(sf_ctx *ctx, int64_t bytes, double *copy_gbs, double *r3w1_gbs);
+
 /* solveStokes! -- replaces scripts/StokesSolver_multixpu.jl:14-96
  * (solveStokes!(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtauPt, Rx, Ry,
  *  dVxdtau, dVydtau, dtauVx, dtauVy, Vx_s, Vy_s, g_y, dx, dy, Nx, Ny, dt, maxdisp, comm;

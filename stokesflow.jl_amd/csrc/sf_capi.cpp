@@ -532,6 +532,46 @@ extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, const double* P_in, cons
 }
 
 // ---------------------------------------------------------------------------------------
+// DRAM probe
+
+extern "C" int sf_probe_dram(sf_ctx* c, int64_t bytes, double* copy_gbs, double* r3w1_gbs)
+{
+    if (!c) return set_err(SF_ERR_ARG, "sf_probe_dram: NULL ctx");
+    if (bytes < ((int64_t)1 << 24) || bytes % 64 != 0)
+        return set_err(SF_ERR_ARG, "sf_probe_dram: bytes must be a multiple of 64 and >= 16 MiB");
+    double* buf[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 4 && e == hipSuccess; ++k) e = dalloc(&buf[k], (size_t)bytes / sizeof(double));
+    hipStream_t s = c->stream;
+    for (int k = 0; k < 4 && e == hipSuccess; ++k) e = launch_fill_f64(buf[k], bytes / 8, 1.0, s);
+    float best_c = 1e30f, best_t = 1e30f;
+    for (int grid : {16384, 65536}) {
+        for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {
+            float ms = 0.f;
+            e = hipEventRecord(c->ev0, s);
+            if (e == hipSuccess) e = launch_probe_copy(buf[0], buf[1], bytes, grid, s);
+            if (e == hipSuccess) e = hipEventRecord(c->ev1, s);
+            if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+            if (e == hipSuccess && ms < best_c) best_c = ms;
+            if (e == hipSuccess) e = hipEventRecord(c->ev0, s);
+            if (e == hipSuccess) e = launch_probe_r3w1(buf[0], buf[1], buf[2], buf[3], bytes, grid, s);
+            if (e == hipSuccess) e = hipEventRecord(c->ev1, s);
+            if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+            if (e == hipSuccess && ms < best_t) best_t = ms;
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    for (double* p : buf)
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) return set_err(SF_ERR_HIP, "sf_probe_dram: %s", hipGetErrorString(e));
+    if (copy_gbs) *copy_gbs = 2.0 * (double)bytes / (1e6 * (double)best_c);
+    if (r3w1_gbs) *r3w1_gbs = 4.0 * (double)bytes / (1e6 * (double)best_t);
+    return SF_OK;
+}
+
+// ---------------------------------------------------------------------------------------
 // marker -> grid
 
 void sf::m2g_free(M2GScratch& S)

@@ -131,6 +131,11 @@ hipError_t launch_fill_f64(double* dst, int64_t n, double v, hipStream_t s);
 hipError_t launch_cast_f64(double* dst, const float* src, int64_t n, hipStream_t s);
 hipError_t launch_reduce_f32(const RedParamsF& p, int narr, int nb, hipStream_t s);
 
+// ---- DRAM probe ----
+hipError_t launch_probe_copy(const double* a, double* b, int64_t bytes, int grid, hipStream_t s);
+hipError_t launch_probe_r3w1(const double* a, const double* b, const double* c, double* d, int64_t bytes, int grid,
+                             hipStream_t s);
+
 // ---- batched ensemble ----
 // One ensemble instance.  pt[0] runs the caller's state into the scratch buffers, pt[1]
 // runs it back (tile parameters complete, flags unused); the rest is what the solve loop

@@ -137,6 +137,14 @@ def autotune_pt(nx, ny, iters=20, state=None, g_y=1.0, dx=None, dy=None):
     return best.as_dict(), [table[i].as_dict() for i in range(n.value)]
 
 
+def probe_dram(nbytes=1 << 30):
+    """DRAM ceiling (sf_probe_dram): best copy and 3-read + 1-write bandwidth in GB/s."""
+    ctx = context(0, 0)
+    cp, tr = C.c_double(), C.c_double()
+    check(ctx.lib.sf_probe_dram(ctx.handle, int(nbytes), C.byref(cp), C.byref(tr)))
+    return {"copy_gbs": cp.value, "r3w1_gbs": tr.value}
+
+
 def context(nx=0, ny=0):
     key = (nx, ny, torch.cuda.current_device())
     ctx = _contexts.get(key)

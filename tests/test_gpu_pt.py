@@ -227,3 +227,8 @@ def test_autotune_keeps_results(gpu, golden):
     ref = oracle_iterations(exp, 63, 47, 9, 1.0, 1.0 / 62, 1.0 / 46, 0.01)
     for k in STATE:
         assert np.array_equal(got[k], ref[k]), k
+
+
+def test_dram_probe(gpu):
+    r = sfhip.probe_dram(1 << 26)
+    assert 500.0 < r["copy_gbs"] < 9000.0 and 500.0 < r["r3w1_gbs"] < 9000.0
