@@ -206,6 +206,15 @@ int sf_marker_to_grid(sf_ctx *ctx, double x_grid_min, double y_grid_min, double 
                       const double *x_m, const double *y_m, const double *val_m, int64_t nm, double dx,
                       double dy, double *val_wt_sum, double *wt_sum, int ws_nx, int ws_ny, sf_m2g_out *out);
 
+/* The three bilinearMarkerToGrid! calls of one timestep (StokesFlow2D_multixpu.jl:164-166) in
+ * one call: rho_vy (nx+1, ny) from rho_m, mu_b (nx, ny) and mu_p (nx-1, ny-1) from mu_m, at the
+ * reference's grid origins.  out[0..2] (may be NULL) get each pass's counters.  Each grid keeps
+ * its own sort: its cell keys come from its own origin in floating point, so one sort cannot
+ * serve the three grids bit for bit. */
+int sf_markers_to_grids3(sf_ctx *ctx, double *rho_vy, double *mu_b, double *mu_p, int nx, int ny, const double *x_m,
+                         const double *y_m, const double *rho_m, const double *mu_m, int64_t nm, double dx, double dy,
+                         sf_m2g_out out[3]);
+
 /* The integer cell keys of bilinearMarkerToGrid! (MarkerToGrid.jl:116-122 with
  * topleftIndexRelDist, common.jl:22-33), for checking them bit for bit: keys[m] =
  * iy*(gnx+1) + ix for marker m's 1-based top-left node (ix, iy) when 0 <= ix <= gnx and

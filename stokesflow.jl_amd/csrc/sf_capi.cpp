@@ -715,6 +715,26 @@ extern "C" int sf_move_markers_rk4(sf_ctx* c, double* x_m, double* y_m, int64_t 
 // ---------------------------------------------------------------------------------------
 // timestep (StokesFlow2D_multixpu.jl:164-181 at one rank)
 
+extern "C" int sf_markers_to_grids3(sf_ctx* c, double* rho_vy, double* mu_b, double* mu_p, int nx, int ny,
+                                    const double* x_m, const double* y_m, const double* rho_m, const double* mu_m,
+                                    int64_t nm, double dx, double dy, sf_m2g_out out[3])
+{
+    if (!c) return set_err(SF_ERR_ARG, "sf_markers_to_grids3: NULL ctx");
+    if (nx < 3 || ny < 3) return set_err(SF_ERR_ARG, "sf_markers_to_grids3: need Nx, Ny >= 3");
+    // grid origins, StokesFlow2D_multixpu.jl:117-126
+    const double x_rho = 0.0 * dx - 0.5 * dx, y_rho = 0.0 * dy;
+    const double x_p = 0.0 * dx + 0.5 * dx, y_p = 0.0 * dy + 0.5 * dy;
+    sf_m2g_out mo[3] = {};
+    int rc = m2g_call(c, x_rho, y_rho, rho_vy, nx + 1, ny, x_m, y_m, rho_m, nm, dx, dy, nullptr, nullptr, 0, 0, &mo[0]);
+    if (rc) return rc;
+    rc = m2g_call(c, 0.0, 0.0, mu_b, nx, ny, x_m, y_m, mu_m, nm, dx, dy, nullptr, nullptr, 0, 0, &mo[1]);
+    if (rc) return rc;
+    rc = m2g_call(c, x_p, y_p, mu_p, nx - 1, ny - 1, x_m, y_m, mu_m, nm, dx, dy, nullptr, nullptr, 0, 0, &mo[2]);
+    if (rc) return rc;
+    if (out) std::memcpy(out, mo, sizeof(mo));
+    return SF_OK;
+}
+
 extern "C" int sf_step(sf_ctx* c, sf_step_args* a, sf_solve_out* so, sf_m2g_out m2g_out[3], sf_rk4_out* ro)
 {
     if (!c || !a) return set_err(SF_ERR_ARG, "sf_step: NULL argument");
@@ -723,17 +743,12 @@ extern "C" int sf_step(sf_ctx* c, sf_step_args* a, sf_solve_out* so, sf_m2g_out 
     const int nx = a->nx, ny = a->ny;
     const double dx = a->dx, dy = a->dy;
     const double lx = (double)(nx - 1) * dx, ly = (double)(ny - 1) * dy;
-    // grid origins, StokesFlow2D_multixpu.jl:117-126
-    const double x_rho = 0.0 * dx - 0.5 * dx, y_rho = 0.0 * dy;
-    const double x_p = 0.0 * dx + 0.5 * dx, y_p = 0.0 * dy + 0.5 * dy;
+    // velocity grid origins, StokesFlow2D_multixpu.jl:121-124
     const double x_vx = -1.0 * dx, y_vx = 0.0 * dy - 0.5 * dy;
     const double x_vy = 0.0 * dx - 0.5 * dx, y_vy = -1.0 * dy;
     sf_m2g_out mo[3] = {};
-    rc = m2g_call(c, x_rho, y_rho, a->rho_vy, nx + 1, ny, a->x_m, a->y_m, a->rho_m, a->nm, dx, dy, nullptr, nullptr, 0, 0, &mo[0]);
-    if (rc) return rc;
-    rc = m2g_call(c, 0.0, 0.0, a->mu_b, nx, ny, a->x_m, a->y_m, a->mu_m, a->nm, dx, dy, nullptr, nullptr, 0, 0, &mo[1]);
-    if (rc) return rc;
-    rc = m2g_call(c, x_p, y_p, a->mu_p, nx - 1, ny - 1, a->x_m, a->y_m, a->mu_m, a->nm, dx, dy, nullptr, nullptr, 0, 0, &mo[2]);
+    rc = sf_markers_to_grids3(c, a->rho_vy, a->mu_b, a->mu_p, nx, ny, a->x_m, a->y_m, a->rho_m, a->mu_m, a->nm, dx, dy,
+                              mo);
     if (rc) return rc;
     sf_solve_out sout{};
     rc = sf_solve_stokes(c, a->P, a->Vx_pad, a->Vy_pad, a->rho_vy, a->mu_b, a->mu_p, nullptr, nullptr, nullptr,

@@ -24,7 +24,7 @@ EXPORTS = (
     "sf_pt_iterations", "sf_marker_to_grid", "sf_move_markers_rk4", "sf_step", "sf_init_markers",
     "sf_plume_properties", "sf_rt_properties", "sf_solve_stokes_refined", "sf_solve_stokes_batched",
     "sf_marker_cell_keys", "sf_ensemble_create", "sf_ensemble_step", "sf_ensemble_destroy",
-    "sf_ctx_autotune_pt", "sf_probe_dram",
+    "sf_ctx_autotune_pt", "sf_probe_dram", "sf_markers_to_grids3",
 )
 
 
@@ -143,6 +143,8 @@ def load():
     L.sf_pt_iterations.argtypes = [_vp, _i64] + [_vp] * 8 + [_d, _d, _d, _i, _i, _d]
     L.sf_marker_to_grid.argtypes = [_vp, _d, _d, _vp, _i, _i, _vp, _vp, _vp, _i64, _d, _d, _vp, _vp, _i, _i,
                                     C.POINTER(M2GOut)]
+    L.sf_markers_to_grids3.argtypes = [_vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _i64, _d, _d,
+                                       C.POINTER(M2GOut)]
     L.sf_probe_dram.argtypes = [_vp, _i64, C.POINTER(_d), C.POINTER(_d)]
     L.sf_ctx_autotune_pt.argtypes = [_vp, _i] + [_vp] * 8 + [_d, _d, _d, C.POINTER(PTTuning), C.POINTER(PTTuning), _i,
                                                              C.POINTER(_i)]

@@ -312,6 +312,24 @@ def bilinearMarkerToGrid(x_grid_min, y_grid_min, val_grid, x_m, y_m, val_m, dx, 
 # --------------------------------------------------------------------------------------
 # moveMarkersRK4!
 
+def markers_to_grids3(state, x_m, y_m, rho_m, mu_m, dx, dy):
+    """The three marker -> grid passes of a timestep into ``state`` (a :class:`FlowState`):
+    rho_vy from rho_m, mu_b and mu_p from mu_m (sf_markers_to_grids3).  Returns the three
+    passes' (n_bad_markers, n_empty_nodes)."""
+    s = state
+    nx, ny = s.nx, s.ny
+    nm = x_m.numel()
+    assert y_m.numel() == nm and rho_m.numel() == nm and mu_m.numel() == nm
+    out = (M2GOut * 3)()
+    ctx = context(0, 0)
+    check(ctx.lib.sf_markers_to_grids3(ctx.handle, _grid_ptr(s.rho_vy, (nx + 1, ny), "rho_vy"),
+                                       _grid_ptr(s.mu_b, (nx, ny), "mu_b"), _grid_ptr(s.mu_p, (nx - 1, ny - 1), "mu_p"),
+                                       nx, ny, _marker_ptr(x_m, "x_m"), _marker_ptr(y_m, "y_m"),
+                                       _marker_ptr(rho_m, "rho_m"), _marker_ptr(mu_m, "mu_m"), nm, float(dx),
+                                       float(dy), out))
+    return [(o.n_bad_markers, o.n_empty_nodes) for o in out]
+
+
 def marker_cell_keys(x_grid_min, y_grid_min, gnx, gny, x_m, y_m, dx, dy):
     """Integer cell keys of the marker -> grid pass (sf_marker_cell_keys): returns (keys, counts,
     n_bad) as int32 CUDA tensors (keys[m] = iy*(gnx+1)+ix of the 1-based top-left node, -1 if

@@ -202,3 +202,21 @@ def test_cell_keys_empty(gpu):
     e = torch.empty(0, dtype=torch.float64, device="cuda")
     keys, counts, nb = sfhip.marker_cell_keys(0.0, 0.0, 4, 3, e, e, 0.25, 0.5)
     assert keys.numel() == 0 and nb == 0 and int(counts.sum()) == 0
+
+
+
+def test_markers_to_grids3_matches_three_calls(gpu):
+    nx, ny = 40, 29
+    dx, dy, x, y, rho, mu = cases.plume_markers(nx, ny, 4, seed=12)
+    xg, yg, rg, mg = (sfhip.from_numpy(a) for a in (x, y, rho, mu))
+    st = sfhip.FlowState(nx, ny)
+    counts = sfhip.markers_to_grids3(st, xg, yg, rg, mg, dx, dy)
+    S = O.solver_arrays(nx, ny)
+    vws, ws = O.zeros(nx + 1, ny), O.zeros(nx + 1, ny)
+    o = O.grid_origins(dx, dy)
+    exp = [O.marker_to_grid(*o["rho"], S["rho_vy"], x, y, rho, dx, dy, vws, ws),
+           O.marker_to_grid(*o["mu_b"], S["mu_b"], x, y, mu, dx, dy, vws, ws),
+           O.marker_to_grid(*o["mu_p"], S["mu_p"], x, y, mu, dx, dy, vws, ws)]
+    assert counts == [tuple(e) for e in exp]
+    for name, k in (("rho_vy", "rho_vy"), ("mu_b", "mu_b"), ("mu_p", "mu_p")):
+        assert np.array_equal(sfhip.to_numpy(getattr(st, name)), S[k]), name
