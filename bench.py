@@ -62,6 +62,39 @@ def bytes_per_iter_fp32(nx, ny):
     return 4 * (2 * (n_vx + n_vy + n_p + n_dvx + n_dvy) + n_p + nx * ny + (nx + 1) * ny + n_dvx + n_dvy + n_p)
 
 
+REF_T_EFF_GBS = {32: 0.00186, 64: 1.417, 128: 5.494, 256: 17.75, 512: 48.50, 1024: 84.65, 2048: 74.80,
+                 4096: 87.45}  # StokesSolver_scaling.jl:186, one GPU
+
+
+def reference_window(sizes=(64, 512, 4096), seed=3000):
+    """The reference's own benchmark window (StokesSolver_scaling.jl:29-152): solveStokes! for 200
+    PT iterations, timed from iteration 11, T_eff = 6 n^2 8 B / t_it; on the plume set-up at each
+    n, beside the reference's published T_eff."""
+    import torch
+
+    import sfhip
+
+    out = {}
+    for n in sizes:
+        dx = dy = 1.0 / (n - 1)
+        x, y = sfhip.init_markers(n, n, 4, dx, dy, 0.25, seed=seed)
+        rho, mu = sfhip.plume_properties(x, y, 0.1, 0.5, 0.8, 0.1, 0.001, 0.9, 1.0, 0.01, 0.1, 1.0)
+        st = sfhip.FlowState(n, n)
+        sfhip.markers_to_grids3(st, x, y, rho, mu, dx, dy)
+        del x, y, rho, mu
+        stats = {}
+        sfhip.solveStokes(st.P, st.Vx, st.Vy, st.rho_vy, st.mu_b, st.mu_p, None, None, None, None, None, None, None,
+                          st.dVxdtau, st.dVydtau, None, None, st.Vx_small, st.Vy_small, 1.0, dx, dy, n, n, 0.0, 0.5,
+                          eps=1e-300, print_info=False, iterMax=200, stats=stats)
+        torch.cuda.synchronize()
+        ref = REF_T_EFF_GBS.get(n)
+        out[str(n)] = {"t_it_us": 1e6 * stats["t_it"], "t_eff_gbs": stats["t_eff"], "alg_gbs": stats["gbs_alg"],
+                       "reference_t_eff_gbs": ref, "t_eff_ratio": stats["t_eff"] / ref if ref else None}
+        del st
+        torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(nx, ny, iters):
     """The CPU oracle (C, OpenMP) timed on `iters` PT iterations of the same grid size."""
     import numpy as np
@@ -338,6 +371,8 @@ def main():
     ap.add_argument("--progress", action="store_true", help="print the solvers' convergence checks (long runs)")
     ap.add_argument("--refined-iters", type=int, default=0,
                     help="also time this many FP32 correction sweeps (defect-correction solver) on the same state")
+    ap.add_argument("--no-ref-window", action="store_true",
+                    help="skip the reference-window record (200 iterations at 64^2, 512^2, 4096^2)")
     ap.add_argument("--tune", action="store_true",
                     help="run the PT launch-shape autotuner first (default: the size default shape)")
     a = ap.parse_args()
@@ -441,6 +476,8 @@ def main():
                                   "s_per_iter": t_r, "iters_per_s": 1.0 / t_r,
                                   "alg_gbs_fp32_kernel_bytes": bytes_per_iter_fp32(nx, ny) / t_r / 1e9,
                                   "speedup_per_iter_vs_fp64": (wall / a.steps) / t_r}
+    if not a.no_ref_window:
+        line["reference_window"] = reference_window()
     if a.tts_n > 0:
         del st
         torch.cuda.empty_cache()
