@@ -585,12 +585,17 @@ int sfo_move_markers_rk4(double *x_m, double *y_m, int64_t nm, const double *Vx,
 }
 
 /* ---------------------------------------------------------------------------------------- */
-/* LinRange element lerpi(j, d, a, b) = fma(t, b, fma(-t, a, a)), t = j/d (Julia Base). */
+/* LinRange element as Julia 1.8 (the reference's CI version, .github/workflows/CI.yml:11)
+ * computes it: lerpi(j, d, a, b) = (1 - t)*a + t*b with t = j/d, unfused.  Later Julia
+ * releases evaluate fma(t, b, fma(-t, a, a)), which moves some elements by one ulp; that
+ * matters here because four markers of the 20 x 26 test lattice (md = 5) sit exactly on the
+ * plume circle of test/test_StokesFlow2D.jl:61-83, and its golden vectors (:86-88) are
+ * reproduced only when all four evaluate to distance^2 == r^2 (not inside). */
 static double linrange_at(double a, double b, int64_t len, int64_t k)
 {
     if (len == 1) return a;
     double t = (double)k / (double)(len - 1);
-    return fma(t, b, fma(-t, a, a));
+    return (1.0 - t) * a + t * b;
 }
 
 int64_t sfo_lattice_markers(int nx, int ny, int md, double lx, double ly, double dx, double dy,
