@@ -27,7 +27,7 @@ extern "C" const char* sf_last_error(void) { return g_err; }
 static void free_ctx(sf_ctx* c)
 {
     if (!c) return;
-    void* bufs[] = {c->vx_b, c->vy_b, c->p_b, c->txx, c->tyy, c->txy, c->divv, c->rx, c->ry, c->part, c->scal,
+    void* bufs[] = {c->dvx_b, c->dvy_b, c->vx_b, c->vy_b, c->p_b, c->txx, c->tyy, c->txy, c->divv, c->rx, c->ry, c->part, c->scal,
                     c->counters, c->m2g.key, c->m2g.count, c->m2g.offset, c->m2g.fill, c->m2g.rec_idx,
                     c->m2g.rec_dx, c->m2g.rec_dy, c->m2g.rec_val, c->m2g.blocksums, c->m2g.counters};
     for (void* b : bufs)
@@ -113,6 +113,124 @@ extern "C" int sf_ctx_set_pt_tuning(sf_ctx* c, int rows, int depth, int max_wave
     c->max_waves = max_waves;
     return SF_OK;
 }
+
+extern "C" int sf_ctx_set_pt_fusion(sf_ctx* c, int iters_per_launch, int rows)
+{
+    if (!c) return set_err(SF_ERR_ARG, "sf_ctx_set_pt_fusion: NULL ctx");
+    if (iters_per_launch < 0 || iters_per_launch > 2)
+        return set_err(SF_ERR_ARG, "sf_ctx_set_pt_fusion: iters_per_launch %d not 0, 1 or 2", iters_per_launch);
+    if (rows < 0 || rows > 4096) return set_err(SF_ERR_ARG, "sf_ctx_set_pt_fusion: rows %d outside [0, 4096]", rows);
+    c->fuse = iters_per_launch;
+    c->pair_rows = rows;
+    return SF_OK;
+}
+
+// The PT loop state shared by the solve and the fixed-count iteration entry point: the
+// ping-pong pair of (Vx_s, Vy_s, P) and, for two-iteration passes, of (dVx, dVy).
+namespace {
+struct PTLoop {
+    sf_ctx* c;
+    PTParams pp;
+    double *cur_vx, *cur_vy, *cur_p, *nxt_vx, *nxt_vy, *nxt_p;
+    double *cur_dvx, *cur_dvy, *nxt_dvx, *nxt_dvy;
+    double *vx0, *vy0, *p0, *dvx0, *dvy0;  // the caller's arrays
+    bool pairs;                             // two-iteration passes enabled
+
+    int init(sf_ctx* ctx, double* Vx_s, double* Vy_s, double* P, double* dVx, double* dVy)
+    {
+        c = ctx;
+        cur_vx = vx0 = Vx_s;
+        cur_vy = vy0 = Vy_s;
+        cur_p = p0 = P;
+        cur_dvx = dvx0 = dVx;
+        cur_dvy = dvy0 = dVy;
+        nxt_vx = c->vx_b;
+        nxt_vy = c->vy_b;
+        nxt_p = c->p_b;
+        pairs = c->fuse != 1;
+        if (pairs && !c->dvx_b) {
+            const size_t nx = c->nx, ny = c->ny;
+            hipError_t e = dalloc(&c->dvx_b, (nx - 2) * (ny - 1));
+            if (e == hipSuccess) e = dalloc(&c->dvy_b, (nx - 1) * (ny - 2));
+            if (e != hipSuccess) {
+                if (c->dvx_b) (void)hipFree(c->dvx_b);
+                c->dvx_b = c->dvy_b = nullptr;
+                return set_err(SF_ERR_HIP, "PT loop: dV buffer allocation failed: %s", hipGetErrorString(e));
+            }
+        }
+        nxt_dvx = c->dvx_b;
+        nxt_dvy = c->dvy_b;
+        return SF_OK;
+    }
+    // One iteration (flags: check-iteration outputs); dV in place.
+    int single(int flags, hipStream_t s)
+    {
+        pp.flags = flags;
+        pp.seg = c->seg;
+        pp.vx_a = cur_vx;
+        pp.vy_a = cur_vy;
+        pp.p_a = cur_p;
+        pp.vx_b = nxt_vx;
+        pp.vy_b = nxt_vy;
+        pp.p_b = nxt_p;
+        pp.dvx = cur_dvx;
+        pp.dvy = cur_dvy;
+        HIPCHK(launch_pt_fused(pp, s));
+        std::swap(cur_vx, nxt_vx);
+        std::swap(cur_vy, nxt_vy);
+        std::swap(cur_p, nxt_p);
+        return SF_OK;
+    }
+    // Two plain iterations in one pass.
+    int pair(hipStream_t s)
+    {
+        pp.flags = 0;
+        pp.seg = c->pair_rows;
+        pp.vx_a = cur_vx;
+        pp.vy_a = cur_vy;
+        pp.p_a = cur_p;
+        pp.vx_b = nxt_vx;
+        pp.vy_b = nxt_vy;
+        pp.p_b = nxt_p;
+        pp.dvx = cur_dvx;
+        pp.dvy = cur_dvy;
+        pp.dvx_o = nxt_dvx;
+        pp.dvy_o = nxt_dvy;
+        HIPCHK(launch_pt_pair(pp, s));
+        std::swap(cur_vx, nxt_vx);
+        std::swap(cur_vy, nxt_vy);
+        std::swap(cur_p, nxt_p);
+        std::swap(cur_dvx, nxt_dvx);
+        std::swap(cur_dvy, nxt_dvy);
+        return SF_OK;
+    }
+    // State back into the caller's arrays.
+    int finish(hipStream_t s)
+    {
+        const size_t nx = c->nx, ny = c->ny;
+        if (cur_vx != vx0) {
+            HIPCHK(hipMemcpyAsync(vx0, cur_vx, sizeof(double) * nx * (ny + 1), hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(vy0, cur_vy, sizeof(double) * (nx + 1) * ny, hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(p0, cur_p, sizeof(double) * (nx - 1) * (ny - 1), hipMemcpyDeviceToDevice, s));
+        }
+        if (cur_dvx != dvx0) {
+            HIPCHK(hipMemcpyAsync(dvx0, cur_dvx, sizeof(double) * (nx - 2) * (ny - 1), hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(dvy0, cur_dvy, sizeof(double) * (nx - 1) * (ny - 2), hipMemcpyDeviceToDevice, s));
+        }
+        cur_vx = vx0;
+        cur_vy = vy0;
+        cur_p = p0;
+        cur_dvx = dvx0;
+        cur_dvy = dvy0;
+        nxt_vx = c->vx_b;
+        nxt_vy = c->vy_b;
+        nxt_p = c->p_b;
+        nxt_dvx = c->dvx_b;
+        nxt_dvy = c->dvy_b;
+        return SF_OK;
+    }
+};
+}  // namespace
 
 extern "C" int sf_ctx_set_stream(sf_ctx* c, void* stream)
 {
@@ -242,12 +360,10 @@ extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_
     pp.ptsc41 = k.ptsc41;
     pp.max_nxy = k.max_nxy;
 
-    double* cur_vx = Vx_s;
-    double* cur_vy = Vy_s;
-    double* cur_p = P;
-    double* nxt_vx = c->vx_b;
-    double* nxt_vy = c->vy_b;
-    double* nxt_p = c->p_b;
+    PTLoop L;
+    rc = L.init(c, Vx_s, Vy_s, P, dVx, dVy);
+    if (rc) return rc;
+    L.pp = pp;
 
     double err = 2.0 * eps;
     int64_t iter = 1, niter = 0;
@@ -257,19 +373,28 @@ extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_
             HIPCHK(hipEventRecord(c->ev0, s));
             timed = true;
         }
+        // iterations iter and iter+1 in one pass when neither is a check, iter is not
+        // followed by a dt refresh, iter+1 is not the last and the timing window does not
+        // start at iter+1
+        if (L.pairs && iter + 1 < iter_max && iter % ncheck != 0 && (iter + 1) % ncheck != 0 &&
+            !(use_fss && iter % ndtupdate == 0) && iter + 1 != itert1) {
+            rc = L.pair(s);
+            if (rc) return rc;
+            if (use_fss && (iter + 1) % ndtupdate == 0) {
+                rc = enqueue_dt(c, L.cur_vx, L.cur_vy, dx, dy, maxdisp, 1);
+                if (rc) return rc;
+            }
+            iter += 2;
+            niter += 2;
+            continue;
+        }
         const bool check = (iter % ncheck) == 0;
         const bool last = iter == iter_max;
-        pp.flags = ((check || last) ? PT_WRITE_TAU : 0) | (check ? PT_WRITE_RX : 0);
-        pp.vx_a = cur_vx;
-        pp.vy_a = cur_vy;
-        pp.p_a = cur_p;
-        pp.vx_b = nxt_vx;
-        pp.vy_b = nxt_vy;
-        pp.p_b = nxt_p;
-        HIPCHK(launch_pt_fused(pp, s));
-        std::swap(cur_vx, nxt_vx);
-        std::swap(cur_vy, nxt_vy);
-        std::swap(cur_p, nxt_p);
+        rc = L.single(((check || last) ? PT_WRITE_TAU : 0) | (check ? PT_WRITE_RX : 0), s);
+        if (rc) return rc;
+        double* cur_vx = L.cur_vx;
+        double* cur_vy = L.cur_vy;
+        double* cur_p = L.cur_p;
 
         const bool dt_upd = use_fss && (iter % ndtupdate) == 0;
         if (dt_upd) {
@@ -325,12 +450,8 @@ extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_
     }
     if (timed) HIPCHK(hipEventRecord(c->ev1, s));
 
-    // state back into the caller's arrays
-    if (cur_vx != Vx_s) {
-        HIPCHK(hipMemcpyAsync(Vx_s, cur_vx, sizeof(double) * (size_t)nx * (ny + 1), hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(Vy_s, cur_vy, sizeof(double) * (size_t)(nx + 1) * ny, hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(P, cur_p, sizeof(double) * (size_t)(nx - 1) * (ny - 1), hipMemcpyDeviceToDevice, s));
-    }
+    rc = L.finish(s);  // state back into the caller's arrays
+    if (rc) return rc;
     HIPCHK(launch_pad_mirror(nx, ny, Vx_s, Vy_s, Vx_pad, Vy_pad, s));
     rc = enqueue_dt(c, Vx_s, Vy_s, dx, dy, maxdisp, 0);
     if (rc) return rc;
@@ -401,25 +522,20 @@ extern "C" int sf_pt_iterations(sf_ctx* c, int64_t n_iter, double* P, double* Vx
     pp.vsc_mdxy2 = k.vsc_mdxy2;
     pp.ptsc41 = k.ptsc41;
     pp.max_nxy = k.max_nxy;
-    double *cv = Vx_s, *cw = Vy_s, *cp = P, *nv = c->vx_b, *nw = c->vy_b, *np = c->p_b;
-    for (int64_t it = 0; it < n_iter; ++it) {
-        pp.vx_a = cv;
-        pp.vy_a = cw;
-        pp.p_a = cp;
-        pp.vx_b = nv;
-        pp.vy_b = nw;
-        pp.p_b = np;
-        HIPCHK(launch_pt_fused(pp, s));
-        std::swap(cv, nv);
-        std::swap(cw, nw);
-        std::swap(cp, np);
+    PTLoop L;
+    rc = L.init(c, Vx_s, Vy_s, P, dVx, dVy);
+    if (rc) return rc;
+    L.pp = pp;
+    int64_t it = 0;
+    for (; L.pairs && it + 2 <= n_iter; it += 2) {
+        rc = L.pair(s);
+        if (rc) return rc;
     }
-    if (cv != Vx_s) {
-        HIPCHK(hipMemcpyAsync(Vx_s, cv, sizeof(double) * (size_t)nx * (ny + 1), hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(Vy_s, cw, sizeof(double) * (size_t)(nx + 1) * ny, hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(P, cp, sizeof(double) * (size_t)(nx - 1) * (ny - 1), hipMemcpyDeviceToDevice, s));
+    for (; it < n_iter; ++it) {
+        rc = L.single(0, s);
+        if (rc) return rc;
     }
-    return SF_OK;
+    return L.finish(s);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -481,7 +597,8 @@ extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, const double* P_in, cons
         release();
         return set_err(SF_ERR_HIP, "%s: %s", who, hipGetErrorString(e));
     }
-    const int saved_seg = c->seg, saved_depth = c->depth, saved_waves = c->max_waves;
+    const int saved_seg = c->seg, saved_depth = c->depth, saved_waves = c->max_waves, saved_fuse = c->fuse;
+    c->fuse = 1;  // the candidates are shapes of the one-iteration kernel
     const int rows_c[] = {1, 2, 4, 8, 16};
     const int depth_c[] = {1, 2};
     const int waves_c[] = {-1, 0};
@@ -517,6 +634,7 @@ extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, const double* P_in, cons
             }
     (void)hipStreamSynchronize(s);
     release();
+    c->fuse = saved_fuse;
     if (rc) {
         c->seg = saved_seg;
         c->depth = saved_depth;

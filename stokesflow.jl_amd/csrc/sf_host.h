@@ -136,4 +136,7 @@ struct sf_ctx {
     int seg = 0;        // rows per tile; 0 = size heuristic
     int depth = 0;      // rows of operands prefetched; 0 = size default
     int max_waves = -1; // 0 persistent (resident slots), > 0 cap, < 0 one wave per tile
+    int fuse = 0;       // PT iterations per launch: 0 default (2), 1, 2
+    int pair_rows = 0;  // rows per tile of the two-iteration pass; 0 = size default
+    double *dvx_b = nullptr, *dvy_b = nullptr;  // second dVx/dVy (two-iteration pass)
 };

@@ -27,6 +27,8 @@ struct PTParams {
     double* p_b;
     double* dvx;          // (nx-2, ny-1), in place
     double* dvy;          // (nx-1, ny-2), in place
+    double* dvx_o;        // two-iteration pass: dVx, dVy out (dvx, dvy are then read only)
+    double* dvy_o;
     const double* rho;    // rho_vy (nx+1, ny)
     const double* mu_b;   // (nx, ny)
     const double* mu_p;   // (nx-1, ny-1)
@@ -65,6 +67,10 @@ int pt_default_depth(int nx, int ny);
 int pt_resident_waves(int depth);
 int red_num_blocks(int64_t nmax);
 hipError_t launch_pt_fused(const PTParams& p, hipStream_t s);
+// Two PT iterations per pass (pt2_tile): state (vx_a, vy_a, p_a, dvx, dvy) -> (vx_b, vy_b, p_b,
+// dvx_o, dvy_o); p.seg = rows per tile (<= 0: default).  Bit-identical to two launch_pt_fused.
+hipError_t launch_pt_pair(const PTParams& p, hipStream_t s);
+int pt2_default_rows(int nx, int ny);
 hipError_t launch_pt_resid_y(const ResYParams& p, hipStream_t s);
 hipError_t launch_reduce(const RedParams& p, int narr, int nb, hipStream_t s);
 hipError_t launch_finalize_dt(const double* part, int nb, double dx, double dy, double maxdisp, double* scal,

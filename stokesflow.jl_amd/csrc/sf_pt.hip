@@ -25,7 +25,7 @@
 // dt are deterministic.
 #include <algorithm>
 
-#include "sf_pt_tile.h"
+#include "sf_pt2_tile.h"
 
 namespace sf {
 
@@ -51,6 +51,13 @@ __global__ __launch_bounds__(64) void pt_tile_f64(PTParams p)
 {
     const int logical = xcd_remap(blockIdx.x, gridDim.x);
     pt_tile<DEPTH, CHECK>(p, logical % p.nstrips, logical / p.nstrips, p.flags);
+}
+
+// Two PT iterations per pass, one wave per tile (sf_pt2_tile.h).
+__global__ __launch_bounds__(64) void pt2_tile_f64(PTParams p)
+{
+    const int logical = xcd_remap(blockIdx.x, gridDim.x);
+    pt2_tile(p, logical % p.nstrips, logical / p.nstrips);
 }
 
 // compute_ResY with post-update V and dt_check (compute_Residuals!, :241-249).
@@ -241,6 +248,29 @@ hipError_t launch_pt_fused(const PTParams& p0, hipStream_t s)
         if (check) hipLaunchKernelGGL((pt_fused_f64<1, true>), dim3(grid), dim3(64), 0, s, p);
         else hipLaunchKernelGGL((pt_fused_f64<1, false>), dim3(grid), dim3(64), 0, s, p);
     }
+    return hipGetLastError();
+}
+
+// Rows per tile of the two-iteration pass: each tile recomputes 3 rows of the first
+// iteration and 1 of the second around its segment, so taller tiles cut the re-reads, while
+// smaller grids need short tiles for enough waves (2 per SIMD at 172 VGPRs).  Measured on
+// MI355X (us per iteration, one-iteration kernel in brackets): 4096^2 16 rows 193-198 (356),
+// 2048^2 12 rows 51.6 (90.7), 1024^2 12 rows 14.2 (19.8), 512^2 4 rows 6.6 (7.0).
+int pt2_default_rows(int nx, int ny)
+{
+    const int n = std::max(nx, ny);
+    const int r = n > 2048 ? 16 : (n > 768 ? 12 : (n > 512 ? 8 : 4));
+    return ny >= r ? r : ny;
+}
+
+hipError_t launch_pt_pair(const PTParams& p0, hipStream_t s)
+{
+    PTParams p = p0;
+    if (p.seg <= 0) p.seg = pt2_default_rows(p.nx, p.ny);
+    p.nstrips = (p.nx - 1 + PT2_OUT_LANES - 1) / PT2_OUT_LANES;  // columns 0..nx-2 carry outputs
+    const int nseg = (p.ny + p.seg - 1) / p.seg;
+    p.ntiles = p.nstrips * nseg;
+    hipLaunchKernelGGL(pt2_tile_f64, dim3(p.ntiles), dim3(64), 0, s, p);
     return hipGetLastError();
 }
 

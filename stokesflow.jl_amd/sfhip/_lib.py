@@ -19,7 +19,7 @@ SF_ERR_SIZE = -4
 
 # every exported symbol of include/sf_hip.h
 EXPORTS = (
-    "sf_version", "sf_last_error", "sf_ctx_create", "sf_ctx_destroy", "sf_ctx_set_stream", "sf_ctx_set_pt_tuning",
+    "sf_version", "sf_last_error", "sf_ctx_create", "sf_ctx_destroy", "sf_ctx_set_stream", "sf_ctx_set_pt_tuning", "sf_ctx_set_pt_fusion",
     "sf_solve_stokes",
     "sf_pt_iterations", "sf_marker_to_grid", "sf_move_markers_rk4", "sf_step", "sf_init_markers",
     "sf_plume_properties", "sf_rt_properties", "sf_solve_stokes_refined", "sf_solve_stokes_batched",
@@ -133,6 +133,7 @@ def load():
     L.sf_ctx_destroy.argtypes = [_vp]
     L.sf_ctx_set_stream.argtypes = [_vp, _vp]
     L.sf_ctx_set_pt_tuning.argtypes = [_vp, _i, _i, _i]
+    L.sf_ctx_set_pt_fusion.argtypes = [_vp, _i, _i]
     L.sf_solve_stokes.argtypes = [_vp] + [_vp] * 19 + [_d, _d, _d, _i, _i, _d, _d, _i, _d, _i64, _i,
                                                         C.POINTER(SolveOut)]
     L.sf_solve_stokes_refined.argtypes = [_vp] + [_vp] * 19 + [_d, _d, _d, _i, _i, _d, _d, _i, _d,

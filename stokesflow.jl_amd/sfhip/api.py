@@ -145,6 +145,19 @@ def probe_dram(nbytes=1 << 30):
     return {"copy_gbs": cp.value, "r3w1_gbs": tr.value}
 
 
+def set_pt_fusion(nx, ny, iters_per_launch=0, rows=0):
+    """PT iterations per launch for the nx x ny context (sf_ctx_set_pt_fusion): 2 runs pairs of
+    plain iterations as one pass, 1 one per launch, 0 the default (2).  Results are identical."""
+    ctx = context(nx, ny)
+    check(ctx.lib.sf_ctx_set_pt_fusion(ctx.handle, int(iters_per_launch), int(rows)))
+
+
+def set_pt_tuning(nx, ny, rows=0, depth=0, max_waves=-1):
+    """Launch shape of the one-iteration PT kernel for the nx x ny context (sf_ctx_set_pt_tuning)."""
+    ctx = context(nx, ny)
+    check(ctx.lib.sf_ctx_set_pt_tuning(ctx.handle, int(rows), int(depth), int(max_waves)))
+
+
 def context(nx=0, ny=0):
     key = (nx, ny, torch.cuda.current_device())
     ctx = _contexts.get(key)

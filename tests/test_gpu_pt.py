@@ -58,6 +58,38 @@ def test_iterations_bitwise_sizes(gpu, nx, ny):
         assert np.array_equal(got[k], exp[k]), (nx, ny, k)
 
 
+# two-iteration passes (sf_ctx_set_pt_fusion): 60-column strips, tiles of `rows` rows; sizes
+# around the strip width and tile heights, minimal grids, odd iteration counts (one plain
+# iteration after the pairs)
+@pytest.mark.parametrize("nx,ny,rows", [(3, 3, 0), (4, 4, 1), (3, 9, 2), (59, 5, 3), (60, 8, 8), (61, 9, 8),
+                                        (62, 17, 16), (63, 31, 0), (121, 12, 5), (122, 40, 8), (200, 7, 4)])
+@pytest.mark.parametrize("n_iter", [2, 5])
+def test_pair_pass_bitwise(gpu, nx, ny, rows, n_iter):
+    S = cases.random_state(nx, ny, seed=nx * 7919 + ny * 31 + rows)
+    try:
+        sfhip.set_pt_fusion(nx, ny, 2, rows)
+        got = run_gpu_iterations(S, nx, ny, n_iter, 1.3, 0.09, 0.05, 0.017)
+    finally:
+        sfhip.set_pt_fusion(nx, ny, 0, 0)
+    exp = oracle_iterations(S, nx, ny, n_iter, 1.3, 0.09, 0.05, 0.017)
+    for k in STATE:
+        assert np.array_equal(got[k], exp[k]), (nx, ny, rows, n_iter, k, np.max(np.abs(got[k] - exp[k])))
+
+
+def test_pair_and_single_launch_agree(gpu):
+    """Forcing one iteration per launch gives the same bits as the default two-iteration passes."""
+    nx, ny = 97, 83
+    S = cases.random_state(nx, ny, seed=5)
+    try:
+        sfhip.set_pt_fusion(nx, ny, 1, 0)
+        one = run_gpu_iterations(S, nx, ny, 9, 0.8, 0.01, 0.012, 0.003)
+    finally:
+        sfhip.set_pt_fusion(nx, ny, 0, 0)
+    two = run_gpu_iterations(S, nx, ny, 9, 0.8, 0.01, 0.012, 0.003)
+    for k in STATE:
+        assert np.array_equal(one[k], two[k]), k
+
+
 def gpu_solve(S, nx, ny, g_y, dx, dy, dt, maxdisp, use_fss, eps, iter_max, scratch=True):
     keys = list(S.keys())
     G = to_gpu(S, keys)
