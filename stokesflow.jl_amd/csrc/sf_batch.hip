@@ -14,7 +14,7 @@
 // prepared on the host) are read through the constant address space: scalar loads that the
 // compiler re-issues where needed instead of holding ~70 uniform values in registers
 // across the solve loop.
-#include "sf_pt_tile.h"
+#include "sf_pt2_tile.h"
 
 namespace sf {
 
@@ -222,11 +222,30 @@ __global__ __launch_bounds__(64) void wide_pt_kernel(WideArgs a)
     const int t = t_glob - a.tile_start[lo];
     CBatch* cinst = (CBatch*)a.desc;
     const int pf = a.slot_pf[lo];
-    const PTParams SF_CONST& pp = cinst[a.slot_inst[lo]].pt[pf & 1];
+    const PTParams SF_CONST& pp = cinst[a.slot_inst[lo]].wpt[pf & 1];
     const int flags = pf >> 1;
     const int nstrips = pp.nstrips;
     if (flags) pt_tile<DEPTH, true>(pp, t % nstrips, t / nstrips, flags);
     else pt_tile<DEPTH, false>(pp, t % nstrips, t / nstrips, 0);
+}
+
+// Two PT iterations of every active instance per launch (pt2_tile, sf_pt2_tile.h).
+__global__ __launch_bounds__(64) void wide_pt2_kernel(WideArgs a)
+{
+    const int total = a.tile_start[a.n_active];
+    const int t_glob = xcd_remap(blockIdx.x, gridDim.x);
+    if (t_glob >= total) return;
+    int lo = 0, hi = a.n_active - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.tile_start[mid] <= t_glob) lo = mid;
+        else hi = mid - 1;
+    }
+    const int t = t_glob - a.tile_start[lo];
+    CBatch* cinst = (CBatch*)a.desc;
+    const PTParams SF_CONST& pp = cinst[a.slot_inst[lo]].wpt[2 + (a.slot_pf[lo] & 1)];
+    const int nstrips = pp.nstrips;
+    pt2_tile(pp, t % nstrips, t / nstrips);
 }
 
 hipError_t launch_wide_pt(const WideArgs& a, int depth, hipStream_t s)
@@ -235,6 +254,14 @@ hipError_t launch_wide_pt(const WideArgs& a, int depth, hipStream_t s)
     if (total <= 0) return hipSuccess;
     if (depth > 1) hipLaunchKernelGGL(wide_pt_kernel<2>, dim3(total), dim3(64), 0, s, a);
     else hipLaunchKernelGGL(wide_pt_kernel<1>, dim3(total), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_wide_pt2(const WideArgs& a, hipStream_t s)
+{
+    const int total = a.tile_start[a.n_active];
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(wide_pt2_kernel, dim3(total), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

@@ -12,7 +12,8 @@ namespace sf {
 size_t batch_scratch_doubles(int nx_, int ny_)
 {
     const size_t nx = nx_, ny = ny_, np = (nx - 1) * (ny - 1);
-    return nx * (ny + 1) + (nx + 1) * ny + np + 3 * np + nx * ny + (nx - 2) * (ny - 1) + (nx - 1) * (ny - 2) + SC_COUNT;
+    const size_t ndv = (nx - 2) * (ny - 1) + (nx - 1) * (ny - 2);
+    return nx * (ny + 1) + (nx + 1) * ny + np + 3 * np + nx * ny + ndv + SC_COUNT + ndv;
 }
 
 int batch_check(const sf_batch_instance& a, int i, const char* who)
@@ -43,7 +44,9 @@ BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, 
     double* t_xy = q; q += (size_t)nx * ny;
     double* r_x = q; q += (size_t)(nx - 2) * (ny - 1);
     double* r_y = q; q += (size_t)(nx - 1) * (ny - 2);
-    double* scal = q;
+    double* scal = q; q += SC_COUNT;
+    double* dvx2 = q; q += (size_t)(nx - 2) * (ny - 1);  // second dV (wide mode)
+    double* dvy2 = q;
     const SolveConsts k = solve_consts(nx, ny, a.dx, a.dy);
     if (rows <= 0) rows = pt_default_rows(nx, ny);
     rows = std::min(rows, ny);
@@ -87,6 +90,21 @@ BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, 
     b.pt[1].vx_b = a.Vx_s;
     b.pt[1].vy_b = a.Vy_s;
     b.pt[1].p_b = a.P;
+    // wide mode: dV ping-pongs with the state, so one parity describes both; the two-iteration
+    // descriptors carry the pass's own tile geometry
+    for (int par = 0; par < 2; ++par) {
+        PTParams w = b.pt[par];
+        w.dvx = par ? dvx2 : a.dVxdtau;
+        w.dvy = par ? dvy2 : a.dVydtau;
+        w.dvx_o = par ? a.dVxdtau : dvx2;
+        w.dvy_o = par ? a.dVydtau : dvy2;
+        b.wpt[par] = w;
+        const int rows2 = pt2_default_rows(nx, ny);
+        w.seg = rows2;
+        w.nstrips = (nx - 1 + 59) / 60;
+        w.ntiles = w.nstrips * ((ny + rows2 - 1) / rows2);
+        b.wpt[2 + par] = w;
+    }
     b.P = a.P;
     b.Vx_s = a.Vx_s;
     b.Vy_s = a.Vy_s;
@@ -156,7 +174,8 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         int id;
         int64_t iter, ncheck, ndtupdate;
         double err, ws;
-        bool checking;
+        bool checking, pair;
+        int64_t nlaunch;  // launches so far: their parity says where the state (V, P, dV) is
     };
     std::vector<Run> act;
     size_t q = 0;
@@ -166,7 +185,7 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         const sf_batch_instance& a = in[id];
         const int mn = std::min(a.nx, a.ny);
         Run r{id, 1, std::min<int64_t>(5 * (int64_t)mn, 2000), 10 * (int64_t)mn, 2.0 * a.eps,
-              batch_working_set(a.nx, a.ny), false};
+              batch_working_set(a.nx, a.ny), false, false, 0};
         HIPCHK(launch_set_scalar(desc[(size_t)id].scal + SC_DT, a.use_fss ? a.dt : 0.0, s));
         act.push_back(r);
         ws_active += r.ws;
@@ -176,12 +195,16 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         const sf_batch_instance& a = in[r.id];
         const BatchInstance& d = desc[(size_t)r.id];
         const int64_t niter = r.iter - 1;
-        if (niter & 1) {  // the last iterate is in the scratch buffers
-            const PTParams& p0 = d.pt[0];
+        if (r.nlaunch & 1) {  // the last iterate is in the scratch buffers
+            const PTParams& p0 = d.wpt[0];
             HIPCHK(hipMemcpyAsync(a.Vx_s, p0.vx_b, sizeof(double) * (size_t)a.nx * (a.ny + 1), hipMemcpyDeviceToDevice, s));
             HIPCHK(hipMemcpyAsync(a.Vy_s, p0.vy_b, sizeof(double) * (size_t)(a.nx + 1) * a.ny, hipMemcpyDeviceToDevice, s));
             HIPCHK(hipMemcpyAsync(a.P, p0.p_b, sizeof(double) * (size_t)(a.nx - 1) * (a.ny - 1), hipMemcpyDeviceToDevice,
                                   s));
+            HIPCHK(hipMemcpyAsync(a.dVxdtau, p0.dvx_o, sizeof(double) * (size_t)(a.nx - 2) * (a.ny - 1),
+                                  hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(a.dVydtau, p0.dvy_o, sizeof(double) * (size_t)(a.nx - 1) * (a.ny - 2),
+                                  hipMemcpyDeviceToDevice, s));
         }
         HIPCHK(launch_pad_mirror(a.nx, a.ny, a.Vx_s, a.Vy_s, a.Vx_pad, a.Vy_pad, s));
         int rc = enqueue_dt_raw(s, a.nx, a.ny, a.Vx_s, a.Vy_s, a.dx, a.dy, a.maxdisp, 0, env.part, d.scal);
@@ -195,37 +218,57 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         return SF_OK;
     };
     int rc = SF_OK;
-    WideArgs wa;
+    WideArgs wa, w2;
     std::memset(&wa, 0, sizeof(wa));
+    std::memset(&w2, 0, sizeof(w2));
     wa.desc = env.desc_dev;
+    w2.desc = env.desc_dev;
     while (!act.empty() || q < order.size()) {
         while (q < order.size() && (int)act.size() < WIDE_MAX &&
                (act.empty() || ws_active + batch_working_set(in[order[q]].nx, in[order[q]].ny) <= budget)) {
             rc = start(order[q++]);
             if (rc) return rc;
         }
-        // one PT iteration of every active instance
-        wa.n_active = (int)act.size();
-        int tiles = 0;
+        // every active instance advances by two PT iterations in one pass where iterations iter
+        // and iter+1 are plain (as in sf_solve_stokes), else by one; the two kinds are two launches
+        wa.n_active = 0;
+        w2.n_active = 0;
+        int tiles = 0, tiles2 = 0;
         for (size_t k = 0; k < act.size(); ++k) {
             Run& r = act[k];
+            const sf_batch_instance& a = in[r.id];
             const bool check = (r.iter % r.ncheck) == 0;
-            const bool last = r.iter == in[r.id].iter_max;
-            const int flags = ((check || last) ? PT_WRITE_TAU : 0) | (check ? PT_WRITE_RX : 0);
-            wa.tile_start[k] = tiles;
-            wa.slot_inst[k] = r.id;
-            wa.slot_pf[k] = (int)((r.iter - 1) & 1) | (flags << 1);
-            tiles += desc[(size_t)r.id].pt[0].ntiles;
+            const bool last = r.iter == a.iter_max;
+            r.pair = r.iter + 1 < a.iter_max && !check && (r.iter + 1) % r.ncheck != 0 &&
+                     !(a.use_fss && r.iter % r.ndtupdate == 0);
             r.checking = check;
+            const int par = (int)(r.nlaunch & 1);
+            if (r.pair) {
+                w2.tile_start[w2.n_active] = tiles2;
+                w2.slot_inst[w2.n_active] = r.id;
+                w2.slot_pf[w2.n_active] = par;
+                tiles2 += desc[(size_t)r.id].wpt[2].ntiles;
+                w2.n_active += 1;
+            } else {
+                const int flags = ((check || last) ? PT_WRITE_TAU : 0) | (check ? PT_WRITE_RX : 0);
+                wa.tile_start[wa.n_active] = tiles;
+                wa.slot_inst[wa.n_active] = r.id;
+                wa.slot_pf[wa.n_active] = par | (flags << 1);
+                tiles += desc[(size_t)r.id].wpt[0].ntiles;
+                wa.n_active += 1;
+            }
         }
-        wa.tile_start[act.size()] = tiles;
-        HIPCHK(launch_wide_pt(wa, depth, s));
+        wa.tile_start[wa.n_active] = tiles;
+        w2.tile_start[w2.n_active] = tiles2;
+        if (w2.n_active) HIPCHK(launch_wide_pt2(w2, s));
+        if (wa.n_active) HIPCHK(launch_wide_pt(wa, depth, s));
         bool any_check = false;
         for (Run& r : act) {
             const sf_batch_instance& a = in[r.id];
             const BatchInstance& d = desc[(size_t)r.id];
-            const PTParams& p = d.pt[(r.iter - 1) & 1];  // its outputs are now the current state
-            const bool dt_upd = a.use_fss && (r.iter % r.ndtupdate) == 0;
+            const PTParams& p = d.wpt[r.nlaunch & 1];  // its outputs are now the current state
+            const int64_t it_done = r.pair ? r.iter + 1 : r.iter;  // the last iteration just run
+            const bool dt_upd = a.use_fss && (it_done % r.ndtupdate) == 0;
             if (dt_upd) {
                 rc = enqueue_dt_raw(s, a.nx, a.ny, p.vx_b, p.vy_b, a.dx, a.dy, a.maxdisp, 1, env.part, d.scal);
                 if (rc) return rc;
@@ -279,7 +322,8 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         keep.reserve(act.size());
         for (Run& r : act) {
             if (r.checking) r.err = env.hscal[(size_t)SC_COUNT * r.id + SC_ERR];
-            r.iter += 1;
+            r.iter += r.pair ? 2 : 1;
+            r.nlaunch += 1;
             const sf_batch_instance& a = in[r.id];
             if (r.err > a.eps && r.iter <= a.iter_max) {
                 keep.push_back(r);
@@ -326,6 +370,8 @@ RefineBufs refine_carve(int nx_, int ny_, float* q, double* part)
     b.fx = q; q += nmx;
     b.fy = q; q += nmy;
     b.fc = q; q += np;
+    b.mx2 = q; q += nmx;
+    b.my2 = q; q += nmy;
     b.part = part;
     return b;
 }
@@ -334,7 +380,7 @@ RefineBufs refine_carve(int nx_, int ny_, float* q, double* part)
 static double refine_working_set(int nx, int ny) { return 4.0 * (double)refine_floats(nx, ny); }
 
 // The FP32 correction descriptors of one instance: [0] runs vx[0] -> vx[1], [1] back.
-void refine_descriptors(const sf_batch_instance& a, const RefineBufs& b, int rows, CorrParams out[2])
+void refine_descriptors(const sf_batch_instance& a, const RefineBufs& b, int rows, CorrParams out[CDESC_PER_INST])
 {
     const SolveConsts k = solve_consts(a.nx, a.ny, a.dx, a.dy);
     CorrParams cp{};
@@ -361,6 +407,9 @@ void refine_descriptors(const sf_batch_instance& a, const RefineBufs& b, int row
     cp.vsc_mdxy2 = (float)k.vsc_mdxy2;
     cp.ptsc41 = (float)k.ptsc41;
     cp.max_nxy = (float)k.max_nxy;
+    float* const mx[2] = {b.mx, b.mx2};
+    float* const my[2] = {b.my, b.my2};
+    const int rows2 = corr2_default_rows(a.nx, a.ny);
     for (int d = 0; d < 2; ++d) {
         out[d] = cp;
         out[d].vx_a = b.vx[d];
@@ -369,6 +418,14 @@ void refine_descriptors(const sf_batch_instance& a, const RefineBufs& b, int row
         out[d].vx_b = b.vx[1 - d];
         out[d].vy_b = b.vy[1 - d];
         out[d].p_b = b.p[1 - d];
+        out[d].mx = mx[d];
+        out[d].my = my[d];
+        out[d].mx_o = mx[1 - d];
+        out[d].my_o = my[1 - d];
+        out[2 + d] = out[d];
+        out[2 + d].seg = rows2;
+        out[2 + d].nstrips = corr2_num_strips(a.nx);
+        out[2 + d].ntiles = out[2 + d].nstrips * ((a.ny + rows2 - 1) / rows2);
     }
 }
 
@@ -392,7 +449,7 @@ static int batch_refine_reserve(sf_ctx* c, int n, size_t nfloats)
         c->b_hdt = nullptr;
         c->b_rcap = 0;
         HIPCHK(dalloc(&c->b_part, (size_t)3 * RED_MAX_BLOCKS * n));
-        HIPCHK(dalloc(&c->b_cdesc, 2 * (size_t)n));
+        HIPCHK(dalloc(&c->b_cdesc, CDESC_PER_INST * (size_t)n));
         HIPCHK(hipHostMalloc((void**)&c->b_hpart, sizeof(double) * 3 * RED_MAX_BLOCKS * (size_t)n, hipHostMallocDefault));
         HIPCHK(hipHostMalloc((void**)&c->b_hdt, sizeof(float) * (size_t)n, hipHostMallocDefault));
         c->b_rcap = n;
@@ -528,14 +585,16 @@ int batch_run_wide_refined(const RefineEnv& env, const sf_batch_instance* in, in
             }
             if (r.n32 > 0 && r.n32 % r.ncheck == 0 && r.n32 != r.checked_at && r.err_k <= a.eps) {
                 r.checked_at = r.n32;
-                HIPCHK(launch_cast_f64(a.dVxdtau, b.mx, nmx, s));
-                HIPCHK(launch_cast_f64(a.dVydtau, b.my, nmy, s));
+                float* const mxc = r.cur ? b.mx2 : b.mx;
+                float* const myc = r.cur ? b.my2 : b.my;
+                HIPCHK(launch_cast_f64(a.dVxdtau, mxc, nmx, s));
+                HIPCHK(launch_cast_f64(a.dVydtau, myc, nmy, s));
                 HIPCHK(launch_set_scalar(d.scal + SC_DT, r.dt_k, s));
                 rc = instance_check_iteration(s, a, d, b.part, env.w.hscal + (size_t)SC_COUNT * r.id, &r.err_ref);
                 if (rc) return rc;
                 r.n64 += 1;
-                HIPCHK(launch_cast_f32(b.mx, a.dVxdtau, nmx, s));
-                HIPCHK(launch_cast_f32(b.my, a.dVydtau, nmy, s));
+                HIPCHK(launch_cast_f32(mxc, a.dVxdtau, nmx, s));
+                HIPCHK(launch_cast_f32(myc, a.dVydtau, nmy, s));
                 if (r.err_ref <= a.eps) {
                     r.converged = true;
                     *done = true;
@@ -551,12 +610,12 @@ int batch_run_wide_refined(const RefineEnv& env, const sf_batch_instance* in, in
                 HIPCHK(launch_fill_f32(b.p[q], np, 0.f, s));
             }
             const float dtf = (float)r.dt_k;
-            if (cdesc[2 * (size_t)r.id].dt != dtf) {
-                cdesc[2 * (size_t)r.id].dt = dtf;
-                cdesc[2 * (size_t)r.id + 1].dt = dtf;
+            if (cdesc[CDESC_PER_INST * (size_t)r.id].dt != dtf) {
+                for (int q = 0; q < CDESC_PER_INST; ++q) cdesc[CDESC_PER_INST * (size_t)r.id + q].dt = dtf;
                 env.hdt[r.id] = dtf;
-                for (int q = 0; q < 2; ++q)
-                    HIPCHK(hipMemcpyAsync((char*)(env.cdesc_dev + 2 * (size_t)r.id + q) + offsetof(CorrParams, dt),
+                for (int q = 0; q < CDESC_PER_INST; ++q)
+                    HIPCHK(hipMemcpyAsync((char*)(env.cdesc_dev + CDESC_PER_INST * (size_t)r.id + q) +
+                                              offsetof(CorrParams, dt),
                                           env.hdt + r.id, sizeof(float), hipMemcpyHostToDevice, s));
             }
             int64_t steps = std::min(r.window, a.iter_max - r.n32);
@@ -564,7 +623,7 @@ int batch_run_wide_refined(const RefineEnv& env, const sf_batch_instance* in, in
             if (a.use_fss) steps = std::min(steps, r.ndtupdate - r.n32 % r.ndtupdate);
             r.left = steps;
             r.wstart = r.n32;
-            r.cur = 0;
+            // both correction buffers are zero, so the window starts at the momentum's parity
             return SF_OK;
         }
     };
@@ -592,8 +651,8 @@ int batch_run_wide_refined(const RefineEnv& env, const sf_batch_instance* in, in
         const BatchInstance& d = desc[(size_t)r.id];
         const RefineBufs& b = rb[(size_t)r.id];
         if (!r.converged) {
-            HIPCHK(launch_cast_f64(a.dVxdtau, b.mx, (int64_t)(a.nx - 2) * (a.ny - 1), s));
-            HIPCHK(launch_cast_f64(a.dVydtau, b.my, (int64_t)(a.nx - 1) * (a.ny - 2), s));
+            HIPCHK(launch_cast_f64(a.dVxdtau, r.cur ? b.mx2 : b.mx, (int64_t)(a.nx - 2) * (a.ny - 1), s));
+            HIPCHK(launch_cast_f64(a.dVydtau, r.cur ? b.my2 : b.my, (int64_t)(a.nx - 1) * (a.ny - 2), s));
         }
         HIPCHK(launch_pad_mirror(a.nx, a.ny, a.Vx_s, a.Vy_s, a.Vx_pad, a.Vy_pad, s));
         int rc = enqueue_dt_raw(s, a.nx, a.ny, a.Vx_s, a.Vy_s, a.dx, a.dy, a.maxdisp, 0, b.part, d.scal);
@@ -610,9 +669,11 @@ int batch_run_wide_refined(const RefineEnv& env, const sf_batch_instance* in, in
     std::vector<Run> act;
     size_t q = 0;
     double ws_active = 0.0;
-    WideCorrArgs wa;
+    WideCorrArgs wa, w2;
     std::memset(&wa, 0, sizeof(wa));
+    std::memset(&w2, 0, sizeof(w2));
     wa.desc = env.cdesc_dev;
+    w2.desc = env.cdesc_dev;
     int rc = SF_OK;
     while (!act.empty() || q < order.size()) {
         while (q < order.size() && (int)act.size() < WIDE_MAX &&
@@ -647,21 +708,35 @@ int batch_run_wide_refined(const RefineEnv& env, const sf_batch_instance* in, in
             }
         }
         if (act.empty()) continue;
-        wa.n_active = (int)act.size();
-        int tiles = 0;
+        // two sweeps per pass for every instance with two or more sweeps left in its window
+        wa.n_active = 0;
+        w2.n_active = 0;
+        int tiles = 0, tiles2 = 0;
         for (size_t k = 0; k < act.size(); ++k) {
-            wa.tile_start[k] = tiles;
-            wa.slot_desc[k] = 2 * act[k].id + act[k].cur;
-            tiles += cdesc[2 * (size_t)act[k].id].ntiles;
+            const size_t base = CDESC_PER_INST * (size_t)act[k].id;
+            if (act[k].left >= 2) {
+                w2.tile_start[w2.n_active] = tiles2;
+                w2.slot_desc[w2.n_active] = (int)base + 2 + act[k].cur;
+                tiles2 += cdesc[base + 2].ntiles;
+                w2.n_active += 1;
+            } else {
+                wa.tile_start[wa.n_active] = tiles;
+                wa.slot_desc[wa.n_active] = (int)base + act[k].cur;
+                tiles += cdesc[base].ntiles;
+                wa.n_active += 1;
+            }
         }
-        wa.tile_start[act.size()] = tiles;
-        HIPCHK(launch_wide_corr(wa, s));
+        wa.tile_start[wa.n_active] = tiles;
+        w2.tile_start[w2.n_active] = tiles2;
+        if (w2.n_active) HIPCHK(launch_wide_corr2(w2, s));
+        if (wa.n_active) HIPCHK(launch_wide_corr(wa, s));
         std::vector<Run> keep;
         keep.reserve(act.size());
         for (Run& r : act) {
+            const int64_t adv = r.left >= 2 ? 2 : 1;
             r.cur ^= 1;
-            r.n32 += 1;
-            r.left -= 1;
+            r.n32 += adv;
+            r.left -= adv;
             bool done = false;
             if (r.left == 0) {
                 rc = end_window(r);
@@ -758,13 +833,14 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
         rc = batch_refine_reserve(c, n, nf);
         if (rc) return rc;
         std::vector<RefineBufs> rb((size_t)n);
-        std::vector<CorrParams> cdesc(2 * (size_t)n);
+        std::vector<CorrParams> cdesc(CDESC_PER_INST * (size_t)n);
         for (int i = 0; i < n; ++i) {
             rb[(size_t)i] = refine_carve(in[i].nx, in[i].ny, c->b_pool32 + foff[(size_t)i],
                                          c->b_part + (size_t)3 * RED_MAX_BLOCKS * i);
-            refine_descriptors(in[i], rb[(size_t)i], prm ? prm->rows : 0, &cdesc[2 * (size_t)i]);
+            refine_descriptors(in[i], rb[(size_t)i], prm ? prm->rows : 0, &cdesc[CDESC_PER_INST * (size_t)i]);
         }
-        HIPCHK(hipMemcpyAsync(c->b_cdesc, cdesc.data(), sizeof(CorrParams) * 2 * (size_t)n, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->b_cdesc, cdesc.data(), sizeof(CorrParams) * CDESC_PER_INST * (size_t)n,
+                              hipMemcpyHostToDevice, s));
         const RefineEnv env{{s, c->b_inst, c->part, c->b_hscal}, &rb, c->b_cdesc, c->b_hpart, c->b_hdt, 0};
         rc = batch_run_wide_refined(env, in, n, host, cdesc, order, budget, res);
         if (rc) return rc;

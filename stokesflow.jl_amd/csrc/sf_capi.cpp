@@ -32,7 +32,7 @@ static void free_ctx(sf_ctx* c)
                     c->m2g.rec_dx, c->m2g.rec_dy, c->m2g.rec_val, c->m2g.blocksums, c->m2g.counters};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
-    void* rbufs[] = {c->r_vx[0], c->r_vx[1], c->r_vy[0], c->r_vy[1], c->r_p[0], c->r_p[1], c->r_mx, c->r_my,
+    void* rbufs[] = {c->r_mx2, c->r_my2, c->r_vx[0], c->r_vx[1], c->r_vy[0], c->r_vy[1], c->r_p[0], c->r_p[1], c->r_mx, c->r_my,
                      c->r_rho, c->r_mub, c->r_mup, c->r_fx, c->r_fy, c->r_fc, c->r_rxo, c->r_ryo, c->r_rco};
     for (void* b : rbufs)
         if (b) (void)hipFree(b);
@@ -175,6 +175,8 @@ struct PTLoop {
         pp.p_b = nxt_p;
         pp.dvx = cur_dvx;
         pp.dvy = cur_dvy;
+        pp.dvx_o = nullptr;  // in place
+        pp.dvy_o = nullptr;
         HIPCHK(launch_pt_fused(pp, s));
         std::swap(cur_vx, nxt_vx);
         std::swap(cur_vy, nxt_vy);
@@ -1082,10 +1084,16 @@ extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, dou
     CorrParams cp{};
     cp.nx = nx;
     cp.ny = ny;
-    cp.seg = c->seg > 0 ? c->seg : 4;  // the context's tile rows, when set
+    const int corr_seg = c->seg > 0 ? c->seg : 4;  // the context's tile rows, when set
     cp.depth = c->depth > 1 ? 2 : 1;
-    cp.mx = c->r_mx;
-    cp.my = c->r_my;
+    // two sweeps per pass unless the context asks for one iteration per launch; the pass
+    // writes the momentum to a second buffer, and the context's pointers are swapped after
+    // each pass so that r_mx / r_my always hold the current momentum
+    const bool corr_pairs = c->fuse != 1;
+    if (corr_pairs && !c->r_mx2) {
+        HIPCHK(dalloc(&c->r_mx2, (size_t)(nx - 2) * (ny - 1)));
+        HIPCHK(dalloc(&c->r_my2, (size_t)(nx - 1) * (ny - 2)));
+    }
     cp.rho = c->r_rho;
     cp.mu_b = c->r_mub;
     cp.mu_p = c->r_mup;
@@ -1177,13 +1185,36 @@ extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, dou
         int64_t steps = std::min(rebase, iter_max - n32);
         steps = std::min(steps, ncheck - n32 % ncheck);
         if (use_fss) steps = std::min(steps, ndtupdate - n32 % ndtupdate);
-        for (int64_t it = 0; it < steps; ++it) {
+        int64_t it = 0;
+        for (; corr_pairs && it + 2 <= steps; it += 2) {
             cp.vx_a = c->r_vx[cur];
             cp.vy_a = c->r_vy[cur];
             cp.p_a = c->r_p[cur];
             cp.vx_b = c->r_vx[1 - cur];
             cp.vy_b = c->r_vy[1 - cur];
             cp.p_b = c->r_p[1 - cur];
+            cp.mx = c->r_mx;
+            cp.my = c->r_my;
+            cp.mx_o = c->r_mx2;
+            cp.my_o = c->r_my2;
+            cp.seg = 0;
+            HIPCHK(launch_pt_corr2(cp, s));
+            cur = 1 - cur;
+            std::swap(c->r_mx, c->r_mx2);
+            std::swap(c->r_my, c->r_my2);
+        }
+        for (; it < steps; ++it) {
+            cp.vx_a = c->r_vx[cur];
+            cp.vy_a = c->r_vy[cur];
+            cp.p_a = c->r_p[cur];
+            cp.vx_b = c->r_vx[1 - cur];
+            cp.vy_b = c->r_vy[1 - cur];
+            cp.p_b = c->r_p[1 - cur];
+            cp.mx = c->r_mx;
+            cp.my = c->r_my;
+            cp.mx_o = nullptr;  // in place
+            cp.my_o = nullptr;
+            cp.seg = corr_seg;
             HIPCHK(launch_pt_corr(cp, false, s));
             cur = 1 - cur;
         }

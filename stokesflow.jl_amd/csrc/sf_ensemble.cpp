@@ -238,7 +238,7 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
         }
         er = dalloc(&e->pool32, nf);
         if (er == hipSuccess) er = dalloc(&e->rpart, (size_t)3 * RED_MAX_BLOCKS * n);
-        if (er == hipSuccess) er = dalloc(&e->cdesc_dev, 2 * (size_t)n);
+        if (er == hipSuccess) er = dalloc(&e->cdesc_dev, CDESC_PER_INST * (size_t)n);
         if (er == hipSuccess)
             er = hipHostMalloc((void**)&e->hpart, sizeof(double) * 3 * RED_MAX_BLOCKS * (size_t)n, hipHostMallocDefault);
         if (er == hipSuccess) er = hipHostMalloc((void**)&e->hdt, sizeof(float) * (size_t)n, hipHostMallocDefault);
@@ -247,13 +247,13 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
             return set_err(SF_ERR_HIP, "%s: %s", who, hipGetErrorString(er));
         }
         e->rb.resize((size_t)n);
-        e->cdesc.resize(2 * (size_t)n);
+        e->cdesc.resize(CDESC_PER_INST * (size_t)n);
         for (int i = 0; i < n; ++i) {
             e->rb[(size_t)i] = refine_carve(members[i].nx, members[i].ny, e->pool32 + foff[(size_t)i],
                                             e->rpart + (size_t)3 * RED_MAX_BLOCKS * i);
-            refine_descriptors(bi[(size_t)i], e->rb[(size_t)i], prm ? prm->rows : 0, &e->cdesc[2 * (size_t)i]);
+            refine_descriptors(bi[(size_t)i], e->rb[(size_t)i], prm ? prm->rows : 0, &e->cdesc[CDESC_PER_INST * (size_t)i]);
         }
-        er = hipMemcpyAsync(e->cdesc_dev, e->cdesc.data(), sizeof(CorrParams) * 2 * (size_t)n, hipMemcpyHostToDevice,
+        er = hipMemcpyAsync(e->cdesc_dev, e->cdesc.data(), sizeof(CorrParams) * CDESC_PER_INST * (size_t)n, hipMemcpyHostToDevice,
                             e->stream);
         if (er != hipSuccess) {
             free_ensemble(e);

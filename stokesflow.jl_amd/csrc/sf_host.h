@@ -58,6 +58,7 @@ struct WideEnv {
 // Per-instance FP32 buffers of the refined wide mode (the layout of sf_solve_stokes_refined's).
 struct RefineBufs {
     float *vx[2], *vy[2], *p[2], *mx, *my, *rho, *mub, *mup, *fx, *fy, *fc;
+    float *mx2, *my2;  // second momentum (ping-pong with the state in the wide mode)
     double* part;  // 3 * RED_MAX_BLOCKS residual partial sums
 };
 
@@ -80,7 +81,7 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n, const
                    const std::vector<int>& order, double budget, int depth, std::vector<BatchResult>& res);
 size_t refine_floats(int nx, int ny);
 RefineBufs refine_carve(int nx, int ny, float* q, double* part);
-void refine_descriptors(const sf_batch_instance& a, const RefineBufs& b, int rows, CorrParams out[2]);
+void refine_descriptors(const sf_batch_instance& a, const RefineBufs& b, int rows, CorrParams out[CDESC_PER_INST]);
 int batch_run_wide_refined(const RefineEnv& env, const sf_batch_instance* in, int n,
                            const std::vector<BatchInstance>& desc, std::vector<CorrParams>& cdesc,
                            const std::vector<int>& order, double budget, std::vector<BatchResult>& res);
@@ -113,6 +114,7 @@ struct sf_ctx {
     float *r_vx[2] = {nullptr, nullptr}, *r_vy[2] = {nullptr, nullptr}, *r_p[2] = {nullptr, nullptr};
     float *r_mx = nullptr, *r_my = nullptr, *r_rho = nullptr, *r_mub = nullptr, *r_mup = nullptr;
     float *r_fx = nullptr, *r_fy = nullptr, *r_fc = nullptr, *r_rxo = nullptr, *r_ryo = nullptr, *r_rco = nullptr;
+    float *r_mx2 = nullptr, *r_my2 = nullptr;  // second momentum (two-sweep pass)
     double* hpart = nullptr;  // pinned host copy of reduction partials
     // batched ensemble scratch
     double* b_pool = nullptr;
@@ -128,7 +130,7 @@ struct sf_ctx {
     float* b_pool32 = nullptr;
     size_t b_pool32_cap = 0;
     double* b_part = nullptr;      // 3 * RED_MAX_BLOCKS per instance
-    sf::CorrParams* b_cdesc = nullptr;  // 2 per instance
+    sf::CorrParams* b_cdesc = nullptr;  // CDESC_PER_INST per instance
     double* b_hpart = nullptr;     // pinned
     float* b_hdt = nullptr;        // pinned
     int b_rcap = 0;

@@ -95,6 +95,8 @@ struct CorrParams {
     float* p_b;
     float* mx;  // PT momentum of the correction (in place)
     float* my;
+    float* mx_o;  // two-sweep pass: momentum out (mx, my are then read only)
+    float* my_o;
     const float* rho;  // FP32 materials
     const float* mu_b;
     const float* mu_p;
@@ -129,6 +131,9 @@ struct RedParamsF {
 };
 
 hipError_t launch_pt_corr(const CorrParams& p, bool check, hipStream_t s);
+// Two correction sweeps per pass: (vx_a, vy_a, p_a, mx, my) -> (vx_b, vy_b, p_b, mx_o, my_o);
+// p.seg = rows per tile (<= 0: default).  Bit-identical to two launch_pt_corr.
+hipError_t launch_pt_corr2(const CorrParams& p, hipStream_t s);
 hipError_t launch_resid_f64(const ResidParams& p, int nb, hipStream_t s);
 hipError_t launch_update_f64(double* a, const float* d, int64_t n, hipStream_t s);
 hipError_t launch_cast_f32(float* dst, const double* src, int64_t n, hipStream_t s);
@@ -147,7 +152,8 @@ hipError_t launch_probe_r3w1(const double* a, const double* b, const double* c, 
 // runs it back (tile parameters complete, flags unused); the rest is what the solve loop
 // and the epilogue need.
 struct BatchInstance {
-    PTParams pt[2];
+    PTParams pt[2];   // one iteration, state ping-pong by parity, dV in place (persistent mode)
+    PTParams wpt[4];  // wide mode, dV ping-pong too: [parity] one iteration, [2 + parity] two
     double *P, *Vx_s, *Vy_s, *Vx_pad, *Vy_pad;  // caller state (in/out)
     double *ry, *scal;                          // library scratch (scal[SC_DT] = stabilisation dt)
     const double* dt_dev;                       // if set, the input dt is read here (device) instead of dt_in
@@ -171,7 +177,10 @@ struct WideArgs {
     int slot_inst[WIDE_MAX];       // descriptor index
     int slot_pf[WIDE_MAX];         // ping-pong parity | PT_WRITE_* flags << 1
 };
+// One PT iteration of every slot (descriptor wpt[parity]).
 hipError_t launch_wide_pt(const WideArgs& a, int depth, hipStream_t s);
+// Two PT iterations of every slot (descriptor wpt[2 + parity]; flags must be 0).
+hipError_t launch_wide_pt2(const WideArgs& a, hipStream_t s);
 
 // Wide mode of the FP32 correction sweeps: slot k runs descriptor slot_desc[k] (two
 // descriptors per instance, one per ping-pong direction).
@@ -182,7 +191,13 @@ struct WideCorrArgs {
     int slot_desc[WIDE_MAX];
 };
 hipError_t launch_wide_corr(const WideCorrArgs& a, hipStream_t s);
+hipError_t launch_wide_corr2(const WideCorrArgs& a, hipStream_t s);  // two sweeps per slot
 int corr_num_strips(int nx);
+int corr2_num_strips(int nx);
+int corr2_default_rows(int nx, int ny);
+// Correction descriptors per instance in the wide refined mode: [parity] one sweep,
+// [2 + parity] two sweeps; the state and the momentum ping-pong together.
+#define CDESC_PER_INST 4
 
 hipError_t launch_batch_solve(const BatchInstance* inst, const int* order, int n_inst, int* queue,
                               BatchResult* res, int threads, int nblocks, hipStream_t s);

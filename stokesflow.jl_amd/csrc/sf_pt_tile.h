@@ -79,6 +79,9 @@ __device__ __forceinline__ void pt_tile(const PP& p, int strip, int segid, int f
     const bool c_vx = c < nx;      // Vx_s, mu_b, txy columns
     const bool c_p = c < nx - 1;   // P, mu_p columns
     const double rdx = p.rdx, rdy = p.rdy;
+    // dV is updated in place unless a second buffer is given (ping-pong in the wide batched mode)
+    double* const dvx_w = p.dvx_o ? p.dvx_o : p.dvx;
+    double* const dvy_w = p.dvy_o ? p.dvy_o : p.dvy;
     LaneCols lc;
     lc.vx = (unsigned)min(c, nx - 1);
     lc.vy = (unsigned)min(c, nx);
@@ -175,7 +178,7 @@ __device__ __forceinline__ void pt_tile(const PP& p, int strip, int segid, int f
             const double vy_new = vy_r_p1 + dtauvy * dvy_new;
 
             if (dvx_ok) {
-                p.dvx[ix2(c, j, nx - 2)] = dvx_new;
+                dvx_w[ix2(c, j, nx - 2)] = dvx_new;
                 p.vx_b[ix2(c + 1, j + 1, lvx)] = vx_new;
                 if (j == 0) p.vx_b[ix2(c + 1, 0, lvx)] = vx_new;        // bc_y_noflux!
                 if (j == ny - 2) p.vx_b[ix2(c + 1, ny, lvx)] = vx_new;
@@ -189,7 +192,7 @@ __device__ __forceinline__ void pt_tile(const PP& p, int strip, int segid, int f
                 if (j == ny - 2) p.vx_b[ix2(col, ny, lvx)] = 0.0;
             }
             if (dvy_ok) {
-                p.dvy[ix2(c, j, nx - 1)] = dvy_new;
+                dvy_w[ix2(c, j, nx - 1)] = dvy_new;
                 p.vy_b[ix2(c + 1, j + 1, lvy)] = vy_new;
                 if (c == 0) p.vy_b[ix2(0, j + 1, lvy)] = vy_new;         // bc_x_noflux!
                 if (c == nx - 2) p.vy_b[ix2(nx, j + 1, lvy)] = vy_new;

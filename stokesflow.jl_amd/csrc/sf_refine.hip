@@ -82,6 +82,9 @@ __device__ __forceinline__ void corr_tile(const PP& p, int strip, int segid)
     const int64_t lvx = nx, lvy = nx + 1, lp = nx - 1;
     const bool c_p = c < nx - 1;
     const float rdx = p.rdx, rdy = p.rdy;
+    // momentum in place unless a second buffer is given (ping-pong in the wide batched mode)
+    float* const mx_w = p.mx_o ? p.mx_o : p.mx;
+    float* const my_w = p.my_o ? p.my_o : p.my;
     CorrCols lc;
     lc.vx = (unsigned)min(c, nx - 1);
     lc.vy = (unsigned)min(c, nx);
@@ -156,7 +159,7 @@ __device__ __forceinline__ void corr_tile(const PP& p, int strip, int segid)
             const float vx_new = vx_r_p1 + dtauvx * mx_new;
             const float vy_new = vy_r_p1 + dtauvy * my_new;
             if (dvx_ok) {
-                p.mx[ix2(c, j, nx - 2)] = mx_new;
+                mx_w[ix2(c, j, nx - 2)] = mx_new;
                 p.vx_b[ix2(c + 1, j + 1, lvx)] = vx_new;
                 if (j == 0) p.vx_b[ix2(c + 1, 0, lvx)] = vx_new;
                 if (j == ny - 2) p.vx_b[ix2(c + 1, ny, lvx)] = vx_new;
@@ -169,7 +172,7 @@ __device__ __forceinline__ void corr_tile(const PP& p, int strip, int segid)
                 if (j == ny - 2) p.vx_b[ix2(col, ny, lvx)] = 0.f;
             }
             if (dvy_ok) {
-                p.my[ix2(c, j, nx - 1)] = my_new;
+                my_w[ix2(c, j, nx - 1)] = my_new;
                 p.vy_b[ix2(c + 1, j + 1, lvy)] = vy_new;
                 if (c == 0) p.vy_b[ix2(0, j + 1, lvy)] = vy_new;
                 if (c == nx - 2) p.vy_b[ix2(nx, j + 1, lvy)] = vy_new;
@@ -202,6 +205,238 @@ __device__ __forceinline__ void corr_tile(const PP& p, int strip, int segid)
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Two correction sweeps per pass (temporal blocking), the FP32 counterpart of pt2_tile
+// (sf_pt2_tile.h, which explains the lane and row scheme): sweep A marches as corr_tile does
+// and keeps its rows in registers, sweep B consumes them one row behind and stores.  Both
+// sweeps evaluate corr_tile's expressions (corr_row), so the result is bit-identical to two
+// pt_corr_f32 launches.  The momentum goes to a second buffer (mx_o, my_o).
+
+#define CORR2_OUT_LANES 60
+
+__device__ inline float lane_up_f(float v, int d) { return __shfl_up(v, (unsigned)d, SF_WAVE); }
+
+struct CorrWin {
+    float vx_r, vx_r_p1, vy_r, vy_r_p1;
+    float rho_r, rho_r_p1, rho_r_p2, rho_rm1_p1;
+    float pn_prev, tyy_prev, txx_prev, pn_prev_p1, txx_prev_p1, txy_prev_p1, mup_prev, mup_prev_p1;
+};
+
+struct CorrRow {
+    float pn, mx, my, vx, vy;
+};
+
+__device__ __forceinline__ void corr_win_init(CorrWin& w, float vx_r, float vy_r, float rho_r)
+{
+    w.vx_r = vx_r;
+    w.vy_r = vy_r;
+    w.rho_r = rho_r;
+    w.vx_r_p1 = lane_down(vx_r, 1);
+    w.vy_r_p1 = lane_down(vy_r, 1);
+    w.rho_r_p1 = lane_down(rho_r, 1);
+    w.rho_r_p2 = lane_down(rho_r, 2);
+    w.rho_rm1_p1 = 0.f;
+    w.pn_prev = w.tyy_prev = w.txx_prev = 0.f;
+    w.pn_prev_p1 = w.txx_prev_p1 = w.txy_prev_p1 = 0.f;
+    w.mup_prev = w.mup_prev_p1 = 0.f;
+}
+
+// One row step of one correction sweep with corr_tile's expressions, then the window slides.
+template <class PP>
+__device__ __forceinline__ CorrRow corr_row(const PP& p, float dt, CorrWin& w, const CorrRowIn& in)
+{
+    const float rdx = p.rdx, rdy = p.rdy;
+    const float vx_r1_p1 = lane_down(in.vx1, 1);
+    const float vy_r1_p1 = lane_down(in.vy1, 1);
+    const float rho_r1_p1 = lane_down(in.rho1, 1);
+    const float rho_r1_p2 = lane_down(in.rho1, 2);
+
+    const float dvxdx = (vx_r1_p1 - in.vx1) * rdx;
+    const float dvydy = (vy_r1_p1 - w.vy_r_p1) * rdy;
+    const float div = dvxdx + dvydy;
+    const float dtaup = (p.ptsc41 * in.mup) / p.max_nxy;
+    const float cres = div + in.fc;
+    const float pn = in.pold - dtaup * cres;
+    const float txx = (2.0f * in.mup) * (dvxdx - (1.0f / 3.0f) * div);
+    const float tyy = (2.0f * in.mup) * (dvydy - (1.0f / 3.0f) * div);
+    const float txy = in.mub * ((in.vx1 - w.vx_r) * rdy + (w.vy_r_p1 - w.vy_r) * rdx);
+    const float txy_p1 = lane_down(txy, 1);
+
+    const float rx = ((w.txx_prev_p1 - w.txx_prev) * rdx + (txy_p1 - w.txy_prev_p1) * rdy) -
+                     (w.pn_prev_p1 - w.pn_prev) * rdx + in.fx;
+    const float av = 0.25f * (((w.vx_r + w.vx_r_p1) + in.vx1) + vx_r1_p1);
+    const float d_xi = w.rho_r_p2 - w.rho_r;
+    const float d_yi = rho_r1_p1 - w.rho_rm1_p1;
+    const float ry = (((tyy - w.tyy_prev) * rdy + (txy_p1 - txy) * rdx) - (pn - w.pn_prev) * rdy) -
+                     p.g * (dt * (((av * d_xi) * 0.5f) * rdx + ((w.vy_r_p1 * d_yi) * 0.5f) * rdy)) + in.fy;
+    CorrRow o;
+    o.pn = pn;
+    o.mx = p.dampx * in.mx + rx;
+    o.my = p.dampy * in.my + ry;
+    const float dtauvx = (p.vsc_mdxy2 / ((w.mup_prev + w.mup_prev_p1) * 0.5f)) / 4.1f;
+    const float dtauvy = (p.vsc_mdxy2 / ((w.mup_prev + in.mup) * 0.5f)) / 4.1f;
+    o.vx = w.vx_r_p1 + dtauvx * o.mx;
+    o.vy = w.vy_r_p1 + dtauvy * o.my;
+
+    w.pn_prev_p1 = lane_down(pn, 1);
+    w.txx_prev_p1 = lane_down(txx, 1);
+    w.mup_prev_p1 = lane_down(in.mup, 1);
+    w.pn_prev = pn;
+    w.txx_prev = txx;
+    w.tyy_prev = tyy;
+    w.txy_prev_p1 = txy_p1;
+    w.mup_prev = in.mup;
+    w.vx_r = in.vx1;
+    w.vx_r_p1 = vx_r1_p1;
+    w.vy_r = in.vy1;
+    w.vy_r_p1 = vy_r1_p1;
+    w.rho_rm1_p1 = w.rho_r_p1;
+    w.rho_r = in.rho1;
+    w.rho_r_p1 = rho_r1_p1;
+    w.rho_r_p2 = rho_r1_p2;
+    return o;
+}
+
+// Stores of sweep B's row step r, with corr_tile's conditions and (homogeneous) BC writes.
+template <class PP>
+__device__ __forceinline__ void corr2_store(const PP& p, const CorrRow& o, int c, int r, int r0, bool out_lane)
+{
+    const int nx = p.nx, ny = p.ny;
+    const int64_t lvx = nx, lvy = nx + 1, lp = nx - 1;
+    if (!out_lane || r < r0) return;
+    if (c < nx - 1 && r <= ny - 2) p.p_b[ix2(c, r, lp)] = o.pn;
+    const int j = r - 1;
+    if (j < 0) return;
+    if (c < nx - 2) {
+        p.mx_o[ix2(c, j, nx - 2)] = o.mx;
+        p.vx_b[ix2(c + 1, j + 1, lvx)] = o.vx;
+        if (j == 0) p.vx_b[ix2(c + 1, 0, lvx)] = o.vx;
+        if (j == ny - 2) p.vx_b[ix2(c + 1, ny, lvx)] = o.vx;
+    }
+    if (c == 0 || c == nx - 2) {
+        const int col = (c == 0) ? 0 : nx - 1;
+        p.vx_b[ix2(col, j + 1, lvx)] = 0.f;
+        if (j == 0) p.vx_b[ix2(col, 0, lvx)] = 0.f;
+        if (j == ny - 2) p.vx_b[ix2(col, ny, lvx)] = 0.f;
+    }
+    if (c < nx - 1 && j <= ny - 3) {
+        p.my_o[ix2(c, j, nx - 1)] = o.my;
+        p.vy_b[ix2(c + 1, j + 1, lvy)] = o.vy;
+        if (c == 0) p.vy_b[ix2(0, j + 1, lvy)] = o.vy;
+        if (c == nx - 2) p.vy_b[ix2(nx, j + 1, lvy)] = o.vy;
+    }
+    if (c < nx - 1 && (j == 0 || j == ny - 2)) {
+        const int row = (j == 0) ? 0 : ny - 1;
+        p.vy_b[ix2(c + 1, row, lvy)] = 0.f;
+        if (c == 0) p.vy_b[ix2(0, row, lvy)] = 0.f;
+        if (c == nx - 2) p.vy_b[ix2(nx, row, lvy)] = 0.f;
+    }
+}
+
+// Two correction sweeps over the tile (strip, segid): state (vx_a, vy_a, p_a, mx, my) ->
+// (vx_b, vy_b, p_b, mx_o, my_o).
+template <class PP>
+__device__ __forceinline__ void corr2_tile(const PP& p, int strip, int segid)
+{
+    const int lane = threadIdx.x & 63;
+    const int c = strip * CORR2_OUT_LANES + lane - 1;
+    const int nx = p.nx, ny = p.ny;
+    const bool out_lane = lane >= 1 && lane <= CORR2_OUT_LANES;
+    const int r0 = segid * p.seg;
+    const int r1 = min(r0 + p.seg, ny);
+    const float dt = p.dt;
+    const int cc = c < 0 ? 0 : c;
+    CorrCols lc;
+    lc.vx = (unsigned)min(cc, nx - 1);
+    lc.vy = (unsigned)min(cc, nx);
+    lc.p = (unsigned)min(cc, nx - 2);
+    lc.dvx = (unsigned)min(cc, nx - 3);
+    lc.dvy = (unsigned)min(cc, nx - 2);
+    const bool vx_zero = (c + 1 == 0) || (c + 1 == nx - 1);
+    const bool vy_left = c + 1 == 0, vy_right = c + 1 == nx;
+
+    const int ra0 = max(r0 - 2, 0);
+    const int ra1 = min(r1 + 1, ny);
+    const int rb0 = max(r0 - 1, 0);
+
+    CorrWin wa, wb;
+    int r = ra0;
+    corr_win_init(wa, (p.vx_a + (int64_t)r * nx)[lc.vx], (p.vy_a + (int64_t)r * (nx + 1))[lc.vy],
+                  (p.rho + (int64_t)r * (nx + 1))[lc.vy]);
+    CorrRowIn nxt = corr_load_row(p, lc, r, r >= r0 - 1);
+    float a_pn = 0.f, a_mup = 0.f, a_mub = 0.f, a_fc = 0.f, a_mx = 0.f, a_my = 0.f, a_fx = 0.f, a_fy = 0.f;
+    float a_rho = 0.f, a_vx = 0.f, a_vy = 0.f, a_vx_last = 0.f;
+
+    for (; r < ra1; ++r) {
+        const CorrRowIn in = nxt;
+        if (r + 1 < ra1) nxt = corr_load_row(p, lc, r + 1, true);
+        const float rho_row = wa.rho_r;
+        const CorrRow oa = corr_row(p, dt, wa, in);
+        const float vx_a = vx_zero ? 0.f : oa.vx;
+        const float vy_dn = lane_down(oa.vy, 1), vy_up = lane_up_f(oa.vy, 1);
+        const float vy_a = (r == ny - 1) ? 0.f : (vy_left ? vy_dn : (vy_right ? vy_up : oa.vy));
+        const float vx_in = lane_up_f(vx_a, 1);
+        const float vy_in = lane_up_f(vy_a, 1);
+
+        const int rb = r - 1;
+        if (rb >= rb0) {
+            if (rb == rb0) {
+                const float vx0 = (rb0 == 0) ? vx_in : lane_up_f(a_vx, 1);
+                const float vy0 = (rb0 == 0) ? 0.f : lane_up_f(a_vy, 1);
+                corr_win_init(wb, vx0, vy0, a_rho);
+            }
+            CorrRowIn ib;
+            ib.vx1 = vx_in;
+            ib.vy1 = vy_in;
+            ib.rho1 = rho_row;
+            ib.pold = a_pn;
+            ib.mup = a_mup;
+            ib.mub = a_mub;
+            ib.fc = a_fc;
+            ib.mx = a_mx;
+            ib.my = a_my;
+            ib.fx = a_fx;
+            ib.fy = a_fy;
+            const CorrRow ob = corr_row(p, dt, wb, ib);
+            corr2_store(p, ob, c, rb, r0, out_lane);
+        }
+        a_pn = oa.pn;
+        a_mup = in.mup;
+        a_mub = in.mub;
+        a_fc = in.fc;
+        a_mx = oa.mx;
+        a_my = oa.my;
+        a_fx = in.fx;
+        a_fy = in.fy;
+        a_rho = rho_row;
+        a_vx = vx_a;
+        a_vy = vy_a;
+        a_vx_last = vx_in;
+    }
+    if (r1 == ny) {
+        CorrRowIn ib;
+        ib.vx1 = a_vx_last;
+        ib.vy1 = 0.f;
+        ib.rho1 = a_rho;
+        ib.pold = a_pn;
+        ib.mup = a_mup;
+        ib.mub = a_mub;
+        ib.fc = a_fc;
+        ib.mx = a_mx;
+        ib.my = a_my;
+        ib.fx = a_fx;
+        ib.fy = a_fy;
+        const CorrRow ob = corr_row(p, dt, wb, ib);
+        corr2_store(p, ob, c, ny - 1, r0, out_lane);
+    }
+}
+
+__global__ __launch_bounds__(64) void pt_corr2_f32(CorrParams p)
+{
+    const int logical = xcd_remap_r(blockIdx.x, gridDim.x);
+    corr2_tile(p, logical % p.nstrips, logical / p.nstrips);
+}
+
 // One wave per tile, XCD-banded tile order (as the FP64 kernel).
 template <int DEPTH, bool CHECK>
 __global__ __launch_bounds__(64) void pt_corr_f32(CorrParams p)
@@ -229,6 +464,35 @@ __global__ __launch_bounds__(64) void wide_corr_kernel(WideCorrArgs a)
     const int nstrips = cp.nstrips;
     corr_tile<1, false>(cp, t % nstrips, t / nstrips);
 }
+
+// Two correction sweeps of every active instance per launch (corr2_tile).
+__global__ __launch_bounds__(64) void wide_corr2_kernel(WideCorrArgs a)
+{
+    const int total = a.tile_start[a.n_active];
+    const int t_glob = xcd_remap_r(blockIdx.x, gridDim.x);
+    if (t_glob >= total) return;
+    int lo = 0, hi = a.n_active - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.tile_start[mid] <= t_glob) lo = mid;
+        else hi = mid - 1;
+    }
+    const int t = t_glob - a.tile_start[lo];
+    typedef const CorrParams __attribute__((address_space(4))) CCorr;
+    CCorr& cp = ((CCorr*)a.desc)[a.slot_desc[lo]];
+    const int nstrips = cp.nstrips;
+    corr2_tile(cp, t % nstrips, t / nstrips);
+}
+
+hipError_t launch_wide_corr2(const WideCorrArgs& a, hipStream_t s)
+{
+    const int total = a.tile_start[a.n_active];
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(wide_corr2_kernel, dim3(total), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+int corr2_num_strips(int nx) { return (nx - 1 + CORR2_OUT_LANES - 1) / CORR2_OUT_LANES; }
 
 int corr_num_strips(int nx) { return (nx + CORR_OUT_LANES - 1) / CORR_OUT_LANES; }
 
@@ -370,6 +634,25 @@ hipError_t launch_pt_corr(const CorrParams& p0, bool check, hipStream_t s)
         if (check) hipLaunchKernelGGL((pt_corr_f32<1, true>), dim3(p.ntiles), dim3(64), 0, s, p);
         else hipLaunchKernelGGL((pt_corr_f32<1, false>), dim3(p.ntiles), dim3(64), 0, s, p);
     }
+    return hipGetLastError();
+}
+
+// Rows per tile of the two-sweep pass (as the FP64 pass: taller tiles cut the halo re-reads).
+int corr2_default_rows(int nx, int ny)
+{
+    const int n = std::max(nx, ny);
+    const int r = n > 2048 ? 16 : (n > 768 ? 12 : (n > 512 ? 8 : 4));
+    return ny >= r ? r : ny;
+}
+
+hipError_t launch_pt_corr2(const CorrParams& p0, hipStream_t s)
+{
+    CorrParams p = p0;
+    if (p.seg <= 0) p.seg = corr2_default_rows(p.nx, p.ny);
+    p.nstrips = (p.nx - 1 + CORR2_OUT_LANES - 1) / CORR2_OUT_LANES;
+    const int nseg = (p.ny + p.seg - 1) / p.seg;
+    p.ntiles = p.nstrips * nseg;
+    hipLaunchKernelGGL(pt_corr2_f32, dim3(p.ntiles), dim3(64), 0, s, p);
     return hipGetLastError();
 }
 

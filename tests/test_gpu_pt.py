@@ -249,6 +249,37 @@ def test_refined_plume(gpu, n, rebase):
         assert np.linalg.norm(a - S[k]) / np.linalg.norm(S[k]) <= 1e-4, k
 
 
+@pytest.mark.parametrize("n,rebase", [(61, 0), (96, 97), (130, 45)])
+def test_refined_pair_sweeps_bitwise(gpu, n, rebase):
+    """FP32 correction sweeps two per pass (default) and one per launch give the same bits:
+    windows of odd and even length (rebase 97 / 45 and ncheck), strips of 60 and 62 columns."""
+    dx, dy, x, y, rho, mu = cases.plume_markers(n, n + 3, 4, seed=78)
+    S = O.solver_arrays(n, n + 3)
+    vws, ws = O.zeros(n + 1, n + 3), O.zeros(n + 1, n + 3)
+    o = O.grid_origins(dx, dy)
+    O.marker_to_grid(*o["rho"], S["rho_vy"], x, y, rho, dx, dy, vws, ws)
+    O.marker_to_grid(*o["mu_b"], S["mu_b"], x, y, mu, dx, dy, vws, ws)
+    O.marker_to_grid(*o["mu_p"], S["mu_p"], x, y, mu, dx, dy, vws, ws)
+    out = []
+    for fuse in (1, 2):
+        G = to_gpu(S, list(S.keys()))
+        st = {}
+        try:
+            sfhip.set_pt_fusion(n, n + 3, fuse, 0)
+            sfhip.solveStokes(G["P"], G["Vx_pad"], G["Vy_pad"], G["rho_vy"], G["mu_b"], G["mu_p"], None, None, None,
+                              None, None, None, None, G["dVx"], G["dVy"], None, None, G["Vx_s"], G["Vy_s"], 1.0, dx, dy,
+                              n, n + 3, 0.0, 0.5, eps=1e-8, print_info=False, stats=st, mixed_precision=True,
+                              refine={"rebase_interval": rebase})
+        finally:
+            sfhip.set_pt_fusion(n, n + 3, 0, 0)
+        torch.cuda.synchronize()
+        out.append((st, {k: sfhip.to_numpy(G[k]) for k in ("Vx_pad", "Vy_pad", "P", "dVx", "dVy")}))
+    (s1, a1), (s2, a2) = out
+    assert s1["n_fp32_iter"] == s2["n_fp32_iter"] and s1["err_fp64"] == s2["err_fp64"]
+    for k in a1:
+        assert np.array_equal(a1[k], a2[k]), k
+
+
 def test_autotune_keeps_results(gpu, golden):
     """The autotuner only picks a launch shape: a solve after tuning is bitwise the oracle's."""
     best, table = sfhip.autotune_pt(63, 47, iters=5)
