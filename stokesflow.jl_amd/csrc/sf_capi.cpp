@@ -117,8 +117,8 @@ extern "C" int sf_ctx_set_pt_tuning(sf_ctx* c, int rows, int depth, int max_wave
 extern "C" int sf_ctx_set_pt_fusion(sf_ctx* c, int iters_per_launch, int rows)
 {
     if (!c) return set_err(SF_ERR_ARG, "sf_ctx_set_pt_fusion: NULL ctx");
-    if (iters_per_launch < 0 || iters_per_launch > 2)
-        return set_err(SF_ERR_ARG, "sf_ctx_set_pt_fusion: iters_per_launch %d not 0, 1 or 2", iters_per_launch);
+    if (iters_per_launch < 0 || iters_per_launch > 3)
+        return set_err(SF_ERR_ARG, "sf_ctx_set_pt_fusion: iters_per_launch %d not 0, 1, 2 or 3", iters_per_launch);
     if (rows < 0 || rows > 4096) return set_err(SF_ERR_ARG, "sf_ctx_set_pt_fusion: rows %d outside [0, 4096]", rows);
     c->fuse = iters_per_launch;
     c->pair_rows = rows;
@@ -134,7 +134,8 @@ struct PTLoop {
     double *cur_vx, *cur_vy, *cur_p, *nxt_vx, *nxt_vy, *nxt_p;
     double *cur_dvx, *cur_dvy, *nxt_dvx, *nxt_dvy;
     double *vx0, *vy0, *p0, *dvx0, *dvy0;  // the caller's arrays
-    bool pairs;                             // two-iteration passes enabled
+    bool pairs;                             // multi-iteration passes enabled
+    int kmax;                               // iterations per pass (2 or 3)
 
     int init(sf_ctx* ctx, double* Vx_s, double* Vy_s, double* P, double* dVx, double* dVy)
     {
@@ -148,6 +149,7 @@ struct PTLoop {
         nxt_vy = c->vy_b;
         nxt_p = c->p_b;
         pairs = c->fuse != 1;
+        kmax = c->fuse == 0 ? pt_default_fusion(c->nx, c->ny) : c->fuse;
         if (pairs && !c->dvx_b) {
             const size_t nx = c->nx, ny = c->ny;
             hipError_t e = dalloc(&c->dvx_b, (nx - 2) * (ny - 1));
@@ -183,8 +185,8 @@ struct PTLoop {
         std::swap(cur_p, nxt_p);
         return SF_OK;
     }
-    // Two plain iterations in one pass.
-    int pair(hipStream_t s)
+    // k plain iterations in one pass.
+    int pair(hipStream_t s, int k = 2)
     {
         pp.flags = 0;
         pp.seg = c->pair_rows;
@@ -198,7 +200,7 @@ struct PTLoop {
         pp.dvy = cur_dvy;
         pp.dvx_o = nxt_dvx;
         pp.dvy_o = nxt_dvy;
-        HIPCHK(launch_pt_pair(pp, s));
+        HIPCHK(launch_pt_multi(pp, k, s));
         std::swap(cur_vx, nxt_vx);
         std::swap(cur_vy, nxt_vy);
         std::swap(cur_p, nxt_p);
@@ -375,19 +377,28 @@ extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_
             HIPCHK(hipEventRecord(c->ev0, s));
             timed = true;
         }
-        // iterations iter and iter+1 in one pass when neither is a check, iter is not
-        // followed by a dt refresh, iter+1 is not the last and the timing window does not
-        // start at iter+1
-        if (L.pairs && iter + 1 < iter_max && iter % ncheck != 0 && (iter + 1) % ncheck != 0 &&
-            !(use_fss && iter % ndtupdate == 0) && iter + 1 != itert1) {
-            rc = L.pair(s);
+        // iterations iter..iter+k-1 in one pass when none is a check iteration, none but the
+        // last is followed by a dt refresh, the last is not iter_max and the timing window does
+        // not start inside the pass
+        auto fusable = [&](int64_t k) {
+            if (iter + k - 1 >= iter_max) return false;
+            for (int64_t j = iter; j < iter + k; ++j) {
+                if (j % ncheck == 0) return false;
+                if (j < iter + k - 1 && use_fss && j % ndtupdate == 0) return false;
+                if (j > iter && j == itert1) return false;
+            }
+            return true;
+        };
+        const int kf = !L.pairs ? 1 : (L.kmax == 3 && fusable(3) ? 3 : (fusable(2) ? 2 : 1));
+        if (kf > 1) {
+            rc = L.pair(s, kf);
             if (rc) return rc;
-            if (use_fss && (iter + 1) % ndtupdate == 0) {
+            if (use_fss && (iter + kf - 1) % ndtupdate == 0) {
                 rc = enqueue_dt(c, L.cur_vx, L.cur_vy, dx, dy, maxdisp, 1);
                 if (rc) return rc;
             }
-            iter += 2;
-            niter += 2;
+            iter += kf;
+            niter += kf;
             continue;
         }
         const bool check = (iter % ncheck) == 0;
@@ -529,8 +540,12 @@ extern "C" int sf_pt_iterations(sf_ctx* c, int64_t n_iter, double* P, double* Vx
     if (rc) return rc;
     L.pp = pp;
     int64_t it = 0;
+    for (; L.pairs && it + L.kmax <= n_iter; it += L.kmax) {
+        rc = L.pair(s, L.kmax);
+        if (rc) return rc;
+    }
     for (; L.pairs && it + 2 <= n_iter; it += 2) {
-        rc = L.pair(s);
+        rc = L.pair(s, 2);
         if (rc) return rc;
     }
     for (; it < n_iter; ++it) {

@@ -71,6 +71,9 @@ hipError_t launch_pt_fused(const PTParams& p, hipStream_t s);
 // dvx_o, dvy_o); p.seg = rows per tile (<= 0: default).  Bit-identical to two launch_pt_fused.
 hipError_t launch_pt_pair(const PTParams& p, hipStream_t s);
 int pt2_default_rows(int nx, int ny);
+// k = 2 or 3 PT iterations per pass (k = 3: ptk_tile<3>, 58 output columns per strip).
+hipError_t launch_pt_multi(const PTParams& p, int k, hipStream_t s);
+int pt_default_fusion(int nx, int ny);  // iterations per pass when the context leaves it open
 hipError_t launch_pt_resid_y(const ResYParams& p, hipStream_t s);
 hipError_t launch_reduce(const RedParams& p, int narr, int nb, hipStream_t s);
 hipError_t launch_finalize_dt(const double* part, int nb, double dx, double dy, double maxdisp, double* scal,

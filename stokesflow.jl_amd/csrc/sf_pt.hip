@@ -25,7 +25,7 @@
 // dt are deterministic.
 #include <algorithm>
 
-#include "sf_pt2_tile.h"
+#include "sf_ptk_tile.h"
 
 namespace sf {
 
@@ -58,6 +58,14 @@ __global__ __launch_bounds__(64) void pt2_tile_f64(PTParams p)
 {
     const int logical = xcd_remap(blockIdx.x, gridDim.x);
     pt2_tile(p, logical % p.nstrips, logical / p.nstrips);
+}
+
+// K PT iterations per pass, one wave per tile (sf_ptk_tile.h).
+template <int K>
+__global__ __launch_bounds__(64) void ptk_tile_f64(PTParams p)
+{
+    const int logical = xcd_remap(blockIdx.x, gridDim.x);
+    ptk_tile<K>(p, logical % p.nstrips, logical / p.nstrips);
 }
 
 // compute_ResY with post-update V and dt_check (compute_Residuals!, :241-249).
@@ -271,6 +279,31 @@ hipError_t launch_pt_pair(const PTParams& p0, hipStream_t s)
     const int nseg = (p.ny + p.seg - 1) / p.seg;
     p.ntiles = p.nstrips * nseg;
     hipLaunchKernelGGL(pt2_tile_f64, dim3(p.ntiles), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+// Iterations per pass by grid size, and the three-iteration pass's tile rows (measured on
+// MI355X, us per iteration: 4096^2 K=3 138.3 with 48 rows (141.8 with 32, 152.9 with 64)
+// against K=2 193.6; 2048^2 K=3 41.7 with 16 rows against K=2 50.4; 1024^2 K=2 14.1 against
+// K=3 14.6 before the stages shared their pseudo-time steps).
+int pt_default_fusion(int nx, int ny) { return std::max(nx, ny) > 1536 ? 3 : 2; }
+
+int pt3_default_rows(int nx, int ny)
+{
+    const int r = std::max(nx, ny) > 3072 ? 48 : 16;
+    return ny >= r ? r : ny;
+}
+
+hipError_t launch_pt_multi(const PTParams& p0, int k, hipStream_t s)
+{
+    if (k == 2) return launch_pt_pair(p0, s);
+    if (k != 3) return hipErrorInvalidValue;
+    PTParams p = p0;
+    if (p.seg <= 0) p.seg = pt3_default_rows(p.nx, p.ny);
+    p.nstrips = (p.nx - 1 + 58 - 1) / 58;  // 64 - 2 K output columns per strip
+    const int nseg = (p.ny + p.seg - 1) / p.seg;
+    p.ntiles = p.nstrips * nseg;
+    hipLaunchKernelGGL(ptk_tile_f64<3>, dim3(p.ntiles), dim3(64), 0, s, p);
     return hipGetLastError();
 }
 

@@ -55,6 +55,12 @@ struct PTRow {
     double pn, dvx, dvy, vx, vy;
 };
 
+// The pseudo-time steps of a row (compute_timesteps!, StokesSolver_multixpu.jl:128-133): they
+// depend on mu_p only, so a later iteration of the same row reuses the first one's.
+struct PTTau {
+    double p, vx, vy;
+};
+
 __device__ __forceinline__ void pt_win_init(PTWin& w, double vx_r, double vy_r, double rho_r)
 {
     w.vx_r = vx_r;
@@ -73,8 +79,8 @@ __device__ __forceinline__ void pt_win_init(PTWin& w, double vx_r, double vy_r, 
 // One row step of one PT iteration with pt_tile's expressions (compute_P_tau!, compute_ResX,
 // compute_ResY, compute_dV!, compute_V!; StokesSolver_multixpu.jl:140-234), then the window
 // slides down one row.  Results of lanes or rows without an output are computed but unused.
-template <class PP>
-__device__ __forceinline__ PTRow pt_row(const PP& p, double dt, PTWin& w, const RowIn& in)
+template <bool TAU_IN, class PP>
+__device__ __forceinline__ PTRow pt_row(const PP& p, double dt, PTWin& w, const RowIn& in, PTTau& tau)
 {
     const double rdx = p.rdx, rdy = p.rdy;
     const double vx_r1_p1 = lane_down(in.vx1, 1);
@@ -85,8 +91,8 @@ __device__ __forceinline__ PTRow pt_row(const PP& p, double dt, PTWin& w, const 
     const double dvxdx = (vx_r1_p1 - in.vx1) * rdx;
     const double dvydy = (vy_r1_p1 - w.vy_r_p1) * rdy;
     const double div = dvxdx + dvydy;
-    const double dtaup = (p.ptsc41 * in.mup) / p.max_nxy;
-    const double pn = in.pold - dtaup * div;
+    if constexpr (!TAU_IN) tau.p = (p.ptsc41 * in.mup) / p.max_nxy;
+    const double pn = in.pold - tau.p * div;
     const double txx = (2.0 * in.mup) * (dvxdx - (1.0 / 3.0) * div);
     const double tyy = (2.0 * in.mup) * (dvydy - (1.0 / 3.0) * div);
     const double txy = ((2.0 * in.mub) * 0.5) * ((in.vx1 - w.vx_r) * rdy + (w.vy_r_p1 - w.vy_r) * rdx);
@@ -103,10 +109,12 @@ __device__ __forceinline__ PTRow pt_row(const PP& p, double dt, PTWin& w, const 
     o.pn = pn;
     o.dvx = p.dampx * in.dvx + rx;
     o.dvy = p.dampy * in.dvy + ry;
-    const double dtauvx = (p.vsc_mdxy2 / ((w.mup_prev + w.mup_prev_p1) * 0.5)) / 4.1;
-    const double dtauvy = (p.vsc_mdxy2 / ((w.mup_prev + in.mup) * 0.5)) / 4.1;
-    o.vx = w.vx_r_p1 + dtauvx * o.dvx;
-    o.vy = w.vy_r_p1 + dtauvy * o.dvy;
+    if constexpr (!TAU_IN) {
+        tau.vx = (p.vsc_mdxy2 / ((w.mup_prev + w.mup_prev_p1) * 0.5)) / 4.1;
+        tau.vy = (p.vsc_mdxy2 / ((w.mup_prev + in.mup) * 0.5)) / 4.1;
+    }
+    o.vx = w.vx_r_p1 + tau.vx * o.dvx;
+    o.vy = w.vy_r_p1 + tau.vy * o.dvy;
 
     w.pn_prev_p1 = lane_down(pn, 1);
     w.txx_prev_p1 = lane_down(txx, 1);
@@ -198,12 +206,14 @@ __device__ __forceinline__ void pt2_tile(const PP& p, int strip, int segid)
     // A's previous row step, as B consumes it
     double a_pn = 0.0, a_mup = 0.0, a_mub = 0.0, a_dvx = 0.0, a_dvy = 0.0, a_rho = 0.0, a_vx = 0.0, a_vy = 0.0;
     double a_vx_last = 0.0;
+    PTTau a_tau{0.0, 0.0, 0.0};
 
     for (; r < ra1; ++r) {
         const RowIn in = nxt;
         if (r + 1 < ra1) nxt = load_row(p, lc, r + 1, true);
         const double rho_row = wa.rho_r;  // rho row r
-        const PTRow oa = pt_row(p, dt, wa, in);
+        PTTau tau_a;
+        const PTRow oa = pt_row<false>(p, dt, wa, in, tau_a);
         // iteration k+1's V row r at column c+1, as pt_tile stores it
         const double vx_a = vx_zero ? 0.0 : oa.vx;
         const double vy_dn = lane_down(oa.vy, 1), vy_up = lane_up(oa.vy, 1);
@@ -228,9 +238,10 @@ __device__ __forceinline__ void pt2_tile(const PP& p, int strip, int segid)
             ib.mub = a_mub;
             ib.dvx = a_dvx;
             ib.dvy = a_dvy;
-            const PTRow ob = pt_row(p, dt, wb, ib);
+            const PTRow ob = pt_row<true>(p, dt, wb, ib, a_tau);
             pt2_store(p, ob, c, rb, r0, out_lane);
         }
+        a_tau = tau_a;
         a_pn = oa.pn;
         a_mup = in.mup;
         a_mub = in.mub;
@@ -254,7 +265,7 @@ __device__ __forceinline__ void pt2_tile(const PP& p, int strip, int segid)
         ib.mub = a_mub;
         ib.dvx = a_dvx;
         ib.dvy = a_dvy;
-        const PTRow ob = pt_row(p, dt, wb, ib);
+        const PTRow ob = pt_row<true>(p, dt, wb, ib, a_tau);
         pt2_store(p, ob, c, rb, r0, out_lane);
     }
 }

@@ -76,6 +76,39 @@ def test_pair_pass_bitwise(gpu, nx, ny, rows, n_iter):
         assert np.array_equal(got[k], exp[k]), (nx, ny, rows, n_iter, k, np.max(np.abs(got[k] - exp[k])))
 
 
+# three iterations per pass: 58-column strips; n_iter 7 = 3 + 3 + 1, 5 = 3 + 2, 4 = 3 + 1 launches
+@pytest.mark.parametrize("nx,ny,rows", [(3, 3, 0), (4, 5, 1), (5, 9, 2), (57, 6, 3), (58, 8, 8), (59, 13, 16),
+                                        (60, 4, 4), (117, 21, 5), (118, 40, 16), (175, 9, 0)])
+@pytest.mark.parametrize("n_iter", [3, 4, 5, 7])
+def test_triple_pass_bitwise(gpu, nx, ny, rows, n_iter):
+    S = cases.random_state(nx, ny, seed=nx * 7001 + ny * 37 + rows)
+    try:
+        sfhip.set_pt_fusion(nx, ny, 3, rows)
+        got = run_gpu_iterations(S, nx, ny, n_iter, 1.1, 0.07, 0.06, 0.013)
+    finally:
+        sfhip.set_pt_fusion(nx, ny, 0, 0)
+    exp = oracle_iterations(S, nx, ny, n_iter, 1.1, 0.07, 0.06, 0.013)
+    for k in STATE:
+        assert np.array_equal(got[k], exp[k]), (nx, ny, rows, n_iter, k, np.max(np.abs(got[k] - exp[k])))
+
+
+def test_triple_pass_solve_golden(gpu, golden):
+    """A full solve with three iterations per pass is bitwise the oracle's (checks, dt refreshes
+    and the timing mark split the passes)."""
+    g = golden["stokes_solver"]
+    nx, ny = g["nx"], g["ny"]
+    S = cases.stokes_solver_inputs(g)
+    try:
+        sfhip.set_pt_fusion(nx, ny, 3, 0)
+        R, _, st = gpu_solve(S, nx, ny, g["g_y"], g["dx"], g["dy"], 0.0, g["maxdisp"], True, g["eps"], 100000)
+    finally:
+        sfhip.set_pt_fusion(nx, ny, 0, 0)
+    exp = O.solve_stokes(S, g["g_y"], g["dx"], g["dy"], nx, ny, 0.0, g["maxdisp"], True, g["eps"])
+    assert st["niter"] == exp["niter"]
+    for k in S:
+        assert np.array_equal(R[k], S[k]), k
+
+
 def test_pair_and_single_launch_agree(gpu):
     """Forcing one iteration per launch gives the same bits as the default two-iteration passes."""
     nx, ny = 97, 83
