@@ -1105,6 +1105,7 @@ extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, dou
     // writes the momentum to a second buffer, and the context's pointers are swapped after
     // each pass so that r_mx / r_my always hold the current momentum
     const bool corr_pairs = c->fuse != 1;
+    const bool corr_k3 = c->fuse == 3 || (c->fuse == 0 && pt_default_fusion(nx, ny) == 3);
     if (corr_pairs && !c->r_mx2) {
         HIPCHK(dalloc(&c->r_mx2, (size_t)(nx - 2) * (ny - 1)));
         HIPCHK(dalloc(&c->r_my2, (size_t)(nx - 1) * (ny - 2)));
@@ -1201,6 +1202,23 @@ extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, dou
         steps = std::min(steps, ncheck - n32 % ncheck);
         if (use_fss) steps = std::min(steps, ndtupdate - n32 % ndtupdate);
         int64_t it = 0;
+        for (; corr_k3 && it + 3 <= steps; it += 3) {
+            cp.vx_a = c->r_vx[cur];
+            cp.vy_a = c->r_vy[cur];
+            cp.p_a = c->r_p[cur];
+            cp.vx_b = c->r_vx[1 - cur];
+            cp.vy_b = c->r_vy[1 - cur];
+            cp.p_b = c->r_p[1 - cur];
+            cp.mx = c->r_mx;
+            cp.my = c->r_my;
+            cp.mx_o = c->r_mx2;
+            cp.my_o = c->r_my2;
+            cp.seg = 0;
+            HIPCHK(launch_pt_corr3(cp, s));
+            cur = 1 - cur;
+            std::swap(c->r_mx, c->r_mx2);
+            std::swap(c->r_my, c->r_my2);
+        }
         for (; corr_pairs && it + 2 <= steps; it += 2) {
             cp.vx_a = c->r_vx[cur];
             cp.vy_a = c->r_vy[cur];

@@ -241,9 +241,14 @@ __device__ __forceinline__ void corr_win_init(CorrWin& w, float vx_r, float vy_r
     w.mup_prev = w.mup_prev_p1 = 0.f;
 }
 
+// The pseudo-time steps of a row (they depend on mu_p only; later sweeps of a row reuse them).
+struct CorrTau {
+    float p, vx, vy;
+};
+
 // One row step of one correction sweep with corr_tile's expressions, then the window slides.
-template <class PP>
-__device__ __forceinline__ CorrRow corr_row(const PP& p, float dt, CorrWin& w, const CorrRowIn& in)
+template <bool TAU_IN, class PP>
+__device__ __forceinline__ CorrRow corr_row(const PP& p, float dt, CorrWin& w, const CorrRowIn& in, CorrTau& tau)
 {
     const float rdx = p.rdx, rdy = p.rdy;
     const float vx_r1_p1 = lane_down(in.vx1, 1);
@@ -254,9 +259,9 @@ __device__ __forceinline__ CorrRow corr_row(const PP& p, float dt, CorrWin& w, c
     const float dvxdx = (vx_r1_p1 - in.vx1) * rdx;
     const float dvydy = (vy_r1_p1 - w.vy_r_p1) * rdy;
     const float div = dvxdx + dvydy;
-    const float dtaup = (p.ptsc41 * in.mup) / p.max_nxy;
+    if constexpr (!TAU_IN) tau.p = (p.ptsc41 * in.mup) / p.max_nxy;
     const float cres = div + in.fc;
-    const float pn = in.pold - dtaup * cres;
+    const float pn = in.pold - tau.p * cres;
     const float txx = (2.0f * in.mup) * (dvxdx - (1.0f / 3.0f) * div);
     const float tyy = (2.0f * in.mup) * (dvydy - (1.0f / 3.0f) * div);
     const float txy = in.mub * ((in.vx1 - w.vx_r) * rdy + (w.vy_r_p1 - w.vy_r) * rdx);
@@ -273,10 +278,12 @@ __device__ __forceinline__ CorrRow corr_row(const PP& p, float dt, CorrWin& w, c
     o.pn = pn;
     o.mx = p.dampx * in.mx + rx;
     o.my = p.dampy * in.my + ry;
-    const float dtauvx = (p.vsc_mdxy2 / ((w.mup_prev + w.mup_prev_p1) * 0.5f)) / 4.1f;
-    const float dtauvy = (p.vsc_mdxy2 / ((w.mup_prev + in.mup) * 0.5f)) / 4.1f;
-    o.vx = w.vx_r_p1 + dtauvx * o.mx;
-    o.vy = w.vy_r_p1 + dtauvy * o.my;
+    if constexpr (!TAU_IN) {
+        tau.vx = (p.vsc_mdxy2 / ((w.mup_prev + w.mup_prev_p1) * 0.5f)) / 4.1f;
+        tau.vy = (p.vsc_mdxy2 / ((w.mup_prev + in.mup) * 0.5f)) / 4.1f;
+    }
+    o.vx = w.vx_r_p1 + tau.vx * o.mx;
+    o.vy = w.vy_r_p1 + tau.vy * o.my;
 
     w.pn_prev_p1 = lane_down(pn, 1);
     w.txx_prev_p1 = lane_down(txx, 1);
@@ -366,12 +373,14 @@ __device__ __forceinline__ void corr2_tile(const PP& p, int strip, int segid)
     CorrRowIn nxt = corr_load_row(p, lc, r, r >= r0 - 1);
     float a_pn = 0.f, a_mup = 0.f, a_mub = 0.f, a_fc = 0.f, a_mx = 0.f, a_my = 0.f, a_fx = 0.f, a_fy = 0.f;
     float a_rho = 0.f, a_vx = 0.f, a_vy = 0.f, a_vx_last = 0.f;
+    CorrTau a_tau{0.f, 0.f, 0.f};
 
     for (; r < ra1; ++r) {
         const CorrRowIn in = nxt;
         if (r + 1 < ra1) nxt = corr_load_row(p, lc, r + 1, true);
         const float rho_row = wa.rho_r;
-        const CorrRow oa = corr_row(p, dt, wa, in);
+        CorrTau tau_a;
+        const CorrRow oa = corr_row<false>(p, dt, wa, in, tau_a);
         const float vx_a = vx_zero ? 0.f : oa.vx;
         const float vy_dn = lane_down(oa.vy, 1), vy_up = lane_up_f(oa.vy, 1);
         const float vy_a = (r == ny - 1) ? 0.f : (vy_left ? vy_dn : (vy_right ? vy_up : oa.vy));
@@ -397,9 +406,10 @@ __device__ __forceinline__ void corr2_tile(const PP& p, int strip, int segid)
             ib.my = a_my;
             ib.fx = a_fx;
             ib.fy = a_fy;
-            const CorrRow ob = corr_row(p, dt, wb, ib);
+            const CorrRow ob = corr_row<true>(p, dt, wb, ib, a_tau);
             corr2_store(p, ob, c, rb, r0, out_lane);
         }
+        a_tau = tau_a;
         a_pn = oa.pn;
         a_mup = in.mup;
         a_mub = in.mub;
@@ -426,9 +436,146 @@ __device__ __forceinline__ void corr2_tile(const PP& p, int strip, int segid)
         ib.my = a_my;
         ib.fx = a_fx;
         ib.fy = a_fy;
-        const CorrRow ob = corr_row(p, dt, wb, ib);
+        const CorrRow ob = corr_row<true>(p, dt, wb, ib, a_tau);
         corr2_store(p, ob, c, ny - 1, r0, out_lane);
     }
+}
+
+// K correction sweeps per pass: the FP32 counterpart of ptk_tile (sf_ptk_tile.h, same stage
+// scheme; the forcing and the materials pass down the stages with the rows).
+struct CorrHand {
+    float pn, mup, mub, fc, mx, my, fx, fy, rho, vx, vy;
+    CorrTau tau;
+};
+
+template <int K, class PP>
+__device__ __forceinline__ void corrk_tile(const PP& p, int strip, int segid)
+{
+    constexpr int OUT = 64 - 2 * K;
+    const int lane = threadIdx.x & 63;
+    const int c = strip * OUT + lane - (K - 1);
+    const int nx = p.nx, ny = p.ny;
+    const bool out_lane = lane >= K - 1 && lane < K - 1 + OUT;
+    const int r0 = segid * p.seg;
+    const int r1 = min(r0 + p.seg, ny);
+    const float dt = p.dt;
+    const int cc = c < 0 ? 0 : c;
+    CorrCols lc;
+    lc.vx = (unsigned)min(cc, nx - 1);
+    lc.vy = (unsigned)min(cc, nx);
+    lc.p = (unsigned)min(cc, nx - 2);
+    lc.dvx = (unsigned)min(cc, nx - 3);
+    lc.dvy = (unsigned)min(cc, nx - 2);
+    const bool vx_zero = (c + 1 == 0) || (c + 1 == nx - 1);
+    const bool vy_left = c + 1 == 0, vy_right = c + 1 == nx;
+
+    int rs0[K], rs1[K];
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        rs0[s] = max(r0 - (K - s), 0);
+        rs1[s] = min(r1 - 1 + (K - 1 - s), ny - 1);
+    }
+    const int r_last = r1 - 1 + (K - 1);
+
+    CorrWin w[K];
+    CorrHand h[K];
+    float vx_last[K];
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        h[s] = CorrHand{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, CorrTau{0.f, 0.f, 0.f}};
+        vx_last[s] = 0.f;
+    }
+    int r = rs0[0];
+    corr_win_init(w[0], (p.vx_a + (int64_t)r * nx)[lc.vx], (p.vy_a + (int64_t)r * (nx + 1))[lc.vy],
+                  (p.rho + (int64_t)r * (nx + 1))[lc.vy]);
+    CorrRowIn nxt = corr_load_row(p, lc, r, r >= r0 - (K - 1));
+
+    for (; r <= r_last; ++r) {
+        float cur_vx[K], cur_vy[K], cur_rho[K];
+        CorrHand hn[K];
+        bool ran[K];
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+            const int rs = r - s;
+            ran[s] = false;
+            cur_vx[s] = cur_vy[s] = cur_rho[s] = 0.f;
+            hn[s] = h[s];
+            if (rs < rs0[s] || rs > rs1[s]) {
+                if (s < K - 1 && rs == ny && rs - 1 >= rs0[s]) {
+                    cur_vx[s] = vx_last[s];
+                    cur_vy[s] = 0.f;
+                    cur_rho[s] = h[s].rho;
+                }
+                continue;
+            }
+            CorrRowIn in;
+            if (s == 0) {
+                in = nxt;
+                if (r + 1 <= rs1[0]) nxt = corr_load_row(p, lc, r + 1, true);
+            } else {
+                in.vx1 = cur_vx[s - 1];
+                in.vy1 = cur_vy[s - 1];
+                in.rho1 = cur_rho[s - 1];
+                in.pold = h[s - 1].pn;
+                in.mup = h[s - 1].mup;
+                in.mub = h[s - 1].mub;
+                in.fc = h[s - 1].fc;
+                in.mx = h[s - 1].mx;
+                in.my = h[s - 1].my;
+                in.fx = h[s - 1].fx;
+                in.fy = h[s - 1].fy;
+                if (rs == rs0[s]) {
+                    const float vx0 = (rs == 0) ? in.vx1 : h[s - 1].vx;
+                    const float vy0 = (rs == 0) ? 0.f : h[s - 1].vy;
+                    corr_win_init(w[s], vx0, vy0, h[s - 1].rho);
+                }
+            }
+            const float rho_row = w[s].rho_r;
+            CorrTau tau;
+            CorrRow o;
+            if (s == 0) {
+                o = corr_row<false>(p, dt, w[s], in, tau);
+            } else {
+                tau = h[s - 1].tau;
+                o = corr_row<true>(p, dt, w[s], in, tau);
+            }
+            if (s == K - 1) {
+                corr2_store(p, o, c, rs, r0, out_lane);
+            } else {
+                const float vx_a = vx_zero ? 0.f : o.vx;
+                const float vy_dn = lane_down(o.vy, 1), vy_up = lane_up_f(o.vy, 1);
+                const float vy_a = (rs == ny - 1) ? 0.f : (vy_left ? vy_dn : (vy_right ? vy_up : o.vy));
+                cur_vx[s] = lane_up_f(vx_a, 1);
+                cur_vy[s] = lane_up_f(vy_a, 1);
+                cur_rho[s] = rho_row;
+                hn[s].pn = o.pn;
+                hn[s].mup = in.mup;
+                hn[s].mub = in.mub;
+                hn[s].fc = in.fc;
+                hn[s].mx = o.mx;
+                hn[s].my = o.my;
+                hn[s].fx = in.fx;
+                hn[s].fy = in.fy;
+                hn[s].rho = rho_row;
+                hn[s].vx = cur_vx[s];
+                hn[s].vy = cur_vy[s];
+                hn[s].tau = tau;
+                ran[s] = true;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < K - 1; ++s) {
+            if (ran[s]) vx_last[s] = cur_vx[s];
+            h[s] = hn[s];
+        }
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void pt_corrk_f32(CorrParams p)
+{
+    const int logical = xcd_remap_r(blockIdx.x, gridDim.x);
+    corrk_tile<K>(p, logical % p.nstrips, logical / p.nstrips);
 }
 
 __global__ __launch_bounds__(64) void pt_corr2_f32(CorrParams p)
@@ -653,6 +800,20 @@ hipError_t launch_pt_corr2(const CorrParams& p0, hipStream_t s)
     const int nseg = (p.ny + p.seg - 1) / p.seg;
     p.ntiles = p.nstrips * nseg;
     hipLaunchKernelGGL(pt_corr2_f32, dim3(p.ntiles), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_pt_corr3(const CorrParams& p0, hipStream_t s)
+{
+    CorrParams p = p0;
+    if (p.seg <= 0) {
+        const int r = std::max(p.nx, p.ny) > 3072 ? 48 : 16;
+        p.seg = p.ny >= r ? r : p.ny;
+    }
+    p.nstrips = (p.nx - 1 + 58 - 1) / 58;
+    const int nseg = (p.ny + p.seg - 1) / p.seg;
+    p.ntiles = p.nstrips * nseg;
+    hipLaunchKernelGGL(pt_corrk_f32<3>, dim3(p.ntiles), dim3(64), 0, s, p);
     return hipGetLastError();
 }
 

@@ -284,8 +284,8 @@ def test_refined_plume(gpu, n, rebase):
 
 @pytest.mark.parametrize("n,rebase", [(61, 0), (96, 97), (130, 45)])
 def test_refined_pair_sweeps_bitwise(gpu, n, rebase):
-    """FP32 correction sweeps two per pass (default) and one per launch give the same bits:
-    windows of odd and even length (rebase 97 / 45 and ncheck), strips of 60 and 62 columns."""
+    """FP32 correction sweeps one, two and three per pass give the same bits: windows of odd and
+    even length (rebase 97 / 45 and ncheck), strips of 58, 60 and 62 columns."""
     dx, dy, x, y, rho, mu = cases.plume_markers(n, n + 3, 4, seed=78)
     S = O.solver_arrays(n, n + 3)
     vws, ws = O.zeros(n + 1, n + 3), O.zeros(n + 1, n + 3)
@@ -294,7 +294,7 @@ def test_refined_pair_sweeps_bitwise(gpu, n, rebase):
     O.marker_to_grid(*o["mu_b"], S["mu_b"], x, y, mu, dx, dy, vws, ws)
     O.marker_to_grid(*o["mu_p"], S["mu_p"], x, y, mu, dx, dy, vws, ws)
     out = []
-    for fuse in (1, 2):
+    for fuse in (1, 2, 3):
         G = to_gpu(S, list(S.keys()))
         st = {}
         try:
@@ -307,10 +307,11 @@ def test_refined_pair_sweeps_bitwise(gpu, n, rebase):
             sfhip.set_pt_fusion(n, n + 3, 0, 0)
         torch.cuda.synchronize()
         out.append((st, {k: sfhip.to_numpy(G[k]) for k in ("Vx_pad", "Vy_pad", "P", "dVx", "dVy")}))
-    (s1, a1), (s2, a2) = out
-    assert s1["n_fp32_iter"] == s2["n_fp32_iter"] and s1["err_fp64"] == s2["err_fp64"]
-    for k in a1:
-        assert np.array_equal(a1[k], a2[k]), k
+    s1, a1 = out[0]
+    for s2, a2 in out[1:]:
+        assert s1["n_fp32_iter"] == s2["n_fp32_iter"] and s1["err_fp64"] == s2["err_fp64"]
+        for k in a1:
+            assert np.array_equal(a1[k], a2[k]), k
 
 
 def test_autotune_keeps_results(gpu, golden):
