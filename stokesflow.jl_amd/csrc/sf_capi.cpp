@@ -117,8 +117,8 @@ extern "C" int sf_ctx_set_pt_tuning(sf_ctx* c, int rows, int depth, int max_wave
 extern "C" int sf_ctx_set_pt_fusion(sf_ctx* c, int iters_per_launch, int rows)
 {
     if (!c) return set_err(SF_ERR_ARG, "sf_ctx_set_pt_fusion: NULL ctx");
-    if (iters_per_launch < 0 || iters_per_launch > 3)
-        return set_err(SF_ERR_ARG, "sf_ctx_set_pt_fusion: iters_per_launch %d not 0, 1, 2 or 3", iters_per_launch);
+    if (iters_per_launch < 0 || iters_per_launch > 4)
+        return set_err(SF_ERR_ARG, "sf_ctx_set_pt_fusion: iters_per_launch %d not in 0..4", iters_per_launch);
     if (rows < 0 || rows > 4096) return set_err(SF_ERR_ARG, "sf_ctx_set_pt_fusion: rows %d outside [0, 4096]", rows);
     c->fuse = iters_per_launch;
     c->pair_rows = rows;
@@ -149,7 +149,7 @@ struct PTLoop {
         nxt_vy = c->vy_b;
         nxt_p = c->p_b;
         pairs = c->fuse != 1;
-        kmax = c->fuse == 0 ? pt_default_fusion(c->nx, c->ny) : c->fuse;
+        kmax = c->fuse == 0 ? pt_default_fusion(c->nx, c->ny) : std::min(c->fuse, 3);
         if (pairs && !c->dvx_b) {
             const size_t nx = c->nx, ny = c->ny;
             hipError_t e = dalloc(&c->dvx_b, (nx - 2) * (ny - 1));
@@ -1105,7 +1105,8 @@ extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, dou
     // writes the momentum to a second buffer, and the context's pointers are swapped after
     // each pass so that r_mx / r_my always hold the current momentum
     const bool corr_pairs = c->fuse != 1;
-    const bool corr_k3 = c->fuse == 3 || (c->fuse == 0 && pt_default_fusion(nx, ny) == 3);
+    // sweeps per pass: the context's setting, or the FP64 solver's size default
+    const int corr_k = c->fuse == 0 ? pt_default_fusion(nx, ny) : c->fuse;
     if (corr_pairs && !c->r_mx2) {
         HIPCHK(dalloc(&c->r_mx2, (size_t)(nx - 2) * (ny - 1)));
         HIPCHK(dalloc(&c->r_my2, (size_t)(nx - 1) * (ny - 2)));
@@ -1202,7 +1203,7 @@ extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, dou
         steps = std::min(steps, ncheck - n32 % ncheck);
         if (use_fss) steps = std::min(steps, ndtupdate - n32 % ndtupdate);
         int64_t it = 0;
-        for (; corr_k3 && it + 3 <= steps; it += 3) {
+        for (; corr_k >= 3 && it + corr_k <= steps; it += corr_k) {
             cp.vx_a = c->r_vx[cur];
             cp.vy_a = c->r_vy[cur];
             cp.p_a = c->r_p[cur];
@@ -1213,8 +1214,8 @@ extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, dou
             cp.my = c->r_my;
             cp.mx_o = c->r_mx2;
             cp.my_o = c->r_my2;
-            cp.seg = 0;
-            HIPCHK(launch_pt_corr3(cp, s));
+            cp.seg = c->pair_rows;  // 0: size default
+            HIPCHK(launch_pt_corrk(cp, corr_k, s));
             cur = 1 - cur;
             std::swap(c->r_mx, c->r_mx2);
             std::swap(c->r_my, c->r_my2);
@@ -1230,7 +1231,7 @@ extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, dou
             cp.my = c->r_my;
             cp.mx_o = c->r_mx2;
             cp.my_o = c->r_my2;
-            cp.seg = 0;
+            cp.seg = c->pair_rows;  // 0: size default
             HIPCHK(launch_pt_corr2(cp, s));
             cur = 1 - cur;
             std::swap(c->r_mx, c->r_mx2);

@@ -137,8 +137,8 @@ hipError_t launch_pt_corr(const CorrParams& p, bool check, hipStream_t s);
 // Two correction sweeps per pass: (vx_a, vy_a, p_a, mx, my) -> (vx_b, vy_b, p_b, mx_o, my_o);
 // p.seg = rows per tile (<= 0: default).  Bit-identical to two launch_pt_corr.
 hipError_t launch_pt_corr2(const CorrParams& p, hipStream_t s);
-// Three correction sweeps per pass (corrk_tile<3>, 58 output columns per strip).
-hipError_t launch_pt_corr3(const CorrParams& p, hipStream_t s);
+// k = 3 or 4 correction sweeps per pass (corrk_tile<k>, 64 - 2k output columns per strip).
+hipError_t launch_pt_corrk(const CorrParams& p, int k, hipStream_t s);
 hipError_t launch_resid_f64(const ResidParams& p, int nb, hipStream_t s);
 hipError_t launch_update_f64(double* a, const float* d, int64_t n, hipStream_t s);
 hipError_t launch_cast_f32(float* dst, const double* src, int64_t n, hipStream_t s);

@@ -803,17 +803,20 @@ hipError_t launch_pt_corr2(const CorrParams& p0, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_pt_corr3(const CorrParams& p0, hipStream_t s)
+hipError_t launch_pt_corrk(const CorrParams& p0, int k, hipStream_t s)
 {
+    if (k != 3 && k != 4) return hipErrorInvalidValue;
     CorrParams p = p0;
     if (p.seg <= 0) {
         const int r = std::max(p.nx, p.ny) > 3072 ? 48 : 16;
         p.seg = p.ny >= r ? r : p.ny;
     }
-    p.nstrips = (p.nx - 1 + 58 - 1) / 58;
+    const int out = 64 - 2 * k;
+    p.nstrips = (p.nx - 1 + out - 1) / out;
     const int nseg = (p.ny + p.seg - 1) / p.seg;
     p.ntiles = p.nstrips * nseg;
-    hipLaunchKernelGGL(pt_corrk_f32<3>, dim3(p.ntiles), dim3(64), 0, s, p);
+    if (k == 3) hipLaunchKernelGGL(pt_corrk_f32<3>, dim3(p.ntiles), dim3(64), 0, s, p);
+    else hipLaunchKernelGGL(pt_corrk_f32<4>, dim3(p.ntiles), dim3(64), 0, s, p);
     return hipGetLastError();
 }
 
