@@ -432,14 +432,18 @@ def main():
     gpu_s = ev0.elapsed_time(ev1) * 1e-3
     finite = bool(torch.isfinite(st.Vx_small).all().item())
 
-    # The dominant kernel is the two-iteration pass (pt2_tile_f64): an even K runs as K/2 - 1 such
-    # launches plus two one-iteration launches (iterations K-1 and K; the last one also writes tau),
-    # so the HIP-event time of the timed region per two iterations is its launch time (to 0.2%).
-    per_launch = gpu_s * 2 / a.steps
+    # The dominant kernel runs k_pass PT iterations per launch (the library's size default: three
+    # above 1536 nodes per side, ptk_tile_f64<3>; two below, pt2_tile_f64). Apart from a few
+    # shorter launches (iteration 10 before the timing mark at 11, and the last iterations), the
+    # timed region is made of such launches, so its HIP-event time per k_pass iterations is the
+    # launch time (to about 0.5%).
+    k_pass = 3 if max(nx, ny) > 1536 else 2
+    kernel = "ptk_tile_f64<3>" if k_pass == 3 else "pt2_tile_f64"
+    per_launch = gpu_s * k_pass / a.steps
     probe = sfhip.probe_dram(1 << 30)  # the box's DRAM ceiling for simple streams
     b_iter = bytes_per_iter(nx, ny)
-    achieved = 2 * b_iter / per_launch / 1e9  # per-iteration algorithmic bytes x 2 iterations per launch
-    traffic = profiled_traffic(nx)
+    achieved = k_pass * b_iter / per_launch / 1e9  # per-iteration algorithmic bytes x iterations per launch
+    traffic = profiled_traffic(nx, "ptk3_tile_f64" if k_pass == 3 else "pt2_tile_f64")
     line = {
         "metric": "PT iters/s + T_eff HBM GB/s; time-to-1e-8 FP32-refined vs FP64; batched solves/s",
         "value": a.steps / wall,
@@ -459,13 +463,13 @@ def main():
         "t_eff_gbs": 6 * nx * ny * 8 / (wall / a.steps) / 1e9,
         "alg_gbs": b_iter / (wall / a.steps) / 1e9,
         "state_finite": finite,
-        "pt_launch_shape": tuned or "size default: two iterations per launch, 16-row tiles above 2048^2",
+        "pt_launch_shape": tuned or (f"size default: {k_pass} PT iterations per launch"),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "pt2_tile_f64 (two PT iterations per launch)",
-                     "bytes_alg_per_launch": 2 * b_iter, "launch_s": per_launch,
+                     "kernel": f"{kernel} ({k_pass} PT iterations per launch)",
+                     "bytes_alg_per_launch": k_pass * b_iter, "launch_s": per_launch,
                      "note": "achieved counts the one-iteration algorithmic bytes (13 x 8 B per node) for each of "
-                             "the two iterations a launch runs; the launch reads and writes the state once, so "
+                             "the iterations a launch runs; the launch reads and writes the state once, so "
                              "achieved can exceed the HBM peak. hbm_gbs is the measured DRAM rate.",
                      "hbm_gbs": (traffic / per_launch / 1e9) if traffic else None,
                      "min_bytes_per_launch": b_iter,
