@@ -88,6 +88,22 @@ int sf_ctx_set_pt_tuning(sf_ctx *ctx, int rows, int depth, int max_waves);
  * reference counterpart (the reference launches each of its kernels once per iteration). */
 int sf_ctx_set_pt_fusion(sf_ctx *ctx, int iters_per_launch, int rows);
 
+/* Snapshot files of the timestep loop, the files viz.jl reads (viz.jl:11-161), as MAT-file
+ * level 5 (MAT.jl reads it; the reference's MAT.jl writes the HDF5-based v7.3 layout).
+ * Replaces saveStats (StokesFlow2D_multixpu.jl:212-223): "dims" (3 x 1 Int64), "nt" (1 x 1
+ * Int64); host only. */
+int sf_save_stats(const char *path, const int64_t dims[3], int64_t nt);
+/* Replaces saveMarkers (:230-240): Float32 "x_m" = x_m + x_shift, "y_m" = y_m + y_shift,
+ * "rho_m" (nm x 1 each) from device arrays (x_shift = (lx - dx) * coords[1] in the reference,
+ * 0 at one rank).  Streams through pinned memory in 32 MiB chunks on the context's stream. */
+int sf_save_markers(sf_ctx *ctx, const char *path, const double *x_m, const double *y_m, const double *rho_m,
+                    int64_t nm, double x_shift, double y_shift);
+/* Replaces saveGrid (:247-299): Float32 node coordinates x, y, x_p, y_p, x_vx, y_vx, x_vy, y_vy
+ * (:117-124) and the device grids mu_b (Nx x Ny), P (Nx-1 x Ny-1), Vx (padded, Nx+2 x Ny+1),
+ * Vy (padded, Nx+1 x Ny+2). */
+int sf_save_grid(sf_ctx *ctx, const char *path, int nx, int ny, double dx, double dy, const double *mu_b,
+                 const double *P, const double *Vx_pad, const double *Vy_pad);
+
 /* Autotuner of that launch shape: times `iters` PT iterations of every candidate (rows 1, 2,
  * 4, 8, 16 x depth 1, 2 x one wave per tile or persistent) on a copy of the caller's problem
  * (P ... mu_p as for sf_pt_iterations; the caller's arrays are not modified) or, when any of

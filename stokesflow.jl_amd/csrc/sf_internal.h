@@ -73,6 +73,8 @@ hipError_t launch_pt_pair(const PTParams& p, hipStream_t s);
 int pt2_default_rows(int nx, int ny);
 // k = 2 or 3 PT iterations per pass (k = 3: ptk_tile<3>, 58 output columns per strip).
 hipError_t launch_pt_multi(const PTParams& p, int k, hipStream_t s);
+// Snapshot staging: dst = (float)(src + shift) (sf_snapshot.hip).
+hipError_t launch_snap_cast(const double* src, float* dst, int64_t n, double shift, hipStream_t s);
 int pt_default_fusion(int nx, int ny);  // iterations per pass when the context leaves it open
 hipError_t launch_pt_resid_y(const ResYParams& p, hipStream_t s);
 hipError_t launch_reduce(const RedParams& p, int narr, int nb, hipStream_t s);

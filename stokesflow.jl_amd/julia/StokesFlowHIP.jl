@@ -8,6 +8,8 @@ argument order and return values:
   * `solveStokes!`          -- scripts/StokesSolver_multixpu.jl:14-96
   * `bilinearMarkerToGrid!` -- scripts/MarkerToGrid.jl:18-38
   * `moveMarkersRK4!`       -- scripts/MoveMarkers.jl:15-23
+  * `saveStats`, `saveMarkers`, `saveGrid` -- scripts/StokesFlow2D_multixpu.jl:212-299
+    (MAT-file level 5, which MAT.jl and viz.jl read)
 
 Arrays are `AMDGPU.ROCArray{Float64}` (column-major, x fastest: the reference's
 `Data.Array` layout), passed to the library as device pointers.  One rank: `comm` and
@@ -20,7 +22,7 @@ module StokesFlowHIP
 
 using AMDGPU
 
-export solveStokes!, bilinearMarkerToGrid!, moveMarkersRK4!
+export solveStokes!, bilinearMarkerToGrid!, moveMarkersRK4!, saveStats, saveMarkers, saveGrid
 
 const libsfhip = joinpath(@__DIR__, "..", "libsfhip.so")
 
@@ -141,6 +143,44 @@ function moveMarkersRK4!(x_m, y_m, Vx, Vy, x_vx_min, y_vx_min, x_vy_min, y_vy_mi
                  Ptr{RK4Out}),
                 h, dptr(x_m), dptr(y_m), Nm, dptr(Vx), size(Vx, 1), size(Vx, 2), dptr(Vy), size(Vy, 1),
                 size(Vy, 2), x_vx_min, y_vx_min, x_vy_min, y_vy_min, dt, lx, ly, dx, dy, out))
+    return nothing
+end
+
+"""
+    saveStats(nt, rank, dims)   -- StokesFlow2D_multixpu.jl:212-223 (writes viz_out/stats.mat on rank 0)
+"""
+function saveStats(nt, rank, dims)
+    (rank != 0) && return nothing
+    d = Int64[dims[1], dims[2], dims[3]]
+    check(ccall((:sf_save_stats, libsfhip), Cint, (Cstring, Ptr{Int64}, Int64), "viz_out/stats.mat", d, nt))
+    return nothing
+end
+
+"""
+    saveMarkers(nt, rank, coords, localDomain, dx, dy, x_m, y_m, ρ_m)   -- :230-240
+"""
+function saveMarkers(nt, rank, coords, localDomain, dx, dy, x_m, y_m, ρ_m)
+    path = string("viz_out/markers_", lpad(nt, 4, '0'), "_", lpad(rank, 4, '0'), ".mat")
+    h = context(0, 0)
+    check(ccall((:sf_save_markers, libsfhip), Cint,
+                (Ptr{Cvoid}, Cstring, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Cdouble, Cdouble),
+                h, path, dptr(x_m), dptr(y_m), dptr(ρ_m), length(x_m), (localDomain[1] - dx) * coords[1],
+                (localDomain[2] - dy) * coords[2]))
+    return nothing
+end
+
+"""
+    saveGrid(nt, rank, x, y, μ_b, x_p, y_p, P, x_vx, y_vx, Vx, x_vy, y_vy, Vy)   -- :247-299
+
+The coordinate vectors are recomputed by the library from the grid step (x[2] - x[1]).
+"""
+function saveGrid(nt, rank, x, y, μ_b, x_p, y_p, P, x_vx, y_vx, Vx, x_vy, y_vy, Vy)
+    path = string("viz_out/grid_", lpad(nt, 4, '0'), "_", lpad(rank, 4, '0'), ".mat")
+    h = context(0, 0)
+    check(ccall((:sf_save_grid, libsfhip), Cint,
+                (Ptr{Cvoid}, Cstring, Cint, Cint, Cdouble, Cdouble, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                 Ptr{Float64}),
+                h, path, length(x), length(y), x[2] - x[1], y[2] - y[1], dptr(μ_b), dptr(P), dptr(Vx), dptr(Vy)))
     return nothing
 end
 

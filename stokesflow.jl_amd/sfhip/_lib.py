@@ -19,7 +19,8 @@ SF_ERR_SIZE = -4
 
 # every exported symbol of include/sf_hip.h
 EXPORTS = (
-    "sf_version", "sf_last_error", "sf_ctx_create", "sf_ctx_destroy", "sf_ctx_set_stream", "sf_ctx_set_pt_tuning", "sf_ctx_set_pt_fusion",
+    "sf_version", "sf_last_error", "sf_ctx_create", "sf_ctx_destroy", "sf_ctx_set_stream", "sf_ctx_set_pt_tuning", "sf_ctx_set_pt_fusion", "sf_save_stats", "sf_save_markers",
+    "sf_save_grid",
     "sf_solve_stokes",
     "sf_pt_iterations", "sf_marker_to_grid", "sf_move_markers_rk4", "sf_step", "sf_init_markers",
     "sf_plume_properties", "sf_rt_properties", "sf_solve_stokes_refined", "sf_solve_stokes_batched",
@@ -134,6 +135,9 @@ def load():
     L.sf_ctx_set_stream.argtypes = [_vp, _vp]
     L.sf_ctx_set_pt_tuning.argtypes = [_vp, _i, _i, _i]
     L.sf_ctx_set_pt_fusion.argtypes = [_vp, _i, _i]
+    L.sf_save_stats.argtypes = [C.c_char_p, C.POINTER(C.c_int64), C.c_int64]
+    L.sf_save_markers.argtypes = [_vp, C.c_char_p, _vp, _vp, _vp, C.c_int64, _d, _d]
+    L.sf_save_grid.argtypes = [_vp, C.c_char_p, _i, _i, _d, _d, _vp, _vp, _vp, _vp]
     L.sf_solve_stokes.argtypes = [_vp] + [_vp] * 19 + [_d, _d, _d, _i, _i, _d, _d, _i, _d, _i64, _i,
                                                         C.POINTER(SolveOut)]
     L.sf_solve_stokes_refined.argtypes = [_vp] + [_vp] * 19 + [_d, _d, _d, _i, _i, _d, _d, _i, _d,

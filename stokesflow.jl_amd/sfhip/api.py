@@ -19,6 +19,7 @@ with a count and returned in ``stats``.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 import torch
@@ -538,22 +539,68 @@ class Ensemble:
             pass
 
 
+def _path(path):
+    return os.fsencode(os.fspath(path))
+
+
+def saveStats(path, dims, nt):
+    """saveStats (StokesFlow2D_multixpu.jl:212-223) through sf_save_stats: "dims" (3 Int64) and
+    "nt" in a MAT-file level 5 file."""
+    d = (C.c_int64 * 3)(*[int(v) for v in dims])
+    check(_lib.load().sf_save_stats(_path(path), d, int(nt)))
+
+
+def saveMarkers(path, x_m, y_m, rho_m, x_shift=0.0, y_shift=0.0):
+    """saveMarkers (:230-240) through sf_save_markers: Float32 x_m + x_shift, y_m + y_shift, rho_m
+    from device arrays (x_shift = (lx - dx) * coords[1] in the reference; 0 at one rank)."""
+    nm = x_m.numel()
+    for a in (y_m, rho_m):
+        if a.numel() != nm:
+            raise ValueError("saveMarkers: marker arrays differ in length")
+    ptrs = [_marker_ptr(a, n) for a, n in ((x_m, "x_m"), (y_m, "y_m"), (rho_m, "rho_m"))]
+    ctx = context(0, 0)
+    check(ctx.lib.sf_save_markers(ctx.handle, _path(path), *ptrs, int(nm), float(x_shift), float(y_shift)))
+
+
+def saveGrid(path, Nx, Ny, dx, dy, mu_b, P, Vx, Vy):
+    """saveGrid (:247-299) through sf_save_grid: Float32 node coordinates (:117-124) and mu_b, P
+    and the padded Vx, Vy."""
+    ptrs = [_grid_ptr(mu_b, (Nx, Ny), "mu_b"), _grid_ptr(P, (Nx - 1, Ny - 1), "P"),
+            _grid_ptr(Vx, (Nx + 2, Ny + 1), "Vx"), _grid_ptr(Vy, (Nx + 1, Ny + 2), "Vy")]
+    ctx = context(0, 0)
+    check(ctx.lib.sf_save_grid(ctx.handle, _path(path), int(Nx), int(Ny), float(dx), float(dy), *ptrs))
+
+
 def StokesFlow2D(Nt, Nx, Ny, Lx_glob, Ly_glob, x_m, y_m, rho_m, mu_m, *, stokes_eps=1e-5, g_y=9.81, maxdisp=0.5,
-                 iterMax=100000, print_info=False, state=None):
+                 iterMax=100000, print_info=False, state=None, save_to_file=False, out_dir="viz_out"):
     """Timestep loop of StokesFlow2D (StokesFlow2D_multixpu.jl:55-235) at one rank.
 
     The reference evaluates density/viscosity callbacks on a marker lattice it builds
     itself; here the caller passes the marker arrays (see :func:`init_markers` and the
     material helpers).  Returns (Vx, Vy, t_tot) with Vx, Vy the padded velocity arrays.
+    save_to_file writes the reference's snapshot files into out_dir (stats.mat, and
+    markers_%04d_0000.mat / grid_%04d_0000.mat for step 0 and after every step, :137-141,
+    :196-200), read by viz.jl.
     """
     dx = Lx_glob / (Nx - 1)
     dy = Ly_glob / (Ny - 1)
     st = state if state is not None else FlowState(Nx, Ny, device=x_m.device)
+
+    def snapshot(t):
+        saveStats(os.path.join(out_dir, "stats.mat"), (1, 1, 1), t)
+        saveMarkers(os.path.join(out_dir, f"markers_{t:04d}_0000.mat"), x_m, y_m, rho_m)
+        saveGrid(os.path.join(out_dir, f"grid_{t:04d}_0000.mat"), Nx, Ny, dx, dy, st.mu_b, st.P, st.Vx, st.Vy)
+
+    if save_to_file:
+        os.makedirs(out_dir, exist_ok=True)
+        snapshot(0)
     dt, t_tot = 0.0, 0.0
     for t in range(Nt):
         if print_info:
             print(f"(t, t_tot) = ({t + 1}, {t_tot})")
         dt, stats = step(st, x_m, y_m, rho_m, mu_m, dx, dy, dt, g_y=g_y, maxdisp=maxdisp, eps=stokes_eps,
                          iterMax=iterMax)
+        if save_to_file:
+            snapshot(t + 1)
         t_tot += dt
     return st.Vx, st.Vy, t_tot
