@@ -34,11 +34,44 @@ __device__ inline double jl_max(double a, double b)
     return a > b ? a : b;
 }
 
-// Value of lane (lane + d) within the 64-wide wave; lanes past the end keep their own value.
-template <typename T>
-__device__ inline T lane_down(T v, int d)
+// One-lane wave shifts as DPP moves (GFX9 DPP wave_shl:1 = 0x130, wave_shr:1 = 0x138): a VALU
+// move instead of an LDS permute.  wave_shl:1 gives lane i the value of lane i+1 and
+// wave_shr:1 that of lane i-1; the lane without a source keeps its own value (bound_ctrl
+// off, old = src), exactly as __shfl_down(v, 1) / __shfl_up(v, 1) do on a 64-wide wave.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v)
 {
-    return __shfl_down(v, (unsigned)d, SF_WAVE);
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift(float v)
+{
+    return __int_as_float(dpp_i32<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double v)
+{
+    const int lo = dpp_i32<CTRL>(__double2loint(v));
+    const int hi = dpp_i32<CTRL>(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// Value of lane (lane + d) within the 64-wide wave.  A shift by d > 1 repeats the one-lane
+// shift, so the last d lanes hold lane 63's value (their own, for d = 1); no kernel stores a
+// result of those halo lanes.
+template <typename T>
+__device__ __forceinline__ T lane_down(T v, int d)
+{
+    for (int k = 0; k < d; ++k) v = dpp_shift<0x130>(v);
+    return v;
+}
+
+// Value of lane (lane - d); the first d lanes hold lane 0's value.
+template <typename T>
+__device__ __forceinline__ T lane_up(T v, int d)
+{
+    for (int k = 0; k < d; ++k) v = dpp_shift<0x138>(v);
+    return v;
 }
 
 }  // namespace sf

@@ -36,13 +36,6 @@ namespace sf {
 
 #define PT2_OUT_LANES 60
 
-// Value of lane (lane - d) within the wave; lanes below d keep their own value.
-template <typename T>
-__device__ inline T lane_up(T v, int d)
-{
-    return __shfl_up(v, (unsigned)d, SF_WAVE);
-}
-
 // The row window one PT iteration carries down a strip (pt_tile's register window).
 struct PTWin {
     double vx_r, vx_r_p1, vy_r, vy_r_p1;

@@ -214,7 +214,7 @@ __device__ __forceinline__ void corr_tile(const PP& p, int strip, int segid)
 
 #define CORR2_OUT_LANES 60
 
-__device__ inline float lane_up_f(float v, int d) { return __shfl_up(v, (unsigned)d, SF_WAVE); }
+__device__ inline float lane_up_f(float v, int d) { return lane_up(v, d); }
 
 struct CorrWin {
     float vx_r, vx_r_p1, vy_r, vy_r_p1;
