@@ -437,13 +437,13 @@ def main():
     # shorter launches (iteration 10 before the timing mark at 11, and the last iterations), the
     # timed region is made of such launches, so its HIP-event time per k_pass iterations is the
     # launch time (to about 0.5%).
-    k_pass = 3 if max(nx, ny) > 1536 else 2
-    kernel = "ptk_tile_f64<3>" if k_pass == 3 else "pt2_tile_f64"
+    k_pass = tuned["iters_per_launch"] if tuned else (3 if max(nx, ny) > 1536 else 2)
+    kernel = {1: "pt_tile_f64", 2: "pt2_tile_f64", 3: "ptk_tile_f64<3>"}[k_pass]
     per_launch = gpu_s * k_pass / a.steps
     probe = sfhip.probe_dram(1 << 30)  # the box's DRAM ceiling for simple streams
     b_iter = bytes_per_iter(nx, ny)
     achieved = k_pass * b_iter / per_launch / 1e9  # per-iteration algorithmic bytes x iterations per launch
-    traffic = profiled_traffic(nx, "ptk3_tile_f64" if k_pass == 3 else "pt2_tile_f64")
+    traffic = profiled_traffic(nx, {1: "pt_tile_f64", 2: "pt2_tile_f64", 3: "ptk3_tile_f64"}[k_pass])
     line = {
         "metric": "PT iters/s + T_eff HBM GB/s; time-to-1e-8 FP32-refined vs FP64; batched solves/s",
         "value": a.steps / wall,

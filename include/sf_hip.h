@@ -88,6 +88,30 @@ int sf_ctx_set_pt_tuning(sf_ctx *ctx, int rows, int depth, int max_waves);
  * reference counterpart (the reference launches each of its kernels once per iteration). */
 int sf_ctx_set_pt_fusion(sf_ctx *ctx, int iters_per_launch, int rows);
 
+/* Autotuner of the launch shapes: times `iters` PT iterations of every candidate on a copy of
+ * the caller's problem (P ... mu_p as for sf_pt_iterations; the caller's arrays are not
+ * modified) or, when any of them is NULL, on a synthetic one (materials 1, state 0), in scratch
+ * it allocates and frees.  Candidates: the one-iteration kernel (rows 1, 2, 4, 8, 16 x depth
+ * 1, 2 x one wave per tile or persistent; iters_per_launch = 1), then passes of 2 and 3
+ * iterations (rows 4, 8, 12, 16, 24, 32, 48).  The fastest one-iteration shape is set on the
+ * context (check iterations use it), and the fastest overall decides the iterations per
+ * launch (sf_ctx_set_pt_fusion).  Timing can depend on the data, so tune on the real problem.
+ * best (may be NULL) receives the fastest overall; table (may be NULL, table_cap entries)
+ * every candidate's time, n_table how many were timed.  No reference counterpart. */
+typedef struct {
+    int rows, depth, max_waves, iters_per_launch;
+    double us_per_iter;
+} sf_pt_tuning;
+int sf_ctx_autotune_pt(sf_ctx *ctx, int iters, const double *P, const double *Vx_s, const double *Vy_s,
+                       const double *dVxdtau, const double *dVydtau, const double *rho_vy, const double *mu_b,
+                       const double *mu_p, double g_y, double dx, double dy, sf_pt_tuning *best, sf_pt_tuning *table,
+                       int table_cap, int *n_table);
+
+/* DRAM probe: best-of-8 bandwidth of a copy (1 read + 1 write) and of a 3-read + 1-write
+ * kernel over four `bytes`-sized scratch arrays (allocated and freed here), in GB/s.  The
+ * ceiling the stencil kernels' bandwidth is read against.  No reference counterpart. */
+int sf_probe_dram(sf_ctx *ctx, int64_t bytes, double *copy_gbs, double *r3w1_gbs);
+
 /* Snapshot files of the timestep loop, the files viz.jl reads (viz.jl:11-161), as MAT-file
  * level 5 (MAT.jl reads it; the reference's MAT.jl writes the HDF5-based v7.3 layout).
  * Replaces saveStats (StokesFlow2D_multixpu.jl:212-223): "dims" (3 x 1 Int64), "nt" (1 x 1
@@ -103,27 +127,6 @@ int sf_save_markers(sf_ctx *ctx, const char *path, const double *x_m, const doub
  * Vy (padded, Nx+1 x Ny+2). */
 int sf_save_grid(sf_ctx *ctx, const char *path, int nx, int ny, double dx, double dy, const double *mu_b,
                  const double *P, const double *Vx_pad, const double *Vy_pad);
-
-/* Autotuner of that launch shape: times `iters` PT iterations of every candidate (rows 1, 2,
- * 4, 8, 16 x depth 1, 2 x one wave per tile or persistent) on a copy of the caller's problem
- * (P ... mu_p as for sf_pt_iterations; the caller's arrays are not modified) or, when any of
- * them is NULL, on a synthetic one (materials 1, state 0), in scratch it allocates and frees;
- * the fastest is set on the context.  Timing can depend on the data, so tune on the real
- * problem.  best (may be NULL) receives it; table (may be NULL, table_cap entries) every
- * candidate's time, n_table how many were timed.  No reference counterpart. */
-typedef struct {
-    int rows, depth, max_waves, reserved;
-    double us_per_iter;
-} sf_pt_tuning;
-int sf_ctx_autotune_pt(sf_ctx *ctx, int iters, const double *P, const double *Vx_s, const double *Vy_s,
-                       const double *dVxdtau, const double *dVydtau, const double *rho_vy, const double *mu_b,
-                       const double *mu_p, double g_y, double dx, double dy, sf_pt_tuning *best, sf_pt_tuning *table,
-                       int table_cap, int *n_table);
-
-/* DRAM probe: best-of-8 bandwidth of a copy (1 read + 1 write) and of a 3-read + 1-write
- * kernel over four `bytes`-sized scratch arrays (allocated and freed here), in GB/s.  The
- * ceiling the stencil kernels' bandwidth is read against.  No reference counterpart. */
-int sf_probe_dram(sf_ctx *ctx, int64_t bytes, double *copy_gbs, double *r3w1_gbs);
 
 /* solveStokes! -- replaces scripts/StokesSolver_multixpu.jl:14-96
  * (solveStokes!(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtauPt, Rx, Ry,

@@ -614,8 +614,9 @@ extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, const double* P_in, cons
         release();
         return set_err(SF_ERR_HIP, "%s: %s", who, hipGetErrorString(e));
     }
-    const int saved_seg = c->seg, saved_depth = c->depth, saved_waves = c->max_waves, saved_fuse = c->fuse;
-    c->fuse = 1;  // the candidates are shapes of the one-iteration kernel
+    const int saved_seg = c->seg, saved_depth = c->depth, saved_waves = c->max_waves, saved_fuse = c->fuse,
+              saved_prows = c->pair_rows;
+    c->fuse = 1;  // first the shapes of the one-iteration kernel
     const int rows_c[] = {1, 2, 4, 8, 16};
     const int depth_c[] = {1, 2};
     const int waves_c[] = {-1, 0};
@@ -644,24 +645,63 @@ extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, const double* P_in, cons
                 t.rows = rows;
                 t.depth = depth;
                 t.max_waves = waves;
+                t.iters_per_launch = 1;
                 t.us_per_iter = 1e3 * (double)ms / iters;
                 if (table && nt < table_cap) table[nt] = t;
                 ++nt;
                 if (t.us_per_iter < bestv.us_per_iter) bestv = t;
             }
+    // then passes of 2 and 3 iterations (the check iterations keep the best one-iteration shape)
+    sf_pt_tuning best_all = bestv;
+    if (rc == SF_OK) {
+        c->seg = bestv.rows;
+        c->depth = bestv.depth;
+        c->max_waves = bestv.max_waves;
+        const int prows_c[] = {4, 8, 12, 16, 24, 32, 48};
+        const int n_it = std::max(6, (iters + 5) / 6 * 6);  // whole passes of either kind
+        for (int k : {2, 3})
+            for (int rows : prows_c) {
+                if (rows > ny && rows != prows_c[0]) continue;
+                c->fuse = k;
+                c->pair_rows = rows;
+                rc = sf_pt_iterations(c, k, P, vx, vy, dvx, dvy, rho, mub, mup, gy, dx, dy, nx, ny, 0.0);
+                if (rc) break;
+                if (hipEventRecord(c->ev0, s) != hipSuccess) { rc = set_err(SF_ERR_HIP, "%s: event", who); break; }
+                rc = sf_pt_iterations(c, n_it, P, vx, vy, dvx, dvy, rho, mub, mup, gy, dx, dy, nx, ny, 0.0);
+                if (rc) break;
+                float ms = 0.f;
+                if (hipEventRecord(c->ev1, s) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) {
+                    rc = set_err(SF_ERR_HIP, "%s: event timing failed", who);
+                    break;
+                }
+                sf_pt_tuning t{};
+                t.rows = rows;
+                t.depth = 1;
+                t.max_waves = -1;
+                t.iters_per_launch = k;
+                t.us_per_iter = 1e3 * (double)ms / n_it;
+                if (table && nt < table_cap) table[nt] = t;
+                ++nt;
+                if (t.us_per_iter < best_all.us_per_iter) best_all = t;
+            }
+    }
     (void)hipStreamSynchronize(s);
     release();
-    c->fuse = saved_fuse;
     if (rc) {
         c->seg = saved_seg;
         c->depth = saved_depth;
         c->max_waves = saved_waves;
+        c->fuse = saved_fuse;
+        c->pair_rows = saved_prows;
         return rc;
     }
     c->seg = bestv.rows;
     c->depth = bestv.depth;
     c->max_waves = bestv.max_waves;
-    if (best) *best = bestv;
+    c->fuse = best_all.iters_per_launch;
+    c->pair_rows = best_all.iters_per_launch > 1 ? best_all.rows : 0;
+    if (best) *best = best_all;
     if (n_table) *n_table = nt;
     return SF_OK;
 }

@@ -95,10 +95,10 @@ class BatchInstance(C.Structure):
 
 
 class PTTuning(C.Structure):
-    _fields_ = [("rows", _i), ("depth", _i), ("max_waves", _i), ("reserved", _i), ("us_per_iter", _d)]
+    _fields_ = [("rows", _i), ("depth", _i), ("max_waves", _i), ("iters_per_launch", _i), ("us_per_iter", _d)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 SF_BATCH_AUTO, SF_BATCH_PERSISTENT, SF_BATCH_WIDE = 0, 1, 2
