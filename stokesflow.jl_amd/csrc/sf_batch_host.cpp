@@ -823,7 +823,9 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
     if (precision == 1 && mode == SF_BATCH_PERSISTENT)
         return set_err(SF_ERR_ARG, "%s: FP32 refinement runs in the wide mode only", who);
     if (precision == 1) {
-        const double budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 192.0 * 1048576.0;
+        // the refined working sets are FP32; a larger active set measured 3-4% faster (64 mixed
+        // instances: 2.92 s at 192 MiB, 2.84 s at 384 MiB, 2.81 s at 768 MiB)
+        const double budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 512.0 * 1048576.0;
         size_t nf = 0;
         std::vector<size_t> foff((size_t)n);
         for (int i = 0; i < n; ++i) {

@@ -198,7 +198,8 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
         return set_err(SF_ERR_ARG, "%s: FP32 refinement needs the wide form (precision %d)", who, prm->precision);
     }
     e->depth = (prm && prm->depth > 1) ? 2 : 1;
-    e->budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 192.0 * 1048576.0;
+    e->budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0
+                                              : (e->refined ? 512.0 : 192.0) * 1048576.0;
     e->mem.assign(members, members + n);
     e->bi = bi;
     hipError_t er = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);

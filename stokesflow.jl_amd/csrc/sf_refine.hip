@@ -18,6 +18,7 @@
 // register row-march, lane shifts for x neighbours) on 16 FP32 arrays per node: read
 // Vd, Pd, md (x, y), mu_p, mu_b, rho, forcing (x, y, C); write Vd, Pd, md.
 #include <algorithm>
+#include <type_traits>
 
 #include "sf_device.h"
 #include "sf_internal.h"
@@ -490,7 +491,11 @@ __device__ __forceinline__ void corrk_tile(const PP& p, int strip, int segid)
                   (p.rho + (int64_t)r * (nx + 1))[lc.vy]);
     CorrRowIn nxt = corr_load_row(p, lc, r, r >= r0 - (K - 1));
 
-    for (; r <= r_last; ++r) {
+    // One outer step (stage s runs row r - s).  ALL: every stage runs a real row that is not its
+    // first and stage 0 prefetches the next row -- the steady middle of the tile, compiled
+    // without the per-stage range tests.
+    auto step = [&](const int r, auto all_tag) {
+        constexpr bool ALL = decltype(all_tag)::value;
         float cur_vx[K], cur_vy[K], cur_rho[K];
         CorrHand hn[K];
         bool ran[K];
@@ -500,18 +505,20 @@ __device__ __forceinline__ void corrk_tile(const PP& p, int strip, int segid)
             ran[s] = false;
             cur_vx[s] = cur_vy[s] = cur_rho[s] = 0.f;
             hn[s] = h[s];
-            if (rs < rs0[s] || rs > rs1[s]) {
-                if (s < K - 1 && rs == ny && rs - 1 >= rs0[s]) {
-                    cur_vx[s] = vx_last[s];
-                    cur_vy[s] = 0.f;
-                    cur_rho[s] = h[s].rho;
+            if constexpr (!ALL) {
+                if (rs < rs0[s] || rs > rs1[s]) {
+                    if (s < K - 1 && rs == ny && rs - 1 >= rs0[s]) {
+                        cur_vx[s] = vx_last[s];
+                        cur_vy[s] = 0.f;
+                        cur_rho[s] = h[s].rho;
+                    }
+                    continue;
                 }
-                continue;
             }
             CorrRowIn in;
             if (s == 0) {
                 in = nxt;
-                if (r + 1 <= rs1[0]) nxt = corr_load_row(p, lc, r + 1, true);
+                if (ALL || r + 1 <= rs1[0]) nxt = corr_load_row(p, lc, r + 1, true);
             } else {
                 in.vx1 = cur_vx[s - 1];
                 in.vy1 = cur_vy[s - 1];
@@ -524,7 +531,7 @@ __device__ __forceinline__ void corrk_tile(const PP& p, int strip, int segid)
                 in.my = h[s - 1].my;
                 in.fx = h[s - 1].fx;
                 in.fy = h[s - 1].fy;
-                if (rs == rs0[s]) {
+                if (!ALL && rs == rs0[s]) {
                     const float vx0 = (rs == 0) ? in.vx1 : h[s - 1].vx;
                     const float vy0 = (rs == 0) ? 0.f : h[s - 1].vy;
                     corr_win_init(w[s], vx0, vy0, h[s - 1].rho);
@@ -565,10 +572,19 @@ __device__ __forceinline__ void corrk_tile(const PP& p, int strip, int segid)
         }
 #pragma unroll
         for (int s = 0; s < K - 1; ++s) {
-            if (ran[s]) vx_last[s] = cur_vx[s];
+            if (ALL || ran[s]) vx_last[s] = cur_vx[s];
             h[s] = hn[s];
         }
+    };
+    int lo = 0, hi = rs1[0] - 1;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        lo = max(lo, rs0[s] + s + 1);
+        hi = min(hi, rs1[s] + s);
     }
+    for (; r < lo && r <= r_last; ++r) step(r, std::false_type{});
+    for (; r <= hi; ++r) step(r, std::true_type{});
+    for (; r <= r_last; ++r) step(r, std::false_type{});
 }
 
 template <int K>
