@@ -29,7 +29,7 @@ static void free_ctx(sf_ctx* c)
     if (!c) return;
     void* bufs[] = {c->dvx_b, c->dvy_b, c->vx_b, c->vy_b, c->p_b, c->txx, c->tyy, c->txy, c->divv, c->rx, c->ry, c->part, c->scal,
                     c->counters, c->m2g.key, c->m2g.count, c->m2g.offset, c->m2g.fill, c->m2g.rec_idx,
-                    c->m2g.rec_dx, c->m2g.rec_dy, c->m2g.rec_val, c->m2g.blocksums, c->m2g.counters};
+                    c->m2g.rec, c->m2g.blocksums, c->m2g.counters};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     void* rbufs[] = {c->r_mx2, c->r_my2, c->r_vx[0], c->r_vx[1], c->r_vy[0], c->r_vy[1], c->r_p[0], c->r_p[1], c->r_mx, c->r_my,
@@ -751,7 +751,7 @@ extern "C" int sf_probe_dram(sf_ctx* c, int64_t bytes, double* copy_gbs, double*
 
 void sf::m2g_free(M2GScratch& S)
 {
-    for (void* p : {(void*)S.key, (void*)S.rec_idx, (void*)S.rec_dx, (void*)S.rec_dy, (void*)S.rec_val, (void*)S.count,
+    for (void* p : {(void*)S.key, (void*)S.rec_idx, (void*)S.rec, (void*)S.count,
                     (void*)S.offset, (void*)S.fill, (void*)S.blocksums, (void*)S.counters})
         if (p) (void)hipFree(p);
     S = M2GScratch();
@@ -760,17 +760,15 @@ void sf::m2g_free(M2GScratch& S)
 int sf::m2g_reserve(M2GScratch& S, int64_t nm, int64_t ncells)
 {
     if (nm > S.cap_markers) {
-        for (void* p : {(void*)S.key, (void*)S.rec_idx, (void*)S.rec_dx, (void*)S.rec_dy, (void*)S.rec_val})
+        for (void* p : {(void*)S.key, (void*)S.rec_idx, (void*)S.rec})
             if (p) HIPCHK(hipFree(p));
         S.key = S.rec_idx = nullptr;
-        S.rec_dx = S.rec_dy = S.rec_val = nullptr;
+        S.rec = nullptr;
         S.cap_markers = 0;
         const int64_t cap = nm + nm / 8;
         HIPCHK(dalloc(&S.key, (size_t)cap));
         HIPCHK(dalloc(&S.rec_idx, (size_t)cap));
-        HIPCHK(dalloc(&S.rec_dx, (size_t)cap));
-        HIPCHK(dalloc(&S.rec_dy, (size_t)cap));
-        HIPCHK(dalloc(&S.rec_val, (size_t)cap));
+        HIPCHK(dalloc(&S.rec, (size_t)cap));
         S.cap_markers = cap;
     }
     if (ncells > S.cap_cells) {

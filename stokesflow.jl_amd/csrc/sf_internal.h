@@ -211,6 +211,13 @@ hipError_t launch_batch_solve(const BatchInstance* inst, const int* order, int n
 int batch_blocks_per_cu(int threads);
 
 // ---- marker -> grid ----
+// One marker's contribution in cell order: the relative position in its cell, its value and
+// its index (the merge key).  32 bytes, so a merge step reads one record from one cache line.
+struct M2GRec {
+    double dx, dy, val;
+    int idx, pad;
+};
+
 struct M2GScratch {
     int64_t cap_markers = 0;
     int64_t cap_cells = 0;
@@ -219,9 +226,7 @@ struct M2GScratch {
     int* offset = nullptr;   // [cells + 1]
     int* fill = nullptr;     // [cells]
     int* rec_idx = nullptr;  // [nm] marker index in cell order
-    double* rec_dx = nullptr;
-    double* rec_dy = nullptr;
-    double* rec_val = nullptr;
+    M2GRec* rec = nullptr;   // [nm] records in cell order
     int* blocksums = nullptr;  // scan scratch
     int64_t cap_blocks = 0;
     unsigned long long* counters = nullptr;  // [4]: bad, empty, nonfinite, spare

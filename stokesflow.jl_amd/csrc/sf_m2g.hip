@@ -162,8 +162,7 @@ __global__ __launch_bounds__(256) void m2g_scatter(const int* __restrict__ key, 
 __global__ __launch_bounds__(256) void m2g_sort_cells(const int* __restrict__ off, int64_t ncells, int* __restrict__ rec_idx,
                                                       const double* __restrict__ x_m, const double* __restrict__ y_m,
                                                       const double* __restrict__ val_m, double xmin, double ymin,
-                                                      double dx, double dy, double* __restrict__ rec_dx,
-                                                      double* __restrict__ rec_dy, double* __restrict__ rec_val)
+                                                      double dx, double dy, M2GRec* __restrict__ rec)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncells) return;
@@ -182,9 +181,13 @@ __global__ __launch_bounds__(256) void m2g_sort_cells(const int* __restrict__ of
         int64_t ix, iy;
         double dxij, dyij;
         topleft(xmin, ymin, x_m[m], y_m[m], dx, dy, ix, iy, dxij, dyij);
-        rec_dx[s] = dxij;
-        rec_dy[s] = dyij;
-        rec_val[s] = val_m[m];
+        M2GRec r;
+        r.dx = dxij;
+        r.dy = dyij;
+        r.val = val_m[m];
+        r.idx = m;
+        r.pad = 0;
+        rec[s] = r;
     }
 }
 
@@ -200,10 +203,8 @@ __device__ inline double corner_weight(int corner, double dxij, double dyij)
     }
 }
 
-__global__ __launch_bounds__(256) void m2g_nodes(const int* __restrict__ off, const int* __restrict__ rec_idx,
-                                                 const double* __restrict__ rec_dx, const double* __restrict__ rec_dy,
-                                                 const double* __restrict__ rec_val, int gnx, int gny,
-                                                 double* __restrict__ val_grid, double* __restrict__ val_wt_sum,
+__global__ __launch_bounds__(256) void m2g_nodes(const int* __restrict__ off, const M2GRec* __restrict__ rec, int gnx,
+                                                 int gny, double* __restrict__ val_grid, double* __restrict__ val_wt_sum,
                                                  double* __restrict__ wt_sum, int ws_nx,
                                                  unsigned long long* __restrict__ counters)
 {
@@ -219,7 +220,7 @@ __global__ __launch_bounds__(256) void m2g_nodes(const int* __restrict__ off, co
     for (int q = 0; q < 4; ++q) {
         h[q] = off[kc[q]];
         e[q] = off[kc[q] + 1];
-        head[q] = (h[q] < e[q]) ? rec_idx[h[q]] : 0x7fffffff;
+        head[q] = (h[q] < e[q]) ? rec[h[q]].idx : 0x7fffffff;
     }
     double wt = 0.0, vw = 0.0;
     while (true) {
@@ -232,11 +233,12 @@ __global__ __launch_bounds__(256) void m2g_nodes(const int* __restrict__ off, co
             }
         if (best == 0x7fffffff) break;
         const int s = h[q];
-        const double w = corner_weight(q, rec_dx[s], rec_dy[s]);
+        const M2GRec r = rec[s];
+        const double w = corner_weight(q, r.dx, r.dy);
         wt = wt + w;
-        vw = vw + w * rec_val[s];
+        vw = vw + w * r.val;
         h[q] = s + 1;
-        head[q] = (h[q] < e[q]) ? rec_idx[h[q]] : 0x7fffffff;
+        head[q] = (h[q] < e[q]) ? rec[h[q]].idx : 0x7fffffff;
     }
     if (wt_sum) {
         wt_sum[(int64_t)a + (int64_t)b * ws_nx] = wt;
@@ -286,9 +288,9 @@ hipError_t m2g_run(M2GScratch& S, double xmin, double ymin, double* val_grid, in
         hipLaunchKernelGGL(m2g_scatter, dim3(grid1d(nm, 256)), dim3(256), 0, st, S.key, nm, S.offset, S.fill,
                            S.rec_idx);
     hipLaunchKernelGGL(m2g_sort_cells, dim3(grid1d(ncells, 256)), dim3(256), 0, st, S.offset, ncells, S.rec_idx, x_m,
-                       y_m, val_m, xmin, ymin, dx, dy, S.rec_dx, S.rec_dy, S.rec_val);
-    hipLaunchKernelGGL(m2g_nodes, dim3(grid1d(gny, 256), gnx), dim3(256), 0, st, S.offset, S.rec_idx, S.rec_dx,
-                       S.rec_dy, S.rec_val, gnx, gny, val_grid, val_wt_sum, wt_sum, ws_nx, S.counters);
+                       y_m, val_m, xmin, ymin, dx, dy, S.rec);
+    hipLaunchKernelGGL(m2g_nodes, dim3(grid1d(gny, 256), gnx), dim3(256), 0, st, S.offset, S.rec, gnx, gny,
+                       val_grid, val_wt_sum, wt_sum, ws_nx, S.counters);
     return hipGetLastError();
 }
 
