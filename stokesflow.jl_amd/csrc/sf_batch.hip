@@ -14,7 +14,7 @@
 // prepared on the host) are read through the constant address space: scalar loads that the
 // compiler re-issues where needed instead of holding ~70 uniform values in registers
 // across the solve loop.
-#include "sf_pt2_tile.h"
+#include "sf_ptk_tile.h"
 
 namespace sf {
 
@@ -254,6 +254,33 @@ hipError_t launch_wide_pt(const WideArgs& a, int depth, hipStream_t s)
     if (total <= 0) return hipSuccess;
     if (depth > 1) hipLaunchKernelGGL(wide_pt_kernel<2>, dim3(total), dim3(64), 0, s, a);
     else hipLaunchKernelGGL(wide_pt_kernel<1>, dim3(total), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+// Three PT iterations of every active instance per launch (ptk_tile<3>).
+__global__ __launch_bounds__(64) void wide_pt3_kernel(WideArgs a)
+{
+    const int total = a.tile_start[a.n_active];
+    const int t_glob = xcd_remap(blockIdx.x, gridDim.x);
+    if (t_glob >= total) return;
+    int lo = 0, hi = a.n_active - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.tile_start[mid] <= t_glob) lo = mid;
+        else hi = mid - 1;
+    }
+    const int t = t_glob - a.tile_start[lo];
+    CBatch* cinst = (CBatch*)a.desc;
+    const PTParams SF_CONST& pp = cinst[a.slot_inst[lo]].wpt[4 + (a.slot_pf[lo] & 1)];
+    const int nstrips = pp.nstrips;
+    ptk_tile<3>(pp, t % nstrips, t / nstrips);
+}
+
+hipError_t launch_wide_pt3(const WideArgs& a, hipStream_t s)
+{
+    const int total = a.tile_start[a.n_active];
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(wide_pt3_kernel, dim3(total), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

@@ -104,6 +104,11 @@ BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, 
         w.nstrips = (nx - 1 + 59) / 60;
         w.ntiles = w.nstrips * ((ny + rows2 - 1) / rows2);
         b.wpt[2 + par] = w;
+        const int rows3 = std::min(ny, std::max(nx, ny) > 256 ? 16 : 8);
+        w.seg = rows3;
+        w.nstrips = (nx - 1 + 57) / 58;
+        w.ntiles = w.nstrips * ((ny + rows3 - 1) / rows3);
+        b.wpt[4 + par] = w;
     }
     b.P = a.P;
     b.Vx_s = a.Vx_s;
@@ -174,7 +179,8 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         int id;
         int64_t iter, ncheck, ndtupdate;
         double err, ws;
-        bool checking, pair;
+        bool checking;
+        int k;            // iterations of this step's launch (1, 2 or 3)
         int64_t nlaunch;  // launches so far: their parity says where the state (V, P, dV) is
     };
     std::vector<Run> act;
@@ -185,7 +191,7 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         const sf_batch_instance& a = in[id];
         const int mn = std::min(a.nx, a.ny);
         Run r{id, 1, std::min<int64_t>(5 * (int64_t)mn, 2000), 10 * (int64_t)mn, 2.0 * a.eps,
-              batch_working_set(a.nx, a.ny), false, false, 0};
+              batch_working_set(a.nx, a.ny), false, 1, 0};
         HIPCHK(launch_set_scalar(desc[(size_t)id].scal + SC_DT, a.use_fss ? a.dt : 0.0, s));
         act.push_back(r);
         ws_active += r.ws;
@@ -218,32 +224,48 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         return SF_OK;
     };
     int rc = SF_OK;
-    WideArgs wa, w2;
+    WideArgs wa, w2, w3;
     std::memset(&wa, 0, sizeof(wa));
     std::memset(&w2, 0, sizeof(w2));
+    std::memset(&w3, 0, sizeof(w3));
     wa.desc = env.desc_dev;
     w2.desc = env.desc_dev;
+    w3.desc = env.desc_dev;
     while (!act.empty() || q < order.size()) {
         while (q < order.size() && (int)act.size() < WIDE_MAX &&
                (act.empty() || ws_active + batch_working_set(in[order[q]].nx, in[order[q]].ny) <= budget)) {
             rc = start(order[q++]);
             if (rc) return rc;
         }
-        // every active instance advances by two PT iterations in one pass where iterations iter
-        // and iter+1 are plain (as in sf_solve_stokes), else by one; the two kinds are two launches
+        // every active instance advances by three or two PT iterations in one pass where its next
+        // iterations are plain (the rule of sf_solve_stokes), else by one; each kind is a launch
         wa.n_active = 0;
         w2.n_active = 0;
-        int tiles = 0, tiles2 = 0;
+        w3.n_active = 0;
+        int tiles = 0, tiles2 = 0, tiles3 = 0;
         for (size_t k = 0; k < act.size(); ++k) {
             Run& r = act[k];
             const sf_batch_instance& a = in[r.id];
             const bool check = (r.iter % r.ncheck) == 0;
             const bool last = r.iter == a.iter_max;
-            r.pair = r.iter + 1 < a.iter_max && !check && (r.iter + 1) % r.ncheck != 0 &&
-                     !(a.use_fss && r.iter % r.ndtupdate == 0);
+            auto fusable = [&](int64_t kk) {
+                if (r.iter + kk - 1 >= a.iter_max) return false;
+                for (int64_t j = r.iter; j < r.iter + kk; ++j) {
+                    if (j % r.ncheck == 0) return false;
+                    if (j < r.iter + kk - 1 && a.use_fss && j % r.ndtupdate == 0) return false;
+                }
+                return true;
+            };
+            r.k = fusable(3) ? 3 : (fusable(2) ? 2 : 1);
             r.checking = check;
             const int par = (int)(r.nlaunch & 1);
-            if (r.pair) {
+            if (r.k == 3) {
+                w3.tile_start[w3.n_active] = tiles3;
+                w3.slot_inst[w3.n_active] = r.id;
+                w3.slot_pf[w3.n_active] = par;
+                tiles3 += desc[(size_t)r.id].wpt[4].ntiles;
+                w3.n_active += 1;
+            } else if (r.k == 2) {
                 w2.tile_start[w2.n_active] = tiles2;
                 w2.slot_inst[w2.n_active] = r.id;
                 w2.slot_pf[w2.n_active] = par;
@@ -260,6 +282,8 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         }
         wa.tile_start[wa.n_active] = tiles;
         w2.tile_start[w2.n_active] = tiles2;
+        w3.tile_start[w3.n_active] = tiles3;
+        if (w3.n_active) HIPCHK(launch_wide_pt3(w3, s));
         if (w2.n_active) HIPCHK(launch_wide_pt2(w2, s));
         if (wa.n_active) HIPCHK(launch_wide_pt(wa, depth, s));
         bool any_check = false;
@@ -267,7 +291,7 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
             const sf_batch_instance& a = in[r.id];
             const BatchInstance& d = desc[(size_t)r.id];
             const PTParams& p = d.wpt[r.nlaunch & 1];  // its outputs are now the current state
-            const int64_t it_done = r.pair ? r.iter + 1 : r.iter;  // the last iteration just run
+            const int64_t it_done = r.iter + r.k - 1;  // the last iteration just run
             const bool dt_upd = a.use_fss && (it_done % r.ndtupdate) == 0;
             if (dt_upd) {
                 rc = enqueue_dt_raw(s, a.nx, a.ny, p.vx_b, p.vy_b, a.dx, a.dy, a.maxdisp, 1, env.part, d.scal);
@@ -322,7 +346,7 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         keep.reserve(act.size());
         for (Run& r : act) {
             if (r.checking) r.err = env.hscal[(size_t)SC_COUNT * r.id + SC_ERR];
-            r.iter += r.pair ? 2 : 1;
+            r.iter += r.k;
             r.nlaunch += 1;
             const sf_batch_instance& a = in[r.id];
             if (r.err > a.eps && r.iter <= a.iter_max) {

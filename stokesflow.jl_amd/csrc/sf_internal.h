@@ -160,7 +160,7 @@ hipError_t launch_probe_r3w1(const double* a, const double* b, const double* c, 
 // and the epilogue need.
 struct BatchInstance {
     PTParams pt[2];   // one iteration, state ping-pong by parity, dV in place (persistent mode)
-    PTParams wpt[4];  // wide mode, dV ping-pong too: [parity] one iteration, [2 + parity] two
+    PTParams wpt[6];  // wide mode, dV ping-pong too: [parity] one iteration, [2 + parity] two, [4 + parity] three
     double *P, *Vx_s, *Vy_s, *Vx_pad, *Vy_pad;  // caller state (in/out)
     double *ry, *scal;                          // library scratch (scal[SC_DT] = stabilisation dt)
     const double* dt_dev;                       // if set, the input dt is read here (device) instead of dt_in
@@ -188,6 +188,8 @@ struct WideArgs {
 hipError_t launch_wide_pt(const WideArgs& a, int depth, hipStream_t s);
 // Two PT iterations of every slot (descriptor wpt[2 + parity]; flags must be 0).
 hipError_t launch_wide_pt2(const WideArgs& a, hipStream_t s);
+// Three PT iterations of every slot (descriptor wpt[4 + parity]).
+hipError_t launch_wide_pt3(const WideArgs& a, hipStream_t s);
 
 // Wide mode of the FP32 correction sweeps: slot k runs descriptor slot_desc[k] (two
 // descriptors per instance, one per ping-pong direction).
