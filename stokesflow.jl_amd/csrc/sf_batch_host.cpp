@@ -450,6 +450,11 @@ void refine_descriptors(const sf_batch_instance& a, const RefineBufs& b, int row
         out[2 + d].seg = rows2;
         out[2 + d].nstrips = corr2_num_strips(a.nx);
         out[2 + d].ntiles = out[2 + d].nstrips * ((a.ny + rows2 - 1) / rows2);
+        const int rows3 = std::min(a.ny, std::max(a.nx, a.ny) > 256 ? 16 : 8);
+        out[4 + d] = out[d];
+        out[4 + d].seg = rows3;
+        out[4 + d].nstrips = (a.nx - 1 + 57) / 58;
+        out[4 + d].ntiles = out[4 + d].nstrips * ((a.ny + rows3 - 1) / rows3);
     }
 }
 
@@ -693,11 +698,13 @@ int batch_run_wide_refined(const RefineEnv& env, const sf_batch_instance* in, in
     std::vector<Run> act;
     size_t q = 0;
     double ws_active = 0.0;
-    WideCorrArgs wa, w2;
+    WideCorrArgs wa, w2, w3;
     std::memset(&wa, 0, sizeof(wa));
     std::memset(&w2, 0, sizeof(w2));
+    std::memset(&w3, 0, sizeof(w3));
     wa.desc = env.cdesc_dev;
     w2.desc = env.cdesc_dev;
+    w3.desc = env.cdesc_dev;
     int rc = SF_OK;
     while (!act.empty() || q < order.size()) {
         while (q < order.size() && (int)act.size() < WIDE_MAX &&
@@ -732,13 +739,19 @@ int batch_run_wide_refined(const RefineEnv& env, const sf_batch_instance* in, in
             }
         }
         if (act.empty()) continue;
-        // two sweeps per pass for every instance with two or more sweeps left in its window
+        // three (or two) sweeps per pass for every instance with that many left in its window
         wa.n_active = 0;
         w2.n_active = 0;
-        int tiles = 0, tiles2 = 0;
+        w3.n_active = 0;
+        int tiles = 0, tiles2 = 0, tiles3 = 0;
         for (size_t k = 0; k < act.size(); ++k) {
             const size_t base = CDESC_PER_INST * (size_t)act[k].id;
-            if (act[k].left >= 2) {
+            if (act[k].left >= 3) {
+                w3.tile_start[w3.n_active] = tiles3;
+                w3.slot_desc[w3.n_active] = (int)base + 4 + act[k].cur;
+                tiles3 += cdesc[base + 4].ntiles;
+                w3.n_active += 1;
+            } else if (act[k].left >= 2) {
                 w2.tile_start[w2.n_active] = tiles2;
                 w2.slot_desc[w2.n_active] = (int)base + 2 + act[k].cur;
                 tiles2 += cdesc[base + 2].ntiles;
@@ -752,12 +765,14 @@ int batch_run_wide_refined(const RefineEnv& env, const sf_batch_instance* in, in
         }
         wa.tile_start[wa.n_active] = tiles;
         w2.tile_start[w2.n_active] = tiles2;
+        w3.tile_start[w3.n_active] = tiles3;
+        if (w3.n_active) HIPCHK(launch_wide_corr3(w3, s));
         if (w2.n_active) HIPCHK(launch_wide_corr2(w2, s));
         if (wa.n_active) HIPCHK(launch_wide_corr(wa, s));
         std::vector<Run> keep;
         keep.reserve(act.size());
         for (Run& r : act) {
-            const int64_t adv = r.left >= 2 ? 2 : 1;
+            const int64_t adv = r.left >= 3 ? 3 : (r.left >= 2 ? 2 : 1);
             r.cur ^= 1;
             r.n32 += adv;
             r.left -= adv;

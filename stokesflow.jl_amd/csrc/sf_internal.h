@@ -201,12 +201,13 @@ struct WideCorrArgs {
 };
 hipError_t launch_wide_corr(const WideCorrArgs& a, hipStream_t s);
 hipError_t launch_wide_corr2(const WideCorrArgs& a, hipStream_t s);  // two sweeps per slot
+hipError_t launch_wide_corr3(const WideCorrArgs& a, hipStream_t s);  // three sweeps per slot
 int corr_num_strips(int nx);
 int corr2_num_strips(int nx);
 int corr2_default_rows(int nx, int ny);
 // Correction descriptors per instance in the wide refined mode: [parity] one sweep,
-// [2 + parity] two sweeps; the state and the momentum ping-pong together.
-#define CDESC_PER_INST 4
+// [2 + parity] two sweeps, [4 + parity] three; the state and the momentum ping-pong together.
+#define CDESC_PER_INST 6
 
 hipError_t launch_batch_solve(const BatchInstance* inst, const int* order, int n_inst, int* queue,
                               BatchResult* res, int threads, int nblocks, hipStream_t s);

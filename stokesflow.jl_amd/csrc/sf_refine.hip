@@ -655,6 +655,33 @@ hipError_t launch_wide_corr2(const WideCorrArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
+// Three correction sweeps of every active instance per launch (corrk_tile<3>).
+__global__ __launch_bounds__(64) void wide_corr3_kernel(WideCorrArgs a)
+{
+    const int total = a.tile_start[a.n_active];
+    const int t_glob = xcd_remap_r(blockIdx.x, gridDim.x);
+    if (t_glob >= total) return;
+    int lo = 0, hi = a.n_active - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.tile_start[mid] <= t_glob) lo = mid;
+        else hi = mid - 1;
+    }
+    const int t = t_glob - a.tile_start[lo];
+    typedef const CorrParams __attribute__((address_space(4))) CCorr;
+    CCorr& cp = ((CCorr*)a.desc)[a.slot_desc[lo]];
+    const int nstrips = cp.nstrips;
+    corrk_tile<3>(cp, t % nstrips, t / nstrips);
+}
+
+hipError_t launch_wide_corr3(const WideCorrArgs& a, hipStream_t s)
+{
+    const int total = a.tile_start[a.n_active];
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(wide_corr3_kernel, dim3(total), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
 int corr2_num_strips(int nx) { return (nx - 1 + CORR2_OUT_LANES - 1) / CORR2_OUT_LANES; }
 
 int corr_num_strips(int nx) { return (nx + CORR_OUT_LANES - 1) / CORR_OUT_LANES; }
