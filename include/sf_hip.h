@@ -210,8 +210,8 @@ typedef struct {
     int mode;         /* SF_BATCH_*; AUTO = WIDE */
     int precision;    /* wide: 0 FP64 PT iterations; 1 FP32 PT sweeps inside FP64 defect correction (the
                          algorithm of sf_solve_stokes_refined, FP64 convergence test) */
-    double budget_mb; /* wide: working-set budget of the active set in MiB (<= 0: 192, the Infinity Cache;
-                         512 with precision 1) */
+    double budget_mb; /* wide: working-set budget of the active set in MiB (<= 0: 768; 512 with
+                         precision 1) */
 } sf_batch_params;
 
 int sf_solve_stokes_batched(sf_ctx *ctx, const sf_batch_instance *inst, int n, const sf_batch_params *prm,

@@ -891,7 +891,10 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
         HIPCHK(hipMemcpyAsync(res.data(), c->b_res, sizeof(BatchResult) * (size_t)n, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     } else if (mode == SF_BATCH_WIDE || mode == SF_BATCH_AUTO) {
-        const double budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 192.0 * 1048576.0;
+        // a working set past the Infinity Cache still pays with multi-iteration passes (mixed 64-512
+        // instances: 3.97 s at 192 MiB, 3.55 s at 384/768 MiB for 64, 17.3 s at 384 and 16.7 s at 768
+        // MiB for 256; eight 1024^2 instances: 8.60 s at 192, 7.40 s at 768 MiB, all eight active)
+        const double budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 768.0 * 1048576.0;
         const WideEnv env{s, c->b_inst, c->part, c->b_hscal};
         rc = batch_run_wide(env, in, n, host, order, budget, (prm && prm->depth > 1) ? 2 : 1, res);
         if (rc) return rc;

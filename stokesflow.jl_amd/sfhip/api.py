@@ -242,11 +242,12 @@ def solveStokes_batched(problems, *, mode="auto", rows=0, depth=0, threads=0, ma
     runtime = wall time of the whole batch).
 
     ``mode`` "persistent": one launch, one workgroup per instance from a device work queue;
-    "wide" (= "auto"): one launch per PT iteration over an active set of instances whose
-    working sets fit ``budget_mb`` (default 192 MiB), refilled as instances converge.
+    "wide" (= "auto"): one to three PT iterations per launch over an active set of instances
+    whose working sets fit ``budget_mb`` (default 768 MiB; 512 MiB with mixed precision),
+    refilled as instances converge.
     ``mixed_precision=True`` (wide mode): every instance is solved as
     ``solveStokes(..., mixed_precision=True)`` solves it (FP32 sweeps inside FP64 defect
-    correction), one launch per FP32 sweep of all active instances.
+    correction), one to three FP32 sweeps per launch of all active instances.
     """
     n = len(problems)
     arr = (BatchInstance * max(n, 1))()
