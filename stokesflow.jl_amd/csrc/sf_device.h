@@ -36,12 +36,13 @@ __device__ inline double jl_max(double a, double b)
 
 // One-lane wave shifts as DPP moves (GFX9 DPP wave_shl:1 = 0x130, wave_shr:1 = 0x138): a VALU
 // move instead of an LDS permute.  wave_shl:1 gives lane i the value of lane i+1 and
-// wave_shr:1 that of lane i-1; the lane without a source keeps its own value (bound_ctrl
-// off, old = src), exactly as __shfl_down(v, 1) / __shfl_up(v, 1) do on a 64-wide wave.
+// wave_shr:1 that of lane i-1, as __shfl_down(v, 1) / __shfl_up(v, 1) do on a 64-wide wave; the
+// lane without a source reads 0 (bound_ctrl), so the move needs no copy of its source first.
+// Only halo lanes lack a source, and no kernel stores a halo lane's result.
 template <int CTRL>
 __device__ __forceinline__ int dpp_i32(int v)
 {
-    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
 }
 template <int CTRL>
 __device__ __forceinline__ float dpp_shift(float v)
@@ -57,8 +58,7 @@ __device__ __forceinline__ double dpp_shift(double v)
 }
 
 // Value of lane (lane + d) within the 64-wide wave.  A shift by d > 1 repeats the one-lane
-// shift, so the last d lanes hold lane 63's value (their own, for d = 1); no kernel stores a
-// result of those halo lanes.
+// shift; the last d lanes read 0, and no kernel stores a result of those halo lanes.
 template <typename T>
 __device__ __forceinline__ T lane_down(T v, int d)
 {
@@ -66,7 +66,7 @@ __device__ __forceinline__ T lane_down(T v, int d)
     return v;
 }
 
-// Value of lane (lane - d); the first d lanes hold lane 0's value.
+// Value of lane (lane - d); the first d lanes read 0.
 template <typename T>
 __device__ __forceinline__ T lane_up(T v, int d)
 {
