@@ -99,7 +99,9 @@ BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, 
         w.dvx_o = par ? a.dVxdtau : dvx2;
         w.dvy_o = par ? a.dVydtau : dvy2;
         b.wpt[par] = w;
-        const int rows2 = pt2_default_rows(nx, ny);
+        // at least 4 rows: the wide mode fills the chip with many instances, so the short tiles
+        // that suit one small grid alone would only add halo rows
+        const int rows2 = std::max(pt2_default_rows(nx, ny), std::min(4, ny));
         w.seg = rows2;
         w.nstrips = (nx - 1 + 59) / 60;
         w.ntiles = w.nstrips * ((ny + rows2 - 1) / rows2);

@@ -657,7 +657,7 @@ extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, const double* P_in, cons
         c->seg = bestv.rows;
         c->depth = bestv.depth;
         c->max_waves = bestv.max_waves;
-        const int prows_c[] = {4, 8, 12, 16, 24, 32, 48};
+        const int prows_c[] = {2, 4, 8, 12, 16, 24, 32, 48};
         const int n_it = std::max(6, (iters + 5) / 6 * 6);  // whole passes of either kind
         for (int k : {2, 3})
             for (int rows : prows_c) {
@@ -1143,8 +1143,8 @@ extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, dou
     // writes the momentum to a second buffer, and the context's pointers are swapped after
     // each pass so that r_mx / r_my always hold the current momentum
     const bool corr_pairs = c->fuse != 1;
-    // sweeps per pass: the context's setting, or the FP64 solver's size default
-    const int corr_k = c->fuse == 0 ? pt_default_fusion(nx, ny) : c->fuse;
+    // sweeps per pass: the context's setting, or the size default of the FP32 sweeps
+    const int corr_k = c->fuse == 0 ? corr_default_fusion(nx, ny) : c->fuse;
     if (corr_pairs && !c->r_mx2) {
         HIPCHK(dalloc(&c->r_mx2, (size_t)(nx - 2) * (ny - 1)));
         HIPCHK(dalloc(&c->r_my2, (size_t)(nx - 1) * (ny - 2)));

@@ -76,6 +76,7 @@ hipError_t launch_pt_multi(const PTParams& p, int k, hipStream_t s);
 // Snapshot staging: dst = (float)(src + shift) (sf_snapshot.hip).
 hipError_t launch_snap_cast(const double* src, float* dst, int64_t n, double shift, hipStream_t s);
 int pt_default_fusion(int nx, int ny);  // iterations per pass when the context leaves it open
+int corr_default_fusion(int nx, int ny);  // FP32 correction sweeps per pass, likewise
 hipError_t launch_pt_resid_y(const ResYParams& p, hipStream_t s);
 hipError_t launch_reduce(const RedParams& p, int narr, int nb, hipStream_t s);
 hipError_t launch_finalize_dt(const double* part, int nb, double dx, double dy, double maxdisp, double* scal,

@@ -263,11 +263,13 @@ hipError_t launch_pt_fused(const PTParams& p0, hipStream_t s)
 // iteration and 1 of the second around its segment, so taller tiles cut the re-reads, while
 // smaller grids need short tiles for enough waves (2 per SIMD at 172 VGPRs).  Measured on
 // MI355X (us per iteration, one-iteration kernel in brackets): 4096^2 16 rows 193-198 (356),
-// 2048^2 12 rows 51.6 (90.7), 1024^2 12 rows 14.2 (19.8), 512^2 4 rows 6.6 (7.0).
+// 2048^2 12 rows 51.6 (90.7), 1024^2 12 rows 14.2 (19.8), 512^2 4 rows 6.6 (7.0).  Up to
+// 256^2 a launch is latency-bound, and 2-row tiles (more waves, shorter row chains) gave 2.8-3.0
+// us per iteration against 3.6-3.9 with 4 rows in the solver loop.
 int pt2_default_rows(int nx, int ny)
 {
     const int n = std::max(nx, ny);
-    const int r = n > 2048 ? 16 : (n > 768 ? 12 : (n > 512 ? 8 : 4));
+    const int r = n > 2048 ? 16 : (n > 768 ? 12 : (n > 512 ? 8 : (n > 256 ? 4 : 2)));
     return ny >= r ? r : ny;
 }
 
@@ -285,8 +287,14 @@ hipError_t launch_pt_pair(const PTParams& p0, hipStream_t s)
 // Iterations per pass by grid size, and the three-iteration pass's tile rows (measured on
 // MI355X, us per iteration: 4096^2 K=3 138.3 with 48 rows (141.8 with 32, 152.9 with 64)
 // against K=2 193.6; 2048^2 K=3 41.7 with 16 rows against K=2 50.4; 1024^2 K=2 14.1 against
-// K=3 14.6 before the stages shared their pseudo-time steps).
-int pt_default_fusion(int nx, int ny) { return std::max(nx, ny) > 1536 ? 3 : 2; }
+// K=3 14.6 before the stages shared their pseudo-time steps).  In the solver loop with shared
+// steps: 1536^2 K=3 20.6 against K=2 23.5, 1280^2 13.8 against 17.5-19.6, 1024^2 13.4 against
+// 12.2, so three iterations per pass from 1025 nodes per side.
+int pt_default_fusion(int nx, int ny) { return std::max(nx, ny) > 1024 ? 3 : 2; }
+
+// Sweeps per pass of the FP32 correction solver when the context leaves it open (three from 1537
+// nodes per side; the FP32 pass holds three waves per SIMD at three sweeps, four at two).
+int corr_default_fusion(int nx, int ny) { return std::max(nx, ny) > 1536 ? 3 : 2; }
 
 int pt3_default_rows(int nx, int ny)
 {
