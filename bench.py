@@ -433,11 +433,11 @@ def main():
     finite = bool(torch.isfinite(st.Vx_small).all().item())
 
     # The dominant kernel runs k_pass PT iterations per launch (the library's size default: three
-    # above 1536 nodes per side, ptk_tile_f64<3>; two below, pt2_tile_f64). Apart from a few
+    # above 1024 nodes per side, ptk_tile_f64<3>; two below, pt2_tile_f64). Apart from a few
     # shorter launches (iteration 10 before the timing mark at 11, and the last iterations), the
     # timed region is made of such launches, so its HIP-event time per k_pass iterations is the
     # launch time (to about 0.5%).
-    k_pass = tuned["iters_per_launch"] if tuned else (3 if max(nx, ny) > 1536 else 2)
+    k_pass = tuned["iters_per_launch"] if tuned else (3 if max(nx, ny) > 1024 else 2)
     kernel = {1: "pt_tile_f64", 2: "pt2_tile_f64", 3: "ptk_tile_f64<3>"}[k_pass]
     per_launch = gpu_s * k_pass / a.steps
     probe = sfhip.probe_dram(1 << 30)  # the box's DRAM ceiling for simple streams
