@@ -275,14 +275,15 @@ def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, b
     return res
 
 
-def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 1.0), eps=1e-5):
+def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 1.0), eps=1e-5, modes=None, progress=False):
     """Timesteps of a Rayleigh-Taylor ensemble (n_inst instances at n x n nodes, 16 markers per
     cell): upper (small-y) layer rho 1.1 and viscosity 10^c, lower layer rho 1 and viscosity 1,
     interface y = 0.5 + 0.02 sin(2 pi k x), k uniform over 1..8, c uniform over ``log10c``.
     The same steps run several ways from the same initial state: the ensemble step in its
     wide and persistent forms, each with hipGraphs and enqueued eagerly, and sf_step per
     instance in a loop.  All must agree bit for bit.  eps defaults to the reference's timestep
-    loop tolerance (stokes_eps = 1e-5, StokesFlow2D_multixpu.jl:56)."""
+    loop tolerance (stokes_eps = 1e-5, StokesFlow2D_multixpu.jl:56).  ``modes`` restricts the
+    runs to the named ones (for large ensembles)."""
     import numpy as np
     import torch
 
@@ -310,6 +311,8 @@ def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 1.0), eps=1e-5)
     if n <= 128:  # one CU per member: only for small members
         runs += [("persistent_graph", "persistent", True, False), ("persistent_eager", "persistent", False, False)]
     runs += [("loop", None, None, False)]
+    if modes:
+        runs = [r for r in runs if r[0] in modes]
     for name, mode, graph, refined in runs:
         ms = members()
         torch.cuda.synchronize()
@@ -322,14 +325,18 @@ def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 1.0), eps=1e-5)
                     dts[i], st = sfhip.step(m["state"], m["x_m"], m["y_m"], m["rho_m"], m["mu_m"], dx, dy, dts[i],
                                             g_y=1.0, eps=eps)
                     iters += st["niter"]
+                    if progress:
+                        print(f"[ens-steps] loop member {i} done", file=sys.stderr, flush=True)
             torch.cuda.synchronize()
             secs = time.perf_counter() - t0
         else:
             ens = sfhip.Ensemble(ms, use_graph=graph, mode=mode, mixed_precision=refined)
             t0 = time.perf_counter()
-            for _ in range(steps):
+            for k in range(steps):
                 outs = ens.step(1)
                 iters += sum(o["niter"] for o in outs)
+                if progress:
+                    print(f"[ens-steps] {name} step {k} done", file=sys.stderr, flush=True)
             torch.cuda.synchronize()
             secs = time.perf_counter() - t0
             ens.close()
@@ -337,14 +344,19 @@ def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 1.0), eps=1e-5)
         finals[name] = [(m["state"].Vx.clone(), m["x_m"].clone()) for m in ms]
         del ms
         torch.cuda.empty_cache()
-    for name, _, _, _ in runs[:-1]:
-        res[name]["speedup_vs_loop"] = res["loop"]["seconds"] / res[name]["seconds"]
+    if "loop" in res:
+        for name, _, _, _ in runs:
+            if name != "loop":
+                res[name]["speedup_vs_loop"] = res["loop"]["seconds"] / res[name]["seconds"]
     fp64 = [name for name, _, _, refined in runs if not refined]
-    res["bitwise_equal"] = all(torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
-                               for name in fp64 for a, b in zip(finals["loop"], finals[name]))
-    res["wide_refined_graph"]["max_rel_l2_vs_fp64"] = max(
-        ((a[0] - b[0]).norm() / b[0].norm().clamp_min(1e-300)).item()
-        for a, b in zip(finals["wide_refined_graph"], finals["loop"]))
+    base = "loop" if "loop" in finals else (fp64[0] if fp64 else None)
+    if len(fp64) > 1:
+        res["bitwise_equal"] = all(torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+                                   for name in fp64 for a, b in zip(finals[base], finals[name]))
+    if "wide_refined_graph" in finals and base:
+        res["wide_refined_graph"]["max_rel_l2_vs_fp64"] = max(
+            ((a[0] - b[0]).norm() / b[0].norm().clamp_min(1e-300)).item()
+            for a, b in zip(finals["wide_refined_graph"], finals[base]))
     return res
 
 
@@ -368,6 +380,9 @@ def main():
                     help="batched modes to time (wide, persistent, wide_refined)")
     ap.add_argument("--ens-steps", type=int, nargs=3, default=None, metavar=("INSTANCES", "N", "STEPS"),
                     help="Rayleigh-Taylor ensemble timesteps: graph-captured vs eager vs per-instance loop")
+    ap.add_argument("--ens-steps-modes", nargs="+", default=None,
+                    choices=["wide_graph", "wide_eager", "wide_refined_graph", "persistent_graph", "persistent_eager",
+                             "loop"], help="run only these forms of the --ens-steps comparison")
     ap.add_argument("--progress", action="store_true", help="print the solvers' convergence checks (long runs)")
     ap.add_argument("--refined-iters", type=int, default=0,
                     help="also time this many FP32 correction sweeps (defect-correction solver) on the same state")
@@ -504,7 +519,7 @@ def main():
         line["ensemble"] = run_ensemble(ensemble_specs(a.ens, log10c=tuple(a.ens_c)), modes=tuple(a.ens_modes))
     if a.ens_steps:
         torch.cuda.empty_cache()
-        line["ensemble_steps"] = run_ensemble_steps(*a.ens_steps)
+        line["ensemble_steps"] = run_ensemble_steps(*a.ens_steps, modes=a.ens_steps_modes, progress=a.progress)
     print(json.dumps(line), flush=True)
 
 
