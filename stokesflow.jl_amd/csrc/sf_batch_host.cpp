@@ -435,7 +435,8 @@ void refine_descriptors(const sf_batch_instance& a, const RefineBufs& b, int row
     cp.max_nxy = (float)k.max_nxy;
     float* const mx[2] = {b.mx, b.mx2};
     float* const my[2] = {b.my, b.my2};
-    const int rows2 = corr2_default_rows(a.nx, a.ny);
+    // at least 4 rows, as the FP64 wide pass: many instances fill the chip
+    const int rows2 = std::max(corr2_default_rows(a.nx, a.ny), std::min(4, a.ny));
     for (int d = 0; d < 2; ++d) {
         out[d] = cp;
         out[d].vx_a = b.vx[d];

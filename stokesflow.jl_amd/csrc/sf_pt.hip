@@ -292,9 +292,10 @@ hipError_t launch_pt_pair(const PTParams& p0, hipStream_t s)
 // 12.2, so three iterations per pass from 1025 nodes per side.
 int pt_default_fusion(int nx, int ny) { return std::max(nx, ny) > 1024 ? 3 : 2; }
 
-// Sweeps per pass of the FP32 correction solver when the context leaves it open (three from 1537
-// nodes per side; the FP32 pass holds three waves per SIMD at three sweeps, four at two).
-int corr_default_fusion(int nx, int ny) { return std::max(nx, ny) > 1536 ? 3 : 2; }
+// Sweeps per pass of the FP32 correction solver when the context leaves it open (three from 769
+// nodes per side; us per sweep, K=3 against K=2: 1024^2 7.82 (8-row tiles) against 8.8, 1280^2
+// 9.54 against 11.2, 1536^2 12.45 against 14.45).
+int corr_default_fusion(int nx, int ny) { return std::max(nx, ny) > 768 ? 3 : 2; }
 
 int pt3_default_rows(int nx, int ny)
 {

@@ -835,11 +835,13 @@ hipError_t launch_pt_corr(const CorrParams& p0, bool check, hipStream_t s)
     return hipGetLastError();
 }
 
-// Rows per tile of the two-sweep pass (as the FP64 pass: taller tiles cut the halo re-reads).
+// Rows per tile of the two-sweep pass (as the FP64 pass: taller tiles cut the halo re-reads,
+// short ones give small grids enough waves; us per sweep in the refined solver: 256^2 2.66 with
+// 2 rows against 3.44 with 4, 512^2 3.93 against 4.12).
 int corr2_default_rows(int nx, int ny)
 {
     const int n = std::max(nx, ny);
-    const int r = n > 2048 ? 16 : (n > 768 ? 12 : (n > 512 ? 8 : 4));
+    const int r = n > 2048 ? 16 : (n > 768 ? 12 : (n > 512 ? 8 : 2));
     return ny >= r ? r : ny;
 }
 
@@ -858,8 +860,9 @@ hipError_t launch_pt_corrk(const CorrParams& p0, int k, hipStream_t s)
 {
     if (k != 3 && k != 4) return hipErrorInvalidValue;
     CorrParams p = p0;
-    if (p.seg <= 0) {
-        const int r = std::max(p.nx, p.ny) > 3072 ? 48 : 16;
+    if (p.seg <= 0) {  // 1024^2: 7.82 us per sweep with 8 rows, 9.48 with 16
+        const int n = std::max(p.nx, p.ny);
+        const int r = n > 3072 ? 48 : (n > 1024 ? 16 : 8);
         p.seg = p.ny >= r ? r : p.ny;
     }
     const int out = 64 - 2 * k;
