@@ -862,7 +862,7 @@ hipError_t launch_pt_corrk(const CorrParams& p0, int k, hipStream_t s)
     CorrParams p = p0;
     if (p.seg <= 0) {  // 1024^2: 7.82 us per sweep with 8 rows, 9.48 with 16
         const int n = std::max(p.nx, p.ny);
-        const int r = n > 3072 ? 48 : (n > 1024 ? 16 : 8);
+        const int r = n > 3072 ? (k == 4 ? 40 : 48) : (n > 1024 ? 16 : 8);
         p.seg = p.ny >= r ? r : p.ny;
     }
     const int out = 64 - 2 * k;
