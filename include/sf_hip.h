@@ -83,7 +83,7 @@ int sf_ctx_set_pt_tuning(sf_ctx *ctx, int rows, int depth, int max_waves);
  * iterations (no check iteration, no dt refresh before the last of them, not the last
  * iteration or the first timed one) as one pass that reads and writes the state once; 1 runs
  * one iteration per launch; 0 is the size default (3 above 1024 nodes per side, else 2; the FP32
- * sweeps of sf_solve_stokes_refined: 3 above 768, else 2).
+ * sweeps of sf_solve_stokes_refined: 4 above 1536, 3 above 768, else 2).
  * rows: grid rows per tile of the pass (0 = size default).  Results are bit-identical either
  * way; the pass needs a second dVx/dVy buffer, allocated in the context on first use.  No
  * reference counterpart (the reference launches each of its kernels once per iteration). */
