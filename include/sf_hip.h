@@ -82,8 +82,10 @@ int sf_ctx_set_pt_tuning(sf_ctx *ctx, int rows, int depth, int max_waves);
 /* Temporal blocking of the PT iteration: iters_per_launch = 2 or 3 runs that many plain
  * iterations (no check iteration, no dt refresh before the last of them, not the last
  * iteration or the first timed one) as one pass that reads and writes the state once; 1 runs
- * one iteration per launch; 0 is the size default (3 above 1024 nodes per side, else 2; the FP32
- * sweeps of sf_solve_stokes_refined: 4 above 1536, 3 above 768, else 2).
+ * one iteration per launch; 0 is the size default (3 above 1024 nodes per side, else 2).  The
+ * FP32 sweeps of sf_solve_stokes_refined take the same setting with 4 and 5 also allowed (the
+ * FP64 passes cap it at 3); their default is 5 above 3072 nodes per side, 4 above 1536, 3 above
+ * 768, else 2.
  * rows: grid rows per tile of the pass (0 = size default).  Results are bit-identical either
  * way; the pass needs a second dVx/dVy buffer, allocated in the context on first use.  No
  * reference counterpart (the reference launches each of its kernels once per iteration). */

@@ -117,8 +117,8 @@ extern "C" int sf_ctx_set_pt_tuning(sf_ctx* c, int rows, int depth, int max_wave
 extern "C" int sf_ctx_set_pt_fusion(sf_ctx* c, int iters_per_launch, int rows)
 {
     if (!c) return set_err(SF_ERR_ARG, "sf_ctx_set_pt_fusion: NULL ctx");
-    if (iters_per_launch < 0 || iters_per_launch > 4)
-        return set_err(SF_ERR_ARG, "sf_ctx_set_pt_fusion: iters_per_launch %d not in 0..4", iters_per_launch);
+    if (iters_per_launch < 0 || iters_per_launch > 5)
+        return set_err(SF_ERR_ARG, "sf_ctx_set_pt_fusion: iters_per_launch %d not in 0..5", iters_per_launch);
     if (rows < 0 || rows > 4096) return set_err(SF_ERR_ARG, "sf_ctx_set_pt_fusion: rows %d outside [0, 4096]", rows);
     c->fuse = iters_per_launch;
     c->pair_rows = rows;

@@ -297,11 +297,12 @@ int pt_default_fusion(int nx, int ny) { return std::max(nx, ny) > 1024 ? 3 : 2; 
 // 9.54 against 11.2, 1536^2 12.45 against 14.45).  Above 1536, four (interleaved repetitions on
 // one box, us per sweep, four against three: 4096^2 79.5 against 87.7, 3072^2 44.9 against 53.7,
 // 2048^2 18.7 against 19.8); four sweeps move fewer bytes per sweep at two waves per SIMD and
-// measured 78-80 us at 4096^2 on every box, where three ranged 76-92.
+// measured 78-80 us at 4096^2 on every box, where three ranged 76-92.  Above 3072, five (225
+// VGPRs, still two waves per SIMD): 4096^2 74.6 against 79.4 for four; 2048^2 18.7 for both.
 int corr_default_fusion(int nx, int ny)
 {
     const int n = std::max(nx, ny);
-    return n > 1536 ? 4 : (n > 768 ? 3 : 2);
+    return n > 3072 ? 5 : (n > 1536 ? 4 : (n > 768 ? 3 : 2));
 }
 
 int pt3_default_rows(int nx, int ny)

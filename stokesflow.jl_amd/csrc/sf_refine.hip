@@ -856,13 +856,14 @@ hipError_t launch_pt_corr2(const CorrParams& p0, hipStream_t s)
     return hipGetLastError();
 }
 
-// Tile rows of the four-sweep pass: the launch runs in rounds of resident waves (two per SIMD at
-// 193 VGPRs), and a last round that is only partly filled costs a whole one, so the height is
-// chosen to minimise rounds x (rows + 7 halo rows) over 24-48 rows.  Measured, us per sweep:
-// 4096^2 40 rows 79.5, 28 81.0, 32 81.6, 64 83.5, 48 85.7 (the model picks 40); 3072^2 28 rows
-// 44.9, 24 47.8, 40 53.2 (the model ranks these in order and picks 44, a two-round launch);
-// 2048^2 40 rows 18.7, 28 22.1, 24 22.9 (picks 40).
-static int corr4_rows(int nx, int ny)
+// Tile rows of the four- and five-sweep passes: the launch runs in rounds of resident waves (two
+// per SIMD at 193 / 225 VGPRs), and a last round that is only partly filled costs a whole one, so
+// the height is chosen to minimise rounds x (rows + 2K-1 halo rows) over 24-48 rows.  Measured,
+// us per sweep, K=4: 4096^2 40 rows 79.5, 28 81.0, 32 81.6, 64 83.5, 48 85.7 (the model picks
+// 40); 3072^2 28 rows 44.9, 24 47.8, 40 53.2 (the model ranks these in order and picks 44, a
+// two-round launch); 2048^2 40 rows 18.7, 28 22.1, 24 22.9 (picks 40).  K=5: 4096^2 40 rows
+// 74.6, 32 77.0, 48 77.0; 2048^2 40 rows 18.7, 48 21.7, 32 23.6 (picks 40 at both).
+static int corrk_rows(int nx, int ny, int k)
 {
     static const int slots = [] {
         int dev = 0, cus = 0;
@@ -871,12 +872,13 @@ static int corr4_rows(int nx, int ny)
             cus = 256;
         return 2 * 4 * cus;
     }();
-    const int64_t strips = (nx - 1 + 55) / 56;
+    const int out = 64 - 2 * k;
+    const int64_t strips = (nx - 1 + out - 1) / out;
     int best = 40;
     double best_cost = 1e300;
     for (int rows = 24; rows <= 48; rows += 4) {
         const int64_t waves = strips * ((ny + rows - 1) / rows);
-        const double cost = (double)((waves + slots - 1) / slots) * (rows + 7);
+        const double cost = (double)((waves + slots - 1) / slots) * (rows + 2 * k - 1);
         if (cost < best_cost) {
             best_cost = cost;
             best = rows;
@@ -887,11 +889,11 @@ static int corr4_rows(int nx, int ny)
 
 hipError_t launch_pt_corrk(const CorrParams& p0, int k, hipStream_t s)
 {
-    if (k != 3 && k != 4) return hipErrorInvalidValue;
+    if (k < 3 || k > 5) return hipErrorInvalidValue;
     CorrParams p = p0;
     if (p.seg <= 0) {  // three sweeps, 1024^2: 7.82 us per sweep with 8 rows, 9.48 with 16
         const int n = std::max(p.nx, p.ny);
-        const int r = k == 4 ? corr4_rows(p.nx, p.ny) : (n > 3072 ? 48 : (n > 1024 ? 16 : 8));
+        const int r = k >= 4 ? corrk_rows(p.nx, p.ny, k) : (n > 3072 ? 48 : (n > 1024 ? 16 : 8));
         p.seg = p.ny >= r ? r : p.ny;
     }
     const int out = 64 - 2 * k;
@@ -899,7 +901,8 @@ hipError_t launch_pt_corrk(const CorrParams& p0, int k, hipStream_t s)
     const int nseg = (p.ny + p.seg - 1) / p.seg;
     p.ntiles = p.nstrips * nseg;
     if (k == 3) hipLaunchKernelGGL(pt_corrk_f32<3>, dim3(p.ntiles), dim3(64), 0, s, p);
-    else hipLaunchKernelGGL(pt_corrk_f32<4>, dim3(p.ntiles), dim3(64), 0, s, p);
+    else if (k == 4) hipLaunchKernelGGL(pt_corrk_f32<4>, dim3(p.ntiles), dim3(64), 0, s, p);
+    else hipLaunchKernelGGL(pt_corrk_f32<5>, dim3(p.ntiles), dim3(64), 0, s, p);
     return hipGetLastError();
 }
 

@@ -284,8 +284,8 @@ def test_refined_plume(gpu, n, rebase):
 
 @pytest.mark.parametrize("n,rebase", [(61, 0), (96, 97), (130, 45)])
 def test_refined_pair_sweeps_bitwise(gpu, n, rebase):
-    """FP32 correction sweeps one to four per pass give the same bits: windows of odd and even
-    length (rebase 97 / 45 and ncheck), strips of 56, 58, 60 and 62 columns."""
+    """FP32 correction sweeps one to five per pass give the same bits: windows of odd and even
+    length (rebase 97 / 45 and ncheck), strips of 54, 56, 58, 60 and 62 columns."""
     dx, dy, x, y, rho, mu = cases.plume_markers(n, n + 3, 4, seed=78)
     S = O.solver_arrays(n, n + 3)
     vws, ws = O.zeros(n + 1, n + 3), O.zeros(n + 1, n + 3)
@@ -294,7 +294,7 @@ def test_refined_pair_sweeps_bitwise(gpu, n, rebase):
     O.marker_to_grid(*o["mu_b"], S["mu_b"], x, y, mu, dx, dy, vws, ws)
     O.marker_to_grid(*o["mu_p"], S["mu_p"], x, y, mu, dx, dy, vws, ws)
     out = []
-    for fuse in (1, 2, 3, 4):
+    for fuse in (1, 2, 3, 4, 5):
         G = to_gpu(S, list(S.keys()))
         st = {}
         try:
