@@ -305,10 +305,34 @@ int corr_default_fusion(int nx, int ny)
     return n > 3072 ? 5 : (n > 1536 ? 4 : (n > 768 ? 3 : 2));
 }
 
+//
+// The tile height comes from a wave-round model: the launch runs in rounds of resident waves (two
+// per SIMD at 236 VGPRs), and a partly filled last round costs a whole one, so the height
+// minimises rounds x (rows + 5 halo rows) over 12-64 rows.  Measured in the solver loop, us per
+// iteration (model pick first): 4096^2 48 rows 136.0, 32 141.8, 56 145.2, 64 152.9; 3072^2 40 rows
+// 74.0, 28 76.7, 20 77.8, 16 79.1; 2048^2 40 rows 34.4, 20 36.6, 16 39.4, 32 45.4; 1536^2 24 rows
+// 19.7, 12 21.2, 16 21.7, 40 28.5; 1280^2 16 rows 13.6, 20 16.0, 12 16.4.
 int pt3_default_rows(int nx, int ny)
 {
-    const int r = std::max(nx, ny) > 3072 ? 48 : 16;
-    return ny >= r ? r : ny;
+    static const int slots = [] {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        return 2 * 4 * cus;
+    }();
+    const int64_t strips = (nx - 1 + 57) / 58;
+    int best = 16;
+    double best_cost = 1e300;
+    for (int rows = 12; rows <= 64; rows += 4) {
+        const int64_t waves = strips * ((ny + rows - 1) / rows);
+        const double cost = (double)((waves + slots - 1) / slots) * (rows + 5);
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = rows;
+        }
+    }
+    return ny >= best ? best : ny;
 }
 
 hipError_t launch_pt_multi(const PTParams& p0, int k, hipStream_t s)
