@@ -96,9 +96,12 @@ int sf_ctx_set_pt_fusion(sf_ctx *ctx, int iters_per_launch, int rows);
  * modified) or, when any of them is NULL, on a synthetic one (materials 1, state 0), in scratch
  * it allocates and frees.  Candidates: the one-iteration kernel (rows 1, 2, 4, 8, 16 x depth
  * 1, 2 x one wave per tile or persistent; iters_per_launch = 1), then passes of 2 and 3
- * iterations (rows 4, 8, 12, 16, 24, 32, 48).  The fastest one-iteration shape is set on the
+ * iterations (rows 2, 4, 8, 12, 16, 24, 32, 48).  The fastest one-iteration shape is set on the
  * context (check iterations use it), and the fastest overall decides the iterations per
- * launch (sf_ctx_set_pt_fusion).  Timing can depend on the data, so tune on the real problem.
+ * launch and the pass rows of the FP64 solver (sf_solve_stokes, sf_pt_iterations); the FP32
+ * sweeps of sf_solve_stokes_refined keep their own size defaults (or the explicit
+ * sf_ctx_set_pt_fusion setting, which also clears the tuned FP64 choice).  Timing can depend
+ * on the data, so tune on the real problem.
  * best (may be NULL) receives the fastest overall; table (may be NULL, table_cap entries)
  * every candidate's time, n_table how many were timed.  No reference counterpart. */
 typedef struct {

@@ -122,119 +122,122 @@ extern "C" int sf_ctx_set_pt_fusion(sf_ctx* c, int iters_per_launch, int rows)
     if (rows < 0 || rows > 4096) return set_err(SF_ERR_ARG, "sf_ctx_set_pt_fusion: rows %d outside [0, 4096]", rows);
     c->fuse = iters_per_launch;
     c->pair_rows = rows;
+    c->fuse64 = c->rows64 = 0;  // an explicit setting replaces the autotuner's FP64 choice
     return SF_OK;
 }
 
-// The PT loop state shared by the solve and the fixed-count iteration entry point: the
-// ping-pong pair of (Vx_s, Vy_s, P) and, for two-iteration passes, of (dVx, dVy).
-namespace {
-struct PTLoop {
-    sf_ctx* c;
-    PTParams pp;
-    double *cur_vx, *cur_vy, *cur_p, *nxt_vx, *nxt_vy, *nxt_p;
-    double *cur_dvx, *cur_dvy, *nxt_dvx, *nxt_dvy;
-    double *vx0, *vy0, *p0, *dvx0, *dvy0;  // the caller's arrays
-    bool pairs;                             // multi-iteration passes enabled
-    int kmax;                               // iterations per pass (2 or 3)
+int PTLoop::single(int flags, hipStream_t s)
+{
+    pp.flags = flags;
+    pp.vx_a = cur_vx;
+    pp.vy_a = cur_vy;
+    pp.p_a = cur_p;
+    pp.vx_b = nxt_vx;
+    pp.vy_b = nxt_vy;
+    pp.p_b = nxt_p;
+    pp.dvx = cur_dvx;
+    pp.dvy = cur_dvy;
+    pp.dvx_o = nullptr;  // in place
+    pp.dvy_o = nullptr;
+    HIPCHK(launch_pt_fused(pp, s));
+    std::swap(cur_vx, nxt_vx);
+    std::swap(cur_vy, nxt_vy);
+    std::swap(cur_p, nxt_p);
+    return SF_OK;
+}
 
-    int init(sf_ctx* ctx, double* Vx_s, double* Vy_s, double* P, double* dVx, double* dVy)
-    {
-        c = ctx;
-        cur_vx = vx0 = Vx_s;
-        cur_vy = vy0 = Vy_s;
-        cur_p = p0 = P;
-        cur_dvx = dvx0 = dVx;
-        cur_dvy = dvy0 = dVy;
-        nxt_vx = c->vx_b;
-        nxt_vy = c->vy_b;
-        nxt_p = c->p_b;
-        pairs = c->fuse != 1;
-        kmax = c->fuse == 0 ? pt_default_fusion(c->nx, c->ny) : std::min(c->fuse, 3);
-        if (pairs && !c->dvx_b) {
-            const size_t nx = c->nx, ny = c->ny;
-            hipError_t e = dalloc(&c->dvx_b, (nx - 2) * (ny - 1));
-            if (e == hipSuccess) e = dalloc(&c->dvy_b, (nx - 1) * (ny - 2));
-            if (e != hipSuccess) {
-                if (c->dvx_b) (void)hipFree(c->dvx_b);
-                c->dvx_b = c->dvy_b = nullptr;
-                return set_err(SF_ERR_HIP, "PT loop: dV buffer allocation failed: %s", hipGetErrorString(e));
-            }
-        }
-        nxt_dvx = c->dvx_b;
-        nxt_dvy = c->dvy_b;
-        return SF_OK;
+int PTLoop::pair(hipStream_t s, int k)
+{
+    const int seg1 = pp.seg;
+    pp.flags = 0;
+    pp.seg = rows;
+    pp.vx_a = cur_vx;
+    pp.vy_a = cur_vy;
+    pp.p_a = cur_p;
+    pp.vx_b = nxt_vx;
+    pp.vy_b = nxt_vy;
+    pp.p_b = nxt_p;
+    pp.dvx = cur_dvx;
+    pp.dvy = cur_dvy;
+    pp.dvx_o = nxt_dvx;
+    pp.dvy_o = nxt_dvy;
+    const hipError_t e = launch_pt_multi(pp, k, s);
+    pp.seg = seg1;  // the one-iteration kernel keeps its own tile rows
+    HIPCHK(e);
+    std::swap(cur_vx, nxt_vx);
+    std::swap(cur_vy, nxt_vy);
+    std::swap(cur_p, nxt_p);
+    std::swap(cur_dvx, nxt_dvx);
+    std::swap(cur_dvy, nxt_dvy);
+    return SF_OK;
+}
+
+int PTLoop::finish(hipStream_t s)
+{
+    const size_t gx = nx, gy = ny;
+    if (cur_vx != vx0) {
+        HIPCHK(hipMemcpyAsync(vx0, cur_vx, sizeof(double) * gx * (gy + 1), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(vy0, cur_vy, sizeof(double) * (gx + 1) * gy, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(p0, cur_p, sizeof(double) * (gx - 1) * (gy - 1), hipMemcpyDeviceToDevice, s));
     }
-    // One iteration (flags: check-iteration outputs); dV in place.
-    int single(int flags, hipStream_t s)
-    {
-        pp.flags = flags;
-        pp.seg = c->seg;
-        pp.vx_a = cur_vx;
-        pp.vy_a = cur_vy;
-        pp.p_a = cur_p;
-        pp.vx_b = nxt_vx;
-        pp.vy_b = nxt_vy;
-        pp.p_b = nxt_p;
-        pp.dvx = cur_dvx;
-        pp.dvy = cur_dvy;
-        pp.dvx_o = nullptr;  // in place
-        pp.dvy_o = nullptr;
-        HIPCHK(launch_pt_fused(pp, s));
-        std::swap(cur_vx, nxt_vx);
-        std::swap(cur_vy, nxt_vy);
-        std::swap(cur_p, nxt_p);
-        return SF_OK;
+    if (cur_dvx != dvx0) {
+        HIPCHK(hipMemcpyAsync(dvx0, cur_dvx, sizeof(double) * (gx - 2) * (gy - 1), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(dvy0, cur_dvy, sizeof(double) * (gx - 1) * (gy - 2), hipMemcpyDeviceToDevice, s));
     }
-    // k plain iterations in one pass.
-    int pair(hipStream_t s, int k = 2)
-    {
-        pp.flags = 0;
-        pp.seg = c->pair_rows;
-        pp.vx_a = cur_vx;
-        pp.vy_a = cur_vy;
-        pp.p_a = cur_p;
-        pp.vx_b = nxt_vx;
-        pp.vy_b = nxt_vy;
-        pp.p_b = nxt_p;
-        pp.dvx = cur_dvx;
-        pp.dvy = cur_dvy;
-        pp.dvx_o = nxt_dvx;
-        pp.dvy_o = nxt_dvy;
-        HIPCHK(launch_pt_multi(pp, k, s));
-        std::swap(cur_vx, nxt_vx);
-        std::swap(cur_vy, nxt_vy);
-        std::swap(cur_p, nxt_p);
-        std::swap(cur_dvx, nxt_dvx);
-        std::swap(cur_dvy, nxt_dvy);
-        return SF_OK;
-    }
-    // State back into the caller's arrays.
-    int finish(hipStream_t s)
-    {
+    cur_vx = vx0;
+    cur_vy = vy0;
+    cur_p = p0;
+    cur_dvx = dvx0;
+    cur_dvy = dvy0;
+    nxt_vx = b.vx_b;
+    nxt_vy = b.vy_b;
+    nxt_p = b.p_b;
+    nxt_dvx = b.dvx_b;
+    nxt_dvy = b.dvy_b;
+    return SF_OK;
+}
+
+int PTLoop::run_fixed(int64_t n_iter, hipStream_t s)
+{
+    int64_t it = 0;
+    int rc = SF_OK;
+    for (; pairs && it + kmax <= n_iter && rc == SF_OK; it += kmax) rc = pair(s, kmax);
+    for (; pairs && it + 2 <= n_iter && rc == SF_OK; it += 2) rc = pair(s, 2);
+    for (; it < n_iter && rc == SF_OK; ++it) rc = single(0, s);
+    if (rc) return rc;
+    return finish(s);
+}
+
+// The context's ping-pong buffers of the PT loop; the second dVx/dVy (multi-iteration passes)
+// is allocated on first use.
+static int ctx_pt_buffers(sf_ctx* c, bool pairs, PTBuffers* b)
+{
+    if (pairs && !c->dvx_b) {
         const size_t nx = c->nx, ny = c->ny;
-        if (cur_vx != vx0) {
-            HIPCHK(hipMemcpyAsync(vx0, cur_vx, sizeof(double) * nx * (ny + 1), hipMemcpyDeviceToDevice, s));
-            HIPCHK(hipMemcpyAsync(vy0, cur_vy, sizeof(double) * (nx + 1) * ny, hipMemcpyDeviceToDevice, s));
-            HIPCHK(hipMemcpyAsync(p0, cur_p, sizeof(double) * (nx - 1) * (ny - 1), hipMemcpyDeviceToDevice, s));
+        hipError_t e = dalloc(&c->dvx_b, (nx - 2) * (ny - 1));
+        if (e == hipSuccess) e = dalloc(&c->dvy_b, (nx - 1) * (ny - 2));
+        if (e != hipSuccess) {
+            if (c->dvx_b) (void)hipFree(c->dvx_b);
+            c->dvx_b = c->dvy_b = nullptr;
+            return set_err(SF_ERR_HIP, "PT loop: dV buffer allocation failed: %s", hipGetErrorString(e));
         }
-        if (cur_dvx != dvx0) {
-            HIPCHK(hipMemcpyAsync(dvx0, cur_dvx, sizeof(double) * (nx - 2) * (ny - 1), hipMemcpyDeviceToDevice, s));
-            HIPCHK(hipMemcpyAsync(dvy0, cur_dvy, sizeof(double) * (nx - 1) * (ny - 2), hipMemcpyDeviceToDevice, s));
-        }
-        cur_vx = vx0;
-        cur_vy = vy0;
-        cur_p = p0;
-        cur_dvx = dvx0;
-        cur_dvy = dvy0;
-        nxt_vx = c->vx_b;
-        nxt_vy = c->vy_b;
-        nxt_p = c->p_b;
-        nxt_dvx = c->dvx_b;
-        nxt_dvy = c->dvy_b;
-        return SF_OK;
     }
-};
-}  // namespace
+    *b = PTBuffers{c->vx_b, c->vy_b, c->p_b, c->dvx_b, c->dvy_b};
+    return SF_OK;
+}
+
+// The PT loop of a context: the autotuner's FP64 choice, else the explicit setting, else the
+// size default.
+static int ctx_pt_loop(sf_ctx* c, PTLoop* L, double* Vx_s, double* Vy_s, double* P, double* dVx, double* dVy)
+{
+    const int fuse = c->fuse64 > 0 ? c->fuse64 : c->fuse;
+    const int rows = c->fuse64 > 0 ? c->rows64 : c->pair_rows;
+    PTBuffers b{};
+    const int rc = ctx_pt_buffers(c, fuse != 1, &b);
+    if (rc) return rc;
+    L->init(b, fuse, rows, c->nx, c->ny, Vx_s, Vy_s, P, dVx, dVy);
+    return SF_OK;
+}
 
 extern "C" int sf_ctx_set_stream(sf_ctx* c, void* stream)
 {
@@ -365,7 +368,7 @@ extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_
     pp.max_nxy = k.max_nxy;
 
     PTLoop L;
-    rc = L.init(c, Vx_s, Vy_s, P, dVx, dVy);
+    rc = ctx_pt_loop(c, &L, Vx_s, Vy_s, P, dVx, dVy);
     if (rc) return rc;
     L.pp = pp;
 
@@ -536,28 +539,16 @@ extern "C" int sf_pt_iterations(sf_ctx* c, int64_t n_iter, double* P, double* Vx
     pp.ptsc41 = k.ptsc41;
     pp.max_nxy = k.max_nxy;
     PTLoop L;
-    rc = L.init(c, Vx_s, Vy_s, P, dVx, dVy);
+    rc = ctx_pt_loop(c, &L, Vx_s, Vy_s, P, dVx, dVy);
     if (rc) return rc;
     L.pp = pp;
-    int64_t it = 0;
-    for (; L.pairs && it + L.kmax <= n_iter; it += L.kmax) {
-        rc = L.pair(s, L.kmax);
-        if (rc) return rc;
-    }
-    for (; L.pairs && it + 2 <= n_iter; it += 2) {
-        rc = L.pair(s, 2);
-        if (rc) return rc;
-    }
-    for (; it < n_iter; ++it) {
-        rc = L.single(0, s);
-        if (rc) return rc;
-    }
-    return L.finish(s);
+    return L.run_fixed(n_iter, s);
 }
 
 // ---------------------------------------------------------------------------------------
-// autotuner of the PT kernel's launch shape: times each candidate on a synthetic state of
-// the context's size (results do not depend on the shape, so any data will do)
+// autotuner of the PT kernel's launch shape: times each candidate on a copy of the caller's
+// problem, or a synthetic one of the context's size (results do not depend on the shape, but
+// timings depend on the data)
 
 extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, const double* P_in, const double* Vx_in, const double* Vy_in,
                                   const double* dVx_in, const double* dVy_in, const double* rho_in,
@@ -614,46 +605,46 @@ extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, const double* P_in, cons
         release();
         return set_err(SF_ERR_HIP, "%s: %s", who, hipGetErrorString(e));
     }
-    const int saved_seg = c->seg, saved_depth = c->depth, saved_waves = c->max_waves, saved_fuse = c->fuse,
-              saved_prows = c->pair_rows;
-    c->fuse = 1;  // first the shapes of the one-iteration kernel
-    const int rows_c[] = {1, 2, 4, 8, 16};
-    const int depth_c[] = {1, 2};
-    const int waves_c[] = {-1, 0};
-    sf_pt_tuning bestv{};
-    bestv.us_per_iter = 1e300;
+    const int saved_seg = c->seg, saved_depth = c->depth, saved_waves = c->max_waves, saved_fuse64 = c->fuse64,
+              saved_rows64 = c->rows64;
     int nt = 0;
-    int rc = SF_OK;
-    for (int rows : rows_c)
-        for (int depth : depth_c)
-            for (int waves : waves_c) {
-                c->seg = rows;
-                c->depth = depth;
-                c->max_waves = waves;
-                rc = sf_pt_iterations(c, 3, P, vx, vy, dvx, dvy, rho, mub, mup, gy, dx, dy, nx, ny, 0.0);
-                if (rc) break;
-                if (hipEventRecord(c->ev0, s) != hipSuccess) { rc = set_err(SF_ERR_HIP, "%s: event", who); break; }
-                rc = sf_pt_iterations(c, iters, P, vx, vy, dvx, dvy, rho, mub, mup, gy, dx, dy, nx, ny, 0.0);
-                if (rc) break;
-                float ms = 0.f;
-                if (hipEventRecord(c->ev1, s) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess ||
-                    hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) {
-                    rc = set_err(SF_ERR_HIP, "%s: event timing failed", who);
-                    break;
+    // times `n` iterations of the context's current shape (after a warm-up of `warm`)
+    auto time_shape = [&](int warm, int n, double* us) -> int {
+        int r = sf_pt_iterations(c, warm, P, vx, vy, dvx, dvy, rho, mub, mup, gy, dx, dy, nx, ny, 0.0);
+        if (r) return r;
+        if (hipEventRecord(c->ev0, s) != hipSuccess) return set_err(SF_ERR_HIP, "%s: event record failed", who);
+        r = sf_pt_iterations(c, n, P, vx, vy, dvx, dvy, rho, mub, mup, gy, dx, dy, nx, ny, 0.0);
+        if (r) return r;
+        float ms = 0.f;
+        if (hipEventRecord(c->ev1, s) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess ||
+            hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess)
+            return set_err(SF_ERR_HIP, "%s: event timing failed", who);
+        *us = 1e3 * (double)ms / n;
+        return SF_OK;
+    };
+    auto record = [&](const sf_pt_tuning& t) {
+        if (table && nt < table_cap) table[nt] = t;
+        ++nt;
+    };
+    sf_pt_tuning bestv{}, best_all{};
+    bestv.us_per_iter = 1e300;
+    // first the shapes of the one-iteration kernel; every loop ends at the first error
+    const int rc = [&]() -> int {
+        c->fuse64 = 1;
+        for (int rows : {1, 2, 4, 8, 16})
+            for (int depth : {1, 2})
+                for (int waves : {-1, 0}) {
+                    c->seg = rows;
+                    c->depth = depth;
+                    c->max_waves = waves;
+                    sf_pt_tuning t{rows, depth, waves, 1, 0.0};
+                    const int r = time_shape(3, iters, &t.us_per_iter);
+                    if (r) return r;
+                    record(t);
+                    if (t.us_per_iter < bestv.us_per_iter) bestv = t;
                 }
-                sf_pt_tuning t{};
-                t.rows = rows;
-                t.depth = depth;
-                t.max_waves = waves;
-                t.iters_per_launch = 1;
-                t.us_per_iter = 1e3 * (double)ms / iters;
-                if (table && nt < table_cap) table[nt] = t;
-                ++nt;
-                if (t.us_per_iter < bestv.us_per_iter) bestv = t;
-            }
-    // then passes of 2 and 3 iterations (the check iterations keep the best one-iteration shape)
-    sf_pt_tuning best_all = bestv;
-    if (rc == SF_OK) {
+        // then passes of 2 and 3 iterations (the check iterations keep the best one-iteration shape)
+        best_all = bestv;
         c->seg = bestv.rows;
         c->depth = bestv.depth;
         c->max_waves = bestv.max_waves;
@@ -662,45 +653,32 @@ extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, const double* P_in, cons
         for (int k : {2, 3})
             for (int rows : prows_c) {
                 if (rows > ny && rows != prows_c[0]) continue;
-                c->fuse = k;
-                c->pair_rows = rows;
-                rc = sf_pt_iterations(c, k, P, vx, vy, dvx, dvy, rho, mub, mup, gy, dx, dy, nx, ny, 0.0);
-                if (rc) break;
-                if (hipEventRecord(c->ev0, s) != hipSuccess) { rc = set_err(SF_ERR_HIP, "%s: event", who); break; }
-                rc = sf_pt_iterations(c, n_it, P, vx, vy, dvx, dvy, rho, mub, mup, gy, dx, dy, nx, ny, 0.0);
-                if (rc) break;
-                float ms = 0.f;
-                if (hipEventRecord(c->ev1, s) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess ||
-                    hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) {
-                    rc = set_err(SF_ERR_HIP, "%s: event timing failed", who);
-                    break;
-                }
-                sf_pt_tuning t{};
-                t.rows = rows;
-                t.depth = 1;
-                t.max_waves = -1;
-                t.iters_per_launch = k;
-                t.us_per_iter = 1e3 * (double)ms / n_it;
-                if (table && nt < table_cap) table[nt] = t;
-                ++nt;
+                c->fuse64 = k;
+                c->rows64 = rows;
+                sf_pt_tuning t{rows, 1, -1, k, 0.0};
+                const int r = time_shape(k, n_it, &t.us_per_iter);
+                if (r) return r;
+                record(t);
                 if (t.us_per_iter < best_all.us_per_iter) best_all = t;
             }
-    }
+        return SF_OK;
+    }();
     (void)hipStreamSynchronize(s);
     release();
     if (rc) {
         c->seg = saved_seg;
         c->depth = saved_depth;
         c->max_waves = saved_waves;
-        c->fuse = saved_fuse;
-        c->pair_rows = saved_prows;
+        c->fuse64 = saved_fuse64;
+        c->rows64 = saved_rows64;
         return rc;
     }
     c->seg = bestv.rows;
     c->depth = bestv.depth;
     c->max_waves = bestv.max_waves;
-    c->fuse = best_all.iters_per_launch;
-    c->pair_rows = best_all.iters_per_launch > 1 ? best_all.rows : 0;
+    // the FP64 passes only: the FP32 sweeps of the refined solver keep their own defaults
+    c->fuse64 = best_all.iters_per_launch;
+    c->rows64 = best_all.iters_per_launch > 1 ? best_all.rows : 0;
     if (best) *best = best_all;
     if (n_table) *n_table = nt;
     return SF_OK;
@@ -979,29 +957,50 @@ extern "C" int sf_rt_properties(sf_ctx* c, const double* x_m, const double* y_m,
 // ---------------------------------------------------------------------------------------
 // FP32 PT sweeps inside FP64 defect correction
 
+// Allocates the refined solver's scratch (all but the second momentum) once per context.
+// hpart is allocated last and is the "reserved" mark: after any failed allocation every
+// buffer is freed and reset, so the next call starts from a consistent (empty) context.
 static int refine_reserve(sf_ctx* c)
 {
-    if (c->r_vx[0]) return SF_OK;
+    if (c->hpart) return SF_OK;
     const size_t nx = c->nx, ny = c->ny;
     const size_t nvx = nx * (ny + 1), nvy = (nx + 1) * ny, np = (nx - 1) * (ny - 1);
     const size_t nmx = (nx - 2) * (ny - 1), nmy = (nx - 1) * (ny - 2);
-    for (int b = 0; b < 2; ++b) {
-        HIPCHK(dalloc(&c->r_vx[b], nvx));
-        HIPCHK(dalloc(&c->r_vy[b], nvy));
-        HIPCHK(dalloc(&c->r_p[b], np));
+    const std::pair<float**, size_t> bufs[] = {
+        {&c->r_vx[0], nvx}, {&c->r_vx[1], nvx}, {&c->r_vy[0], nvy},        {&c->r_vy[1], nvy}, {&c->r_p[0], np},
+        {&c->r_p[1], np},   {&c->r_mx, nmx},    {&c->r_my, nmy},           {&c->r_rho, (nx + 1) * ny},
+        {&c->r_mub, nx * ny}, {&c->r_mup, np},  {&c->r_fx, nmx},           {&c->r_fy, nmy},    {&c->r_fc, np},
+        {&c->r_rxo, nmx},   {&c->r_ryo, nmy},   {&c->r_rco, np}};
+    hipError_t e = hipSuccess;
+    for (const auto& b : bufs)
+        if (e == hipSuccess && !*b.first) e = dalloc(b.first, b.second);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&c->hpart, 3 * (size_t)RED_MAX_BLOCKS * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        for (const auto& b : bufs) {
+            if (*b.first) (void)hipFree(*b.first);
+            *b.first = nullptr;
+        }
+        c->hpart = nullptr;
+        return set_err(SF_ERR_HIP, "sf_solve_stokes_refined: scratch allocation failed: %s", hipGetErrorString(e));
     }
-    HIPCHK(dalloc(&c->r_mx, nmx));
-    HIPCHK(dalloc(&c->r_my, nmy));
-    HIPCHK(dalloc(&c->r_rho, (nx + 1) * ny));
-    HIPCHK(dalloc(&c->r_mub, nx * ny));
-    HIPCHK(dalloc(&c->r_mup, np));
-    HIPCHK(dalloc(&c->r_fx, nmx));
-    HIPCHK(dalloc(&c->r_fy, nmy));
-    HIPCHK(dalloc(&c->r_fc, np));
-    HIPCHK(dalloc(&c->r_rxo, nmx));
-    HIPCHK(dalloc(&c->r_ryo, nmy));
-    HIPCHK(dalloc(&c->r_rco, np));
-    HIPCHK(hipHostMalloc((void**)&c->hpart, 3 * (size_t)RED_MAX_BLOCKS * sizeof(double), hipHostMallocDefault));
+    return SF_OK;
+}
+
+// The second PT momentum of the multi-sweep passes: both buffers or neither.
+static int refine_reserve_m2(sf_ctx* c)
+{
+    if (c->r_mx2 && c->r_my2) return SF_OK;
+    hipError_t e = c->r_mx2 ? hipSuccess : dalloc(&c->r_mx2, (size_t)(c->nx - 2) * (c->ny - 1));
+    if (e == hipSuccess && !c->r_my2) e = dalloc(&c->r_my2, (size_t)(c->nx - 1) * (c->ny - 2));
+    if (e != hipSuccess) {
+        for (float** b : {&c->r_mx2, &c->r_my2}) {
+            if (*b) (void)hipFree(*b);
+            *b = nullptr;
+        }
+        return set_err(SF_ERR_HIP, "sf_solve_stokes_refined: momentum buffer allocation failed: %s",
+                       hipGetErrorString(e));
+    }
     return SF_OK;
 }
 
@@ -1141,13 +1140,14 @@ extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, dou
     cp.depth = c->depth > 1 ? 2 : 1;
     // two sweeps per pass unless the context asks for one iteration per launch; the pass
     // writes the momentum to a second buffer, and the context's pointers are swapped after
-    // each pass so that r_mx / r_my always hold the current momentum
+    // each pass so that r_mx / r_my always hold the current momentum.  The setting is the
+    // explicit one of sf_ctx_set_pt_fusion (the autotuner's FP64 choice does not apply here).
     const bool corr_pairs = c->fuse != 1;
     // sweeps per pass: the context's setting, or the size default of the FP32 sweeps
     const int corr_k = c->fuse == 0 ? corr_default_fusion(nx, ny) : c->fuse;
-    if (corr_pairs && !c->r_mx2) {
-        HIPCHK(dalloc(&c->r_mx2, (size_t)(nx - 2) * (ny - 1)));
-        HIPCHK(dalloc(&c->r_my2, (size_t)(nx - 1) * (ny - 2)));
+    if (corr_pairs) {
+        rc = refine_reserve_m2(c);
+        if (rc) return rc;
     }
     cp.rho = c->r_rho;
     cp.mu_b = c->r_mub;

@@ -95,6 +95,61 @@ int batch_run_wide_refined(const RefineEnv& env, const sf_batch_instance* in, in
                                                  __FILE__, __LINE__);                                     \
     } while (0)
 
+namespace sf {
+
+// Ping-pong buffers of the PT loop: a second copy of (Vx_s, Vy_s, P) and, for the
+// multi-iteration passes, of (dVx, dVy).
+struct PTBuffers {
+    double *vx_b, *vy_b, *p_b, *dvx_b, *dvy_b;
+};
+
+// The PT loop state shared by the solve and the fixed-count iteration entry points: the
+// ping-pong pair of (Vx_s, Vy_s, P) and, for multi-iteration passes, of (dVx, dVy).  pp holds
+// the constants, materials and dt pointer; the state pointers are managed here.
+struct PTLoop {
+    PTParams pp;
+    int nx, ny;
+    double *cur_vx, *cur_vy, *cur_p, *nxt_vx, *nxt_vy, *nxt_p;
+    double *cur_dvx, *cur_dvy, *nxt_dvx, *nxt_dvy;
+    double *vx0, *vy0, *p0, *dvx0, *dvy0;  // the caller's arrays
+    PTBuffers b;
+    bool pairs;  // multi-iteration passes enabled (b.dvx_b, b.dvy_b must then be set)
+    int kmax;    // iterations per pass (2 or 3)
+    int rows;    // tile rows of the passes (0: size default)
+
+    // fuse: iterations per pass (0 size default, 1 one per launch, 2 or 3; larger values cap at 3)
+    void init(const PTBuffers& buf, int fuse, int pass_rows, int nx_, int ny_, double* Vx_s, double* Vy_s, double* P,
+              double* dVx, double* dVy)
+    {
+        nx = nx_;
+        ny = ny_;
+        b = buf;
+        cur_vx = vx0 = Vx_s;
+        cur_vy = vy0 = Vy_s;
+        cur_p = p0 = P;
+        cur_dvx = dvx0 = dVx;
+        cur_dvy = dvy0 = dVy;
+        nxt_vx = b.vx_b;
+        nxt_vy = b.vy_b;
+        nxt_p = b.p_b;
+        nxt_dvx = b.dvx_b;
+        nxt_dvy = b.dvy_b;
+        rows = pass_rows;
+        pairs = fuse != 1;
+        kmax = fuse == 0 ? pt_default_fusion(nx, ny) : std::min(fuse, 3);
+    }
+    // One iteration (flags: check-iteration outputs); dV in place.
+    int single(int flags, hipStream_t s);
+    // k plain iterations in one pass.
+    int pair(hipStream_t s, int k = 2);
+    // State back into the caller's arrays.
+    int finish(hipStream_t s);
+    // n_iter plain iterations (passes of kmax while they fit, then 2, then one), then finish.
+    int run_fixed(int64_t n_iter, hipStream_t s);
+};
+
+}  // namespace sf
+
 // The context: device scratch for one grid size and the stream work is enqueued on.
 struct sf_ctx {
     int device = 0;
@@ -138,7 +193,9 @@ struct sf_ctx {
     int seg = 0;        // rows per tile; 0 = size heuristic
     int depth = 0;      // rows of operands prefetched; 0 = size default
     int max_waves = -1; // 0 persistent (resident slots), > 0 cap, < 0 one wave per tile
-    int fuse = 0;       // PT iterations per launch: 0 default (2), 1, 2
-    int pair_rows = 0;  // rows per tile of the two-iteration pass; 0 = size default
+    int fuse = 0;       // sf_ctx_set_pt_fusion: iterations / FP32 sweeps per pass (0 size default, 1..5)
+    int pair_rows = 0;  // rows per tile of the multi-iteration passes; 0 = size default
+    int fuse64 = 0;     // autotuner's choice for the FP64 passes only (0: none; then fuse / pair_rows apply)
+    int rows64 = 0;
     double *dvx_b = nullptr, *dvy_b = nullptr;  // second dVx/dVy (two-iteration pass)
 };
