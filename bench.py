@@ -9,15 +9,17 @@ first, then exactly K timed ones.  This is the reference's own headline measurem
 iteration 11, T_eff = 6 Nx Ny 8 B / t_it).
 
 Printed: one JSON line with PT iterations/s (value), the reference's T_eff, the roofline of
-the fused iteration kernel (algorithmic bytes / average launch time against 8 TB/s), and the
-CPU oracle timed on a bounded sample of iterations on the host cores.  Two further records
-ride on the same line (both skippable):
+the dominant pass kernel (its minimum bytes per launch over its own launch time, timed with HIP
+events around each launch in the timed region, against 8 TB/s), the same for the FP32 pass of
+the refined solver (roofline_fp32), and the CPU oracle timed on a bounded sample of iterations
+on the host cores.  Further records ride on the same line, each skipped when the bench's time
+budget (--budget-s) would not hold it:
 
-* time_to_1e-8 (--tts-n, default 2048): the FP64 solver against FP32 PT sweeps inside FP64
-  defect correction, from the same state to the same FP64 tolerance;
-* ensemble (--ens, default 64 instances): a mixed-size plume ensemble (64..512 nodes per
-  side) solved by the batched solver and by one solveStokes call per instance, bit for bit
-  the same results; solves/s for both.
+* ensemble (--ens, default 256 instances): the mixed-size plume ensemble (64..512 nodes per
+  side) solved by the batched solver (FP64 and FP32-refined) and by one solveStokes call per
+  instance, bit for bit the same FP64 results; solves/s for each, all and converged only;
+* time_to_1e-8 (--tts-n, default 4096): the FP64 solver against FP32 PT sweeps inside FP64
+  defect correction, from the same state to the same FP64 tolerance.
 """
 import argparse
 import json
@@ -60,6 +62,35 @@ def bytes_per_iter_fp32(nx, ny):
     n_vx, n_vy, n_p = nx * (ny + 1), (nx + 1) * ny, (nx - 1) * (ny - 1)
     n_dvx, n_dvy = (nx - 2) * (ny - 1), (nx - 1) * (ny - 2)
     return 4 * (2 * (n_vx + n_vy + n_p + n_dvx + n_dvy) + n_p + nx * ny + (nx + 1) * ny + n_dvx + n_dvy + n_p)
+
+
+def pass_timing(nx, ny, precision):
+    """The library's per-launch timing of the pass kernels since sfhip.time_passes(nx, ny): the
+    kind (iterations per launch) with the most launch time, its launch count and mean seconds."""
+    import sfhip
+
+    kinds = [t for t in sfhip.pass_times(nx, ny) if t["precision"] == precision]
+    if not kinds:
+        return None
+    t = max(kinds, key=lambda e: e["seconds"])
+    return {"iters_per_launch": t["iters_per_launch"], "launches": t["launches"],
+            "launch_s": t["seconds"] / t["launches"]}
+
+
+def roofline_entry(kernel, k, launch_s, launches, b_min, b_iter, traffic, probe):
+    """Roofline of one pass kernel: a launch reads and writes each node's arrays once for its k
+    iterations, so its algorithmic bytes are the pass minimum b_min (b_iter, the bytes of one
+    iteration, counted once); the k x b_iter rate is the per-iteration equivalent."""
+    achieved = b_min / launch_s / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": f"{kernel} ({k} iterations per launch)", "bytes_per_launch": b_min, "launch_s": launch_s,
+            "launches_timed": launches,
+            "per_iteration_equiv_gbs": k * b_iter / launch_s / 1e9,
+            "hbm_gbs": (traffic / launch_s / 1e9) if traffic else None,
+            "frac_hbm_traffic": (traffic / launch_s / 1e9 / HBM_PEAK_GBS) if traffic else None,
+            "dram_probe_gbs": probe,
+            "frac_of_probe_copy": (traffic / launch_s / 1e9 / probe["copy_gbs"]) if traffic and probe else None}
 
 
 REF_T_EFF_GBS = {32: 0.00186, 64: 1.417, 128: 5.494, 256: 17.75, 512: 48.50, 1024: 84.65, 2048: 74.80,
@@ -183,7 +214,7 @@ def build_instance(spec):
     return st, dx, x.numel()
 
 
-def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, budget_mb=0.0):
+def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, budget_mb=0.0, refined_loop=False):
     """Solve every instance to err <= eps with the batched solver in each of ``modes``
     ("wide", "persistent"), then (``loop``) one solveStokes call at a time, each from the same
     zero state; all runs must agree bit for bit."""
@@ -218,8 +249,9 @@ def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, b
         t = time.perf_counter() - t0
         bpi = bytes_per_iter_fp32 if mode.endswith("_refined") else bytes_per_iter
         b_alg = sum(o["niter"] * bpi(st.nx, st.ny) for o, (st, _, _) in zip(outs, insts))
-        res[mode] = {"seconds": t, "solves_per_s": len(specs) / t, "alg_gbs": b_alg / t / 1e9,
-                     "converged": sum(int(o["converged"]) for o in outs), "pt_iters": sum(o["niter"] for o in outs)}
+        conv = sum(int(o["converged"]) for o in outs)
+        res[mode] = {"seconds": t, "solves_per_s": len(specs) / t, "converged_solves_per_s": conv / t,
+                     "alg_gbs": b_alg / t / 1e9, "converged": conv, "pt_iters": sum(o["niter"] for o in outs)}
         sols[mode], iters[mode] = snapshot(), [o["niter"] for o in outs]
         if mode.endswith("_refined"):
             continue
@@ -235,20 +267,22 @@ def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, b
         reset()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        lit = []
+        lit, lconv = [], 0
         for st, dx, _ in insts:
             stats = {}
             sfhip.solveStokes(st.P, st.Vx, st.Vy, st.rho_vy, st.mu_b, st.mu_p, None, None, None, None, None, None,
                               None, st.dVxdtau, st.dVydtau, None, None, st.Vx_small, st.Vy_small, 1.0, dx, dx,
                               st.nx, st.ny, 0.0, 0.5, eps=eps, print_info=False, iterMax=iter_max, stats=stats)
             lit.append(stats["niter"])
+            lconv += int(stats["converged"])
         torch.cuda.synchronize()
         t_loop = time.perf_counter() - t0
-        res["per_solve_loop"] = {"seconds": t_loop, "solves_per_s": len(specs) / t_loop}
+        res["per_solve_loop"] = {"seconds": t_loop, "solves_per_s": len(specs) / t_loop,
+                                 "converged_solves_per_s": lconv / t_loop, "converged": lconv}
         sols["loop"], iters["loop"] = snapshot(), lit
         for mode in modes:
             res[mode]["speedup_vs_loop"] = t_loop / res[mode]["seconds"]
-        if any(m.endswith("_refined") for m in modes):  # the same loop with the refined single solver
+        if refined_loop and any(m.endswith("_refined") for m in modes):  # the loop with the refined solver
             reset()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -361,6 +395,7 @@ def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 1.0), eps=1e-5,
 
 
 def main():
+    t_start = time.perf_counter()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
@@ -368,28 +403,35 @@ def main():
     ap.add_argument("--n", type=int, default=4096, help="nodes per direction")
     ap.add_argument("--cpu-iters", type=int, default=300, help="oracle iterations timed for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tts-n", type=int, default=2048,
+    ap.add_argument("--tts-n", type=int, default=4096,
                     help="nodes per direction of the time-to-1e-8 comparison (FP64 vs FP32-refined); 0 skips it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy first (reference size for rocprofv3 byte counters)")
-    ap.add_argument("--ens", type=int, default=64,
+    ap.add_argument("--ens", type=int, default=256,
                     help="instances of the mixed-size plume ensemble (batched vs per-solve loop); 0 skips it")
     ap.add_argument("--ens-c", type=float, nargs=2, default=(0.5, 1.25), metavar=("LO", "HI"),
                     help="range of log10 viscosity contrast of the ensemble plumes")
     ap.add_argument("--ens-modes", nargs="+", default=["wide", "wide_refined"],
                     help="batched modes to time (wide, persistent, wide_refined)")
+    ap.add_argument("--ens-refined-loop", action="store_true",
+                    help="also time the per-solve loop with the refined single solver")
+    ap.add_argument("--ens-no-loop", action="store_true", help="time the batched modes only (no per-solve loop)")
     ap.add_argument("--ens-steps", type=int, nargs=3, default=None, metavar=("INSTANCES", "N", "STEPS"),
                     help="Rayleigh-Taylor ensemble timesteps: graph-captured vs eager vs per-instance loop")
     ap.add_argument("--ens-steps-modes", nargs="+", default=None,
                     choices=["wide_graph", "wide_eager", "wide_refined_graph", "persistent_graph", "persistent_eager",
                              "loop"], help="run only these forms of the --ens-steps comparison")
     ap.add_argument("--progress", action="store_true", help="print the solvers' convergence checks (long runs)")
-    ap.add_argument("--refined-iters", type=int, default=0,
-                    help="also time this many FP32 correction sweeps (defect-correction solver) on the same state")
+    ap.add_argument("--refined-iters", type=int, default=4000,
+                    help="FP32 correction sweeps of the refined solver timed on the same state (roofline_fp32); 0 "
+                         "skips them")
     ap.add_argument("--no-ref-window", action="store_true",
                     help="skip the reference-window record (200 iterations at 64^2, 512^2, 4096^2)")
     ap.add_argument("--tune", action="store_true",
                     help="run the PT launch-shape autotuner first (default: the size default shape)")
+    ap.add_argument("--budget-s", type=float, default=480.0,
+                    help="wall-time budget of the whole run: a record whose expected time would exceed it is "
+                         "skipped (and named in 'skipped')")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
@@ -426,17 +468,18 @@ def main():
 
     tuned = sfhip.autotune_pt(nx, ny, iters=30, state=st, g_y=1.0, dx=dx, dy=dy)[0] if a.tune else None
 
-    def solve(iters):
+    def solve(iters, refined=False):
         stats = {}
         sfhip.solveStokes(st.P, st.Vx, st.Vy, st.rho_vy, st.mu_b, st.mu_p, None, None, None, None, None, None, None,
                           st.dVxdtau, st.dVydtau, None, None, st.Vx_small, st.Vy_small, 1.0, dx, dy, nx, ny, 0.0, 0.5,
-                          eps=1e-300, print_info=False, iterMax=iters, stats=stats)
+                          eps=1e-300, print_info=False, iterMax=iters, stats=stats, mixed_precision=refined)
         return stats
 
     if a.warmup > 0:
         solve(a.warmup)
     stream = torch.cuda.current_stream()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    sfhip.time_passes(nx, ny, True)  # HIP events around each pass launch, on the library's stream
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev0.record(stream)
@@ -445,20 +488,18 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu_s = ev0.elapsed_time(ev1) * 1e-3
+    pt64 = pass_timing(nx, ny, 0)
+    sfhip.time_passes(nx, ny, False)
     finite = bool(torch.isfinite(st.Vx_small).all().item())
 
-    # The dominant kernel runs k_pass PT iterations per launch (the library's size default: three
-    # above 1024 nodes per side, ptk_tile_f64<3>; two below, pt2_tile_f64). Apart from a few
-    # shorter launches (iteration 10 before the timing mark at 11, and the last iterations), the
-    # timed region is made of such launches, so its HIP-event time per k_pass iterations is the
-    # launch time (to about 0.5%).
-    k_pass = tuned["iters_per_launch"] if tuned else (3 if max(nx, ny) > 1024 else 2)
-    kernel = {1: "pt_tile_f64", 2: "pt2_tile_f64", 3: "ptk_tile_f64<3>"}[k_pass]
-    per_launch = gpu_s * k_pass / a.steps
     probe = sfhip.probe_dram(1 << 30)  # the box's DRAM ceiling for simple streams
     b_iter = bytes_per_iter(nx, ny)
-    achieved = k_pass * b_iter / per_launch / 1e9  # per-iteration algorithmic bytes x iterations per launch
-    traffic = profiled_traffic(nx, {1: "pt_tile_f64", 2: "pt2_tile_f64", 3: "ptk3_tile_f64"}[k_pass])
+    names = {1: ("pt_tile_f64", "pt_tile_f64"), 2: ("pt2_tile_f64", "pt2_tile_f64"),
+             3: ("ptk_tile_f64<3>", "ptk3_tile_f64")}
+    if pt64 is None:  # no pass launch in the timed region (one iteration per launch): the region itself
+        pt64 = {"iters_per_launch": 1, "launches": a.steps, "launch_s": gpu_s / a.steps}
+    k_pass = pt64["iters_per_launch"]
+    kernel, prof = names[k_pass]
     line = {
         "metric": "PT iters/s + T_eff HBM GB/s; time-to-1e-8 FP32-refined vs FP64; batched solves/s",
         "value": a.steps / wall,
@@ -477,49 +518,59 @@ def main():
                    "nx": nx, "ny": ny, "markers": n_markers, "parallelism": "single GPU"},
         "t_eff_gbs": 6 * nx * ny * 8 / (wall / a.steps) / 1e9,
         "alg_gbs": b_iter / (wall / a.steps) / 1e9,
+        "gpu_event_s": gpu_s,
         "state_finite": finite,
         "pt_launch_shape": tuned or (f"size default: {k_pass} PT iterations per launch"),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"{kernel} ({k_pass} PT iterations per launch)",
-                     "bytes_alg_per_launch": k_pass * b_iter, "launch_s": per_launch,
-                     "note": "achieved counts the one-iteration algorithmic bytes (13 x 8 B per node) for each of "
-                             "the iterations a launch runs; the launch reads and writes the state once, so "
-                             "achieved can exceed the HBM peak. hbm_gbs is the measured DRAM rate.",
-                     "hbm_gbs": (traffic / per_launch / 1e9) if traffic else None,
-                     "min_bytes_per_launch": b_iter,
-                     "frac_min_bytes": b_iter / per_launch / 1e9 / HBM_PEAK_GBS,
-                     "dram_probe_gbs": probe,
-                     "frac_of_probe_copy": (traffic / per_launch / 1e9 / probe["copy_gbs"]) if traffic else None},
+        "roofline": roofline_entry(kernel, k_pass, pt64["launch_s"], pt64["launches"], b_iter, b_iter,
+                                   profiled_traffic(nx, prof), probe),
         "cpu_baseline": None if a.no_cpu else cpu_baseline(nx, ny, a.cpu_iters),
     }
-    if a.refined_iters > 0:
+    if a.refined_iters > 0:  # the FP32 passes of the refined solver on the same grid, timed the same way
         for t in (st.P, st.Vx, st.Vy, st.dVxdtau, st.dVydtau, st.Vx_small, st.Vy_small):
             t.zero_()
-        rstats = {}
-        sfhip.solveStokes(st.P, st.Vx, st.Vy, st.rho_vy, st.mu_b, st.mu_p, None, None, None, None, None, None, None,
-                          st.dVxdtau, st.dVydtau, None, None, st.Vx_small, st.Vy_small, 1.0, dx, dy, nx, ny, 0.0, 0.5,
-                          eps=1e-300, print_info=False, iterMax=a.refined_iters, stats=rstats, mixed_precision=True)
+        sfhip.time_passes(nx, ny, True)
+        rstats = solve(a.refined_iters, refined=True)
         torch.cuda.synchronize()
+        c32 = pass_timing(nx, ny, 1)
+        sfhip.time_passes(nx, ny, False)
         t_r = rstats["seconds"] / max(1, rstats["n_fp32_iter"])
         line["refined_sweeps"] = {"fp32_iters": rstats["n_fp32_iter"], "windows": rstats["n_outer"],
                                   "s_per_iter": t_r, "iters_per_s": 1.0 / t_r,
-                                  "alg_gbs_fp32_kernel_bytes": bytes_per_iter_fp32(nx, ny) / t_r / 1e9,
                                   "speedup_per_iter_vs_fp64": (wall / a.steps) / t_r}
+        if c32:
+            k32 = c32["iters_per_launch"]
+            b32 = bytes_per_iter_fp32(nx, ny)
+            kname = f"pt_corrk_f32<{k32}>" if k32 >= 3 else {1: "pt_corr_f32", 2: "pt_corr2_f32"}[k32]
+            line["roofline_fp32"] = roofline_entry(kname, k32, c32["launch_s"], c32["launches"], b32, b32,
+                                                   profiled_traffic(nx, f"pt_corrk{k32}_f32"), probe)
+            line["refined_sweeps"]["kernel_s_per_sweep"] = c32["launch_s"] / k32
     if not a.no_ref_window:
         line["reference_window"] = reference_window()
-    if a.tts_n > 0:
-        del st
+    del st
+    torch.cuda.empty_cache()
+
+    # the north-star records, each only if the remaining budget holds its expected time (seconds
+    # measured on MI355X: 256-instance ensemble about 90, time to 1e-8 at 4096^2 about 190)
+    skipped = []
+
+    def fits(name, expect_s):
+        left = a.budget_s - (time.perf_counter() - t_start)
+        if expect_s > left:
+            skipped.append({"record": name, "expected_s": expect_s, "budget_left_s": left})
+            return False
+        return True
+
+    if a.ens > 0 and fits("ensemble", 20.0 + 0.3 * a.ens):
+        line["ensemble"] = run_ensemble(ensemble_specs(a.ens, log10c=tuple(a.ens_c)), modes=tuple(a.ens_modes),
+                                        loop=not a.ens_no_loop, refined_loop=a.ens_refined_loop)
         torch.cuda.empty_cache()
+    if a.tts_n > 0 and fits("time_to_1e-8", 190.0 * (a.tts_n / 4096.0) ** 3):
         line["time_to_1e-8"] = time_to_solution(a.tts_n, progress=a.progress)
-    if a.ens > 0:
-        if "time_to_1e-8" not in line:
-            del st
         torch.cuda.empty_cache()
-        line["ensemble"] = run_ensemble(ensemble_specs(a.ens, log10c=tuple(a.ens_c)), modes=tuple(a.ens_modes))
-    if a.ens_steps:
-        torch.cuda.empty_cache()
+    if a.ens_steps and fits("ensemble_steps", 0.0):
         line["ensemble_steps"] = run_ensemble_steps(*a.ens_steps, modes=a.ens_steps_modes, progress=a.progress)
+    line["skipped"] = skipped
+    line["bench_wall_s"] = time.perf_counter() - t_start
     print(json.dumps(line), flush=True)
 
 

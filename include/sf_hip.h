@@ -113,6 +113,21 @@ int sf_ctx_autotune_pt(sf_ctx *ctx, int iters, const double *P, const double *Vx
                        const double *mu_p, double g_y, double dx, double dy, sf_pt_tuning *best, sf_pt_tuning *table,
                        int table_cap, int *n_table);
 
+/* Per-launch timing of the pass kernels, for roofline figures.  enable = 1 brackets every later
+ * launch of the FP64 multi-iteration passes (sf_solve_stokes, sf_pt_iterations) and of the FP32
+ * correction passes (sf_solve_stokes_refined) on this context with two HIP events on its stream
+ * and clears the totals; enable = 0 stops.  Results do not change.  sf_ctx_pass_times returns the
+ * totals since the last enable, one entry per (precision, iterations per launch) launched; it
+ * waits for the timed launches.  No reference counterpart (instrumentation). */
+typedef struct {
+    int32_t precision;        /* 0: FP64 PT iterations, 1: FP32 correction sweeps */
+    int32_t iters_per_launch; /* PT iterations (FP32 sweeps) per launch */
+    int64_t launches;
+    double seconds;           /* summed durations of those launches */
+} sf_pass_time;
+int sf_ctx_time_passes(sf_ctx *ctx, int enable);
+int sf_ctx_pass_times(sf_ctx *ctx, sf_pass_time *out, int cap, int *n_out);
+
 /* DRAM probe: best-of-8 bandwidth of a copy (1 read + 1 write) and of a 3-read + 1-write
  * kernel over four `bytes`-sized scratch arrays (allocated and freed here), in GB/s.  The
  * ceiling the stencil kernels' bandwidth is read against.  No reference counterpart. */
@@ -325,6 +340,36 @@ int sf_plume_properties(sf_ctx *ctx, const double *x_m, const double *y_m, int64
                         double *rho_m, double *mu_m);
 int sf_rt_properties(sf_ctx *ctx, const double *x_m, const double *y_m, int64_t nm, const double mat[8],
                      double *rho_m, double *mu_m);
+
+/* initializeMarkersCPU -- replaces scripts/common.jl:40-87 at one rank: the deterministic marker
+ * lattice LinRange(dxm/2, lx - dxm/2, md (Nx-1)) x LinRange(dym/2, ly - dym/2, md (Ny-1)),
+ * dxm = dx / md, x outer / y inner, each element evaluated as Julia 1.8's LinRange does (bit for
+ * bit).  rand_pos = 1 adds (u - 0.5) dxm and (u - 0.5) dym as RAND_MARKER_POS does, with u from
+ * the build's splitmix64(seed, index) (Julia's RNG stream is not reproduced).  x_m, y_m hold
+ * md^2 (nx-1)(ny-1) values. */
+int sf_lattice_markers(sf_ctx *ctx, double *x_m, double *y_m, int nx, int ny, int md, double lx, double ly,
+                       double dx, double dy, int rand_pos, uint64_t seed);
+
+/* setInitialMarkerProperties! -- replaces scripts/common.jl:104-110 with a material descriptor
+ * in place of the density / viscosity callbacks: rho_m, mu_m of the first shape (in order) that
+ * contains the global position (x0 + x_m, y0 + y_m), else rho_bg, mu_bg.  Shapes (a..e their
+ * parameters):
+ *   SF_SHAPE_Y_BELOW    y < a                      SF_SHAPE_Y_ABOVE   y >= a
+ *   SF_SHAPE_CIRCLE     (x - a)^2 + (y - b)^2 < c^2
+ *   SF_SHAPE_RECT       a <= x < b and c <= y < d
+ *   SF_SHAPE_SINE_BELOW y < a + b sin(2 pi c x / d)
+ * The plume callbacks of StokesFlow2D_multixpu.jl:387-404 are {Y_BELOW air_height, CIRCLE}. */
+enum { SF_SHAPE_Y_BELOW = 1, SF_SHAPE_Y_ABOVE = 2, SF_SHAPE_CIRCLE = 3, SF_SHAPE_RECT = 4, SF_SHAPE_SINE_BELOW = 5 };
+#define SF_MAX_SHAPES 16
+typedef struct {
+    int32_t kind; /* SF_SHAPE_* */
+    int32_t reserved;
+    double a, b, c, d, e;
+    double rho, mu; /* material inside the shape */
+} sf_shape;
+int sf_marker_properties(sf_ctx *ctx, const double *x_m, const double *y_m, int64_t nm, double x0, double y0,
+                         const sf_shape *shapes, int n_shapes, double rho_bg, double mu_bg, double *rho_m,
+                         double *mu_m);
 
 #ifdef __cplusplus
 }

@@ -48,6 +48,7 @@ static void free_ctx(sf_ctx* c)
     if (c->hcounters) (void)hipHostFree(c->hcounters);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    c->timer.release();
     delete c;
 }
 
@@ -161,9 +162,17 @@ int PTLoop::pair(hipStream_t s, int k)
     pp.dvy = cur_dvy;
     pp.dvx_o = nxt_dvx;
     pp.dvy_o = nxt_dvy;
+    if (tm && tm->on) {
+        const int r = tm->begin(s, 0, k);
+        if (r) return r;
+    }
     const hipError_t e = launch_pt_multi(pp, k, s);
     pp.seg = seg1;  // the one-iteration kernel keeps its own tile rows
     HIPCHK(e);
+    if (tm && tm->on) {
+        const int r = tm->end(s);
+        if (r) return r;
+    }
     std::swap(cur_vx, nxt_vx);
     std::swap(cur_vy, nxt_vy);
     std::swap(cur_p, nxt_p);
@@ -236,6 +245,101 @@ static int ctx_pt_loop(sf_ctx* c, PTLoop* L, double* Vx_s, double* Vy_s, double*
     const int rc = ctx_pt_buffers(c, fuse != 1, &b);
     if (rc) return rc;
     L->init(b, fuse, rows, c->nx, c->ny, Vx_s, Vy_s, P, dVx, dVy);
+    L->tm = &c->timer;
+    return SF_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// per-launch timing of the pass kernels
+
+int PassTimer::begin(hipStream_t s, int precision, int k)
+{
+    if (count == RING) {  // resolve the oldest launch (recorded RING launches ago)
+        const int r = flush_one();
+        if (r) return r;
+    }
+    const int slot = (head + count) % RING;
+    for (int q = 0; q < 2; ++q)
+        if (!ev[2 * slot + q]) HIPCHK(hipEventCreate(&ev[2 * slot + q]));
+    kind[slot] = (precision ? 8 : 0) + std::min(std::max(k, 0), 7);
+    HIPCHK(hipEventRecord(ev[2 * slot], s));
+    return SF_OK;
+}
+
+int PassTimer::end(hipStream_t s)
+{
+    const int slot = (head + count) % RING;
+    HIPCHK(hipEventRecord(ev[2 * slot + 1], s));
+    ++count;
+    return SF_OK;
+}
+
+int PassTimer::flush_one()
+{
+    float ms = 0.f;
+    HIPCHK(hipEventSynchronize(ev[2 * head + 1]));
+    HIPCHK(hipEventElapsedTime(&ms, ev[2 * head], ev[2 * head + 1]));
+    sec[kind[head]] += 1e-3 * (double)ms;
+    n[kind[head]] += 1;
+    head = (head + 1) % RING;
+    --count;
+    return SF_OK;
+}
+
+int PassTimer::flush()
+{
+    while (count > 0) {
+        const int r = flush_one();
+        if (r) return r;
+    }
+    return SF_OK;
+}
+
+void PassTimer::reset()
+{
+    head = count = 0;
+    for (int q = 0; q < KINDS; ++q) {
+        sec[q] = 0.0;
+        n[q] = 0;
+    }
+}
+
+void PassTimer::release()
+{
+    for (hipEvent_t& e : ev)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+    reset();
+    on = false;
+}
+
+extern "C" int sf_ctx_time_passes(sf_ctx* c, int enable)
+{
+    if (!c) return set_err(SF_ERR_ARG, "sf_ctx_time_passes: NULL ctx");
+    if (c->timer.count) {  // drop what an earlier timing left pending
+        if (c->stream) HIPCHK(hipStreamSynchronize(c->stream));
+        else HIPCHK(hipDeviceSynchronize());
+    }
+    c->timer.reset();
+    c->timer.on = enable != 0;
+    return SF_OK;
+}
+
+extern "C" int sf_ctx_pass_times(sf_ctx* c, sf_pass_time* out, int cap, int* n_out)
+{
+    if (!c) return set_err(SF_ERR_ARG, "sf_ctx_pass_times: NULL ctx");
+    if (cap > 0 && !out) return set_err(SF_ERR_ARG, "sf_ctx_pass_times: NULL output with cap > 0");
+    const int rc = c->timer.flush();
+    if (rc) return rc;
+    int m = 0;
+    for (int q = 0; q < PassTimer::KINDS; ++q) {
+        if (!c->timer.n[q]) continue;
+        if (m < cap) out[m] = sf_pass_time{q >= 8 ? 1 : 0, q % 8, c->timer.n[q], c->timer.sec[q]};
+        ++m;
+    }
+    if (n_out) *n_out = m;
     return SF_OK;
 }
 
@@ -309,50 +413,51 @@ static int enqueue_dt(sf_ctx* c, const double* vx, const double* vy, double dx, 
     return enqueue_dt_raw(c->stream, c->nx, c->ny, vx, vy, dx, dy, maxdisp, update, c->part, c->scal);
 }
 
-extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_pad, const double* rho_vy,
-                               const double* mu_b, const double* mu_p, double* txx, double* tyy, double* txy,
-                               double* divV, double* dtauP, double* Rx, double* Ry, double* dVx, double* dVy,
-                               double* dtauVx, double* dtauVy, double* Vx_s, double* Vy_s, double g_y, double dx,
-                               double dy, int nx, int ny, double dt, double maxdisp, int use_fss, double eps,
-                               int64_t iter_max, int print_info, sf_solve_out* out)
+int sf::solve_core(const SolveScratch& w, const char* who, double* P, double* Vx_pad, double* Vy_pad,
+                   const double* rho_vy, const double* mu_b, const double* mu_p, double* txx, double* tyy, double* txy,
+                   double* divV, double* dtauP, double* Rx, double* Ry, double* dVx, double* dVy, double* dtauVx,
+                   double* dtauVy, double* Vx_s, double* Vy_s, double g_y, double dx, double dy, int nx, int ny,
+                   double dt, double maxdisp, int use_fss, double eps, int64_t iter_max, int print_info,
+                   sf_solve_out* out)
 {
-    int rc = check_ctx(c, nx, ny, "sf_solve_stokes");
-    if (rc) return rc;
     if (!P || !Vx_pad || !Vy_pad || !rho_vy || !mu_b || !mu_p || !dVx || !dVy || !Vx_s || !Vy_s)
-        return set_err(SF_ERR_ARG, "sf_solve_stokes: a state or material array is NULL");
-    if (!(dx > 0.0) || !(dy > 0.0)) return set_err(SF_ERR_ARG, "sf_solve_stokes: dx, dy must be > 0");
-    hipStream_t s = c->stream;
+        return set_err(SF_ERR_ARG, "%s: a state or material array is NULL", who);
+    if (!(dx > 0.0) || !(dy > 0.0)) return set_err(SF_ERR_ARG, "%s: dx, dy must be > 0", who);
+    hipStream_t s = w.s;
     const SolveConsts k = solve_consts(nx, ny, dx, dy);
     if (!use_fss) dt = 0.0;
 
-    double* t_xx = txx ? txx : c->txx;
-    double* t_yy = tyy ? tyy : c->tyy;
-    double* t_xy = txy ? txy : c->txy;
-    double* d_v = divV ? divV : c->divv;
-    double* r_x = Rx ? Rx : c->rx;
-    double* r_y = Ry ? Ry : c->ry;
+    double* t_xx = txx ? txx : w.txx;
+    double* t_yy = tyy ? tyy : w.tyy;
+    double* t_xy = txy ? txy : w.txy;
+    double* d_v = divV ? divV : w.divv;
+    double* r_x = Rx ? Rx : w.rx;
+    double* r_y = Ry ? Ry : w.ry;
     if (dtauP || dtauVx || dtauVy)
         HIPCHK(launch_fill_timesteps(nx, ny, mu_p, dtauVx, dtauVy, dtauP, k.vsc_mdxy2, k.ptsc41, k.max_nxy, s));
-    HIPCHK(launch_set_scalar(c->scal + SC_DT, dt, s));
+    HIPCHK(launch_set_scalar(w.scal + SC_DT, dt, s));
 
     const int mn = std::min(nx, ny);
     const int64_t ncheck = std::min<int64_t>(5 * (int64_t)mn, 2000);
     const int64_t ndtupdate = 10 * (int64_t)mn;
     const int64_t itert1 = 11;
     const double n_rx = (double)(nx - 2) * (ny - 1), n_ry = (double)(nx - 1) * (ny - 2), n_dv = (double)(nx - 1) * (ny - 1);
+    auto dt_of = [&](const double* vx, const double* vy, int update) {
+        return enqueue_dt_raw(s, nx, ny, vx, vy, dx, dy, maxdisp, update, w.part, w.scal);
+    };
 
     PTParams pp{};
     pp.nx = nx;
     pp.ny = ny;
-    pp.seg = c->seg;
-    pp.depth = c->depth;
-    pp.max_waves = c->max_waves;
+    pp.seg = w.seg;
+    pp.depth = w.depth;
+    pp.max_waves = w.max_waves;
     pp.dvx = dVx;
     pp.dvy = dVy;
     pp.rho = rho_vy;
     pp.mu_b = mu_b;
     pp.mu_p = mu_p;
-    pp.dt = c->scal + SC_DT;
+    pp.dt = w.scal + SC_DT;
     pp.txx = t_xx;
     pp.tyy = t_yy;
     pp.txy = t_xy;
@@ -368,16 +473,23 @@ extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_
     pp.max_nxy = k.max_nxy;
 
     PTLoop L;
-    rc = ctx_pt_loop(c, &L, Vx_s, Vy_s, P, dVx, dVy);
-    if (rc) return rc;
+    L.init(w.pt, w.fuse, w.rows, nx, ny, Vx_s, Vy_s, P, dVx, dVy);
+    L.tm = w.tm;
     L.pp = pp;
 
     double err = 2.0 * eps;
     int64_t iter = 1, niter = 0;
     bool timed = false;
+    auto t_host0 = std::chrono::steady_clock::now();
+    int rc = SF_OK;
     while (err > eps && iter <= iter_max) {
-        if (iter == itert1) {
-            HIPCHK(hipEventRecord(c->ev0, s));
+        if (iter == itert1) {  // the reference's timing window starts here (:44)
+            if (w.ev0) {
+                HIPCHK(hipEventRecord(w.ev0, s));
+            } else {
+                HIPCHK(hipStreamSynchronize(s));
+                t_host0 = std::chrono::steady_clock::now();
+            }
             timed = true;
         }
         // iterations iter..iter+k-1 in one pass when none is a check iteration, none but the
@@ -397,7 +509,7 @@ extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_
             rc = L.pair(s, kf);
             if (rc) return rc;
             if (use_fss && (iter + kf - 1) % ndtupdate == 0) {
-                rc = enqueue_dt(c, L.cur_vx, L.cur_vy, dx, dy, maxdisp, 1);
+                rc = dt_of(L.cur_vx, L.cur_vy, 1);
                 if (rc) return rc;
             }
             iter += kf;
@@ -414,17 +526,17 @@ extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_
 
         const bool dt_upd = use_fss && (iter % ndtupdate) == 0;
         if (dt_upd) {
-            rc = enqueue_dt(c, cur_vx, cur_vy, dx, dy, maxdisp, 1);
+            rc = dt_of(cur_vx, cur_vy, 1);
             if (rc) return rc;
         }
         if (check) {
             if (use_fss) {
                 if (!dt_upd) {
-                    rc = enqueue_dt(c, cur_vx, cur_vy, dx, dy, maxdisp, 0);
+                    rc = dt_of(cur_vx, cur_vy, 0);
                     if (rc) return rc;
                 }
             } else {
-                HIPCHK(launch_set_scalar(c->scal + SC_DT_CHECK, 0.0, s));
+                HIPCHK(launch_set_scalar(w.scal + SC_DT_CHECK, 0.0, s));
             }
             ResYParams rq{};
             rq.nx = nx;
@@ -435,7 +547,7 @@ extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_
             rq.rho = rho_vy;
             rq.vx = cur_vx;
             rq.vy = cur_vy;
-            rq.dt = c->scal + SC_DT_CHECK;
+            rq.dt = w.scal + SC_DT_CHECK;
             rq.ry = r_y;
             rq.g = g_y;
             rq.rdx = k.rdx;
@@ -448,14 +560,14 @@ extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_
             rp.n[1] = (int64_t)n_ry;
             rp.a[2] = d_v;
             rp.n[2] = (int64_t)n_dv;
-            rp.part = c->part;
+            rp.part = w.part;
             rp.mode = RED_SUM;
             const int nb = red_num_blocks((int64_t)n_dv);
             HIPCHK(launch_reduce(rp, 3, nb, s));
-            HIPCHK(launch_finalize_err(c->part, nb, n_rx, n_ry, n_dv, c->scal, s));
-            HIPCHK(hipMemcpyAsync(c->hscal, c->scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHK(launch_finalize_err(w.part, nb, n_rx, n_ry, n_dv, w.scal, s));
+            HIPCHK(hipMemcpyAsync(w.hscal, w.scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
-            err = c->hscal[SC_ERR];
+            err = w.hscal[SC_ERR];
             if (print_info >= 2) {
                 printf("  iter = %lld, err = %1.3e\n", (long long)iter, err);
                 fflush(stdout);
@@ -464,20 +576,27 @@ extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_
         iter += 1;
         niter += 1;
     }
-    if (timed) HIPCHK(hipEventRecord(c->ev1, s));
+    double runtime = 0.0;
+    if (timed) {
+        if (w.ev0) {
+            HIPCHK(hipEventRecord(w.ev1, s));
+        } else {
+            HIPCHK(hipStreamSynchronize(s));
+            runtime = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_host0).count();
+        }
+    }
 
     rc = L.finish(s);  // state back into the caller's arrays
     if (rc) return rc;
     HIPCHK(launch_pad_mirror(nx, ny, Vx_s, Vy_s, Vx_pad, Vy_pad, s));
-    rc = enqueue_dt(c, Vx_s, Vy_s, dx, dy, maxdisp, 0);
+    rc = dt_of(Vx_s, Vy_s, 0);
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(c->hscal, c->scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(w.hscal, w.scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
 
-    double runtime = 0.0;
-    if (timed) {
+    if (timed && w.ev0) {
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        HIPCHK(hipEventElapsedTime(&ms, w.ev0, w.ev1));
         runtime = 1e-3 * (double)ms;
     }
     const double t_it = runtime / (double)(niter - itert1 + 1);
@@ -486,18 +605,41 @@ extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_
         printf("Total steps = %lld, err = %1.3e, time = %1.3e sec (@ T_eff = %1.2f GB/s) \n", (long long)niter, err,
                runtime, timed ? a_eff / t_it : 0.0);
     if (out) {
-        out->dt_new = c->hscal[SC_DT_CHECK];
+        out->dt_new = w.hscal[SC_DT_CHECK];
         out->err = err;
         out->t_eff = timed ? a_eff / t_it : 0.0;
         out->runtime = runtime;
         out->niter = niter;
         out->converged = err <= eps;
         out->reserved = 0;
-        out->dt_last = c->hscal[SC_DT];
+        out->dt_last = w.hscal[SC_DT];
         out->t_it = timed ? t_it : 0.0;
         out->gbs_alg = timed ? bytes_per_iter(nx, ny) / t_it / 1e9 : 0.0;
     }
     return SF_OK;
+}
+
+extern "C" int sf_solve_stokes(sf_ctx* c, double* P, double* Vx_pad, double* Vy_pad, const double* rho_vy,
+                               const double* mu_b, const double* mu_p, double* txx, double* tyy, double* txy,
+                               double* divV, double* dtauP, double* Rx, double* Ry, double* dVx, double* dVy,
+                               double* dtauVx, double* dtauVy, double* Vx_s, double* Vy_s, double g_y, double dx,
+                               double dy, int nx, int ny, double dt, double maxdisp, int use_fss, double eps,
+                               int64_t iter_max, int print_info, sf_solve_out* out)
+{
+    int rc = check_ctx(c, nx, ny, "sf_solve_stokes");
+    if (rc) return rc;
+    // the autotuner's FP64 choice, else the explicit setting, else the size default
+    const int fuse = c->fuse64 > 0 ? c->fuse64 : c->fuse;
+    const int rows = c->fuse64 > 0 ? c->rows64 : c->pair_rows;
+    PTBuffers b{};
+    rc = ctx_pt_buffers(c, fuse != 1, &b);
+    if (rc) return rc;
+    const SolveScratch w{c->stream, b,       c->txx,    c->tyy,       c->txy, c->divv, c->rx,
+                         c->ry,     c->part, c->scal,   c->hscal,     c->ev0, c->ev1,  c->seg,
+                         c->depth,  c->max_waves, fuse, rows,         &c->timer};
+    return solve_core(w, "sf_solve_stokes", P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtauP, Rx,
+                      Ry, dVx, dVy, dtauVx, dtauVy, Vx_s, Vy_s, g_y, dx, dy, nx, ny, dt, maxdisp, use_fss, eps,
+                      iter_max, print_info, out);
 }
 
 extern "C" int sf_pt_iterations(sf_ctx* c, int64_t n_iter, double* P, double* Vx_s, double* Vy_s, double* dVx,
@@ -942,6 +1084,39 @@ extern "C" int sf_plume_properties(sf_ctx* c, const double* x_m, const double* y
     return SF_OK;
 }
 
+extern "C" int sf_lattice_markers(sf_ctx* c, double* x_m, double* y_m, int nx, int ny, int md, double lx, double ly,
+                                  double dx, double dy, int rand_pos, uint64_t seed)
+{
+    if (!c || !x_m || !y_m) return set_err(SF_ERR_ARG, "sf_lattice_markers: NULL argument");
+    if (nx < 2 || ny < 2 || md < 1) return set_err(SF_ERR_ARG, "sf_lattice_markers: bad sizes");
+    if (!(dx > 0.0) || !(dy > 0.0)) return set_err(SF_ERR_ARG, "sf_lattice_markers: dx, dy must be > 0");
+    HIPCHK(launch_lattice_markers(x_m, y_m, nx, ny, md, lx, ly, dx, dy, rand_pos, seed, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SF_OK;
+}
+
+extern "C" int sf_marker_properties(sf_ctx* c, const double* x_m, const double* y_m, int64_t nm, double x0, double y0,
+                                    const sf_shape* shapes, int n_shapes, double rho_bg, double mu_bg, double* rho_m,
+                                    double* mu_m)
+{
+    if (!c || (nm > 0 && (!x_m || !y_m || !rho_m || !mu_m)) || (n_shapes > 0 && !shapes))
+        return set_err(SF_ERR_ARG, "sf_marker_properties: NULL argument");
+    if (n_shapes < 0 || n_shapes > SF_MAX_SHAPES)
+        return set_err(SF_ERR_ARG, "sf_marker_properties: %d shapes (0..%d)", n_shapes, SF_MAX_SHAPES);
+    ShapeSet set{};
+    set.n = n_shapes;
+    set.rho_bg = rho_bg;
+    set.mu_bg = mu_bg;
+    for (int k = 0; k < n_shapes; ++k) {
+        if (shapes[k].kind < SF_SHAPE_Y_BELOW || shapes[k].kind > SF_SHAPE_SINE_BELOW)
+            return set_err(SF_ERR_ARG, "sf_marker_properties: shape %d has unknown kind %d", k, shapes[k].kind);
+        set.s[k] = shapes[k];
+    }
+    HIPCHK(launch_shape_properties(x_m, y_m, nm, x0, y0, set, rho_m, mu_m, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SF_OK;
+}
+
 extern "C" int sf_rt_properties(sf_ctx* c, const double* x_m, const double* y_m, int64_t nm, const double mat[8],
                                 double* rho_m, double* mu_m)
 {
@@ -1169,6 +1344,15 @@ extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, dou
 
     const double n_rx = (double)nmx, n_ry = (double)nmy, n_dv = (double)np;
     const int nbr = red_num_blocks(np);
+    // one FP32 pass of kk sweeps, bracketed by the context's pass timer when it is on
+    auto timed_corr = [&](int kk, auto&& launch) -> int {
+        if (c->timer.on) {
+            const int r = c->timer.begin(s, 1, kk);
+            if (r) return r;
+        }
+        HIPCHK(launch());
+        return c->timer.on ? c->timer.end(s) : SF_OK;
+    };
     double dt_k = dt;
     int64_t n32 = 0, n64 = 0;
     int rebases = 0;
@@ -1253,7 +1437,8 @@ extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, dou
             cp.mx_o = c->r_mx2;
             cp.my_o = c->r_my2;
             cp.seg = c->pair_rows;  // 0: size default
-            HIPCHK(launch_pt_corrk(cp, corr_k, s));
+            rc = timed_corr(corr_k, [&] { return launch_pt_corrk(cp, corr_k, s); });
+            if (rc) return rc;
             cur = 1 - cur;
             std::swap(c->r_mx, c->r_mx2);
             std::swap(c->r_my, c->r_my2);
@@ -1270,7 +1455,8 @@ extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, dou
             cp.mx_o = c->r_mx2;
             cp.my_o = c->r_my2;
             cp.seg = c->pair_rows;  // 0: size default
-            HIPCHK(launch_pt_corr2(cp, s));
+            rc = timed_corr(2, [&] { return launch_pt_corr2(cp, s); });
+            if (rc) return rc;
             cur = 1 - cur;
             std::swap(c->r_mx, c->r_mx2);
             std::swap(c->r_my, c->r_my2);
@@ -1287,7 +1473,8 @@ extern "C" int sf_solve_stokes_refined(sf_ctx* c, double* P, double* Vx_pad, dou
             cp.mx_o = nullptr;  // in place
             cp.my_o = nullptr;
             cp.seg = corr_seg;
-            HIPCHK(launch_pt_corr(cp, false, s));
+            rc = timed_corr(1, [&] { return launch_pt_corr(cp, false, s); });
+            if (rc) return rc;
             cur = 1 - cur;
         }
         const int64_t before = n32;

@@ -103,6 +103,26 @@ struct PTBuffers {
     double *vx_b, *vy_b, *p_b, *dvx_b, *dvy_b;
 };
 
+// Per-launch timing of the pass kernels on one stream: two HIP events around each launch, kept
+// in a ring and resolved lazily (a slot is resolved when the ring wraps, long after its launch
+// ended), summed per (precision, iterations per launch).  Instrumentation only.
+struct PassTimer {
+    enum { RING = 256, KINDS = 16 };
+    bool on = false;
+    hipEvent_t ev[2 * RING] = {};
+    int kind[RING] = {};
+    int head = 0, count = 0;
+    double sec[KINDS] = {};
+    int64_t n[KINDS] = {};
+    // precision 0: FP64 PT iterations, 1: FP32 correction sweeps; k iterations per launch
+    int begin(hipStream_t s, int precision, int k);
+    int end(hipStream_t s);
+    int flush_one();  // resolves the oldest pending launch (waits for it)
+    int flush();      // resolves every pending launch
+    void reset();
+    void release();
+};
+
 // The PT loop state shared by the solve and the fixed-count iteration entry points: the
 // ping-pong pair of (Vx_s, Vy_s, P) and, for multi-iteration passes, of (dVx, dVy).  pp holds
 // the constants, materials and dt pointer; the state pointers are managed here.
@@ -116,6 +136,7 @@ struct PTLoop {
     bool pairs;  // multi-iteration passes enabled (b.dvx_b, b.dvy_b must then be set)
     int kmax;    // iterations per pass (2 or 3)
     int rows;    // tile rows of the passes (0: size default)
+    PassTimer* tm = nullptr;  // times the pass launches when set and on
 
     // fuse: iterations per pass (0 size default, 1 one per launch, 2 or 3; larger values cap at 3)
     void init(const PTBuffers& buf, int fuse, int pass_rows, int nx_, int ny_, double* Vx_s, double* Vy_s, double* P,
@@ -147,6 +168,29 @@ struct PTLoop {
     // n_iter plain iterations (passes of kmax while they fit, then 2, then one), then finish.
     int run_fixed(int64_t n_iter, hipStream_t s);
 };
+
+// Scratch of one FP64 solve (sf_solve_stokes: the context's).
+struct SolveScratch {
+    hipStream_t s;
+    PTBuffers pt;                              // ping-pong state (dvx_b, dvy_b: NULL when fuse == 1)
+    double *txx, *tyy, *txy, *divv, *rx, *ry;  // used where the caller passes NULL
+    double* part;                              // 3 * RED_MAX_BLOCKS reduction partials
+    double* scal;                              // SC_COUNT device scalars
+    double* hscal;                             // host mirror of scal
+    hipEvent_t ev0, ev1;                       // timing window; NULL: taken on the host clock
+    int seg, depth, max_waves;                 // one-iteration kernel shape
+    int fuse, rows;                            // iterations per pass and pass tile rows (0: defaults)
+    PassTimer* tm;                             // may be NULL
+};
+
+// The body of solveStokes! (StokesSolver_multixpu.jl:14-96) on the given scratch; `who` names the
+// entry point in error messages.
+int solve_core(const SolveScratch& w, const char* who, double* P, double* Vx_pad, double* Vy_pad,
+               const double* rho_vy, const double* mu_b, const double* mu_p, double* txx, double* tyy, double* txy,
+               double* divV, double* dtauP, double* Rx, double* Ry, double* dVx, double* dVy, double* dtauVx,
+               double* dtauVy, double* Vx_s, double* Vy_s, double g_y, double dx, double dy, int nx, int ny,
+               double dt, double maxdisp, int use_fss, double eps, int64_t iter_max, int print_info,
+               sf_solve_out* out);
 
 }  // namespace sf
 
@@ -198,4 +242,5 @@ struct sf_ctx {
     int fuse64 = 0;     // autotuner's choice for the FP64 passes only (0: none; then fuse / pair_rows apply)
     int rows64 = 0;
     double *dvx_b = nullptr, *dvy_b = nullptr;  // second dVx/dVy (two-iteration pass)
+    sf::PassTimer timer;  // sf_ctx_time_passes
 };

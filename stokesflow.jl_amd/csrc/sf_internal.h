@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/sf_hip.h"
+
 namespace sf {
 
 enum { PT_WRITE_TAU = 1, PT_WRITE_RX = 2 };
@@ -257,5 +259,15 @@ hipError_t launch_plume_properties(const double* x_m, const double* y_m, int64_t
                                    double* mu_m, hipStream_t s);
 hipError_t launch_rt_properties(const double* x_m, const double* y_m, int64_t nm, Mat10 mat, double* rho_m,
                                 double* mu_m, hipStream_t s);
+hipError_t launch_lattice_markers(double* x_m, double* y_m, int nx, int ny, int md, double lx, double ly, double dx,
+                                  double dy, int rand_pos, uint64_t seed, hipStream_t s);
+// A material descriptor passed by value to the kernel.
+struct ShapeSet {
+    int n;
+    double rho_bg, mu_bg;
+    sf_shape s[SF_MAX_SHAPES];
+};
+hipError_t launch_shape_properties(const double* x_m, const double* y_m, int64_t nm, double x0, double y0,
+                                   const ShapeSet& set, double* rho_m, double* mu_m, hipStream_t s);
 
 }  // namespace sf

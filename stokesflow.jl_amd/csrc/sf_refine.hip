@@ -47,11 +47,82 @@ __device__ inline int crow_r(int r, int nrows) { return r < 0 ? 0 : (r >= nrows 
 
 struct CorrRowIn {
     float vx1, vy1, rho1, pold, mup, mub, fc, mx, my, fx, fy;
+    // the next row's values one lane down (columns c + 1, c + 2), when the caller has them
+    float vx1_p1, vy1_p1, rho1_p1, rho1_p2;
 };
 
 struct CorrCols {
     unsigned vx, vy, p, dvx, dvy;
 };
+
+// The FP32 sweep's arithmetic, shared by every FP32 kernel so that all sweep groupings give the
+// same bits.  It is the PT iteration's (StokesSolver_multixpu.jl:140-234) on the correction, in an
+// order of its own: products with a common factor are grouped, constants folded and a*b+c fused,
+// which the FP64 kernels may not do (they reproduce the reference bit for bit) but the correction
+// sweeps may: they only have to drive the FP64 residual down, and convergence is decided in FP64.
+struct CorrC {
+    float rdx, rdy;
+    float kx, ky;  // g dt / 8 / dx and g dt / 2 / dy: the free-surface term's factors
+    float dampx, dampy;
+};
+
+template <class PP>
+__device__ __forceinline__ CorrC corr_consts(const PP& p, float dt)
+{
+    const float gdt = p.g * dt;
+    return CorrC{p.rdx, p.rdy, (gdt * 0.125f) * p.rdx, (gdt * 0.5f) * p.rdy, p.dampx, p.dampy};
+}
+
+// The P update and the stresses at one node (compute_P_tau!, :140-169): m2 = 2 mu_p, taup = dtauP.
+struct CorrPT {
+    float dvxdx, dvydy, cres, pn, txx, tyy, txy;
+};
+
+__device__ __forceinline__ CorrPT corr_ptau(const CorrC& k, float vx1, float vx1_p1, float vx_r, float vy1_p1,
+                                            float vy_r, float vy_r_p1, float m2, float mub, float fc, float pold,
+                                            float taup)
+{
+    CorrPT o;
+    o.dvxdx = (vx1_p1 - vx1) * k.rdx;
+    o.dvydy = (vy1_p1 - vy_r_p1) * k.rdy;
+    const float div = o.dvxdx + o.dvydy;
+    o.cres = div + fc;  // continuity residual of the full state
+    o.pn = __builtin_fmaf(-taup, o.cres, pold);
+    const float d3 = div * (1.0f / 3.0f);
+    o.txx = m2 * (o.dvxdx - d3);
+    o.tyy = m2 * (o.dvydy - d3);
+    o.txy = mub * __builtin_fmaf(vx1 - vx_r, k.rdy, (vy_r_p1 - vy_r) * k.rdx);
+    return o;
+}
+
+// x momentum residual (compute_ResX, :176-182) plus the forcing.
+__device__ __forceinline__ float corr_rx(const CorrC& k, float txx_p1, float txx, float pn_p1, float pn, float txy_p1,
+                                         float txy_prev_p1, float fx)
+{
+    return __builtin_fmaf((txx_p1 - txx) - (pn_p1 - pn), k.rdx, __builtin_fmaf(txy_p1 - txy_prev_p1, k.rdy, fx));
+}
+
+// y momentum residual with the free-surface term (compute_ResY, :189-202) plus the forcing;
+// av4 is four times the Vx average, d_xi and d_yi the two rho differences.
+__device__ __forceinline__ float corr_ry(const CorrC& k, float tyy, float tyy_prev, float pn, float pn_prev,
+                                         float txy_p1, float txy, float av4, float d_xi, float vy, float d_yi,
+                                         float fy)
+{
+    const float stab = __builtin_fmaf(av4 * d_xi, k.kx, (vy * d_yi) * k.ky);
+    return __builtin_fmaf((tyy - tyy_prev) - (pn - pn_prev), k.rdy, __builtin_fmaf(txy_p1 - txy, k.rdx, fy)) - stab;
+}
+
+// The pseudo-time steps (compute_timesteps!, :128-133) from mu_p.
+template <class PP>
+__device__ __forceinline__ float corr_taup(const PP& p, float mup)
+{
+    return (p.ptsc41 * mup) * tau_rcp(p.max_nxy);
+}
+template <class PP>
+__device__ __forceinline__ float corr_tauv(const PP& p, float mup_a, float mup_b)
+{
+    return p.vsc_mdxy2 * tau_rcp((mup_a + mup_b) * 2.05f);
+}
 
 template <class PP>
 __device__ __forceinline__ CorrRowIn corr_load_row(const PP& p, const CorrCols& lc, int r, bool need_j)
@@ -90,7 +161,7 @@ __device__ __forceinline__ void corr_tile(const PP& p, int strip, int segid)
     const float dt = p.dt;
     const int64_t lvx = nx, lvy = nx + 1, lp = nx - 1;
     const bool c_p = c < nx - 1;
-    const float rdx = p.rdx, rdy = p.rdy;
+    const CorrC k = corr_consts(p, dt);
     // momentum in place unless a second buffer is given (ping-pong in the wide batched mode)
     float* const mx_w = p.mx_o ? p.mx_o : p.mx;
     float* const my_w = p.my_o ? p.my_o : p.my;
@@ -138,15 +209,9 @@ __device__ __forceinline__ void corr_tile(const PP& p, int strip, int segid)
         const float rho_r1_p1 = lane_down(in.rho1, 1);
         const float rho_r1_p2 = lane_down(in.rho1, 2);
 
-        const float dvxdx = (vx_r1_p1 - in.vx1) * rdx;
-        const float dvydy = (vy_r1_p1 - vy_r_p1) * rdy;
-        const float div = dvxdx + dvydy;
-        const float dtaup = (p.ptsc41 * in.mup) * tau_rcp(p.max_nxy);
-        const float cres = div + in.fc;                // continuity residual of the full state
-        const float pn = in.pold - dtaup * cres;
-        const float txx = (2.0f * in.mup) * (dvxdx - (1.0f / 3.0f) * div);
-        const float tyy = (2.0f * in.mup) * (dvydy - (1.0f / 3.0f) * div);
-        const float txy = in.mub * ((in.vx1 - vx_r) * rdy + (vy_r_p1 - vy_r) * rdx);
+        const CorrPT t = corr_ptau(k, in.vx1, vx_r1_p1, vx_r, vy_r1_p1, vy_r, vy_r_p1, 2.0f * in.mup, in.mub, in.fc,
+                                   in.pold, corr_taup(p, in.mup));
+        const float pn = t.pn, cres = t.cres, txx = t.txx, tyy = t.tyy, txy = t.txy;
         const float txy_p1 = lane_down(txy, 1);
 
         if (emit && out_lane && c_p && cell_row) {
@@ -154,19 +219,14 @@ __device__ __forceinline__ void corr_tile(const PP& p, int strip, int segid)
             if constexpr (CHECK) p.rco[ix2(c, r, lp)] = cres;
         }
         if (has_out && out_lane) {
-            const float rx = ((txx_prev_p1 - txx_prev) * rdx + (txy_p1 - txy_prev_p1) * rdy) -
-                             (pn_prev_p1 - pn_prev) * rdx + in.fx;
-            const float av = 0.25f * (((vx_r + vx_r_p1) + in.vx1) + vx_r1_p1);
-            const float d_xi = rho_r_p2 - rho_r_;
-            const float d_yi = rho_r1_p1 - rho_rm1_p1;
-            const float ry = (((tyy - tyy_prev) * rdy + (txy_p1 - txy) * rdx) - (pn - pn_prev) * rdy) -
-                             p.g * (dt * (((av * d_xi) * 0.5f) * rdx + ((vy_r_p1 * d_yi) * 0.5f) * rdy)) + in.fy;
-            const float mx_new = p.dampx * in.mx + rx;
-            const float my_new = p.dampy * in.my + ry;
-            const float dtauvx = p.vsc_mdxy2 * tau_rcp(((mup_prev + mup_prev_p1) * 0.5f) * 4.1f);
-            const float dtauvy = p.vsc_mdxy2 * tau_rcp(((mup_prev + in.mup) * 0.5f) * 4.1f);
-            const float vx_new = vx_r_p1 + dtauvx * mx_new;
-            const float vy_new = vy_r_p1 + dtauvy * my_new;
+            const float rx = corr_rx(k, txx_prev_p1, txx_prev, pn_prev_p1, pn_prev, txy_p1, txy_prev_p1, in.fx);
+            const float av4 = ((vx_r + vx_r_p1) + in.vx1) + vx_r1_p1;
+            const float ry = corr_ry(k, tyy, tyy_prev, pn, pn_prev, txy_p1, txy, av4, rho_r_p2 - rho_r_, vy_r_p1,
+                                     rho_r1_p1 - rho_rm1_p1, in.fy);
+            const float mx_new = __builtin_fmaf(k.dampx, in.mx, rx);
+            const float my_new = __builtin_fmaf(k.dampy, in.my, ry);
+            const float vx_new = __builtin_fmaf(corr_tauv(p, mup_prev, mup_prev_p1), mx_new, vx_r_p1);
+            const float vy_new = __builtin_fmaf(corr_tauv(p, mup_prev, in.mup), my_new, vy_r_p1);
             if (dvx_ok) {
                 mx_w[ix2(c, j, nx - 2)] = mx_new;
                 p.vx_b[ix2(c + 1, j + 1, lvx)] = vx_new;
@@ -250,58 +310,62 @@ __device__ __forceinline__ void corr_win_init(CorrWin& w, float vx_r, float vy_r
     w.mup_prev = w.mup_prev_p1 = 0.f;
 }
 
-// The pseudo-time steps of a row (they depend on mu_p only; later sweeps of a row reuse them).
+// The pseudo-time steps of a row and 2 mu_p (they depend on mu_p only; later sweeps of a row
+// reuse them).
 struct CorrTau {
-    float p, vx, vy;
+    float p, vx, vy, m2;
 };
 
 // One row step of one correction sweep with corr_tile's expressions, then the window slides.
-template <bool TAU_IN, class PP>
-__device__ __forceinline__ CorrRow corr_row(const PP& p, float dt, CorrWin& w, const CorrRowIn& in, CorrTau& tau)
+// SHIFTED: in carries the next row's lane-shifted values (a later sweep of a multi-sweep pass
+// has them from the sweep before it), else they are shifted here.
+template <bool TAU_IN, bool SHIFTED, class PP>
+__device__ __forceinline__ CorrRow corr_row(const PP& p, const CorrC& k, CorrWin& w, const CorrRowIn& in, CorrTau& tau)
 {
-    const float rdx = p.rdx, rdy = p.rdy;
-    const float vx_r1_p1 = lane_down(in.vx1, 1);
-    const float vy_r1_p1 = lane_down(in.vy1, 1);
-    const float rho_r1_p1 = lane_down(in.rho1, 1);
-    const float rho_r1_p2 = lane_down(in.rho1, 2);
-
-    const float dvxdx = (vx_r1_p1 - in.vx1) * rdx;
-    const float dvydy = (vy_r1_p1 - w.vy_r_p1) * rdy;
-    const float div = dvxdx + dvydy;
-    if constexpr (!TAU_IN) tau.p = (p.ptsc41 * in.mup) * tau_rcp(p.max_nxy);
-    const float cres = div + in.fc;
-    const float pn = in.pold - tau.p * cres;
-    const float txx = (2.0f * in.mup) * (dvxdx - (1.0f / 3.0f) * div);
-    const float tyy = (2.0f * in.mup) * (dvydy - (1.0f / 3.0f) * div);
-    const float txy = in.mub * ((in.vx1 - w.vx_r) * rdy + (w.vy_r_p1 - w.vy_r) * rdx);
-    const float txy_p1 = lane_down(txy, 1);
-
-    const float rx = ((w.txx_prev_p1 - w.txx_prev) * rdx + (txy_p1 - w.txy_prev_p1) * rdy) -
-                     (w.pn_prev_p1 - w.pn_prev) * rdx + in.fx;
-    const float av = 0.25f * (((w.vx_r + w.vx_r_p1) + in.vx1) + vx_r1_p1);
-    const float d_xi = w.rho_r_p2 - w.rho_r;
-    const float d_yi = rho_r1_p1 - w.rho_rm1_p1;
-    const float ry = (((tyy - w.tyy_prev) * rdy + (txy_p1 - txy) * rdx) - (pn - w.pn_prev) * rdy) -
-                     p.g * (dt * (((av * d_xi) * 0.5f) * rdx + ((w.vy_r_p1 * d_yi) * 0.5f) * rdy)) + in.fy;
-    CorrRow o;
-    o.pn = pn;
-    o.mx = p.dampx * in.mx + rx;
-    o.my = p.dampy * in.my + ry;
-    if constexpr (!TAU_IN) {
-        tau.vx = p.vsc_mdxy2 * tau_rcp(((w.mup_prev + w.mup_prev_p1) * 0.5f) * 4.1f);
-        tau.vy = p.vsc_mdxy2 * tau_rcp(((w.mup_prev + in.mup) * 0.5f) * 4.1f);
+    float vx_r1_p1, vy_r1_p1, rho_r1_p1, rho_r1_p2;
+    if constexpr (SHIFTED) {
+        vx_r1_p1 = in.vx1_p1;
+        vy_r1_p1 = in.vy1_p1;
+        rho_r1_p1 = in.rho1_p1;
+        rho_r1_p2 = in.rho1_p2;
+    } else {
+        vx_r1_p1 = lane_down(in.vx1, 1);
+        vy_r1_p1 = lane_down(in.vy1, 1);
+        rho_r1_p1 = lane_down(in.rho1, 1);
+        rho_r1_p2 = lane_down(in.rho1, 2);
     }
-    o.vx = w.vx_r_p1 + tau.vx * o.mx;
-    o.vy = w.vy_r_p1 + tau.vy * o.my;
+    if constexpr (!TAU_IN) {
+        tau.p = corr_taup(p, in.mup);
+        tau.m2 = 2.0f * in.mup;
+    }
+    const CorrPT t = corr_ptau(k, in.vx1, vx_r1_p1, w.vx_r, vy_r1_p1, w.vy_r, w.vy_r_p1, tau.m2, in.mub, in.fc,
+                               in.pold, tau.p);
+    const float txy_p1 = lane_down(t.txy, 1);
+    const float rx = corr_rx(k, w.txx_prev_p1, w.txx_prev, w.pn_prev_p1, w.pn_prev, txy_p1, w.txy_prev_p1, in.fx);
+    const float av4 = ((w.vx_r + w.vx_r_p1) + in.vx1) + vx_r1_p1;
+    const float ry = corr_ry(k, t.tyy, w.tyy_prev, t.pn, w.pn_prev, txy_p1, t.txy, av4, w.rho_r_p2 - w.rho_r,
+                             w.vy_r_p1, rho_r1_p1 - w.rho_rm1_p1, in.fy);
+    CorrRow o;
+    o.pn = t.pn;
+    o.mx = __builtin_fmaf(k.dampx, in.mx, rx);
+    o.my = __builtin_fmaf(k.dampy, in.my, ry);
+    if constexpr (!TAU_IN) {
+        tau.vx = corr_tauv(p, w.mup_prev, w.mup_prev_p1);
+        tau.vy = corr_tauv(p, w.mup_prev, in.mup);
+    }
+    o.vx = __builtin_fmaf(tau.vx, o.mx, w.vx_r_p1);
+    o.vy = __builtin_fmaf(tau.vy, o.my, w.vy_r_p1);
 
-    w.pn_prev_p1 = lane_down(pn, 1);
-    w.txx_prev_p1 = lane_down(txx, 1);
-    w.mup_prev_p1 = lane_down(in.mup, 1);
-    w.pn_prev = pn;
-    w.txx_prev = txx;
-    w.tyy_prev = tyy;
+    w.pn_prev_p1 = lane_down(t.pn, 1);
+    w.txx_prev_p1 = lane_down(t.txx, 1);
+    if constexpr (!TAU_IN) {
+        w.mup_prev_p1 = lane_down(in.mup, 1);
+        w.mup_prev = in.mup;
+    }
+    w.pn_prev = t.pn;
+    w.txx_prev = t.txx;
+    w.tyy_prev = t.tyy;
     w.txy_prev_p1 = txy_p1;
-    w.mup_prev = in.mup;
     w.vx_r = in.vx1;
     w.vx_r_p1 = vx_r1_p1;
     w.vy_r = in.vy1;
@@ -360,7 +424,7 @@ __device__ __forceinline__ void corr2_tile(const PP& p, int strip, int segid)
     const bool out_lane = lane >= 1 && lane <= CORR2_OUT_LANES;
     const int r0 = segid * p.seg;
     const int r1 = min(r0 + p.seg, ny);
-    const float dt = p.dt;
+    const CorrC k = corr_consts(p, p.dt);
     const int cc = c < 0 ? 0 : c;
     CorrCols lc;
     lc.vx = (unsigned)min(cc, nx - 1);
@@ -382,14 +446,14 @@ __device__ __forceinline__ void corr2_tile(const PP& p, int strip, int segid)
     CorrRowIn nxt = corr_load_row(p, lc, r, r >= r0 - 1);
     float a_pn = 0.f, a_mup = 0.f, a_mub = 0.f, a_fc = 0.f, a_mx = 0.f, a_my = 0.f, a_fx = 0.f, a_fy = 0.f;
     float a_rho = 0.f, a_vx = 0.f, a_vy = 0.f, a_vx_last = 0.f;
-    CorrTau a_tau{0.f, 0.f, 0.f};
+    CorrTau a_tau{0.f, 0.f, 0.f, 0.f};
 
     for (; r < ra1; ++r) {
         const CorrRowIn in = nxt;
         if (r + 1 < ra1) nxt = corr_load_row(p, lc, r + 1, true);
         const float rho_row = wa.rho_r;
         CorrTau tau_a;
-        const CorrRow oa = corr_row<false>(p, dt, wa, in, tau_a);
+        const CorrRow oa = corr_row<false, false>(p, k, wa, in, tau_a);
         const float vx_a = vx_zero ? 0.f : oa.vx;
         const float vy_dn = lane_down(oa.vy, 1), vy_up = lane_up_f(oa.vy, 1);
         const float vy_a = (r == ny - 1) ? 0.f : (vy_left ? vy_dn : (vy_right ? vy_up : oa.vy));
@@ -415,7 +479,7 @@ __device__ __forceinline__ void corr2_tile(const PP& p, int strip, int segid)
             ib.my = a_my;
             ib.fx = a_fx;
             ib.fy = a_fy;
-            const CorrRow ob = corr_row<true>(p, dt, wb, ib, a_tau);
+            const CorrRow ob = corr_row<true, false>(p, k, wb, ib, a_tau);
             corr2_store(p, ob, c, rb, r0, out_lane);
         }
         a_tau = tau_a;
@@ -445,7 +509,7 @@ __device__ __forceinline__ void corr2_tile(const PP& p, int strip, int segid)
         ib.my = a_my;
         ib.fx = a_fx;
         ib.fy = a_fy;
-        const CorrRow ob = corr_row<true>(p, dt, wb, ib, a_tau);
+        const CorrRow ob = corr_row<true, false>(p, k, wb, ib, a_tau);
         corr2_store(p, ob, c, ny - 1, r0, out_lane);
     }
 }
@@ -455,6 +519,13 @@ __device__ __forceinline__ void corr2_tile(const PP& p, int strip, int segid)
 struct CorrHand {
     float pn, mup, mub, fc, mx, my, fx, fy, rho, vx, vy;
     CorrTau tau;
+};
+
+// The row a stage hands to the next one within an outer step: V (lane-shifted for the next
+// stage, and unshifted, which is the next stage's one-lane-down view), rho and its lane-down
+// views (materials sit on the same columns in every stage).
+struct CorrPass {
+    float vx, vy, rho, vx_p1, vy_p1, rho_p1, rho_p2;
 };
 
 template <int K, class PP>
@@ -467,7 +538,7 @@ __device__ __forceinline__ void corrk_tile(const PP& p, int strip, int segid)
     const bool out_lane = lane >= K - 1 && lane < K - 1 + OUT;
     const int r0 = segid * p.seg;
     const int r1 = min(r0 + p.seg, ny);
-    const float dt = p.dt;
+    const CorrC kc = corr_consts(p, p.dt);
     const int cc = c < 0 ? 0 : c;
     CorrCols lc;
     lc.vx = (unsigned)min(cc, nx - 1);
@@ -491,7 +562,7 @@ __device__ __forceinline__ void corrk_tile(const PP& p, int strip, int segid)
     float vx_last[K];
 #pragma unroll
     for (int s = 0; s < K; ++s) {
-        h[s] = CorrHand{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, CorrTau{0.f, 0.f, 0.f}};
+        h[s] = CorrHand{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, CorrTau{0.f, 0.f, 0.f, 0.f}};
         vx_last[s] = 0.f;
     }
     int r = rs0[0];
@@ -504,21 +575,23 @@ __device__ __forceinline__ void corrk_tile(const PP& p, int strip, int segid)
     // without the per-stage range tests.
     auto step = [&](const int r, auto all_tag) {
         constexpr bool ALL = decltype(all_tag)::value;
-        float cur_vx[K], cur_vy[K], cur_rho[K];
+        CorrPass cur[K];
         CorrHand hn[K];
         bool ran[K];
 #pragma unroll
         for (int s = 0; s < K; ++s) {
             const int rs = r - s;
             ran[s] = false;
-            cur_vx[s] = cur_vy[s] = cur_rho[s] = 0.f;
+            cur[s] = CorrPass{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             hn[s] = h[s];
             if constexpr (!ALL) {
                 if (rs < rs0[s] || rs > rs1[s]) {
-                    if (s < K - 1 && rs == ny && rs - 1 >= rs0[s]) {
-                        cur_vx[s] = vx_last[s];
-                        cur_vy[s] = 0.f;
-                        cur_rho[s] = h[s].rho;
+                    if (s < K - 1 && rs == ny && rs - 1 >= rs0[s]) {  // the virtual row ny
+                        cur[s].vx = vx_last[s];
+                        cur[s].vx_p1 = lane_down(vx_last[s], 1);
+                        cur[s].rho = h[s].rho;
+                        cur[s].rho_p1 = lane_down(h[s].rho, 1);
+                        cur[s].rho_p2 = lane_down(h[s].rho, 2);
                     }
                     continue;
                 }
@@ -528,9 +601,13 @@ __device__ __forceinline__ void corrk_tile(const PP& p, int strip, int segid)
                 in = nxt;
                 if (ALL || r + 1 <= rs1[0]) nxt = corr_load_row(p, lc, r + 1, true);
             } else {
-                in.vx1 = cur_vx[s - 1];
-                in.vy1 = cur_vy[s - 1];
-                in.rho1 = cur_rho[s - 1];
+                in.vx1 = cur[s - 1].vx;
+                in.vy1 = cur[s - 1].vy;
+                in.rho1 = cur[s - 1].rho;
+                in.vx1_p1 = cur[s - 1].vx_p1;
+                in.vy1_p1 = cur[s - 1].vy_p1;
+                in.rho1_p1 = cur[s - 1].rho_p1;
+                in.rho1_p2 = cur[s - 1].rho_p2;
                 in.pold = h[s - 1].pn;
                 in.mup = h[s - 1].mup;
                 in.mub = h[s - 1].mub;
@@ -545,14 +622,14 @@ __device__ __forceinline__ void corrk_tile(const PP& p, int strip, int segid)
                     corr_win_init(w[s], vx0, vy0, h[s - 1].rho);
                 }
             }
-            const float rho_row = w[s].rho_r;
+            const float rho_row = w[s].rho_r, rho_row_p1 = w[s].rho_r_p1, rho_row_p2 = w[s].rho_r_p2;
             CorrTau tau;
             CorrRow o;
             if (s == 0) {
-                o = corr_row<false>(p, dt, w[s], in, tau);
+                o = corr_row<false, false>(p, kc, w[s], in, tau);
             } else {
                 tau = h[s - 1].tau;
-                o = corr_row<true>(p, dt, w[s], in, tau);
+                o = corr_row<true, true>(p, kc, w[s], in, tau);
             }
             if (s == K - 1) {
                 corr2_store(p, o, c, rs, r0, out_lane);
@@ -560,9 +637,7 @@ __device__ __forceinline__ void corrk_tile(const PP& p, int strip, int segid)
                 const float vx_a = vx_zero ? 0.f : o.vx;
                 const float vy_dn = lane_down(o.vy, 1), vy_up = lane_up_f(o.vy, 1);
                 const float vy_a = (rs == ny - 1) ? 0.f : (vy_left ? vy_dn : (vy_right ? vy_up : o.vy));
-                cur_vx[s] = lane_up_f(vx_a, 1);
-                cur_vy[s] = lane_up_f(vy_a, 1);
-                cur_rho[s] = rho_row;
+                cur[s] = CorrPass{lane_up_f(vx_a, 1), lane_up_f(vy_a, 1), rho_row, vx_a, vy_a, rho_row_p1, rho_row_p2};
                 hn[s].pn = o.pn;
                 hn[s].mup = in.mup;
                 hn[s].mub = in.mub;
@@ -572,15 +647,15 @@ __device__ __forceinline__ void corrk_tile(const PP& p, int strip, int segid)
                 hn[s].fx = in.fx;
                 hn[s].fy = in.fy;
                 hn[s].rho = rho_row;
-                hn[s].vx = cur_vx[s];
-                hn[s].vy = cur_vy[s];
+                hn[s].vx = cur[s].vx;
+                hn[s].vy = cur[s].vy;
                 hn[s].tau = tau;
                 ran[s] = true;
             }
         }
 #pragma unroll
         for (int s = 0; s < K - 1; ++s) {
-            if (ALL || ran[s]) vx_last[s] = cur_vx[s];
+            if (ALL || ran[s]) vx_last[s] = cur[s].vx;
             h[s] = hn[s];
         }
     };
@@ -591,7 +666,16 @@ __device__ __forceinline__ void corrk_tile(const PP& p, int strip, int segid)
         hi = min(hi, rs1[s] + s);
     }
     for (; r < lo && r <= r_last; ++r) step(r, std::false_type{});
-    for (; r <= hi; ++r) step(r, std::true_type{});
+    // the steady middle two rows per trip: a value carried from one row step to the next then
+    // alternates between two registers instead of being copied back each step
+    for (; r + 1 <= hi; r += 2) {
+        step(r, std::true_type{});
+        step(r + 1, std::true_type{});
+    }
+    if (r <= hi) {  // at most one steady row left
+        step(r, std::true_type{});
+        ++r;
+    }
     for (; r <= r_last; ++r) step(r, std::false_type{});
 }
 

@@ -183,7 +183,94 @@ __global__ __launch_bounds__(256) void rt_props_kernel(const double* __restrict_
     mu_m[m] = up ? mat.v[5] : mat.v[7];
 }
 
+// The reference's deterministic marker lattice (initializeMarkersCPU, common.jl:40-83, one rank:
+// all four boundary flags true): LinRange(dxm/2, lx - dxm/2, md (Nx-1)) in x, likewise in y,
+// x outer / y inner.  A LinRange element is evaluated as Julia 1.8 does, (1-t) a + t b with
+// t = j / (len - 1), unfused.  rand_pos adds (u - 0.5) dxm, (u - 0.5) dym (common.jl:85-87) with u
+// from splitmix64(seed, index) (the reference's Julia RNG stream is not reproduced).
+__device__ inline double linrange_at(double a, double b, int64_t len, int64_t j)
+{
+    if (len == 1) return a;
+    const double t = (double)j / (double)(len - 1);
+    return (1.0 - t) * a + t * b;
+}
+
+__global__ __launch_bounds__(256) void lattice_markers_kernel(double* __restrict__ x_m, double* __restrict__ y_m,
+                                                              int64_t nmx, int64_t nmy, double xlo, double xhi,
+                                                              double ylo, double yhi, double dxm, double dym,
+                                                              int rand_pos, uint64_t seed)
+{
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmx * nmy) return;
+    const int64_t i = m / nmy, j = m - i * nmy;
+    double x = linrange_at(xlo, xhi, nmx, i);
+    double y = linrange_at(ylo, yhi, nmy, j);
+    if (rand_pos) {
+        x = x + (uniform01(seed, 2 * (uint64_t)m) - 0.5) * dxm;
+        y = y + (uniform01(seed, 2 * (uint64_t)m + 1) - 0.5) * dym;
+    }
+    x_m[m] = x;
+    y_m[m] = y;
+}
+
+// Material descriptor in place of the reference's density / viscosity callbacks
+// (setInitialMarkerProperties!, common.jl:104-110): the first shape that contains the global
+// position (x0 + x_m, y0 + y_m) gives rho and mu, else the background.  Each test is written as
+// the reference's example callbacks write theirs (StokesFlow2D_multixpu.jl:387-404:
+// y < air_height, (x - cx)^2 + (y - cy)^2 < r^2).
+__global__ __launch_bounds__(256) void shape_props_kernel(const double* __restrict__ x_m, const double* __restrict__ y_m,
+                                                          int64_t nm, double x0, double y0, ShapeSet set,
+                                                          double* __restrict__ rho_m, double* __restrict__ mu_m)
+{
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nm) return;
+    const double x = x0 + x_m[m], y = y0 + y_m[m];
+    double rho = set.rho_bg, mu = set.mu_bg;
+    for (int k = 0; k < set.n; ++k) {
+        const sf_shape& sh = set.s[k];
+        bool in = false;
+        switch (sh.kind) {
+        case SF_SHAPE_Y_BELOW: in = y < sh.a; break;
+        case SF_SHAPE_Y_ABOVE: in = y >= sh.a; break;
+        case SF_SHAPE_CIRCLE: {
+            const double ddx = x - sh.a, ddy = y - sh.b;
+            in = ddx * ddx + ddy * ddy < sh.c * sh.c;
+            break;
+        }
+        case SF_SHAPE_RECT: in = x >= sh.a && x < sh.b && y >= sh.c && y < sh.d; break;
+        case SF_SHAPE_SINE_BELOW: in = y < sh.a + sh.b * sin(2.0 * M_PI * sh.c * x / sh.d); break;
+        default: break;
+        }
+        if (in) {
+            rho = sh.rho;
+            mu = sh.mu;
+            break;
+        }
+    }
+    rho_m[m] = rho;
+    mu_m[m] = mu;
+}
+
 static int g1(int64_t n) { return (int)((n + 255) / 256); }
+
+hipError_t launch_lattice_markers(double* x_m, double* y_m, int nx, int ny, int md, double lx, double ly, double dx,
+                                  double dy, int rand_pos, uint64_t seed, hipStream_t s)
+{
+    const double dxm = dx / md, dym = dy / md;
+    const int64_t nmx = (int64_t)md * (nx - 1), nmy = (int64_t)md * (ny - 1);
+    if (nmx * nmy <= 0) return hipSuccess;
+    hipLaunchKernelGGL(lattice_markers_kernel, dim3(g1(nmx * nmy)), dim3(256), 0, s, x_m, y_m, nmx, nmy, dxm / 2,
+                       lx - dxm / 2, dym / 2, ly - dym / 2, dxm, dym, rand_pos, seed);
+    return hipGetLastError();
+}
+
+hipError_t launch_shape_properties(const double* x_m, const double* y_m, int64_t nm, double x0, double y0,
+                                   const ShapeSet& set, double* rho_m, double* mu_m, hipStream_t s)
+{
+    if (nm <= 0) return hipSuccess;
+    hipLaunchKernelGGL(shape_props_kernel, dim3(g1(nm)), dim3(256), 0, s, x_m, y_m, nm, x0, y0, set, rho_m, mu_m);
+    return hipGetLastError();
+}
 
 hipError_t launch_rk4(double* x_m, double* y_m, int64_t nm, const double* vx, int vx_n1, int vx_n2, const double* vy,
                       int vy_n1, int vy_n2, double x_vx_min, double y_vx_min, double x_vy_min, double y_vy_min,

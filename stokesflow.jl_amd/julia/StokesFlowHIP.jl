@@ -10,6 +10,8 @@ argument order and return values:
   * `moveMarkersRK4!`       -- scripts/MoveMarkers.jl:15-23
   * `saveStats`, `saveMarkers`, `saveGrid` -- scripts/StokesFlow2D_multixpu.jl:212-299
     (MAT-file level 5, which MAT.jl and viz.jl read)
+  * `initializeMarkersCPU`, `setInitialMarkerProperties!` -- scripts/common.jl:40-110 (on the
+    device; the density / viscosity callbacks become a `Materials` descriptor)
 
 Arrays are `AMDGPU.ROCArray{Float64}` (column-major, x fastest: the reference's
 `Data.Array` layout), passed to the library as device pointers.  One rank: `comm` and
@@ -22,7 +24,8 @@ module StokesFlowHIP
 
 using AMDGPU
 
-export solveStokes!, bilinearMarkerToGrid!, moveMarkersRK4!, saveStats, saveMarkers, saveGrid
+export solveStokes!, bilinearMarkerToGrid!, moveMarkersRK4!, saveStats, saveMarkers, saveGrid,
+       initializeMarkersCPU, setInitialMarkerProperties!, Materials
 
 const libsfhip = joinpath(@__DIR__, "..", "libsfhip.so")
 
@@ -181,6 +184,82 @@ function saveGrid(nt, rank, x, y, μ_b, x_p, y_p, P, x_vx, y_vx, Vx, x_vy, y_vy,
                 (Ptr{Cvoid}, Cstring, Cint, Cint, Cdouble, Cdouble, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
                  Ptr{Float64}),
                 h, path, length(x), length(y), x[2] - x[1], y[2] - y[1], dptr(μ_b), dptr(P), dptr(Vx), dptr(Vy)))
+    return nothing
+end
+
+"""
+    initializeMarkersCPU(comm, dims, coords, marker_density, lx, ly, dx, dy, Nx, Ny, RAND_MARKER_POS; seed=0)
+        -- common.jl:40-100 at one rank
+
+The marker lattice, built on the device bit for bit as the reference builds it; returns
+(x_m, y_m, ρ_m, μ_m) as ROCArrays (ρ_m, μ_m zero).  With RAND_MARKER_POS the jitter comes from
+the library's seeded generator (`seed`), not from Julia's RNG.
+"""
+function initializeMarkersCPU(comm, dims, coords, marker_density::Integer, lx, ly, dx, dy, Nx, Ny, RAND_MARKER_POS;
+                              seed=0)
+    nm = marker_density^2 * (Nx - 1) * (Ny - 1)
+    x_m = AMDGPU.zeros(Float64, nm)
+    y_m = AMDGPU.zeros(Float64, nm)
+    h = context(0, 0)
+    check(ccall((:sf_lattice_markers, libsfhip), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Cint, Cint, Cdouble, Cdouble, Cdouble, Cdouble, Cint,
+                 UInt64),
+                h, dptr(x_m), dptr(y_m), Nx, Ny, marker_density, lx, ly, dx, dy, RAND_MARKER_POS ? 1 : 0, seed))
+    return x_m, y_m, AMDGPU.zeros(Float64, nm), AMDGPU.zeros(Float64, nm)
+end
+
+struct Shape
+    kind::Int32
+    reserved::Int32
+    a::Cdouble
+    b::Cdouble
+    c::Cdouble
+    d::Cdouble
+    e::Cdouble
+    ρ::Cdouble
+    μ::Cdouble
+end
+
+const SHAPE_KINDS = Dict(:y_below => 1, :y_above => 2, :circle => 3, :rect => 4, :sine_below => 5)
+
+"""
+    Materials(shapes, ρ_bg, μ_bg)
+
+Material descriptor in place of the `density(x, y)` / `viscosity(x, y)` callbacks: shapes in
+order, `(:y_below, h, ρ, μ)`, `(:y_above, h, ρ, μ)`, `(:circle, cx, cy, r, ρ, μ)`,
+`(:rect, x0, x1, y0, y1, ρ, μ)`, `(:sine_below, y0, amp, k, lx, ρ, μ)`; the first shape that
+contains a point decides, else the background.  The callbacks of StokesFlow2D_multixpu.jl:387-404
+are `Materials([(:y_below, air_height, ρ_air, μ_air), (:circle, plume_x, plume_y, plume_r,
+ρ_plume, μ_plume)], ρ_matrix, μ_matrix)`.
+"""
+struct Materials
+    shapes::Vector{Shape}
+    ρ_bg::Float64
+    μ_bg::Float64
+end
+function Materials(shapes::AbstractVector, ρ_bg, μ_bg)
+    v = Shape[]
+    for sh in shapes
+        kind, vals... = sh
+        p = collect(Float64, vals[1:end-2])
+        append!(p, zeros(5 - length(p)))
+        push!(v, Shape(SHAPE_KINDS[kind], 0, p[1], p[2], p[3], p[4], p[5], vals[end-1], vals[end]))
+    end
+    return Materials(v, ρ_bg, μ_bg)
+end
+
+"""
+    setInitialMarkerProperties!(x_m, y_m, ρ_m, μ_m, x0, y0, materials::Materials)   -- common.jl:104-110
+"""
+function setInitialMarkerProperties!(x_m, y_m, ρ_m, μ_m, x0, y0, materials::Materials)
+    Nm = length(x_m)
+    @assert (length(y_m) == Nm) && (length(ρ_m) == Nm) && (length(μ_m) == Nm)
+    h = context(0, 0)
+    check(ccall((:sf_marker_properties, libsfhip), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Cdouble, Cdouble, Ptr{Shape}, Cint, Cdouble, Cdouble,
+                 Ptr{Float64}, Ptr{Float64}),
+                h, dptr(x_m), dptr(y_m), Nm, x0, y0, materials.shapes, length(materials.shapes), materials.ρ_bg,
+                materials.μ_bg, dptr(ρ_m), dptr(μ_m)))
     return nothing
 end
 

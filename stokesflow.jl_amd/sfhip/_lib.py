@@ -25,7 +25,8 @@ EXPORTS = (
     "sf_pt_iterations", "sf_marker_to_grid", "sf_move_markers_rk4", "sf_step", "sf_init_markers",
     "sf_plume_properties", "sf_rt_properties", "sf_solve_stokes_refined", "sf_solve_stokes_batched",
     "sf_marker_cell_keys", "sf_ensemble_create", "sf_ensemble_step", "sf_ensemble_destroy",
-    "sf_ctx_autotune_pt", "sf_probe_dram", "sf_markers_to_grids3",
+    "sf_ctx_autotune_pt", "sf_probe_dram", "sf_markers_to_grids3", "sf_ctx_time_passes", "sf_ctx_pass_times",
+    "sf_lattice_markers", "sf_marker_properties",
 )
 
 
@@ -101,6 +102,23 @@ class PTTuning(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class PassTime(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("iters_per_launch", C.c_int32), ("launches", C.c_int64),
+                ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Shape(C.Structure):
+    """sf_shape: one shape of a material descriptor."""
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("a", C.c_double), ("b", C.c_double),
+                ("c", C.c_double), ("d", C.c_double), ("e", C.c_double), ("rho", C.c_double), ("mu", C.c_double)]
+
+
+SF_SHAPE_Y_BELOW, SF_SHAPE_Y_ABOVE, SF_SHAPE_CIRCLE, SF_SHAPE_RECT, SF_SHAPE_SINE_BELOW = 1, 2, 3, 4, 5
+SF_MAX_SHAPES = 16
+
 SF_BATCH_AUTO, SF_BATCH_PERSISTENT, SF_BATCH_WIDE = 0, 1, 2
 
 
@@ -140,6 +158,8 @@ def load():
     L.sf_save_grid.argtypes = [_vp, C.c_char_p, _i, _i, _d, _d, _vp, _vp, _vp, _vp]
     L.sf_solve_stokes.argtypes = [_vp] + [_vp] * 19 + [_d, _d, _d, _i, _i, _d, _d, _i, _d, _i64, _i,
                                                         C.POINTER(SolveOut)]
+    L.sf_ctx_time_passes.argtypes = [_vp, _i]
+    L.sf_ctx_pass_times.argtypes = [_vp, C.POINTER(PassTime), _i, C.POINTER(_i)]
     L.sf_solve_stokes_refined.argtypes = [_vp] + [_vp] * 19 + [_d, _d, _d, _i, _i, _d, _d, _i, _d,
                                                                 C.POINTER(RefineParams), C.POINTER(SolveOut),
                                                                 C.POINTER(RefineOut)]
@@ -163,6 +183,8 @@ def load():
     L.sf_init_markers.argtypes = [_vp, _vp, _vp, _i, _i, _i, _d, _d, _d, C.c_uint64]
     L.sf_plume_properties.argtypes = [_vp, _vp, _vp, _i64, C.POINTER(_d), _vp, _vp]
     L.sf_rt_properties.argtypes = [_vp, _vp, _vp, _i64, C.POINTER(_d), _vp, _vp]
+    L.sf_lattice_markers.argtypes = [_vp, _vp, _vp, _i, _i, _i, _d, _d, _d, _d, _i, C.c_uint64]
+    L.sf_marker_properties.argtypes = [_vp, _vp, _vp, _i64, _d, _d, C.POINTER(Shape), _i, _d, _d, _vp, _vp]
     for name in EXPORTS:
         if name not in ("sf_version", "sf_last_error"):
             getattr(L, name).restype = _i

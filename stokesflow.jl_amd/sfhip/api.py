@@ -154,6 +154,24 @@ def set_pt_fusion(nx, ny, iters_per_launch=0, rows=0):
     check(ctx.lib.sf_ctx_set_pt_fusion(ctx.handle, int(iters_per_launch), int(rows)))
 
 
+def time_passes(nx, ny, enable=True):
+    """Per-launch timing of the pass kernels of the nx x ny context (sf_ctx_time_passes): from now
+    on every FP64 multi-iteration pass and every FP32 correction pass is bracketed by two HIP
+    events on the context's stream; the totals restart at zero."""
+    ctx = context(nx, ny)
+    check(ctx.lib.sf_ctx_time_passes(ctx.handle, int(bool(enable))))
+
+
+def pass_times(nx, ny):
+    """Totals of the timed pass launches of the nx x ny context (sf_ctx_pass_times): a list of
+    dicts with precision (0 FP64, 1 FP32), iters_per_launch, launches and seconds."""
+    ctx = context(nx, ny)
+    buf = (_lib.PassTime * 16)()
+    n = C.c_int(0)
+    check(ctx.lib.sf_ctx_pass_times(ctx.handle, buf, 16, C.byref(n)))
+    return [buf[i].as_dict() for i in range(min(n.value, 16))]
+
+
 def set_pt_tuning(nx, ny, rows=0, depth=0, max_waves=-1):
     """Launch shape of the one-iteration PT kernel for the nx x ny context (sf_ctx_set_pt_tuning)."""
     ctx = context(nx, ny)
@@ -201,8 +219,8 @@ def solveStokes(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtau
                 divV=divV, dtauPt=dtauPt, Rx=Rx, Ry=Ry, dVxdtau=dVxdtau, dVydtau=dVydtau, dtauVx=dtauVx,
                 dtauVy=dtauVy, Vx_s=Vx_s, Vy_s=Vy_s)
     ptrs = [_grid_ptr(args[k], sh[k], k) for k in sh]
-    ctx = context(nx, ny)
     out = SolveOut()
+    ctx = context(nx, ny)
     if mixed_precision:
         prm = RefineParams()
         prm.iter_max = int(iterMax)
@@ -404,6 +422,81 @@ def plume_properties(x_m, y_m, air_h, cx, cy, r, rho_air, rho_plume, rho_bg, mu_
     check(ctx.lib.sf_plume_properties(ctx.handle, _marker_ptr(x_m, "x_m"), _marker_ptr(y_m, "y_m"), x_m.numel(),
                                       mat, rho.data_ptr(), mu.data_ptr()))
     return rho, mu
+
+
+def lattice_markers(Nx, Ny, md, lx, ly, dx, dy, rand_pos=False, seed=0, device="cuda"):
+    """The reference's marker lattice on the device (sf_lattice_markers, common.jl:40-87 at one
+    rank): LinRange(dxm/2, lx - dxm/2, md (Nx-1)) x LinRange(dym/2, ly - dym/2, md (Ny-1)), x outer,
+    bit for bit as Julia 1.8 evaluates it; rand_pos adds (u - 0.5) dxm / dym from the build's
+    seeded generator."""
+    n = md * (Nx - 1) * md * (Ny - 1)
+    x = torch.empty(n, dtype=torch.float64, device=device)
+    y = torch.empty(n, dtype=torch.float64, device=device)
+    ctx = context()
+    check(ctx.lib.sf_lattice_markers(ctx.handle, x.data_ptr(), y.data_ptr(), int(Nx), int(Ny), int(md), float(lx),
+                                     float(ly), float(dx), float(dy), int(bool(rand_pos)),
+                                     int(seed) & 0xFFFFFFFFFFFFFFFF))
+    return x, y
+
+
+_SHAPE_KINDS = {"y_below": _lib.SF_SHAPE_Y_BELOW, "y_above": _lib.SF_SHAPE_Y_ABOVE, "circle": _lib.SF_SHAPE_CIRCLE,
+                "rect": _lib.SF_SHAPE_RECT, "sine_below": _lib.SF_SHAPE_SINE_BELOW}
+
+
+class Materials:
+    """Material descriptor in place of the reference's density(x, y) / viscosity(x, y) callbacks:
+    an ordered list of shapes, each with its (rho, mu); the first shape that contains a point
+    decides, else the background.  Shapes: ("y_below", h), ("y_above", h), ("circle", cx, cy, r),
+    ("rect", x0, x1, y0, y1), ("sine_below", y0, amp, k, lx), each followed by rho, mu:
+    ``Materials([("y_below", 2.0, 1e-3, 1e-2), ("circle", 5, 5, 2, 3.2, 0.1)], rho_bg=3.3, mu_bg=1)``
+    is the plume of StokesFlow2D_multixpu.jl:387-404."""
+
+    def __init__(self, shapes, rho_bg, mu_bg):
+        if len(shapes) > _lib.SF_MAX_SHAPES:
+            raise ValueError(f"at most {_lib.SF_MAX_SHAPES} shapes")
+        self.shapes = [tuple(sh) for sh in shapes]
+        self.rho_bg, self.mu_bg = float(rho_bg), float(mu_bg)
+
+    def as_array(self):
+        arr = (_lib.Shape * max(len(self.shapes), 1))()
+        for i, sh in enumerate(self.shapes):
+            kind, *vals = sh
+            params, (rho, mu) = vals[:-2], vals[-2:]
+            if kind not in _SHAPE_KINDS:
+                raise ValueError(f"unknown shape {kind!r}")
+            arr[i].kind = _SHAPE_KINDS[kind]
+            for name, v in zip("abcde", params):
+                setattr(arr[i], name, float(v))
+            arr[i].rho, arr[i].mu = float(rho), float(mu)
+        return arr
+
+
+def marker_properties(x_m, y_m, materials, x0=0.0, y0=0.0):
+    """rho_m, mu_m of every marker from a :class:`Materials` descriptor at global positions
+    (x0 + x_m, y0 + y_m) (sf_marker_properties)."""
+    rho = torch.empty_like(x_m)
+    mu = torch.empty_like(x_m)
+    setInitialMarkerProperties(x_m, y_m, rho, mu, x0, y0, materials)
+    return rho, mu
+
+
+def initializeMarkers(marker_density, lx, ly, dx, dy, Nx, Ny, RAND_MARKER_POS=False, seed=0):
+    """initializeMarkersCPU (common.jl:40-100) at one rank, on the device: the marker lattice and
+    zeroed rho_m, mu_m.  Returns (x_m, y_m, rho_m, mu_m)."""
+    x, y = lattice_markers(Nx, Ny, marker_density, lx, ly, dx, dy, rand_pos=RAND_MARKER_POS, seed=seed)
+    return x, y, torch.zeros_like(x), torch.zeros_like(x)
+
+
+def setInitialMarkerProperties(x_m, y_m, rho_m, mu_m, x0, y0, materials):
+    """setInitialMarkerProperties! (common.jl:104-110) with a :class:`Materials` descriptor in
+    place of the density / viscosity callbacks; rho_m, mu_m are filled in place."""
+    nm = x_m.numel()
+    assert y_m.numel() == nm and rho_m.numel() == nm and mu_m.numel() == nm
+    arr = materials.as_array()
+    ctx = context()
+    check(ctx.lib.sf_marker_properties(ctx.handle, _marker_ptr(x_m, "x_m"), _marker_ptr(y_m, "y_m"), nm, float(x0),
+                                       float(y0), arr, len(materials.shapes), materials.rho_bg, materials.mu_bg,
+                                       _marker_ptr(rho_m, "rho_m"), _marker_ptr(mu_m, "mu_m")))
 
 
 def rt_properties(x_m, y_m, y0, amp, k, lx, rho_up, mu_up, rho_low, mu_low):

@@ -113,3 +113,28 @@ def test_ensemble_refined_matches_single_refined_steps(gpu, use_graph):
         for a, b in zip(ens_m, ref_m):
             same(a, b)
     ens.close()
+
+
+@pytest.mark.parametrize("mode", ["persistent", "wide"])
+def test_ensemble_graph_against_oracle_loop(gpu, mode):
+    """The captured ensemble step against the oracle's own timestep loop (StokesFlow2D_multixpu.jl:
+    155-218 at one rank) per member: 2 FP64 steps, fields and markers bitwise, same iterations and
+    dt."""
+    ms = [member(nx, ny, 90 + k) for k, (nx, ny) in enumerate(SIZES)]
+    host = []
+    for k, (nx, ny) in enumerate(SIZES):
+        dx, dy, x, y, rho, mu = cases.plume_markers(nx, ny, 4, seed=90 + k)
+        host.append((nx, ny, dx, dy, x, y, rho, mu))
+    ens = sfhip.Ensemble(ms, use_graph=True, mode=mode)
+    got = [ens.step(1) for _ in range(2)]
+    torch.cuda.synchronize()
+    ens.close()
+    for k, (nx, ny, dx, dy, x, y, rho, mu) in enumerate(host):
+        S, _, outs = O.stokesflow2d(2, nx, ny, 1.0, 1.0, x, y, rho, mu, g_y=1.0, eps=1e-8)  # dx = 1 / (nx - 1)
+        for t in range(2):
+            assert got[t][k]["niter"] == outs[t]["niter"] and got[t][k]["dt_new"] == outs[t]["dt_new"], (k, t)
+        st = ms[k]["state"]
+        for f, o in (("Vx", "Vx_pad"), ("Vy", "Vy_pad"), ("P", "P"), ("rho_vy", "rho_vy"), ("mu_b", "mu_b"),
+                     ("mu_p", "mu_p"), ("dVxdtau", "dVx"), ("dVydtau", "dVy")):
+            assert np.array_equal(sfhip.to_numpy(getattr(st, f)), S[o]), (k, f)
+        assert np.array_equal(sfhip.to_numpy(ms[k]["x_m"]), x) and np.array_equal(sfhip.to_numpy(ms[k]["y_m"]), y)

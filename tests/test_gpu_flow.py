@@ -13,10 +13,15 @@ import cases
 pytestmark = pytest.mark.gpu
 
 
-def run_both(nt, nx, ny, lx_glob, ly_glob, x, y, rho, mu, g_y, eps):
-    """Run nt steps on the GPU (sf_step) and in the oracle; compare after every step."""
+def run_both(nt, nx, ny, lx_glob, ly_glob, x, y, rho, mu, g_y, eps, device_markers=None):
+    """Run nt steps on the GPU (sf_step) and in the oracle; compare after every step.  The GPU
+    runs on device_markers (x, y, rho, mu already on the device) when given, else on copies of
+    the host arrays."""
     dx, dy = lx_glob / (nx - 1), ly_glob / (ny - 1)
-    xg, yg, rg, mg = (sfhip.from_numpy(a) for a in (x, y, rho, mu))
+    if device_markers is not None:
+        xg, yg, rg, mg = device_markers
+    else:
+        xg, yg, rg, mg = (sfhip.from_numpy(a) for a in (x, y, rho, mu))
     st = sfhip.FlowState(nx, ny)
     S = O.solver_arrays(nx, ny)
     vws, ws = O.zeros(nx + 1, ny), O.zeros(nx + 1, ny)
@@ -58,6 +63,32 @@ def test_flow_golden_pipeline(gpu, golden):
     st, S, worst = run_both(f["nt"], nx, ny, L, f["ly"], x, y, rho, mu, f["g_y"], f["eps"])
     inds = np.asarray(f["inds"]) - 1
     # test/test_StokesFlow2D.jl:107-108: isapprox(..., atol=1e-6) per element
+    np.testing.assert_allclose(sfhip.to_numpy(st.Vx).ravel(order="F")[inds], f["vx_ref"], rtol=0, atol=f["atol"])
+    np.testing.assert_allclose(sfhip.to_numpy(st.Vy).ravel(order="F")[inds], f["vy_ref"], rtol=0, atol=f["atol"])
+
+
+def test_flow_golden_pipeline_device_markers(gpu, golden):
+    """The end-to-end golden run with the markers set up on the device: the reference's lattice
+    (initializeMarkersCPU) and its plume callbacks as a material descriptor
+    (setInitialMarkerProperties!), then 5 timesteps, against the oracle loop bit for bit and the
+    reference's vectors at its own tolerance."""
+    f = golden["stokes_flow_2d"]
+    nx, ny, L = f["nx"], f["ny"], f["lx"]
+    dx, dy = L / (nx - 1), f["ly"] / (ny - 1)
+    lx, ly = (nx - 1) * dx, (ny - 1) * dy
+    xg, yg, rg, mg = sfhip.initializeMarkers(f["marker_density"], lx, ly, dx, dy, nx, ny)
+    mats = sfhip.Materials([("y_below", f["air_height"], f["rho_air"], f["mu_air"]),
+                            ("circle", f["plume_x"], f["plume_y"], f["plume_r"], f["rho_plume"], f["mu_plume"])],
+                           rho_bg=f["rho_matrix"], mu_bg=f["mu_matrix"])
+    sfhip.setInitialMarkerProperties(xg, yg, rg, mg, 0.0, 0.0, mats)
+    x, y = O.lattice_markers(nx, ny, f["marker_density"], lx, ly, dx, dy)
+    rho, mu = O.plume_properties(x, y, f["air_height"], f["plume_x"], f["plume_y"], f["plume_r"], f["rho_air"],
+                                 f["rho_plume"], f["rho_matrix"], f["mu_air"], f["mu_plume"], f["mu_matrix"])
+    for a, b in ((xg, x), (yg, y), (rg, rho), (mg, mu)):
+        assert np.array_equal(sfhip.to_numpy(a), b)
+    st, S, worst = run_both(f["nt"], nx, ny, L, f["ly"], x, y, rho, mu, f["g_y"], f["eps"],
+                            device_markers=(xg, yg, rg, mg))
+    inds = np.asarray(f["inds"]) - 1
     np.testing.assert_allclose(sfhip.to_numpy(st.Vx).ravel(order="F")[inds], f["vx_ref"], rtol=0, atol=f["atol"])
     np.testing.assert_allclose(sfhip.to_numpy(st.Vy).ravel(order="F")[inds], f["vy_ref"], rtol=0, atol=f["atol"])
 

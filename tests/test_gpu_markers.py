@@ -220,3 +220,66 @@ def test_markers_to_grids3_matches_three_calls(gpu):
     assert counts == [tuple(e) for e in exp]
     for name, k in (("rho_vy", "rho_vy"), ("mu_b", "mu_b"), ("mu_p", "mu_p")):
         assert np.array_equal(sfhip.to_numpy(getattr(st, name)), S[k]), name
+
+
+# ------------------------------------------------------------------ marker set-up (common.jl:40-110)
+
+@pytest.mark.parametrize("nx,ny,md", [(20, 26, 5), (64, 48, 4), (33, 17, 3), (7, 9, 1), (3, 2, 2)])
+def test_lattice_bitwise_vs_oracle(gpu, nx, ny, md):
+    """initializeMarkersCPU's LinRange lattice on the device, bit for bit the oracle's (which the
+    reference's end-to-end golden vectors pin)."""
+    lx, ly = 10.0, 7.5
+    dx, dy = lx / (nx - 1), ly / (ny - 1)
+    x, y = sfhip.lattice_markers(nx, ny, md, (nx - 1) * dx, (ny - 1) * dy, dx, dy)
+    xo, yo = O.lattice_markers(nx, ny, md, (nx - 1) * dx, (ny - 1) * dy, dx, dy)
+    assert x.numel() == xo.size == md * md * (nx - 1) * (ny - 1)
+    assert np.array_equal(sfhip.to_numpy(x), xo) and np.array_equal(sfhip.to_numpy(y), yo)
+
+
+def test_lattice_random_positions(gpu):
+    """RAND_MARKER_POS: each marker moves by less than half a marker spacing from its lattice
+    position, stays in the domain, and the seed decides the positions."""
+    nx, ny, md = 40, 30, 4
+    dx, dy = 1.0 / (nx - 1), 0.75 / (ny - 1)
+    lx, ly = (nx - 1) * dx, (ny - 1) * dy
+    x0, y0 = sfhip.lattice_markers(nx, ny, md, lx, ly, dx, dy)
+    x1, y1 = sfhip.lattice_markers(nx, ny, md, lx, ly, dx, dy, rand_pos=True, seed=9)
+    x2, y2 = sfhip.lattice_markers(nx, ny, md, lx, ly, dx, dy, rand_pos=True, seed=9)
+    x3, _ = sfhip.lattice_markers(nx, ny, md, lx, ly, dx, dy, rand_pos=True, seed=10)
+    assert torch.equal(x1, x2) and torch.equal(y1, y2) and not torch.equal(x1, x3)
+    assert ((x1 - x0).abs() <= 0.5 * dx / md).all() and ((y1 - y0).abs() <= 0.5 * dy / md).all()
+    assert (x1 >= 0).all() and (x1 <= lx).all() and (y1 >= 0).all() and (y1 <= ly).all()
+
+
+def test_materials_descriptor(gpu):
+    """The material descriptor gives the plume and Rayleigh-Taylor helpers' values bit for bit, and
+    rectangles and half planes as numpy evaluates them; the first containing shape wins."""
+    nx = ny = 50
+    dx = dy = 1.0 / (nx - 1)
+    x, y = sfhip.init_markers(nx, ny, 4, dx, dy, 0.25, seed=4)
+    rho_p, mu_p = sfhip.plume_properties(x, y, 0.1, 0.5, 0.8, 0.1, 0.001, 0.9, 1.0, 0.01, 0.1, 1.0)
+    mats = sfhip.Materials([("y_below", 0.1, 0.001, 0.01), ("circle", 0.5, 0.8, 0.1, 0.9, 0.1)], rho_bg=1.0, mu_bg=1.0)
+    rho, mu = sfhip.marker_properties(x, y, mats)
+    assert torch.equal(rho, rho_p) and torch.equal(mu, mu_p)
+    rho_r, mu_r = sfhip.rt_properties(x, y, 0.5, 0.02, 3, 1.0, 1.1, 10.0, 1.0, 1.0)
+    rho, mu = sfhip.marker_properties(x, y, sfhip.Materials([("sine_below", 0.5, 0.02, 3, 1.0, 1.1, 10.0)], 1.0, 1.0))
+    assert torch.equal(rho, rho_r) and torch.equal(mu, mu_r)
+    mats = sfhip.Materials([("rect", 0.2, 0.4, 0.3, 0.6, 2.0, 20.0), ("y_above", 0.5, 3.0, 30.0)], rho_bg=1.0,
+                           mu_bg=10.0)
+    rho, mu = sfhip.marker_properties(x, y, mats, x0=0.25, y0=-0.1)
+    xh, yh = sfhip.to_numpy(x) + 0.25, sfhip.to_numpy(y) + (-0.1)
+    inr = (xh >= 0.2) & (xh < 0.4) & (yh >= 0.3) & (yh < 0.6)
+    above = yh >= 0.5
+    exp_rho = np.where(inr, 2.0, np.where(above, 3.0, 1.0))
+    exp_mu = np.where(inr, 20.0, np.where(above, 30.0, 10.0))
+    assert inr.any() and (above & ~inr).any()
+    assert np.array_equal(sfhip.to_numpy(rho), exp_rho) and np.array_equal(sfhip.to_numpy(mu), exp_mu)
+    # an unknown shape kind is rejected by the C ABI before anything runs
+    from sfhip import _lib
+    bad = (_lib.Shape * 1)()
+    bad[0].kind = 99
+    L = sfhip.load()
+    ctx = sfhip.context()
+    rc = L.sf_marker_properties(ctx.handle, x.data_ptr(), y.data_ptr(), x.numel(), 0.0, 0.0, bad, 1, 1.0, 1.0,
+                                rho.data_ptr(), mu.data_ptr())
+    assert rc == _lib.SF_ERR_ARG and b"unknown kind" in L.sf_last_error()

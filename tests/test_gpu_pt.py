@@ -77,8 +77,10 @@ def test_pair_pass_bitwise(gpu, nx, ny, rows, n_iter):
 
 
 # three iterations per pass: 58-column strips; n_iter 7 = 3 + 3 + 1, 5 = 3 + 2, 4 = 3 + 1 launches
+# the tall tiles of the size defaults (24-64 rows): several tiles per strip, a partial last one
 @pytest.mark.parametrize("nx,ny,rows", [(3, 3, 0), (4, 5, 1), (5, 9, 2), (57, 6, 3), (58, 8, 8), (59, 13, 16),
-                                        (60, 4, 4), (117, 21, 5), (118, 40, 16), (175, 9, 0)])
+                                        (60, 4, 4), (117, 21, 5), (118, 40, 16), (175, 9, 0), (175, 150, 24),
+                                        (175, 150, 40), (175, 150, 48), (176, 203, 64)])
 @pytest.mark.parametrize("n_iter", [3, 4, 5, 7])
 def test_triple_pass_bitwise(gpu, nx, ny, rows, n_iter):
     S = cases.random_state(nx, ny, seed=nx * 7001 + ny * 37 + rows)
@@ -312,6 +314,60 @@ def test_refined_pair_sweeps_bitwise(gpu, n, rebase):
         assert s1["n_fp32_iter"] == s2["n_fp32_iter"] and s1["err_fp64"] == s2["err_fp64"]
         for k in a1:
             assert np.array_equal(a1[k], a2[k]), k
+
+
+@pytest.mark.parametrize("rows", [24, 40])
+def test_refined_sweeps_tall_tiles_bitwise(gpu, rows):
+    """Four and five FP32 sweeps per pass with the tile heights of the size defaults (40 rows at
+    2048^2 and 4096^2) on a grid several tiles tall: the same bits as one sweep per launch."""
+    nx, ny = 130, 197
+    dx, dy, x, y, rho, mu = cases.plume_markers(nx, ny, 4, seed=79)
+    S = O.solver_arrays(nx, ny)
+    vws, ws = O.zeros(nx + 1, ny), O.zeros(nx + 1, ny)
+    o = O.grid_origins(dx, dy)
+    O.marker_to_grid(*o["rho"], S["rho_vy"], x, y, rho, dx, dy, vws, ws)
+    O.marker_to_grid(*o["mu_b"], S["mu_b"], x, y, mu, dx, dy, vws, ws)
+    O.marker_to_grid(*o["mu_p"], S["mu_p"], x, y, mu, dx, dy, vws, ws)
+    out = []
+    for fuse in (1, 4, 5):
+        G = to_gpu(S, list(S.keys()))
+        st = {}
+        try:
+            sfhip.set_pt_fusion(nx, ny, fuse, rows if fuse > 1 else 0)
+            sfhip.solveStokes(G["P"], G["Vx_pad"], G["Vy_pad"], G["rho_vy"], G["mu_b"], G["mu_p"], None, None, None,
+                              None, None, None, None, G["dVx"], G["dVy"], None, None, G["Vx_s"], G["Vy_s"], 1.0, dx, dy,
+                              nx, ny, 0.0, 0.5, eps=1e-8, print_info=False, iterMax=2999, stats=st,
+                              mixed_precision=True, refine={"rebase_interval": 333})
+        finally:
+            sfhip.set_pt_fusion(nx, ny, 0, 0)
+        torch.cuda.synchronize()
+        out.append((st, {k: sfhip.to_numpy(G[k]) for k in ("Vx_pad", "Vy_pad", "P", "dVx", "dVy")}))
+    s1, a1 = out[0]
+    for s2, a2 in out[1:]:
+        assert s1["n_fp32_iter"] == s2["n_fp32_iter"] and s1["err_fp64"] == s2["err_fp64"]
+        for k in a1:
+            assert np.array_equal(a1[k], a2[k]), k
+
+
+def test_pass_timer_counts_launches_and_keeps_results(gpu):
+    """sf_ctx_time_passes brackets each pass launch with events; the totals count the launches of
+    each kind, and timing does not change a bit."""
+    nx, ny = 150, 97
+    S = cases.random_state(nx, ny, seed=12)
+    exp = oracle_iterations({k: v.copy(order="F") for k, v in S.items()}, nx, ny, 23, 1.0, 0.01, 0.012, 0.003)
+    try:
+        sfhip.set_pt_fusion(nx, ny, 3, 8)
+        sfhip.time_passes(nx, ny, True)
+        got = run_gpu_iterations(S, nx, ny, 23, 1.0, 0.01, 0.012, 0.003)  # 7 passes of 3, one of 2
+        t = sfhip.pass_times(nx, ny)
+        assert sfhip.pass_times(nx, ny) == t  # reading twice gives the same totals
+    finally:
+        sfhip.time_passes(nx, ny, False)
+        sfhip.set_pt_fusion(nx, ny, 0, 0)
+    assert [(e["precision"], e["iters_per_launch"], e["launches"]) for e in t] == [(0, 2, 1), (0, 3, 7)]
+    assert all(e["seconds"] > 0.0 for e in t)
+    for k in STATE:
+        assert np.array_equal(got[k], exp[k]), k
 
 
 def test_autotune_keeps_results(gpu, golden):
