@@ -165,6 +165,26 @@ int sf_solve_stokes(sf_ctx *ctx, double *P, double *Vx_pad, double *Vy_pad, cons
                     double dy, int nx, int ny, double dt, double maxdisp, int use_fss, double eps,
                     int64_t iter_max, int print_info, sf_solve_out *out);
 
+/* sf_solve_stokes without a context and without allocating: the same solve (size-default launch
+ * shapes, results bit-identical to sf_solve_stokes on a new context), with its scratch -- the PT
+ * ping-pong state, the check-iteration arrays the caller passes as NULL, reduction partials and
+ * device scalars -- carved from one static device array of SF_NOALLOC_SCRATCH_BYTES per GPU,
+ * held until the process ends.  It calls no hipMalloc, hipFree, hipHostMalloc or hipEventCreate;
+ * the timing window (runtime, T_eff) is taken on the host clock.  A grid whose scratch does not
+ * fit returns SF_ERR_SIZE (about 88 B per node with every scratch argument NULL, 40 B with
+ * txx, tyy, txy, divV, Rx, Ry passed: up to about 860^2 or 1290^2 nodes).  Work runs on
+ * hip_stream (NULL: the default stream).  Every call on one GPU shares that GPU's array, whose
+ * contents persist from call to call: calls that overlap on one GPU race.  No reference
+ * counterpart: for callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc). */
+#define SF_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int sf_solve_stokes_noalloc(void *hip_stream, double *P, double *Vx_pad, double *Vy_pad, const double *rho_vy,
+                            const double *mu_b, const double *mu_p, double *txx, double *tyy, double *txy,
+                            double *divV, double *dtauP, double *Rx, double *Ry, double *dVxdtau, double *dVydtau,
+                            double *dtauVx, double *dtauVy, double *Vx_s, double *Vy_s, double g_y, double dx,
+                            double dy, int nx, int ny, double dt, double maxdisp, int use_fss, double eps,
+                            int64_t iter_max, int print_info, sf_solve_out *out);
+
 /* FP32 PT sweeps inside FP64 defect correction.  Same arguments, shapes and outputs as
  * sf_solve_stokes.  The PT operator is affine in (V, P) for a fixed stabilisation dt.
  * Every rebase_interval iterations the FP64 residual of the current state is computed

@@ -169,7 +169,8 @@ struct PTLoop {
     int run_fixed(int64_t n_iter, hipStream_t s);
 };
 
-// Scratch of one FP64 solve (sf_solve_stokes: the context's).
+// Scratch of one FP64 solve: the context's (sf_solve_stokes) or carved from a static device
+// array (sf_solve_stokes_noalloc).
 struct SolveScratch {
     hipStream_t s;
     PTBuffers pt;                              // ping-pong state (dvx_b, dvy_b: NULL when fuse == 1)

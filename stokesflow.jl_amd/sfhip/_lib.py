@@ -26,7 +26,7 @@ EXPORTS = (
     "sf_plume_properties", "sf_rt_properties", "sf_solve_stokes_refined", "sf_solve_stokes_batched",
     "sf_marker_cell_keys", "sf_ensemble_create", "sf_ensemble_step", "sf_ensemble_destroy",
     "sf_ctx_autotune_pt", "sf_probe_dram", "sf_markers_to_grids3", "sf_ctx_time_passes", "sf_ctx_pass_times",
-    "sf_lattice_markers", "sf_marker_properties",
+    "sf_solve_stokes_noalloc", "sf_lattice_markers", "sf_marker_properties",
 )
 
 
@@ -158,6 +158,8 @@ def load():
     L.sf_save_grid.argtypes = [_vp, C.c_char_p, _i, _i, _d, _d, _vp, _vp, _vp, _vp]
     L.sf_solve_stokes.argtypes = [_vp] + [_vp] * 19 + [_d, _d, _d, _i, _i, _d, _d, _i, _d, _i64, _i,
                                                         C.POINTER(SolveOut)]
+    L.sf_solve_stokes_noalloc.argtypes = [_vp] + [_vp] * 19 + [_d, _d, _d, _i, _i, _d, _d, _i, _d, _i64, _i,
+                                                                C.POINTER(SolveOut)]
     L.sf_ctx_time_passes.argtypes = [_vp, _i]
     L.sf_ctx_pass_times.argtypes = [_vp, C.POINTER(PassTime), _i, C.POINTER(_i)]
     L.sf_solve_stokes_refined.argtypes = [_vp] + [_vp] * 19 + [_d, _d, _d, _i, _i, _d, _d, _i, _d,

@@ -194,7 +194,7 @@ def context(nx=0, ny=0):
 def solveStokes(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtauPt, Rx, Ry, dVxdtau, dVydtau,
                 dtauVx, dtauVy, Vx_s, Vy_s, g_y, dx, dy, Nx, Ny, dt, maxdisp, comm=None, *,
                 use_free_surface_stabilization=True, eps=1e-8, print_info=True, iterMax=100000, stats=None,
-                mixed_precision=False, refine=None):
+                mixed_precision=False, refine=None, no_alloc=False):
     """Pseudo-transient Stokes solve (StokesSolver_multixpu.jl:14-96).  Returns (dt, T_eff, runtime).
 
     ``comm`` is accepted for signature compatibility and must be None (one rank).  The
@@ -204,6 +204,10 @@ def solveStokes(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtau
     (``sf_solve_stokes_refined``); ``refine`` may hold ``rebase_interval`` and ``verbose``;
     ``iterMax`` caps the FP32 iterations.  Then T_eff is 0, runtime is the wall time and
     ``stats`` also receives the refinement counters.
+
+    ``no_alloc=True`` runs the FP64 solve through ``sf_solve_stokes_noalloc``: no context and no
+    device allocation (its scratch is a static device array of the library), on torch's current
+    stream; grids whose scratch does not fit raise ``SFError`` (SF_ERR_SIZE).
     """
     if comm is not None:
         raise ValueError("one-rank build: comm must be None")
@@ -220,6 +224,17 @@ def solveStokes(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtau
                 dtauVy=dtauVy, Vx_s=Vx_s, Vy_s=Vy_s)
     ptrs = [_grid_ptr(args[k], sh[k], k) for k in sh]
     out = SolveOut()
+    if no_alloc:
+        if mixed_precision:
+            raise ValueError("no_alloc applies to the FP64 solve only")
+        lib = _lib.load()
+        stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        check(lib.sf_solve_stokes_noalloc(stream, *ptrs, float(g_y), float(dx), float(dy), nx, ny, float(dt),
+                                          float(maxdisp), int(bool(use_free_surface_stabilization)), float(eps),
+                                          int(iterMax), int(print_info), C.byref(out)))
+        if stats is not None:
+            stats.update(out.as_dict())
+        return out.dt_new, out.t_eff, out.runtime
     ctx = context(nx, ny)
     if mixed_precision:
         prm = RefineParams()
