@@ -12,8 +12,9 @@ Printed: one JSON line with PT iterations/s (value), the reference's T_eff, the 
 the dominant pass kernel (its minimum bytes per launch over its own launch time, timed with HIP
 events around each launch in the timed region, against 8 TB/s), the same for the FP32 pass of
 the refined solver (roofline_fp32), and the CPU oracle timed on a bounded sample of iterations
-on the host cores.  Further records ride on the same line, each skipped when the bench's time
-budget (--budget-s) would not hold it:
+on the host cores.  The stages of one timestep at the same size (marker -> grid, a capped solve,
+RK4) are timed too (timestep).  Further records ride on the same line, each skipped when the
+bench's time budget (--budget-s) would not hold it:
 
 * ensemble (--ens, default 256 instances): the mixed-size plume ensemble (64..512 nodes per
   side) solved by the batched solver (FP64 and FP32-refined) and by one solveStokes call per
@@ -185,6 +186,49 @@ def time_to_solution(n, seed=3000, progress=False):
             "refined_rel_l2_vs_fp64": rel}
 
 
+def timestep_stages(n, iters=31, seed=3000):
+    """One StokesFlow2D timestep at n x n nodes (16 markers per cell) through the reference-named
+    calls, each stage timed with HIP events: the three marker -> grid passes, a solve capped at
+    `iters` PT iterations, and RK4 (StokesFlow2D_multixpu.jl:164-181)."""
+    import torch
+
+    import sfhip
+
+    dx = dy = 1.0 / (n - 1)
+    x, y = sfhip.init_markers(n, n, 4, dx, dy, 0.25, seed=seed)
+    rho, mu = sfhip.plume_properties(x, y, 0.1, 0.5, 0.8, 0.1, 0.001, 0.9, 1.0, 0.01, 0.1, 1.0)
+    st = sfhip.FlowState(n, n)
+    o = sfhip.grid_origins(dx, dy)
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+    out = {}
+    for rep in range(2):  # the first step warms up; the second is reported
+        ev[0].record(stream)
+        sfhip.bilinearMarkerToGrid(*o["rho"], st.rho_vy, x, y, rho, dx, dy, print_info=False)
+        ev[1].record(stream)
+        sfhip.bilinearMarkerToGrid(*o["mu_b"], st.mu_b, x, y, mu, dx, dy, print_info=False)
+        ev[2].record(stream)
+        sfhip.bilinearMarkerToGrid(*o["mu_p"], st.mu_p, x, y, mu, dx, dy, print_info=False)
+        ev[3].record(stream)
+        dt, _, _ = sfhip.solveStokes(st.P, st.Vx, st.Vy, st.rho_vy, st.mu_b, st.mu_p, None, None, None, None, None,
+                                     None, None, st.dVxdtau, st.dVydtau, None, None, st.Vx_small, st.Vy_small, 1.0, dx,
+                                     dy, n, n, 0.0, 0.5, eps=1e-300, print_info=False, iterMax=iters)
+        ev[4].record(stream)
+        sfhip.moveMarkersRK4(x, y, st.Vx, st.Vy, *o["vx"], *o["vy"], dt, (n - 1) * dx, (n - 1) * dy, dx, dy,
+                             print_info=False)
+        ev[5].record(stream)
+        torch.cuda.synchronize()
+        ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(5)]
+        nm = x.numel()
+        out = {"n": n, "markers": nm, "solve_iters": iters,
+               "m2g_ms": {"rho_vy": ms[0], "mu_b": ms[1], "mu_p": ms[2]}, "solve_ms": ms[3], "rk4_ms": ms[4],
+               "m2g_markers_per_s": 3 * nm / (1e-3 * (ms[0] + ms[1] + ms[2])), "rk4_markers_per_s": nm / (1e-3 * ms[4]),
+               "rk4_gbs": 32 * nm / (1e-3 * ms[4]) / 1e9}
+    del x, y, rho, mu, st
+    torch.cuda.empty_cache()
+    return out
+
+
 def ensemble_specs(n_inst, seed=4000, sizes=(64, 128, 256, 512), log10c=(1.0, 4.0)):
     """The mixed-size plume ensemble: n uniform over ``sizes``, plume centre uniform over
     [0.2,0.8]x[0.6,0.9], radius uniform over [0.05,0.15], plume viscosity 10^-c with c
@@ -214,7 +258,8 @@ def build_instance(spec):
     return st, dx, x.numel()
 
 
-def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, budget_mb=0.0, refined_loop=False):
+def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, budget_mb=0.0, refined_loop=False,
+                 tune=True):
     """Solve every instance to err <= eps with the batched solver in each of ``modes``
     ("wide", "persistent"), then (``loop``) one solveStokes call at a time, each from the same
     zero state; all runs must agree bit for bit."""
@@ -239,6 +284,22 @@ def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, b
                  "iterMax": iter_max} for st, dx, _ in insts]
     res = {"instances": len(specs), "sizes": sorted({s["n"] for s in specs}), "markers_total": n_markers}
     sols, iters = {}, {}
+    if tune:  # the wide passes' shapes per grid size (sf_batch_autotune), kept for the solves below
+        reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        table = sfhip.autotune_batched(problems, budget_mb=budget_mb)
+        torch.cuda.synchronize()
+        best = {}
+        for t in table:
+            key = f"{t['nx']}"
+            if key not in best or t["ns_per_node_iter"] < best[key]["ns_per_node_iter"]:
+                best[key] = {"iters_per_launch": t["iters_per_launch"], "rows": t["rows"],
+                             "ns_per_node_iter": t["ns_per_node_iter"], "instances": t["instances"]}
+        kept = {f"{t['nx']}": {"iters_per_launch": t["iters_per_launch"], "rows": t["rows"]}
+                for t in table if t["kept"]}
+        res["autotune"] = {"seconds": time.perf_counter() - t0, "candidates": len(table),
+                           "fastest_alone": best, "kept_after_batch_check": kept}
     for mode in modes:
         reset()
         torch.cuda.synchronize()
@@ -297,6 +358,8 @@ def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, b
             for mode in modes:
                 if mode.endswith("_refined"):
                     res[mode]["speedup_vs_refined_loop"] = t_lr / res[mode]["seconds"]
+    if tune:
+        sfhip.clear_batched_tuning()
     names = [m for m in sols if not m.endswith("_refined")]
     res["bitwise_equal"] = all(
         iters[m] == iters[names[0]] and all(torch.equal(a, b) for x, y in zip(sols[m], sols[names[0]])
@@ -416,6 +479,8 @@ def main():
     ap.add_argument("--ens-refined-loop", action="store_true",
                     help="also time the per-solve loop with the refined single solver")
     ap.add_argument("--ens-no-loop", action="store_true", help="time the batched modes only (no per-solve loop)")
+    ap.add_argument("--ens-no-tune", action="store_true",
+                    help="batched modes with the size-default pass shapes (no sf_batch_autotune first)")
     ap.add_argument("--ens-steps", type=int, nargs=3, default=None, metavar=("INSTANCES", "N", "STEPS"),
                     help="Rayleigh-Taylor ensemble timesteps: graph-captured vs eager vs per-instance loop")
     ap.add_argument("--ens-steps-modes", nargs="+", default=None,
@@ -425,6 +490,9 @@ def main():
     ap.add_argument("--refined-iters", type=int, default=4000,
                     help="FP32 correction sweeps of the refined solver timed on the same state (roofline_fp32); 0 "
                          "skips them")
+    ap.add_argument("--timestep", type=int, default=1,
+                    help="1: time the stages of one timestep at the bench size (marker -> grid x3, a 31-iteration "
+                         "solve, RK4); 0 skips it")
     ap.add_argument("--no-ref-window", action="store_true",
                     help="skip the reference-window record (200 iterations at 64^2, 512^2, 4096^2)")
     ap.add_argument("--tune", action="store_true",
@@ -548,6 +616,9 @@ def main():
         line["reference_window"] = reference_window()
     del st
     torch.cuda.empty_cache()
+    if a.timestep:
+        line["timestep"] = timestep_stages(nx)
+    torch.cuda.empty_cache()
 
     # the north-star records, each only if the remaining budget holds its expected time (seconds
     # measured on MI355X: 256-instance ensemble about 90, time to 1e-8 at 4096^2 about 190)
@@ -562,7 +633,8 @@ def main():
 
     if a.ens > 0 and fits("ensemble", 20.0 + 0.3 * a.ens):
         line["ensemble"] = run_ensemble(ensemble_specs(a.ens, log10c=tuple(a.ens_c)), modes=tuple(a.ens_modes),
-                                        loop=not a.ens_no_loop, refined_loop=a.ens_refined_loop)
+                                        loop=not a.ens_no_loop, refined_loop=a.ens_refined_loop,
+                                        tune=not a.ens_no_tune)
         torch.cuda.empty_cache()
     if a.tts_n > 0 and fits("time_to_1e-8", 190.0 * (a.tts_n / 4096.0) ** 3):
         line["time_to_1e-8"] = time_to_solution(a.tts_n, progress=a.progress)

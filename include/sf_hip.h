@@ -258,6 +258,28 @@ typedef struct {
 int sf_solve_stokes_batched(sf_ctx *ctx, const sf_batch_instance *inst, int n, const sf_batch_params *prm,
                             sf_solve_out *outs);
 
+/* Autotuner of the batched solver's wide FP64 passes.  For every grid size in inst it times the
+ * wide two- and three-iteration passes with tile heights of 4..64 rows over an active set of that
+ * size's instances, as large as prm->budget_mb holds, `iters` PT iterations per candidate, and
+ * takes the fastest (iterations per pass, rows) per size.  Each size's choice is then checked on
+ * the whole batch -- every instance capped at 240 PT iterations, with the size defaults and with
+ * the choice added -- and kept on the context only where the capped run is faster (one launch
+ * mixes the sizes' tiles).  Copies of the caller's states are used (allocated and freed here;
+ * the caller's arrays are not modified).  Later sf_solve_stokes_batched calls (wide mode) and
+ * ensembles created from the context use the kept shapes; results do not depend on them.
+ * table (may be NULL, table_cap entries) gets every candidate's time, n_table how many were
+ * timed.  sf_batch_tuning_clear returns to the size defaults.  No reference counterpart. */
+typedef struct {
+    int nx, ny;
+    int iters_per_launch, rows;
+    int instances;           /* active-set size the candidate was timed on */
+    int kept;                /* 1: this size's choice, kept after the check on the whole batch */
+    double ns_per_node_iter; /* time per grid node and PT iteration */
+} sf_batch_tuning;
+int sf_batch_autotune(sf_ctx *ctx, const sf_batch_instance *inst, int n, const sf_batch_params *prm, int iters,
+                      sf_batch_tuning *table, int table_cap, int *n_table);
+int sf_batch_tuning_clear(sf_ctx *ctx);
+
 /* n_iter iterations of the PT loop body (StokesSolver_multixpu.jl:48-58) at a fixed
  * stabilisation dt, with no dt refresh and no convergence check.  Building block for
  * fixed-iteration benchmarks and for bit-exact per-iteration parity. */

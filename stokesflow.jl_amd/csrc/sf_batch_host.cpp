@@ -29,7 +29,16 @@ int batch_check(const sf_batch_instance& a, int i, const char* who)
 }
 
 // The device descriptor of one instance; scratch carved from q (batch_scratch_doubles long).
-BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, const double* dt_dev)
+BatchShape batch_shape(const sf_ctx* c, int nx, int ny)
+{
+    if (c)
+        for (const BatchTune& t : c->btune)
+            if (t.nx == nx && t.ny == ny) return t.sh;
+    return BatchShape();
+}
+
+BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, const double* dt_dev,
+                               const BatchShape& sh)
 {
     BatchInstance b;
     std::memset(&b, 0, sizeof(b));
@@ -101,12 +110,12 @@ BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, 
         b.wpt[par] = w;
         // at least 4 rows: the wide mode fills the chip with many instances, so the short tiles
         // that suit one small grid alone would only add halo rows
-        const int rows2 = std::max(pt2_default_rows(nx, ny), std::min(4, ny));
+        const int rows2 = sh.rows2 > 0 ? std::min(sh.rows2, ny) : std::max(pt2_default_rows(nx, ny), std::min(4, ny));
         w.seg = rows2;
         w.nstrips = (nx - 1 + 59) / 60;
         w.ntiles = w.nstrips * ((ny + rows2 - 1) / rows2);
         b.wpt[2 + par] = w;
-        const int rows3 = std::min(ny, std::max(nx, ny) > 256 ? 16 : 8);
+        const int rows3 = std::min(ny, sh.rows3 > 0 ? sh.rows3 : (std::max(nx, ny) > 256 ? 16 : 8));
         w.seg = rows3;
         w.nstrips = (nx - 1 + 57) / 58;
         w.ntiles = w.nstrips * ((ny + rows3 - 1) / rows3);
@@ -123,6 +132,7 @@ BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, 
     b.nx = nx;
     b.ny = ny;
     b.use_fss = a.use_fss;
+    b.kmax = std::min(std::max(sh.kmax, 1), 3);
     b.dx = a.dx;
     b.dy = a.dy;
     b.dt_in = a.dt;
@@ -258,7 +268,8 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
                 }
                 return true;
             };
-            r.k = fusable(3) ? 3 : (fusable(2) ? 2 : 1);
+            const int kmax = desc[(size_t)r.id].kmax;
+            r.k = (kmax >= 3 && fusable(3)) ? 3 : ((kmax >= 2 && fusable(2)) ? 2 : 1);
             r.checking = check;
             const int par = (int)(r.nlaunch & 1);
             if (r.k == 3) {
@@ -845,7 +856,8 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
         c->b_cap = n;
     }
     std::vector<BatchInstance> host((size_t)n);
-    for (int i = 0; i < n; ++i) host[(size_t)i] = batch_descriptor(in[i], rows, c->b_pool + off[(size_t)i], nullptr);
+    for (int i = 0; i < n; ++i)
+        host[(size_t)i] = batch_descriptor(in[i], rows, c->b_pool + off[(size_t)i], nullptr, batch_shape(c, in[i].nx, in[i].ny));
     std::vector<std::pair<int, int>> dims((size_t)n);
     for (int i = 0; i < n; ++i) dims[(size_t)i] = {in[i].nx, in[i].ny};
     const std::vector<int> order = batch_order(dims);
@@ -919,3 +931,240 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
     return SF_OK;
 }
 
+
+// ---------------------------------------------------------------------------------------
+// autotuner of the wide passes: per grid size, the multi-iteration pass (2 or 3 PT iterations
+// per launch) and its tile rows, timed over an active set of that size's instances (copies of
+// the caller's problems in scratch allocated here), as the wide scheduler launches them
+
+extern "C" int sf_batch_autotune(sf_ctx* c, const sf_batch_instance* in, int n, const sf_batch_params* prm, int iters,
+                                 sf_batch_tuning* table, int table_cap, int* n_table)
+{
+    const char* who = "sf_batch_autotune";
+    if (!c || (n > 0 && !in)) return set_err(SF_ERR_ARG, "%s: NULL argument", who);
+    if (table_cap > 0 && !table) return set_err(SF_ERR_ARG, "%s: NULL table with table_cap > 0", who);
+    if (n_table) *n_table = 0;
+    if (iters < 1) iters = 30;
+    for (int i = 0; i < n; ++i) {
+        const int rc = batch_check(in[i], i, who);
+        if (rc) return rc;
+    }
+    hipStream_t s = c->stream;
+    const double budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 768.0 * 1048576.0;
+    std::vector<std::pair<int, int>> sizes;
+    for (int i = 0; i < n; ++i)
+        if (std::find(sizes.begin(), sizes.end(), std::make_pair(in[i].nx, in[i].ny)) == sizes.end())
+            sizes.push_back({in[i].nx, in[i].ny});
+    int nt = 0;
+    std::vector<BatchTune> found;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double* pool = nullptr;
+    BatchInstance* dev_desc = nullptr;
+    int rc = SF_OK;
+    auto fail = [&](hipError_t e) { return set_err(SF_ERR_HIP, "%s: %s", who, hipGetErrorString(e)); };
+    hipError_t e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    if (e == hipSuccess) e = dalloc(&dev_desc, (size_t)WIDE_MAX);
+    if (e != hipSuccess) rc = fail(e);
+    for (size_t z = 0; z < sizes.size() && rc == SF_OK; ++z) {
+        const int nx = sizes[z].first, ny = sizes[z].second;
+        std::vector<int> members;
+        for (int i = 0; i < n; ++i)
+            if (in[i].nx == nx && in[i].ny == ny) members.push_back(i);
+        const double ws = batch_working_set(nx, ny);
+        // an active set as the scheduler forms it: as many of this size as the budget holds
+        const int R = std::max(1, std::min({(int)members.size(), (int)WIDE_MAX, (int)(budget / ws)}));
+        const size_t gx = nx, gy = ny, np = (gx - 1) * (gy - 1), nvx = gx * (gy + 1), nvy = (gx + 1) * gy;
+        const size_t ndx = (gx - 2) * (gy - 1), ndy = (gx - 1) * (gy - 2);
+        const size_t lens[10] = {np, nvx, nvy, ndx, ndy, (gx + 2) * (gy + 1), (gx + 1) * (gy + 2), nvy, gx * gy, np};
+        size_t per = batch_scratch_doubles(nx, ny);
+        for (size_t l : lens) per += l;
+        if (pool) (void)hipFree(pool);
+        pool = nullptr;
+        e = dalloc(&pool, per * (size_t)R);
+        if (e != hipSuccess) {
+            rc = fail(e);
+            break;
+        }
+        // copies of the caller's instances of this size (state and materials)
+        std::vector<sf_batch_instance> cp((size_t)R);
+        std::vector<double*> scr((size_t)R);
+        for (int j = 0; j < R && e == hipSuccess; ++j) {
+            const sf_batch_instance& src = in[members[(size_t)j % members.size()]];
+            double* q = pool + per * (size_t)j;
+            sf_batch_instance a = src;
+            double** dst[10] = {&a.P, &a.Vx_s, &a.Vy_s, &a.dVxdtau, &a.dVydtau, &a.Vx_pad, &a.Vy_pad, nullptr, nullptr,
+                                nullptr};
+            const double* from[10] = {src.P, src.Vx_s, src.Vy_s, src.dVxdtau, src.dVydtau, src.Vx_pad, src.Vy_pad,
+                                      src.rho_vy, src.mu_b, src.mu_p};
+            double* mat[3] = {nullptr, nullptr, nullptr};
+            for (int t = 0; t < 10 && e == hipSuccess; ++t) {
+                e = hipMemcpyAsync(q, from[t], lens[t] * sizeof(double), hipMemcpyDeviceToDevice, s);
+                if (t < 7) *dst[t] = q;
+                else mat[t - 7] = q;
+                q += lens[t];
+            }
+            a.rho_vy = mat[0];
+            a.mu_b = mat[1];
+            a.mu_p = mat[2];
+            cp[(size_t)j] = a;
+            scr[(size_t)j] = q;
+        }
+        if (e != hipSuccess) {
+            rc = fail(e);
+            break;
+        }
+        BatchTune best{nx, ny, BatchShape()};
+        double best_t[4] = {1e300, 1e300, 1e300, 1e300};  // per k: ns per node and iteration
+        int best_rows[4] = {0, 0, 0, 0};
+        const int rows_c[] = {4, 8, 12, 16, 24, 32, 48, 64};
+        for (int k : {2, 3}) {
+            for (int rows : rows_c) {
+                if (rows > ny && rows != rows_c[0]) continue;
+                BatchShape sh;
+                sh.kmax = k;
+                sh.rows2 = rows;
+                sh.rows3 = rows;
+                std::vector<BatchInstance> desc((size_t)R);
+                for (int j = 0; j < R; ++j) {
+                    cp[(size_t)j].dt = 0.0;
+                    desc[(size_t)j] = batch_descriptor(cp[(size_t)j], 0, scr[(size_t)j], nullptr, sh);
+                }
+                e = hipMemcpyAsync(dev_desc, desc.data(), sizeof(BatchInstance) * (size_t)R, hipMemcpyHostToDevice, s);
+                for (int j = 0; j < R && e == hipSuccess; ++j) e = launch_set_scalar(desc[(size_t)j].scal + SC_DT, 0.0, s);
+                WideArgs w;
+                std::memset(&w, 0, sizeof(w));
+                w.desc = dev_desc;
+                w.n_active = R;
+                int tiles = 0;
+                for (int j = 0; j < R; ++j) {
+                    w.tile_start[j] = tiles;
+                    w.slot_inst[j] = j;
+                    tiles += desc[(size_t)j].wpt[k == 3 ? 4 : 2].ntiles;
+                }
+                w.tile_start[R] = tiles;
+                const int reps = std::max(4, (iters + k - 1) / k) & ~1;  // even: the state ends where it began
+                float ms = 0.f;
+                for (int rep = -2; rep < reps && e == hipSuccess; ++rep) {  // two untimed launches first
+                    if (rep == 0) e = hipEventRecord(ev[0], s);
+                    for (int j = 0; j < R; ++j) w.slot_pf[j] = rep & 1;
+                    if (e == hipSuccess) e = (k == 3) ? launch_wide_pt3(w, s) : launch_wide_pt2(w, s);
+                }
+                if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+                if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+                if (e != hipSuccess) break;
+                const double t_ns = 1e6 * (double)ms / ((double)reps * k * R * (double)nx * ny);
+                if (table && nt < table_cap) table[nt] = sf_batch_tuning{nx, ny, k, rows, R, 0, t_ns};
+                ++nt;
+                if (t_ns < best_t[k]) {
+                    best_t[k] = t_ns;
+                    best_rows[k] = rows;
+                }
+            }
+            if (e != hipSuccess) break;
+        }
+        if (e != hipSuccess) {
+            rc = fail(e);
+            break;
+        }
+        best.sh.kmax = best_t[3] <= best_t[2] ? 3 : 2;
+        best.sh.rows2 = best_rows[2];
+        best.sh.rows3 = best_rows[3];
+        found.push_back(best);
+    }
+    (void)hipStreamSynchronize(s);
+    if (pool) (void)hipFree(pool);
+    pool = nullptr;
+    if (dev_desc) (void)hipFree(dev_desc);
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (rc) return rc;
+
+    // The choices are checked on the whole batch: every instance capped at a few hundred PT
+    // iterations (copies of the caller's states; the materials are only read), once with the size
+    // defaults and once per size with that size's choice added; a choice stays only where the
+    // capped run gets faster, since one size's tiles share each launch with the others'.
+    std::vector<BatchTune> saved = c->btune;
+    for (const BatchTune& t : found)
+        saved.erase(std::remove_if(saved.begin(), saved.end(),
+                                   [&](const BatchTune& u) { return u.nx == t.nx && u.ny == t.ny; }),
+                    saved.end());
+    std::vector<sf_batch_instance> cp(in, in + n);
+    std::vector<size_t> soff((size_t)n);
+    size_t need = 0;
+    for (int i = 0; i < n; ++i) {
+        const size_t gx = in[i].nx, gy = in[i].ny;
+        soff[(size_t)i] = need;
+        need += (gx - 1) * (gy - 1) + gx * (gy + 1) + (gx + 1) * gy + (gx - 2) * (gy - 1) + (gx - 1) * (gy - 2) +
+                (gx + 2) * (gy + 1) + (gx + 1) * (gy + 2);
+    }
+    hipError_t e2 = dalloc(&pool, need);
+    if (e2 != hipSuccess) {
+        c->btune = saved;
+        return set_err(SF_ERR_HIP, "%s: %s", who, hipGetErrorString(e2));
+    }
+    const int64_t cap_iters = 240;
+    auto capped_run = [&](double* secs) -> int {
+        for (int i = 0; i < n && e2 == hipSuccess; ++i) {
+            const size_t gx = in[i].nx, gy = in[i].ny;
+            double* q = pool + soff[(size_t)i];
+            const size_t len[7] = {(gx - 1) * (gy - 1), gx * (gy + 1), (gx + 1) * gy, (gx - 2) * (gy - 1),
+                                   (gx - 1) * (gy - 2), (gx + 2) * (gy + 1), (gx + 1) * (gy + 2)};
+            const double* from[7] = {in[i].P, in[i].Vx_s, in[i].Vy_s, in[i].dVxdtau, in[i].dVydtau, in[i].Vx_pad,
+                                     in[i].Vy_pad};
+            double** to[7] = {&cp[(size_t)i].P, &cp[(size_t)i].Vx_s, &cp[(size_t)i].Vy_s, &cp[(size_t)i].dVxdtau,
+                              &cp[(size_t)i].dVydtau, &cp[(size_t)i].Vx_pad, &cp[(size_t)i].Vy_pad};
+            for (int t = 0; t < 7 && e2 == hipSuccess; ++t) {
+                e2 = hipMemcpyAsync(q, from[t], len[t] * sizeof(double), hipMemcpyDeviceToDevice, s);
+                *to[t] = q;
+                q += len[t];
+            }
+            cp[(size_t)i].iter_max = std::min<int64_t>(in[i].iter_max, cap_iters);
+        }
+        if (e2 == hipSuccess) e2 = hipStreamSynchronize(s);
+        if (e2 != hipSuccess) return set_err(SF_ERR_HIP, "%s: %s", who, hipGetErrorString(e2));
+        sf_batch_params wp = prm ? *prm : sf_batch_params{};
+        wp.mode = SF_BATCH_WIDE;
+        wp.precision = 0;
+        std::vector<sf_solve_out> outs((size_t)n);
+        const auto t0 = std::chrono::steady_clock::now();
+        const int r = sf_solve_stokes_batched(c, cp.data(), n, &wp, outs.data());
+        *secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return r;
+    };
+    std::vector<BatchTune> kept;
+    double t_best = 0.0;
+    c->btune = saved;
+    rc = capped_run(&t_best);
+    for (size_t z = 0; z < found.size() && rc == SF_OK; ++z) {
+        c->btune = saved;
+        c->btune.insert(c->btune.end(), kept.begin(), kept.end());
+        c->btune.push_back(found[z]);
+        double t = 0.0;
+        rc = capped_run(&t);
+        if (rc == SF_OK && t < 0.99 * t_best) {
+            kept.push_back(found[z]);
+            t_best = t;
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(pool);
+    c->btune = saved;
+    if (rc) return rc;
+    c->btune.insert(c->btune.end(), kept.begin(), kept.end());
+    for (int q = 0; q < nt && q < table_cap; ++q)  // mark the candidates the check kept
+        for (const BatchTune& t : kept)
+            if (table[q].nx == t.nx && table[q].ny == t.ny && table[q].iters_per_launch == t.sh.kmax &&
+                table[q].rows == (t.sh.kmax == 3 ? t.sh.rows3 : t.sh.rows2))
+                table[q].kept = 1;
+    if (n_table) *n_table = nt;
+    return SF_OK;
+}
+
+extern "C" int sf_batch_tuning_clear(sf_ctx* c)
+{
+    if (!c) return set_err(SF_ERR_ARG, "sf_batch_tuning_clear: NULL ctx");
+    c->btune.clear();
+    return SF_OK;
+}

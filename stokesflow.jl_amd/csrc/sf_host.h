@@ -73,7 +73,20 @@ struct RefineEnv {
 
 size_t batch_scratch_doubles(int nx, int ny);
 int batch_check(const sf_batch_instance& a, int i, const char* who);
-BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, const double* dt_dev);
+
+// Launch shape of an instance's wide FP64 passes (results do not depend on it): the most PT
+// iterations per pass (2 or 3) and the tile rows of the two- and three-iteration passes
+// (0: size default).  sf_batch_autotune keeps one per grid size on the context.
+struct BatchShape {
+    int kmax = 3, rows2 = 0, rows3 = 0;
+};
+struct BatchTune {
+    int nx, ny;
+    BatchShape sh;
+};
+BatchShape batch_shape(const sf_ctx* c, int nx, int ny);
+BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, const double* dt_dev,
+                               const BatchShape& sh = BatchShape());
 std::vector<int> batch_order(const std::vector<std::pair<int, int>>& dims);
 int batch_grid(sf_ctx* c, int n, int threads, int max_blocks);
 int batch_threads(const sf_batch_params* prm, int* threads, int* rows, const char* who);
@@ -244,4 +257,5 @@ struct sf_ctx {
     int rows64 = 0;
     double *dvx_b = nullptr, *dvy_b = nullptr;  // second dVx/dVy (two-iteration pass)
     sf::PassTimer timer;  // sf_ctx_time_passes
+    std::vector<sf::BatchTune> btune;  // sf_batch_autotune: wide-pass shapes per grid size
 };

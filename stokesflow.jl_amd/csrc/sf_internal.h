@@ -167,7 +167,7 @@ struct BatchInstance {
     double *P, *Vx_s, *Vy_s, *Vx_pad, *Vy_pad;  // caller state (in/out)
     double *ry, *scal;                          // library scratch (scal[SC_DT] = stabilisation dt)
     const double* dt_dev;                       // if set, the input dt is read here (device) instead of dt_in
-    int nx, ny, use_fss, pad0;
+    int nx, ny, use_fss, kmax;  // kmax: most PT iterations per wide pass (2 or 3)
     double dx, dy, dt_in, maxdisp, eps;
     int64_t iter_max;
 };

@@ -56,33 +56,59 @@ __global__ void zero_counters(unsigned long long* c)
     if (threadIdx.x < 4) c[threadIdx.x] = 0ull;
 }
 
+// Runs of equal keys in consecutive lanes (markers in index order share cells: the x-outer /
+// y-inner set-up puts md consecutive markers in one cell): per lane, whether it heads a run, the
+// run length (heads only) and the lane of its run's head.  Every lane of the wave calls this.
+struct KeyRun {
+    bool head;
+    int len, head_lane;
+};
+__device__ inline KeyRun key_run(int k)
+{
+    const int lane = (int)__lane_id();
+    const int kp = dpp_i32<0x138>(k);  // the key of lane - 1 (wave_shr:1; lane 0 reads 0)
+    const bool valid = k >= 0;
+    const bool head = valid && (lane == 0 || kp != k);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long brk = __ballot(head || !valid);  // a run ends where the next one starts
+    KeyRun r;
+    r.head = head;
+    const unsigned long long after = lane == 63 ? 0ull : (brk >> (lane + 1));
+    r.len = head ? (after ? __builtin_ctzll(after) + 1 : 64 - lane) : 0;
+    const unsigned long long upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+    r.head_lane = upto ? 63 - __builtin_clzll(upto) : lane;
+    return r;
+}
+
 __global__ __launch_bounds__(256) void m2g_keys(const double* __restrict__ x_m, const double* __restrict__ y_m, int64_t nm,
                                                 double xmin, double ymin, double dx, double dy, int gnx, int gny,
                                                 int colmajor, int* __restrict__ key, int* __restrict__ count,
                                                 unsigned long long* __restrict__ counters)
 {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= nm) return;
-    const double x = x_m[m], y = y_m[m];
-    if (!key_ok((x - xmin) / dx) || !key_ok((y - ymin) / dy)) {
-        key[m] = -1;
-        atomicAdd(&counters[CNT_NONFINITE], 1ull);
-        return;
+    int k = -1;
+    if (m < nm) {
+        const double x = x_m[m], y = y_m[m];
+        if (!key_ok((x - xmin) / dx) || !key_ok((y - ymin) / dy)) {
+            atomicAdd(&counters[CNT_NONFINITE], 1ull);
+        } else {
+            int64_t ix, iy;
+            double dxij, dyij;
+            topleft(xmin, ymin, x, y, dx, dy, ix, iy, dxij, dyij);
+            if (!(ix >= 0 && ix <= gnx && iy >= 0 && iy <= gny)) {
+                atomicAdd(&counters[CNT_BAD], 1ull);
+            } else {
+                // the sort numbers cells column-major (ix slowest), the order in which the x-outer /
+                // y-inner marker set-up visits them, so the per-cell gathers below read nearly in
+                // order; the exported keys are row-major (sf_marker_cell_keys)
+                k = colmajor ? (int)(ix * (int64_t)(gny + 1) + iy) : (int)(iy * (int64_t)(gnx + 1) + ix);
+            }
+        }
+        key[m] = k;
     }
-    int64_t ix, iy;
-    double dxij, dyij;
-    topleft(xmin, ymin, x, y, dx, dy, ix, iy, dxij, dyij);
-    if (!(ix >= 0 && ix <= gnx && iy >= 0 && iy <= gny)) {
-        key[m] = -1;
-        atomicAdd(&counters[CNT_BAD], 1ull);
-        return;
-    }
-    // the sort numbers cells column-major (ix slowest), the order in which the x-outer /
-    // y-inner marker set-up visits them, so the per-cell gathers below read nearly in order;
-    // the exported keys are row-major (sf_marker_cell_keys)
-    const int k = colmajor ? (int)(ix * (int64_t)(gny + 1) + iy) : (int)(iy * (int64_t)(gnx + 1) + ix);
-    key[m] = k;
-    atomicAdd(&count[k], 1);
+    // one counter atomic per run of equal keys instead of one per marker
+    const KeyRun run = key_run(k);
+    if (run.head) atomicAdd(&count[k], run.len);
 }
 
 __global__ __launch_bounds__(SCAN_BLOCK) void scan_block_sums(const int* __restrict__ cnt, int64_t n,
@@ -150,19 +176,19 @@ __global__ __launch_bounds__(256) void m2g_scatter(const int* __restrict__ key, 
                                                    int* __restrict__ fill, int* __restrict__ rec_idx)
 {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= nm) return;
-    const int k = key[m];
+    const int k = m < nm ? key[m] : -1;
+    // a run of equal keys takes its slots with one atomic; its markers fill them in index order
+    const KeyRun run = key_run(k);
+    int base = 0;
+    if (run.head) base = atomicAdd(&fill[k], run.len);
+    base = __shfl(base, run.head_lane);
     if (k < 0) return;
-    const int slot = off[k] + atomicAdd(&fill[k], 1);
-    rec_idx[slot] = (int)m;
+    rec_idx[off[k] + base + ((int)__lane_id() - run.head_lane)] = (int)m;
 }
 
-// Sort each cell segment by marker index (insertion sort: segments hold ~md^2 markers),
-// then store the per-slot records used by the node merge.
-__global__ __launch_bounds__(256) void m2g_sort_cells(const int* __restrict__ off, int64_t ncells, int* __restrict__ rec_idx,
-                                                      const double* __restrict__ x_m, const double* __restrict__ y_m,
-                                                      const double* __restrict__ val_m, double xmin, double ymin,
-                                                      double dx, double dy, M2GRec* __restrict__ rec)
+// Sort each cell segment by marker index (insertion sort: segments hold ~md^2 markers in a few
+// index-ordered runs).
+__global__ __launch_bounds__(256) void m2g_sort_cells(const int* __restrict__ off, int64_t ncells, int* __restrict__ rec_idx)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncells) return;
@@ -176,19 +202,27 @@ __global__ __launch_bounds__(256) void m2g_sort_cells(const int* __restrict__ of
         }
         rec_idx[q + 1] = v;
     }
-    for (int s = b; s < e; ++s) {
-        const int m = rec_idx[s];
-        int64_t ix, iy;
-        double dxij, dyij;
-        topleft(xmin, ymin, x_m[m], y_m[m], dx, dy, ix, iy, dxij, dyij);
-        M2GRec r;
-        r.dx = dxij;
-        r.dy = dyij;
-        r.val = val_m[m];
-        r.idx = m;
-        r.pad = 0;
-        rec[s] = r;
-    }
+}
+
+// The per-slot records used by the node merge, one thread per slot (coalesced in slot order).
+__global__ __launch_bounds__(256) void m2g_records(const int* __restrict__ rec_idx, const int* __restrict__ nrec,
+                                                   const double* __restrict__ x_m, const double* __restrict__ y_m,
+                                                   const double* __restrict__ val_m, double xmin, double ymin,
+                                                   double dx, double dy, M2GRec* __restrict__ rec)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= *nrec) return;  // the valid markers' slots (the scan's total)
+    const int m = rec_idx[s];
+    int64_t ix, iy;
+    double dxij, dyij;
+    topleft(xmin, ymin, x_m[m], y_m[m], dx, dy, ix, iy, dxij, dyij);
+    M2GRec r;
+    r.dx = dxij;
+    r.dy = dyij;
+    r.val = val_m[m];
+    r.idx = m;
+    r.pad = 0;
+    rec[s] = r;
 }
 
 
@@ -287,8 +321,10 @@ hipError_t m2g_run(M2GScratch& S, double xmin, double ymin, double* val_grid, in
     if (nm > 0)
         hipLaunchKernelGGL(m2g_scatter, dim3(grid1d(nm, 256)), dim3(256), 0, st, S.key, nm, S.offset, S.fill,
                            S.rec_idx);
-    hipLaunchKernelGGL(m2g_sort_cells, dim3(grid1d(ncells, 256)), dim3(256), 0, st, S.offset, ncells, S.rec_idx, x_m,
-                       y_m, val_m, xmin, ymin, dx, dy, S.rec);
+    hipLaunchKernelGGL(m2g_sort_cells, dim3(grid1d(ncells, 256)), dim3(256), 0, st, S.offset, ncells, S.rec_idx);
+    if (nm > 0)  // the valid markers fill slots 0 .. offset[ncells] - 1 (at most nm)
+        hipLaunchKernelGGL(m2g_records, dim3(grid1d(nm, 256)), dim3(256), 0, st, S.rec_idx, S.offset + ncells, x_m,
+                           y_m, val_m, xmin, ymin, dx, dy, S.rec);
     hipLaunchKernelGGL(m2g_nodes, dim3(grid1d(gny, 256), gnx), dim3(256), 0, st, S.offset, S.rec, gnx, gny,
                        val_grid, val_wt_sum, wt_sum, ws_nx, S.counters);
     return hipGetLastError();

@@ -941,12 +941,11 @@ hipError_t launch_pt_corr2(const CorrParams& p0, hipStream_t s)
 }
 
 // Tile rows of the four- and five-sweep passes: the launch runs in rounds of resident waves (two
-// per SIMD at 193 / 225 VGPRs), and a last round that is only partly filled costs a whole one, so
-// the height is chosen to minimise rounds x (rows + 2K-1 halo rows) over 24-48 rows.  Measured,
-// us per sweep, K=4: 4096^2 40 rows 79.5, 28 81.0, 32 81.6, 64 83.5, 48 85.7 (the model picks
-// 40); 3072^2 28 rows 44.9, 24 47.8, 40 53.2 (the model ranks these in order and picks 44, a
-// two-round launch); 2048^2 40 rows 18.7, 28 22.1, 24 22.9 (picks 40).  K=5: 4096^2 40 rows
-// 74.6, 32 77.0, 48 77.0; 2048^2 40 rows 18.7, 48 21.7, 32 23.6 (picks 40 at both).
+// per SIMD at 217 / 253 VGPRs), and a last round that is only partly filled costs a whole one, so
+// the height is chosen to minimise rounds x (rows + 2K-1 halo rows) over 24-96 rows.  Measured,
+// us per sweep: K=5 4096^2 80 rows 59.8-60.2 (the pick, a two-round launch), 40 62.3-63.2, 32 69.2,
+// 48 70.1, 56 66.3, 64 72.2, 96 70.1, 128 87.3; K=4 3072^2 84 rows 39.0 (the pick, one round), 28
+// 40.0, 44 40.4; 2560^2 60 rows 28.3 (the pick), 40 30.7; 2048^2 40 rows 15.1 (the pick), 80 21.4.
 static int corrk_rows(int nx, int ny, int k)
 {
     static const int slots = [] {
@@ -960,7 +959,7 @@ static int corrk_rows(int nx, int ny, int k)
     const int64_t strips = (nx - 1 + out - 1) / out;
     int best = 40;
     double best_cost = 1e300;
-    for (int rows = 24; rows <= 48; rows += 4) {
+    for (int rows = 24; rows <= 96; rows += 4) {
         const int64_t waves = strips * ((ny + rows - 1) / rows);
         const double cost = (double)((waves + slots - 1) / slots) * (rows + 2 * k - 1);
         if (cost < best_cost) {

@@ -26,7 +26,8 @@ EXPORTS = (
     "sf_plume_properties", "sf_rt_properties", "sf_solve_stokes_refined", "sf_solve_stokes_batched",
     "sf_marker_cell_keys", "sf_ensemble_create", "sf_ensemble_step", "sf_ensemble_destroy",
     "sf_ctx_autotune_pt", "sf_probe_dram", "sf_markers_to_grids3", "sf_ctx_time_passes", "sf_ctx_pass_times",
-    "sf_solve_stokes_noalloc", "sf_lattice_markers", "sf_marker_properties",
+    "sf_solve_stokes_noalloc", "sf_lattice_markers", "sf_marker_properties", "sf_batch_autotune",
+    "sf_batch_tuning_clear",
 )
 
 
@@ -110,6 +111,14 @@ class PassTime(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("nx", _i), ("ny", _i), ("iters_per_launch", _i), ("rows", _i), ("instances", _i), ("kept", _i),
+                ("ns_per_node_iter", _d)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class Shape(C.Structure):
     """sf_shape: one shape of a material descriptor."""
     _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("a", C.c_double), ("b", C.c_double),
@@ -161,6 +170,9 @@ def load():
     L.sf_solve_stokes_noalloc.argtypes = [_vp] + [_vp] * 19 + [_d, _d, _d, _i, _i, _d, _d, _i, _d, _i64, _i,
                                                                 C.POINTER(SolveOut)]
     L.sf_ctx_time_passes.argtypes = [_vp, _i]
+    L.sf_batch_autotune.argtypes = [_vp, C.POINTER(BatchInstance), _i, C.POINTER(BatchParams), _i,
+                                    C.POINTER(BatchTuning), _i, C.POINTER(_i)]
+    L.sf_batch_tuning_clear.argtypes = [_vp]
     L.sf_ctx_pass_times.argtypes = [_vp, C.POINTER(PassTime), _i, C.POINTER(_i)]
     L.sf_solve_stokes_refined.argtypes = [_vp] + [_vp] * 19 + [_d, _d, _d, _i, _i, _d, _d, _i, _d,
                                                                 C.POINTER(RefineParams), C.POINTER(SolveOut),

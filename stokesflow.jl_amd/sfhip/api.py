@@ -283,6 +283,40 @@ def solveStokes_batched(problems, *, mode="auto", rows=0, depth=0, threads=0, ma
     correction), one to three FP32 sweeps per launch of all active instances.
     """
     n = len(problems)
+    arr = _batch_instances(problems)
+    modes = {"auto": _lib.SF_BATCH_AUTO, "persistent": _lib.SF_BATCH_PERSISTENT, "wide": _lib.SF_BATCH_WIDE}
+    prm = BatchParams(int(rows), int(depth), int(threads), int(max_blocks), modes[mode], int(bool(mixed_precision)),
+                      float(budget_mb))
+    outs = (SolveOut * max(n, 1))()
+    ctx = context(0, 0)
+    check(ctx.lib.sf_solve_stokes_batched(ctx.handle, arr, n, C.byref(prm), outs))
+    keys = ("dt_new", "err", "niter", "converged", "dt_last", "runtime")
+    return [{k: getattr(outs[i], k) for k in keys} for i in range(n)]
+
+
+def autotune_batched(problems, *, budget_mb=0.0, iters=30):
+    """Autotuner of the batched solver's wide passes (sf_batch_autotune): per grid size among
+    ``problems`` (as for :func:`solveStokes_batched`; their arrays are copied, not modified), the
+    fastest iterations per pass and tile rows, kept for later batched solves and ensembles.
+    Returns the table of every timed candidate."""
+    n = len(problems)
+    arr = _batch_instances(problems)
+    prm = BatchParams(0, 0, 0, 0, _lib.SF_BATCH_WIDE, 0, float(budget_mb))
+    table = (_lib.BatchTuning * 512)()
+    nt = C.c_int(0)
+    ctx = context(0, 0)
+    check(ctx.lib.sf_batch_autotune(ctx.handle, arr, n, C.byref(prm), int(iters), table, 512, C.byref(nt)))
+    return [table[i].as_dict() for i in range(min(nt.value, 512))]
+
+
+def clear_batched_tuning():
+    """Back to the size-default shapes of the wide passes (sf_batch_tuning_clear)."""
+    ctx = context(0, 0)
+    check(ctx.lib.sf_batch_tuning_clear(ctx.handle))
+
+
+def _batch_instances(problems):
+    n = len(problems)
     arr = (BatchInstance * max(n, 1))()
     for i, pb in enumerate(problems):
         nx, ny = int(pb["Nx"]), int(pb["Ny"])
@@ -298,14 +332,7 @@ def solveStokes_batched(problems, *, mode="auto", rows=0, depth=0, threads=0, ma
         b.eps = float(pb.get("eps", 1e-8))
         b.iter_max = int(pb.get("iterMax", 100000))
         b.use_fss = int(bool(pb.get("use_free_surface_stabilization", True)))
-    modes = {"auto": _lib.SF_BATCH_AUTO, "persistent": _lib.SF_BATCH_PERSISTENT, "wide": _lib.SF_BATCH_WIDE}
-    prm = BatchParams(int(rows), int(depth), int(threads), int(max_blocks), modes[mode], int(bool(mixed_precision)),
-                      float(budget_mb))
-    outs = (SolveOut * max(n, 1))()
-    ctx = context(0, 0)
-    check(ctx.lib.sf_solve_stokes_batched(ctx.handle, arr, n, C.byref(prm), outs))
-    keys = ("dt_new", "err", "niter", "converged", "dt_last", "runtime")
-    return [{k: getattr(outs[i], k) for k in keys} for i in range(n)]
+    return arr
 
 
 def pt_iterations(n_iter, P, Vx_s, Vy_s, dVxdtau, dVydtau, rho_vy, mu_b, mu_p, g_y, dx, dy, Nx, Ny, dt):

@@ -235,3 +235,35 @@ def test_wide_refined_refills_beyond_active_set_limit(gpu):
         assert o["niter"] == st["niter"] and o["dt_new"] == st["dt_new"], (nx, ny)
         for k in OUT:
             assert np.array_equal(sfhip.to_numpy(G[k]), exp[k]), (nx, ny, k)
+
+
+def test_batch_autotune_keeps_results(gpu):
+    """sf_batch_autotune times the wide passes per grid size (two and three iterations per pass,
+    several tile heights) and keeps the fastest; a batched solve afterwards is still bitwise equal
+    to one solve per instance, and the caller's arrays are untouched by the tuning."""
+    sizes = [(24, 24), (33, 47), (24, 24), (64, 64)]
+    insts = [(nx, ny, *plume_instance(nx, ny, seed=500 + k)) for k, (nx, ny) in enumerate(sizes)]
+    problems, gs = [], []
+    for nx, ny, S, dx, dy in insts:
+        pb, G = problem(S, nx, ny, dx, dy)
+        problems.append(pb)
+        gs.append(G)
+    before = [{k: v.clone() for k, v in G.items()} for G in gs]
+    try:
+        table = sfhip.autotune_batched(problems, iters=12)
+        torch.cuda.synchronize()
+        for G, B in zip(gs, before):
+            assert all(torch.equal(G[k], B[k]) for k in G)
+        assert {(t["nx"], t["ny"]) for t in table} == {(24, 24), (33, 47), (64, 64)}
+        assert {t["iters_per_launch"] for t in table} == {2, 3}
+        assert all(t["ns_per_node_iter"] > 0 for t in table)
+        assert [t["instances"] for t in table if (t["nx"], t["ny"]) == (24, 24)][0] == 2
+        outs = sfhip.solveStokes_batched(problems, mode="wide")
+        torch.cuda.synchronize()
+    finally:
+        sfhip.clear_batched_tuning()
+    for (nx, ny, S, dx, dy), o, G in zip(insts, outs, gs):
+        st, exp = single(S, nx, ny, dx, dy)
+        assert o["niter"] == st["niter"] and o["dt_new"] == st["dt_new"], (nx, ny)
+        for k in OUT:
+            assert np.array_equal(sfhip.to_numpy(G[k]), exp[k]), (nx, ny, k)
