@@ -34,6 +34,12 @@ for _p in (ROOT, os.path.join(ROOT, "stokesflow.jl_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+_T0 = time.perf_counter()
+
+
+def note(msg):
+    """One line on stderr per finished stage, so that a long run shows it is alive."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 REF_PT_ITERS_PER_S = 108.5  # reference at 4096^2 on 1 GPU: 190 timed iterations in 1.7516 s
 
 
@@ -176,6 +182,7 @@ def time_to_solution(n, seed=3000, progress=False):
         torch.cuda.synchronize()
         res[mode] = {"seconds": time.perf_counter() - t0, "converged": bool(stats["converged"]),
                      "err": stats["err"], "pt_iters": stats["niter"]}
+        note(f"time to 1e-8 at {n}^2, {mode}: {res[mode]['seconds']:.1f} s")
         if mode != "fp64":
             res[mode]["rebases"] = stats["n_outer"]
         fields[mode] = (st.Vx.clone(), st.Vy.clone(), st.P.clone())
@@ -298,6 +305,7 @@ def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, b
                              "ns_per_node_iter": t["ns_per_node_iter"], "instances": t["instances"]}
         kept = {f"{t['nx']}": {"iters_per_launch": t["iters_per_launch"], "rows": t["rows"]}
                 for t in table if t["kept"]}
+        note("ensemble autotune done")
         res["autotune"] = {"seconds": time.perf_counter() - t0, "candidates": len(table),
                            "fastest_alone": best, "kept_after_batch_check": kept}
     for mode in modes:
@@ -314,6 +322,7 @@ def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, b
         res[mode] = {"seconds": t, "solves_per_s": len(specs) / t, "converged_solves_per_s": conv / t,
                      "alg_gbs": b_alg / t / 1e9, "converged": conv, "pt_iters": sum(o["niter"] for o in outs)}
         sols[mode], iters[mode] = snapshot(), [o["niter"] for o in outs]
+        note(f"ensemble {mode}: {t:.1f} s")
         if mode.endswith("_refined"):
             continue
         by_size = {}
@@ -336,6 +345,8 @@ def run_ensemble(specs, modes=("wide",), loop=True, iter_max=100000, eps=1e-8, b
                               st.nx, st.ny, 0.0, 0.5, eps=eps, print_info=False, iterMax=iter_max, stats=stats)
             lit.append(stats["niter"])
             lconv += int(stats["converged"])
+            if len(lit) % 64 == 0:
+                note(f"ensemble per-solve loop: {len(lit)} of {len(insts)} solves")
         torch.cuda.synchronize()
         t_loop = time.perf_counter() - t0
         res["per_solve_loop"] = {"seconds": t_loop, "solves_per_s": len(specs) / t_loop,
@@ -437,8 +448,12 @@ def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 1.0), eps=1e-5,
             torch.cuda.synchronize()
             secs = time.perf_counter() - t0
             ens.close()
-        res[name] = {"seconds": secs, "steps_per_s": steps / secs, "pt_iters": iters}
-        finals[name] = [(m["state"].Vx.clone(), m["x_m"].clone()) for m in ms]
+        res[name] = {"seconds": secs, "steps_per_s": steps / secs, "pt_iters": iters,
+                     "markers_per_s": n_inst * res["markers_per_instance"] * steps / secs}
+        if mode is not None:
+            res[name]["converged_last_step"] = sum(int(o["converged"]) for o in outs)
+        finals[name] = [(m["state"].Vx.clone(), m["x_m"].clone(), m["state"].Vy.clone(), m["state"].P.clone())
+                        for m in ms]
         del ms
         torch.cuda.empty_cache()
     if "loop" in res:
@@ -451,9 +466,12 @@ def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 1.0), eps=1e-5,
         res["bitwise_equal"] = all(torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
                                    for name in fp64 for a, b in zip(finals[base], finals[name]))
     if "wide_refined_graph" in finals and base:
-        res["wide_refined_graph"]["max_rel_l2_vs_fp64"] = max(
-            ((a[0] - b[0]).norm() / b[0].norm().clamp_min(1e-300)).item()
-            for a, b in zip(finals["wide_refined_graph"], finals[base]))
+        def rel(a, b):
+            return ((a - b).norm() / b.norm().clamp_min(1e-300)).item()
+        per = [{"Vx": rel(a[0], b[0]), "Vy": rel(a[2], b[2]), "P": rel(a[3], b[3])}
+               for a, b in zip(finals["wide_refined_graph"], finals[base])]
+        res["wide_refined_graph"]["max_rel_l2_vs_fp64"] = max(p["Vx"] for p in per)
+        res["wide_refined_graph"]["rel_l2_vs_fp64_per_instance"] = per
     return res
 
 
@@ -479,10 +497,14 @@ def main():
     ap.add_argument("--ens-refined-loop", action="store_true",
                     help="also time the per-solve loop with the refined single solver")
     ap.add_argument("--ens-no-loop", action="store_true", help="time the batched modes only (no per-solve loop)")
+    ap.add_argument("--ens-iter-max", type=int, default=100000,
+                    help="iterMax of every ensemble solve (the reference's default; lower only to profile)")
     ap.add_argument("--ens-no-tune", action="store_true",
                     help="batched modes with the size-default pass shapes (no sf_batch_autotune first)")
     ap.add_argument("--ens-steps", type=int, nargs=3, default=None, metavar=("INSTANCES", "N", "STEPS"),
                     help="Rayleigh-Taylor ensemble timesteps: graph-captured vs eager vs per-instance loop")
+    ap.add_argument("--ens-steps-eps", type=float, default=1e-5,
+                    help="solver tolerance of the --ens-steps timesteps (the reference's timestep loop uses 1e-5)")
     ap.add_argument("--ens-steps-modes", nargs="+", default=None,
                     choices=["wide_graph", "wide_eager", "wide_refined_graph", "persistent_graph", "persistent_eager",
                              "loop"], help="run only these forms of the --ens-steps comparison")
@@ -533,6 +555,7 @@ def main():
     n_markers = x_m.numel()
     del x_m, y_m, rho_m, mu_m
     torch.cuda.empty_cache()
+    note(f"set-up: {n_markers} markers, material grids")
 
     tuned = sfhip.autotune_pt(nx, ny, iters=30, state=st, g_y=1.0, dx=dx, dy=dy)[0] if a.tune else None
 
@@ -558,6 +581,7 @@ def main():
     gpu_s = ev0.elapsed_time(ev1) * 1e-3
     pt64 = pass_timing(nx, ny, 0)
     sfhip.time_passes(nx, ny, False)
+    note(f"timed region: {a.steps} PT iterations in {wall:.3f} s")
     finite = bool(torch.isfinite(st.Vx_small).all().item())
 
     probe = sfhip.probe_dram(1 << 30)  # the box's DRAM ceiling for simple streams
@@ -612,12 +636,15 @@ def main():
             line["roofline_fp32"] = roofline_entry(kname, k32, c32["launch_s"], c32["launches"], b32, b32,
                                                    profiled_traffic(nx, f"pt_corrk{k32}_f32"), probe)
             line["refined_sweeps"]["kernel_s_per_sweep"] = c32["launch_s"] / k32
+    note("cpu baseline and refined passes done")
     if not a.no_ref_window:
         line["reference_window"] = reference_window()
+        note("reference window done")
     del st
     torch.cuda.empty_cache()
     if a.timestep:
         line["timestep"] = timestep_stages(nx)
+        note("timestep stages done")
     torch.cuda.empty_cache()
 
     # the north-star records, each only if the remaining budget holds its expected time (seconds
@@ -634,13 +661,14 @@ def main():
     if a.ens > 0 and fits("ensemble", 20.0 + 0.3 * a.ens):
         line["ensemble"] = run_ensemble(ensemble_specs(a.ens, log10c=tuple(a.ens_c)), modes=tuple(a.ens_modes),
                                         loop=not a.ens_no_loop, refined_loop=a.ens_refined_loop,
-                                        tune=not a.ens_no_tune)
+                                        tune=not a.ens_no_tune, iter_max=a.ens_iter_max)
         torch.cuda.empty_cache()
     if a.tts_n > 0 and fits("time_to_1e-8", 190.0 * (a.tts_n / 4096.0) ** 3):
         line["time_to_1e-8"] = time_to_solution(a.tts_n, progress=a.progress)
         torch.cuda.empty_cache()
     if a.ens_steps and fits("ensemble_steps", 0.0):
-        line["ensemble_steps"] = run_ensemble_steps(*a.ens_steps, modes=a.ens_steps_modes, progress=a.progress)
+        line["ensemble_steps"] = run_ensemble_steps(*a.ens_steps, modes=a.ens_steps_modes, progress=a.progress,
+                                                    eps=a.ens_steps_eps)
     line["skipped"] = skipped
     line["bench_wall_s"] = time.perf_counter() - t_start
     print(json.dumps(line), flush=True)

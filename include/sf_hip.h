@@ -251,7 +251,7 @@ typedef struct {
     int mode;         /* SF_BATCH_*; AUTO = WIDE */
     int precision;    /* wide: 0 FP64 PT iterations; 1 FP32 PT sweeps inside FP64 defect correction (the
                          algorithm of sf_solve_stokes_refined, FP64 convergence test) */
-    double budget_mb; /* wide: working-set budget of the active set in MiB (<= 0: 768; 512 with
+    double budget_mb; /* wide: working-set budget of the active set in MiB (<= 0: 1536; 1024 with
                          precision 1) */
 } sf_batch_params;
 

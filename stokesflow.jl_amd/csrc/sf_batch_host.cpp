@@ -877,9 +877,9 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
     if (precision == 1 && mode == SF_BATCH_PERSISTENT)
         return set_err(SF_ERR_ARG, "%s: FP32 refinement runs in the wide mode only", who);
     if (precision == 1) {
-        // the refined working sets are FP32; a larger active set measured 3-4% faster (64 mixed
-        // instances: 2.92 s at 192 MiB, 2.84 s at 384 MiB, 2.81 s at 768 MiB)
-        const double budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 512.0 * 1048576.0;
+        // the refined working sets are FP32; 256 mixed 64..512 instances: 11.10 s at 256 MiB, 11.07 s at
+        // 512, 10.44-10.47 s at 1024 and 2048 MiB (one box, one process)
+        const double budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 1024.0 * 1048576.0;
         size_t nf = 0;
         std::vector<size_t> foff((size_t)n);
         for (int i = 0; i < n; ++i) {
@@ -906,10 +906,11 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
         HIPCHK(hipMemcpyAsync(res.data(), c->b_res, sizeof(BatchResult) * (size_t)n, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     } else if (mode == SF_BATCH_WIDE || mode == SF_BATCH_AUTO) {
-        // a working set past the Infinity Cache still pays with multi-iteration passes (mixed 64-512
-        // instances: 3.97 s at 192 MiB, 3.55 s at 384/768 MiB for 64, 17.3 s at 384 and 16.7 s at 768
-        // MiB for 256; eight 1024^2 instances: 8.60 s at 192, 7.40 s at 768 MiB, all eight active)
-        const double budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 768.0 * 1048576.0;
+        // a working set past the Infinity Cache still pays with multi-iteration passes: 256 mixed
+        // 64..512 instances, one box, one process: 17.69 s at 256 MiB, 16.93 at 384, 16.24 at 768, 15.35-
+        // 15.38 at 1536 and 3072 MiB (64 slots full of 512^2 instances: 18,432 waves, nine full rounds of
+        // the 2,048 resident ones); 128 slots were slower (15.5 s)
+        const double budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 1536.0 * 1048576.0;
         const WideEnv env{s, c->b_inst, c->part, c->b_hscal};
         rc = batch_run_wide(env, in, n, host, order, budget, (prm && prm->depth > 1) ? 2 : 1, res);
         if (rc) return rc;
@@ -950,7 +951,7 @@ extern "C" int sf_batch_autotune(sf_ctx* c, const sf_batch_instance* in, int n, 
         if (rc) return rc;
     }
     hipStream_t s = c->stream;
-    const double budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 768.0 * 1048576.0;
+    const double budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0 : 1536.0 * 1048576.0;
     std::vector<std::pair<int, int>> sizes;
     for (int i = 0; i < n; ++i)
         if (std::find(sizes.begin(), sizes.end(), std::make_pair(in[i].nx, in[i].ny)) == sizes.end())

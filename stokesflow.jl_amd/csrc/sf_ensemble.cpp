@@ -199,7 +199,7 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
     }
     e->depth = (prm && prm->depth > 1) ? 2 : 1;
     e->budget = (prm && prm->budget_mb > 0.0) ? prm->budget_mb * 1048576.0
-                                              : (e->refined ? 512.0 : 768.0) * 1048576.0;
+                                              : (e->refined ? 1024.0 : 1536.0) * 1048576.0;
     e->mem.assign(members, members + n);
     e->bi = bi;
     hipError_t er = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);

@@ -276,7 +276,7 @@ def solveStokes_batched(problems, *, mode="auto", rows=0, depth=0, threads=0, ma
 
     ``mode`` "persistent": one launch, one workgroup per instance from a device work queue;
     "wide" (= "auto"): one to three PT iterations per launch over an active set of instances
-    whose working sets fit ``budget_mb`` (default 768 MiB; 512 MiB with mixed precision),
+    whose working sets fit ``budget_mb`` (default 1536 MiB; 1024 MiB with mixed precision),
     refilled as instances converge.
     ``mixed_precision=True`` (wide mode): every instance is solved as
     ``solveStokes(..., mixed_precision=True)`` solves it (FP32 sweeps inside FP64 defect
