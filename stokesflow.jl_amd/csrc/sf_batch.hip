@@ -257,7 +257,7 @@ hipError_t launch_wide_pt(const WideArgs& a, int depth, hipStream_t s)
     return hipGetLastError();
 }
 
-// Three PT iterations of every active instance per launch (ptk_tile<3>).
+// Three PT iterations of every active instance per launch (ptk_tile<3>, split loop form).
 __global__ __launch_bounds__(64) void wide_pt3_kernel(WideArgs a)
 {
     const int total = a.tile_start[a.n_active];
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(64) void wide_pt3_kernel(WideArgs a)
     CBatch* cinst = (CBatch*)a.desc;
     const PTParams SF_CONST& pp = cinst[a.slot_inst[lo]].wpt[4 + (a.slot_pf[lo] & 1)];
     const int nstrips = pp.nstrips;
-    ptk_tile<3>(pp, t % nstrips, t / nstrips);
+    ptk_tile<3, true>(pp, t % nstrips, t / nstrips);
 }
 
 hipError_t launch_wide_pt3(const WideArgs& a, hipStream_t s)

@@ -65,7 +65,7 @@ template <int K>
 __global__ __launch_bounds__(64) void ptk_tile_f64(PTParams p)
 {
     const int logical = xcd_remap(blockIdx.x, gridDim.x);
-    ptk_tile<K>(p, logical % p.nstrips, logical / p.nstrips);
+    ptk_tile<K, false>(p, logical % p.nstrips, logical / p.nstrips);
 }
 
 // compute_ResY with post-update V and dt_check (compute_Residuals!, :241-249).

@@ -36,11 +36,14 @@ namespace sf {
 
 #define PT2_OUT_LANES 60
 
-// The row window one PT iteration carries down a strip (pt_tile's register window).
+// The row window one PT iteration carries down a strip (pt_tile's register window).  Only the
+// lane's own values are carried; their neighbours one or two lanes down are moved in again
+// on every row (a DPP move is exact, so the values are the same), which keeps the window at
+// 9 doubles instead of 16 and leaves a three-stage pass at 198 VGPRs instead of 236.
 struct PTWin {
-    double vx_r, vx_r_p1, vy_r, vy_r_p1;
-    double rho_r, rho_r_p1, rho_r_p2, rho_rm1_p1;
-    double pn_prev, tyy_prev, txx_prev, pn_prev_p1, txx_prev_p1, txy_prev_p1, mup_prev, mup_prev_p1;
+    double vx_r, vy_r;
+    double rho_r, rho_rm1_p1;
+    double pn_prev, tyy_prev, txx_prev, txy_prev_p1, mup_prev;
 };
 
 // One row step's results: P at (c, r); dVx, dVy at (c, r-1); Vx, Vy at (c+1, r).
@@ -59,14 +62,10 @@ __device__ __forceinline__ void pt_win_init(PTWin& w, double vx_r, double vy_r, 
     w.vx_r = vx_r;
     w.vy_r = vy_r;
     w.rho_r = rho_r;
-    w.vx_r_p1 = lane_down(vx_r, 1);
-    w.vy_r_p1 = lane_down(vy_r, 1);
-    w.rho_r_p1 = lane_down(rho_r, 1);
-    w.rho_r_p2 = lane_down(rho_r, 2);
     w.rho_rm1_p1 = 0.0;
     w.pn_prev = w.tyy_prev = w.txx_prev = 0.0;
-    w.pn_prev_p1 = w.txx_prev_p1 = w.txy_prev_p1 = 0.0;
-    w.mup_prev = w.mup_prev_p1 = 0.0;
+    w.txy_prev_p1 = 0.0;
+    w.mup_prev = 0.0;
 }
 
 // One row step of one PT iteration with pt_tile's expressions (compute_P_tau!, compute_ResX,
@@ -76,55 +75,54 @@ template <bool TAU_IN, class PP>
 __device__ __forceinline__ PTRow pt_row(const PP& p, double dt, PTWin& w, const RowIn& in, PTTau& tau)
 {
     const double rdx = p.rdx, rdy = p.rdy;
+    const double w_vx_r_p1 = lane_down(w.vx_r, 1);
+    const double w_vy_r_p1 = lane_down(w.vy_r, 1);
+    const double w_rho_r_p1 = lane_down(w.rho_r, 1);
+    const double w_rho_r_p2 = lane_down(w.rho_r, 2);
+    const double w_pn_prev_p1 = lane_down(w.pn_prev, 1);
+    const double w_txx_prev_p1 = lane_down(w.txx_prev, 1);
+    const double w_mup_prev_p1 = lane_down(w.mup_prev, 1);
     const double vx_r1_p1 = lane_down(in.vx1, 1);
     const double vy_r1_p1 = lane_down(in.vy1, 1);
     const double rho_r1_p1 = lane_down(in.rho1, 1);
-    const double rho_r1_p2 = lane_down(in.rho1, 2);
 
     const double dvxdx = (vx_r1_p1 - in.vx1) * rdx;
-    const double dvydy = (vy_r1_p1 - w.vy_r_p1) * rdy;
+    const double dvydy = (vy_r1_p1 - w_vy_r_p1) * rdy;
     const double div = dvxdx + dvydy;
     if constexpr (!TAU_IN) tau.p = (p.ptsc41 * in.mup) / p.max_nxy;
     const double pn = in.pold - tau.p * div;
     const double txx = (2.0 * in.mup) * (dvxdx - (1.0 / 3.0) * div);
     const double tyy = (2.0 * in.mup) * (dvydy - (1.0 / 3.0) * div);
-    const double txy = ((2.0 * in.mub) * 0.5) * ((in.vx1 - w.vx_r) * rdy + (w.vy_r_p1 - w.vy_r) * rdx);
+    const double txy = ((2.0 * in.mub) * 0.5) * ((in.vx1 - w.vx_r) * rdy + (w_vy_r_p1 - w.vy_r) * rdx);
     const double txy_p1 = lane_down(txy, 1);
 
-    const double rx = ((w.txx_prev_p1 - w.txx_prev) * rdx + (txy_p1 - w.txy_prev_p1) * rdy) -
-                      (w.pn_prev_p1 - w.pn_prev) * rdx;
-    const double av = 0.25 * (((w.vx_r + w.vx_r_p1) + in.vx1) + vx_r1_p1);
-    const double d_xi = w.rho_r_p2 - w.rho_r;
+    const double rx = ((w_txx_prev_p1 - w.txx_prev) * rdx + (txy_p1 - w.txy_prev_p1) * rdy) -
+                      (w_pn_prev_p1 - w.pn_prev) * rdx;
+    const double av = 0.25 * (((w.vx_r + w_vx_r_p1) + in.vx1) + vx_r1_p1);
+    const double d_xi = w_rho_r_p2 - w.rho_r;
     const double d_yi = rho_r1_p1 - w.rho_rm1_p1;
     const double ry = (((tyy - w.tyy_prev) * rdy + (txy_p1 - txy) * rdx) - (pn - w.pn_prev) * rdy) +
-                      p.g * (w.rho_r_p1 - dt * (((av * d_xi) * 0.5) * rdx + ((w.vy_r_p1 * d_yi) * 0.5) * rdy));
+                      p.g * (w_rho_r_p1 - dt * (((av * d_xi) * 0.5) * rdx + ((w_vy_r_p1 * d_yi) * 0.5) * rdy));
     PTRow o;
     o.pn = pn;
     o.dvx = p.dampx * in.dvx + rx;
     o.dvy = p.dampy * in.dvy + ry;
     if constexpr (!TAU_IN) {
-        tau.vx = (p.vsc_mdxy2 / ((w.mup_prev + w.mup_prev_p1) * 0.5)) / 4.1;
+        tau.vx = (p.vsc_mdxy2 / ((w.mup_prev + w_mup_prev_p1) * 0.5)) / 4.1;
         tau.vy = (p.vsc_mdxy2 / ((w.mup_prev + in.mup) * 0.5)) / 4.1;
     }
-    o.vx = w.vx_r_p1 + tau.vx * o.dvx;
-    o.vy = w.vy_r_p1 + tau.vy * o.dvy;
+    o.vx = w_vx_r_p1 + tau.vx * o.dvx;
+    o.vy = w_vy_r_p1 + tau.vy * o.dvy;
 
-    w.pn_prev_p1 = lane_down(pn, 1);
-    w.txx_prev_p1 = lane_down(txx, 1);
-    w.mup_prev_p1 = lane_down(in.mup, 1);
     w.pn_prev = pn;
     w.txx_prev = txx;
     w.tyy_prev = tyy;
     w.txy_prev_p1 = txy_p1;
     w.mup_prev = in.mup;
     w.vx_r = in.vx1;
-    w.vx_r_p1 = vx_r1_p1;
     w.vy_r = in.vy1;
-    w.vy_r_p1 = vy_r1_p1;
-    w.rho_rm1_p1 = w.rho_r_p1;
+    w.rho_rm1_p1 = w_rho_r_p1;
     w.rho_r = in.rho1;
-    w.rho_r_p1 = rho_r1_p1;
-    w.rho_r_p2 = rho_r1_p2;
     return o;
 }
 
