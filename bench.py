@@ -589,7 +589,7 @@ def main():
     probe = sfhip.probe_dram(1 << 30)  # the box's DRAM ceiling for simple streams
     b_iter = bytes_per_iter(nx, ny)
     names = {1: ("pt_tile_f64", "pt_tile_f64"), 2: ("pt2_tile_f64", "pt2_tile_f64"),
-             3: ("ptk_tile_f64<3>", "ptk3_tile_f64")}
+             3: ("ptk_tile_f64<3>", "ptk3_tile_f64"), 4: ("ptk_tile_f64<4>", "ptk4_tile_f64")}
     if pt64 is None:  # no pass launch in the timed region (one iteration per launch): the region itself
         pt64 = {"iters_per_launch": 1, "launches": a.steps, "launch_s": gpu_s / a.steps}
     k_pass = pt64["iters_per_launch"]

@@ -79,13 +79,13 @@ int sf_ctx_set_stream(sf_ctx *ctx, void *hip_stream);
  * that many waves, < 0 = one wave per tile.  No reference counterpart (ParallelStencil picks launch
  * shapes itself). */
 int sf_ctx_set_pt_tuning(sf_ctx *ctx, int rows, int depth, int max_waves);
-/* Temporal blocking of the PT iteration: iters_per_launch = 2 or 3 runs that many plain
+/* Temporal blocking of the PT iteration: iters_per_launch = 2, 3 or 4 runs that many plain
  * iterations (no check iteration, no dt refresh before the last of them, not the last
- * iteration or the first timed one) as one pass that reads and writes the state once; 1 runs
- * one iteration per launch; 0 is the size default (3 above 1024 nodes per side, else 2).  The
- * FP32 sweeps of sf_solve_stokes_refined take the same setting with 4 and 5 also allowed (the
- * FP64 passes cap it at 3); their default is 5 above 3072 nodes per side, 4 above 1536, 3 above
- * 768, else 2.
+ * iteration or the first timed one) as one pass that reads and writes the state once (where
+ * they do not fit, shorter passes); 1 runs one iteration per launch; 0 is the size default (4
+ * above 96 nodes per side, else 2).  The FP32 sweeps of sf_solve_stokes_refined take the same
+ * setting with 5 also allowed (the FP64 passes cap it at 4); their default is 5 above 3072
+ * nodes per side, 4 above 1536, 3 above 768, else 2.
  * rows: grid rows per tile of the pass (0 = size default).  Results are bit-identical either
  * way; the pass needs a second dVx/dVy buffer, allocated in the context on first use.  No
  * reference counterpart (the reference launches each of its kernels once per iteration). */
@@ -95,7 +95,7 @@ int sf_ctx_set_pt_fusion(sf_ctx *ctx, int iters_per_launch, int rows);
  * the caller's problem (P ... mu_p as for sf_pt_iterations; the caller's arrays are not
  * modified) or, when any of them is NULL, on a synthetic one (materials 1, state 0), in scratch
  * it allocates and frees.  Candidates: the one-iteration kernel (rows 1, 2, 4, 8, 16 x depth
- * 1, 2 x one wave per tile or persistent; iters_per_launch = 1), then passes of 2 and 3
+ * 1, 2 x one wave per tile or persistent; iters_per_launch = 1), then passes of 2, 3 and 4
  * iterations (rows 2, 4, 8, 12, 16, 24, 32, 48).  The fastest one-iteration shape is set on the
  * context (check iterations use it), and the fastest overall decides the iterations per
  * launch and the pass rows of the FP64 solver (sf_solve_stokes, sf_pt_iterations); the FP32
@@ -259,7 +259,7 @@ int sf_solve_stokes_batched(sf_ctx *ctx, const sf_batch_instance *inst, int n, c
                             sf_solve_out *outs);
 
 /* Autotuner of the batched solver's wide FP64 passes.  For every grid size in inst it times the
- * wide two- and three-iteration passes with tile heights of 4..64 rows over an active set of that
+ * wide two-, three- and four-iteration passes with tile heights of 4..64 rows over an active set of that
  * size's instances, as large as prm->budget_mb holds, `iters` PT iterations per candidate, and
  * takes the fastest (iterations per pass, rows) per size.  Each size's choice is then checked on
  * the whole batch -- every instance capped at 240 PT iterations, with the size defaults and with

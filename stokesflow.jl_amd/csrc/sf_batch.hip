@@ -257,8 +257,11 @@ hipError_t launch_wide_pt(const WideArgs& a, int depth, hipStream_t s)
     return hipGetLastError();
 }
 
-// Three PT iterations of every active instance per launch (ptk_tile<3>, split loop form).
-__global__ __launch_bounds__(64) void wide_pt3_kernel(WideArgs a)
+// K = 3 or 4 PT iterations of every active instance per launch (ptk_tile<K>; descriptor
+// wpt[2K - 2 + parity]).  K = 3 runs the split loop form (214 VGPRs); K = 4 the plain one (238
+// VGPRs; its split form needs 260, past the two waves per SIMD).
+template <int K, bool SPLIT>
+__global__ __launch_bounds__(64) void wide_ptk_kernel(WideArgs a)
 {
     const int total = a.tile_start[a.n_active];
     const int t_glob = xcd_remap(blockIdx.x, gridDim.x);
@@ -271,16 +274,24 @@ __global__ __launch_bounds__(64) void wide_pt3_kernel(WideArgs a)
     }
     const int t = t_glob - a.tile_start[lo];
     CBatch* cinst = (CBatch*)a.desc;
-    const PTParams SF_CONST& pp = cinst[a.slot_inst[lo]].wpt[4 + (a.slot_pf[lo] & 1)];
+    const PTParams SF_CONST& pp = cinst[a.slot_inst[lo]].wpt[2 * K - 2 + (a.slot_pf[lo] & 1)];
     const int nstrips = pp.nstrips;
-    ptk_tile<3, true>(pp, t % nstrips, t / nstrips);
+    ptk_tile<K, SPLIT>(pp, t % nstrips, t / nstrips);
 }
 
 hipError_t launch_wide_pt3(const WideArgs& a, hipStream_t s)
 {
     const int total = a.tile_start[a.n_active];
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(wide_pt3_kernel, dim3(total), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((wide_ptk_kernel<3, true>), dim3(total), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_wide_pt4(const WideArgs& a, hipStream_t s)
+{
+    const int total = a.tile_start[a.n_active];
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL((wide_ptk_kernel<4, false>), dim3(total), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

@@ -120,6 +120,12 @@ BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, 
         w.nstrips = (nx - 1 + 57) / 58;
         w.ntiles = w.nstrips * ((ny + rows3 - 1) / rows3);
         b.wpt[4 + par] = w;
+        const int n = std::max(nx, ny);
+        const int rows4 = std::min(ny, sh.rows4 > 0 ? sh.rows4 : (n > 384 ? 64 : (n > 192 ? 48 : 8)));
+        w.seg = rows4;
+        w.nstrips = (nx - 1 + 55) / 56;
+        w.ntiles = w.nstrips * ((ny + rows4 - 1) / rows4);
+        b.wpt[6 + par] = w;
     }
     b.P = a.P;
     b.Vx_s = a.Vx_s;
@@ -132,7 +138,11 @@ BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, 
     b.nx = nx;
     b.ny = ny;
     b.use_fss = a.use_fss;
-    b.kmax = std::min(std::max(sh.kmax, 1), 3);
+    // size default: four iterations per pass above 192 nodes per side, three below (wide passes,
+    // ns per node and iteration on the 256-instance ensemble's active sets, best tile heights,
+    // K = 4 / 3: 512^2 0.00691 / 0.00783; 256^2 0.00833 / 0.00905; 128^2 0.01306 / 0.01219;
+    // 64^2 0.03259 / 0.03056)
+    b.kmax = sh.kmax > 0 ? std::min(sh.kmax, 4) : (std::max(nx, ny) > 192 ? 4 : 3);
     b.dx = a.dx;
     b.dy = a.dy;
     b.dt_in = a.dt;
@@ -192,7 +202,7 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         int64_t iter, ncheck, ndtupdate;
         double err, ws;
         bool checking;
-        int k;            // iterations of this step's launch (1, 2 or 3)
+        int k;            // iterations of this step's launch (1 to 4)
         int64_t nlaunch;  // launches so far: their parity says where the state (V, P, dV) is
     };
     std::vector<Run> act;
@@ -236,13 +246,15 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         return SF_OK;
     };
     int rc = SF_OK;
-    WideArgs wa, w2, w3;
+    WideArgs wa, w2, w3, w4;
     std::memset(&wa, 0, sizeof(wa));
     std::memset(&w2, 0, sizeof(w2));
     std::memset(&w3, 0, sizeof(w3));
+    std::memset(&w4, 0, sizeof(w4));
     wa.desc = env.desc_dev;
     w2.desc = env.desc_dev;
     w3.desc = env.desc_dev;
+    w4.desc = env.desc_dev;
     while (!act.empty() || q < order.size()) {
         while (q < order.size() && (int)act.size() < WIDE_MAX &&
                (act.empty() || ws_active + batch_working_set(in[order[q]].nx, in[order[q]].ny) <= budget)) {
@@ -254,7 +266,8 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         wa.n_active = 0;
         w2.n_active = 0;
         w3.n_active = 0;
-        int tiles = 0, tiles2 = 0, tiles3 = 0;
+        w4.n_active = 0;
+        int tiles = 0, tiles2 = 0, tiles3 = 0, tiles4 = 0;
         for (size_t k = 0; k < act.size(); ++k) {
             Run& r = act[k];
             const sf_batch_instance& a = in[r.id];
@@ -269,10 +282,18 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
                 return true;
             };
             const int kmax = desc[(size_t)r.id].kmax;
-            r.k = (kmax >= 3 && fusable(3)) ? 3 : ((kmax >= 2 && fusable(2)) ? 2 : 1);
+            r.k = 1;
+            for (int kk = kmax; kk >= 2 && r.k == 1; --kk)
+                if (fusable(kk)) r.k = kk;
             r.checking = check;
             const int par = (int)(r.nlaunch & 1);
-            if (r.k == 3) {
+            if (r.k == 4) {
+                w4.tile_start[w4.n_active] = tiles4;
+                w4.slot_inst[w4.n_active] = r.id;
+                w4.slot_pf[w4.n_active] = par;
+                tiles4 += desc[(size_t)r.id].wpt[6].ntiles;
+                w4.n_active += 1;
+            } else if (r.k == 3) {
                 w3.tile_start[w3.n_active] = tiles3;
                 w3.slot_inst[w3.n_active] = r.id;
                 w3.slot_pf[w3.n_active] = par;
@@ -296,6 +317,8 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         wa.tile_start[wa.n_active] = tiles;
         w2.tile_start[w2.n_active] = tiles2;
         w3.tile_start[w3.n_active] = tiles3;
+        w4.tile_start[w4.n_active] = tiles4;
+        if (w4.n_active) HIPCHK(launch_wide_pt4(w4, s));
         if (w3.n_active) HIPCHK(launch_wide_pt3(w3, s));
         if (w2.n_active) HIPCHK(launch_wide_pt2(w2, s));
         if (wa.n_active) HIPCHK(launch_wide_pt(wa, depth, s));
@@ -934,7 +957,7 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
 
 
 // ---------------------------------------------------------------------------------------
-// autotuner of the wide passes: per grid size, the multi-iteration pass (2 or 3 PT iterations
+// autotuner of the wide passes: per grid size, the multi-iteration pass (2, 3 or 4 PT iterations
 // per launch) and its tile rows, timed over an active set of that size's instances (copies of
 // the caller's problems in scratch allocated here), as the wide scheduler launches them
 
@@ -1016,16 +1039,17 @@ extern "C" int sf_batch_autotune(sf_ctx* c, const sf_batch_instance* in, int n, 
             break;
         }
         BatchTune best{nx, ny, BatchShape()};
-        double best_t[4] = {1e300, 1e300, 1e300, 1e300};  // per k: ns per node and iteration
-        int best_rows[4] = {0, 0, 0, 0};
+        double best_t[5] = {1e300, 1e300, 1e300, 1e300, 1e300};  // per k: ns per node and iteration
+        int best_rows[5] = {0, 0, 0, 0, 0};
         const int rows_c[] = {4, 8, 12, 16, 24, 32, 48, 64};
-        for (int k : {2, 3}) {
+        for (int k : {2, 3, 4}) {
             for (int rows : rows_c) {
                 if (rows > ny && rows != rows_c[0]) continue;
                 BatchShape sh;
                 sh.kmax = k;
                 sh.rows2 = rows;
                 sh.rows3 = rows;
+                sh.rows4 = rows;
                 std::vector<BatchInstance> desc((size_t)R);
                 for (int j = 0; j < R; ++j) {
                     cp[(size_t)j].dt = 0.0;
@@ -1041,7 +1065,7 @@ extern "C" int sf_batch_autotune(sf_ctx* c, const sf_batch_instance* in, int n, 
                 for (int j = 0; j < R; ++j) {
                     w.tile_start[j] = tiles;
                     w.slot_inst[j] = j;
-                    tiles += desc[(size_t)j].wpt[k == 3 ? 4 : 2].ntiles;
+                    tiles += desc[(size_t)j].wpt[2 * k - 2].ntiles;
                 }
                 w.tile_start[R] = tiles;
                 const int reps = std::max(4, (iters + k - 1) / k) & ~1;  // even: the state ends where it began
@@ -1049,7 +1073,8 @@ extern "C" int sf_batch_autotune(sf_ctx* c, const sf_batch_instance* in, int n, 
                 for (int rep = -2; rep < reps && e == hipSuccess; ++rep) {  // two untimed launches first
                     if (rep == 0) e = hipEventRecord(ev[0], s);
                     for (int j = 0; j < R; ++j) w.slot_pf[j] = rep & 1;
-                    if (e == hipSuccess) e = (k == 3) ? launch_wide_pt3(w, s) : launch_wide_pt2(w, s);
+                    if (e == hipSuccess)
+                        e = (k == 4) ? launch_wide_pt4(w, s) : (k == 3) ? launch_wide_pt3(w, s) : launch_wide_pt2(w, s);
                 }
                 if (e == hipSuccess) e = hipEventRecord(ev[1], s);
                 if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
@@ -1069,9 +1094,12 @@ extern "C" int sf_batch_autotune(sf_ctx* c, const sf_batch_instance* in, int n, 
             rc = fail(e);
             break;
         }
-        best.sh.kmax = best_t[3] <= best_t[2] ? 3 : 2;
+        best.sh.kmax = 2;
+        for (int k = 3; k <= 4; ++k)
+            if (best_t[k] <= best_t[best.sh.kmax]) best.sh.kmax = k;
         best.sh.rows2 = best_rows[2];
         best.sh.rows3 = best_rows[3];
+        best.sh.rows4 = best_rows[4];
         found.push_back(best);
     }
     (void)hipStreamSynchronize(s);
@@ -1157,7 +1185,7 @@ extern "C" int sf_batch_autotune(sf_ctx* c, const sf_batch_instance* in, int n, 
     for (int q = 0; q < nt && q < table_cap; ++q)  // mark the candidates the check kept
         for (const BatchTune& t : kept)
             if (table[q].nx == t.nx && table[q].ny == t.ny && table[q].iters_per_launch == t.sh.kmax &&
-                table[q].rows == (t.sh.kmax == 3 ? t.sh.rows3 : t.sh.rows2))
+                table[q].rows == (t.sh.kmax == 4 ? t.sh.rows4 : t.sh.kmax == 3 ? t.sh.rows3 : t.sh.rows2))
                 table[q].kept = 1;
     if (n_table) *n_table = nt;
     return SF_OK;

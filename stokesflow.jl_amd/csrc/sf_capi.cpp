@@ -210,8 +210,8 @@ int PTLoop::run_fixed(int64_t n_iter, hipStream_t s)
 {
     int64_t it = 0;
     int rc = SF_OK;
-    for (; pairs && it + kmax <= n_iter && rc == SF_OK; it += kmax) rc = pair(s, kmax);
-    for (; pairs && it + 2 <= n_iter && rc == SF_OK; it += 2) rc = pair(s, 2);
+    for (int k = kmax; pairs && k >= 2; --k)
+        for (; it + k <= n_iter && rc == SF_OK; it += k) rc = pair(s, k);
     for (; it < n_iter && rc == SF_OK; ++it) rc = single(0, s);
     if (rc) return rc;
     return finish(s);
@@ -504,7 +504,9 @@ int sf::solve_core(const SolveScratch& w, const char* who, double* P, double* Vx
             }
             return true;
         };
-        const int kf = !L.pairs ? 1 : (L.kmax == 3 && fusable(3) ? 3 : (fusable(2) ? 2 : 1));
+        int kf = 1;
+        for (int k = L.pairs ? L.kmax : 1; k >= 2 && kf == 1; --k)
+            if (fusable(k)) kf = k;
         if (kf > 1) {
             rc = L.pair(s, kf);
             if (rc) return rc;
@@ -785,14 +787,14 @@ extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, const double* P_in, cons
                     record(t);
                     if (t.us_per_iter < bestv.us_per_iter) bestv = t;
                 }
-        // then passes of 2 and 3 iterations (the check iterations keep the best one-iteration shape)
+        // then passes of 2 to 4 iterations (the check iterations keep the best one-iteration shape)
         best_all = bestv;
         c->seg = bestv.rows;
         c->depth = bestv.depth;
         c->max_waves = bestv.max_waves;
         const int prows_c[] = {2, 4, 8, 12, 16, 24, 32, 48};
-        const int n_it = std::max(6, (iters + 5) / 6 * 6);  // whole passes of either kind
-        for (int k : {2, 3})
+        const int n_it = std::max(12, (iters + 11) / 12 * 12);  // whole passes of every kind
+        for (int k : {2, 3, 4})
             for (int rows : prows_c) {
                 if (rows > ny && rows != prows_c[0]) continue;
                 c->fuse64 = k;

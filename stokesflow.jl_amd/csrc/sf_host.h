@@ -75,10 +75,10 @@ size_t batch_scratch_doubles(int nx, int ny);
 int batch_check(const sf_batch_instance& a, int i, const char* who);
 
 // Launch shape of an instance's wide FP64 passes (results do not depend on it): the most PT
-// iterations per pass (2 or 3) and the tile rows of the two- and three-iteration passes
-// (0: size default).  sf_batch_autotune keeps one per grid size on the context.
+// iterations per pass (2, 3 or 4) and the tile rows of the two-, three- and four-iteration
+// passes (0: size default).  sf_batch_autotune keeps one per grid size on the context.
 struct BatchShape {
-    int kmax = 3, rows2 = 0, rows3 = 0;
+    int kmax = 0, rows2 = 0, rows3 = 0, rows4 = 0;
 };
 struct BatchTune {
     int nx, ny;
@@ -147,11 +147,11 @@ struct PTLoop {
     double *vx0, *vy0, *p0, *dvx0, *dvy0;  // the caller's arrays
     PTBuffers b;
     bool pairs;  // multi-iteration passes enabled (b.dvx_b, b.dvy_b must then be set)
-    int kmax;    // iterations per pass (2 or 3)
+    int kmax;    // iterations per pass (2, 3 or 4)
     int rows;    // tile rows of the passes (0: size default)
     PassTimer* tm = nullptr;  // times the pass launches when set and on
 
-    // fuse: iterations per pass (0 size default, 1 one per launch, 2 or 3; larger values cap at 3)
+    // fuse: iterations per pass (0 size default, 1 one per launch, 2 to 4; larger values cap at 4)
     void init(const PTBuffers& buf, int fuse, int pass_rows, int nx_, int ny_, double* Vx_s, double* Vy_s, double* P,
               double* dVx, double* dVy)
     {
@@ -170,7 +170,7 @@ struct PTLoop {
         nxt_dvy = b.dvy_b;
         rows = pass_rows;
         pairs = fuse != 1;
-        kmax = fuse == 0 ? pt_default_fusion(nx, ny) : std::min(fuse, 3);
+        kmax = fuse == 0 ? pt_default_fusion(nx, ny) : std::min(fuse, 4);
     }
     // One iteration (flags: check-iteration outputs); dV in place.
     int single(int flags, hipStream_t s);
@@ -178,7 +178,7 @@ struct PTLoop {
     int pair(hipStream_t s, int k = 2);
     // State back into the caller's arrays.
     int finish(hipStream_t s);
-    // n_iter plain iterations (passes of kmax while they fit, then 2, then one), then finish.
+    // n_iter plain iterations (passes of kmax while they fit, then smaller ones, then one), then finish.
     int run_fixed(int64_t n_iter, hipStream_t s);
 };
 

@@ -163,11 +163,12 @@ hipError_t launch_probe_r3w1(const double* a, const double* b, const double* c, 
 // and the epilogue need.
 struct BatchInstance {
     PTParams pt[2];   // one iteration, state ping-pong by parity, dV in place (persistent mode)
-    PTParams wpt[6];  // wide mode, dV ping-pong too: [parity] one iteration, [2 + parity] two, [4 + parity] three
+    PTParams wpt[8];  // wide mode, dV ping-pong too: [parity] one iteration, [2 + parity] two, [4 + parity]
+                      // three, [6 + parity] four
     double *P, *Vx_s, *Vy_s, *Vx_pad, *Vy_pad;  // caller state (in/out)
     double *ry, *scal;                          // library scratch (scal[SC_DT] = stabilisation dt)
     const double* dt_dev;                       // if set, the input dt is read here (device) instead of dt_in
-    int nx, ny, use_fss, kmax;  // kmax: most PT iterations per wide pass (2 or 3)
+    int nx, ny, use_fss, kmax;  // kmax: most PT iterations per wide pass (2, 3 or 4)
     double dx, dy, dt_in, maxdisp, eps;
     int64_t iter_max;
 };
@@ -193,6 +194,8 @@ hipError_t launch_wide_pt(const WideArgs& a, int depth, hipStream_t s);
 hipError_t launch_wide_pt2(const WideArgs& a, hipStream_t s);
 // Three PT iterations of every slot (descriptor wpt[4 + parity]).
 hipError_t launch_wide_pt3(const WideArgs& a, hipStream_t s);
+// Four PT iterations of every slot (descriptor wpt[6 + parity]).
+hipError_t launch_wide_pt4(const WideArgs& a, hipStream_t s);
 
 // Wide mode of the FP32 correction sweeps: slot k runs descriptor slot_desc[k] (two
 // descriptors per instance, one per ping-pong direction).

@@ -284,13 +284,11 @@ hipError_t launch_pt_pair(const PTParams& p0, hipStream_t s)
     return hipGetLastError();
 }
 
-// Iterations per pass by grid size, and the three-iteration pass's tile rows (measured on
-// MI355X, us per iteration: 4096^2 K=3 138.3 with 48 rows (141.8 with 32, 152.9 with 64)
-// against K=2 193.6; 2048^2 K=3 41.7 with 16 rows against K=2 50.4; 1024^2 K=2 14.1 against
-// K=3 14.6 before the stages shared their pseudo-time steps).  In the solver loop with shared
-// steps: 1536^2 K=3 20.6 against K=2 23.5, 1280^2 13.8 against 17.5-19.6, 1024^2 13.4 against
-// 12.2, so three iterations per pass from 1025 nodes per side.
-int pt_default_fusion(int nx, int ny) { return std::max(nx, ny) > 1024 ? 3 : 2; }
+// Iterations per pass by grid size (pass timer, us per iteration, best tile height of each;
+// K = 4 / 3 / 2): 4096^2 108.7 / 129.1 / 190.4; 2048^2 30.0 / 34.3 / 50.7; 1024^2 11.2 / 11.5 / 12.7;
+// 512^2 5.5 / 6.7 / 6.4; 256^2 3.79 / 3.83 / 4.07; 128^2 3.56 / 3.57 / 4.23; 64^2 3.52 / 3.54 / 3.34.
+// Four iterations per pass from 97 nodes per side, two below (where a pass is mostly launch).
+int pt_default_fusion(int nx, int ny) { return std::max(nx, ny) > 96 ? 4 : 2; }
 
 // Sweeps per pass of the FP32 correction solver when the context leaves it open (three from 769
 // nodes per side; us per sweep, K=3 against K=2: 1024^2 7.82 (8-row tiles) against 8.8, 1280^2
@@ -306,27 +304,35 @@ int corr_default_fusion(int nx, int ny)
 }
 
 //
-// The tile height comes from a wave-round model: the launch runs in rounds of resident waves (two
-// per SIMD at 236 VGPRs), and a partly filled last round costs a whole one, so the height
-// minimises rounds x (rows + 5 halo rows) over 12-64 rows.  Measured in the solver loop, us per
-// iteration (model pick first): 4096^2 48 rows 136.0, 32 141.8, 56 145.2, 64 152.9; 3072^2 40 rows
-// 74.0, 28 76.7, 20 77.8, 16 79.1; 2048^2 40 rows 34.4, 20 36.6, 16 39.4, 32 45.4; 1536^2 24 rows
-// 19.7, 12 21.2, 16 21.7, 40 28.5; 1280^2 16 rows 13.6, 20 16.0, 12 16.4.
-int pt3_default_rows(int nx, int ny)
+// The tile height of a K-iteration pass comes from a wave-round model.  A launch runs in rounds of
+// resident waves (two per SIMD: 198 VGPRs at K = 3, 238 at K = 4) and a partly filled last round
+// costs a whole one, so the height minimises rounds x (rows + 2K - 1 halo rows); a launch of at most
+// one wave per SIMD runs each wave 1.38x faster (measured at 512^2, K = 4: 6-row tiles, 860 waves,
+// 1.70 us per row step; 4-row tiles, 1280 waves, 2.35 us), which is what small grids reach with
+// short tiles.  Candidates: 2-12 rows in steps of 2, then 16-64 in steps of 4.  Measured (pass
+// timer, us per iteration, model pick first), K = 4: 4096^2 52 rows 108.7, 44 111.7, 40 112.2, 60
+// 110.1, 48 118.1; 3584^2 56 rows 80.2, 28 84.6; 3072^2 44 rows 63.5, 28 64.2, 24 66.5; 2560^2 60
+// rows 43.5, 32 47.7; 2048^2 40 rows 30.0, 16 34.7, 48 34.1; 1536^2 24 rows 18.2, 16 19.8; 1280^2
+// 16 rows 13.5, 8 16.0; 1024^2 12 rows 11.2, 24 12.6; 768^2 12 rows 7.9, 8 8.9; 512^2 6 rows 5.5,
+// 8 6.3, 4 6.5, 12 7.9; 64^2-256^2 2 rows.  K = 3 in the solver loop: 4096^2 48 rows 136.0, 32
+// 141.8; 3072^2 40 rows 74.0, 28 76.7; 2048^2 40 rows 34.4, 20 36.6; 1536^2 24 rows 19.7, 12 21.2.
+int ptk_default_rows(int k, int nx, int ny)
 {
-    static const int slots = [] {
+    static const int simds = [] {
         int dev = 0, cus = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             cus = 256;
-        return 2 * 4 * cus;
+        return 4 * cus;
     }();
-    const int64_t strips = (nx - 1 + 57) / 58;
+    const int out = 64 - 2 * k;
+    const int64_t strips = (nx - 1 + out - 1) / out;
     int best = 16;
     double best_cost = 1e300;
-    for (int rows = 12; rows <= 64; rows += 4) {
+    for (int rows = 2; rows <= 64; rows += rows < 12 ? 2 : 4) {
         const int64_t waves = strips * ((ny + rows - 1) / rows);
-        const double cost = (double)((waves + slots - 1) / slots) * (rows + 5);
+        const double steps = rows + 2 * k - 1;
+        const double cost = waves <= simds ? steps : 1.38 * (double)((waves + 2 * simds - 1) / (2 * simds)) * steps;
         if (cost < best_cost) {
             best_cost = cost;
             best = rows;
@@ -338,13 +344,14 @@ int pt3_default_rows(int nx, int ny)
 hipError_t launch_pt_multi(const PTParams& p0, int k, hipStream_t s)
 {
     if (k == 2) return launch_pt_pair(p0, s);
-    if (k != 3) return hipErrorInvalidValue;
+    if (k != 3 && k != 4) return hipErrorInvalidValue;
     PTParams p = p0;
-    if (p.seg <= 0) p.seg = pt3_default_rows(p.nx, p.ny);
-    p.nstrips = (p.nx - 1 + 58 - 1) / 58;  // 64 - 2 K output columns per strip
+    if (p.seg <= 0) p.seg = ptk_default_rows(k, p.nx, p.ny);
+    p.nstrips = (p.nx - 1 + (64 - 2 * k) - 1) / (64 - 2 * k);  // 64 - 2 K output columns per strip
     const int nseg = (p.ny + p.seg - 1) / p.seg;
     p.ntiles = p.nstrips * nseg;
-    hipLaunchKernelGGL(ptk_tile_f64<3>, dim3(p.ntiles), dim3(64), 0, s, p);
+    if (k == 3) hipLaunchKernelGGL(ptk_tile_f64<3>, dim3(p.ntiles), dim3(64), 0, s, p);
+    else hipLaunchKernelGGL(ptk_tile_f64<4>, dim3(p.ntiles), dim3(64), 0, s, p);
     return hipGetLastError();
 }
 

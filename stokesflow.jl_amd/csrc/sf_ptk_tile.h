@@ -16,9 +16,9 @@
 // steps in between, where every stage has a row of its range other than its first and no
 // stage reaches row ny-1, through a body without them, two steps per loop trip: the
 // hand-over then needs no copies between loop-carried registers, about a quarter fewer VALU
-// instructions per step.  The batched pass over many small tiles gains from it (5% on a
-// mixed ensemble of 64-512 node grids); the 2048^2-4096^2 single pass does not, and keeps
-// the plain form.
+// instructions per step.  The batched three-iteration pass over many small tiles gains from it
+// (5% on a mixed ensemble of 64-512 node grids); the 2048^2-4096^2 single pass does not, and
+// keeps the plain form, as does every four-iteration pass (the split form needs 260 VGPRs).
 #pragma once
 
 #include "sf_pt2_tile.h"

@@ -147,9 +147,9 @@ def probe_dram(nbytes=1 << 30):
 
 
 def set_pt_fusion(nx, ny, iters_per_launch=0, rows=0):
-    """PT iterations per launch for the nx x ny context (sf_ctx_set_pt_fusion): 2 or 3 runs that
-    many plain iterations as one pass, 1 one per launch, 0 the size default; the FP32 sweeps of
-    mixed_precision solves also take 4 or 5.  Results are identical."""
+    """PT iterations per launch for the nx x ny context (sf_ctx_set_pt_fusion): 2, 3 or 4 runs
+    that many plain iterations as one pass, 1 one per launch, 0 the size default; the FP32 sweeps
+    of mixed_precision solves also take 5.  Results are identical."""
     ctx = context(nx, ny)
     check(ctx.lib.sf_ctx_set_pt_fusion(ctx.handle, int(iters_per_launch), int(rows)))
 

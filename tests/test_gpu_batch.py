@@ -238,7 +238,7 @@ def test_wide_refined_refills_beyond_active_set_limit(gpu):
 
 
 def test_batch_autotune_keeps_results(gpu):
-    """sf_batch_autotune times the wide passes per grid size (two and three iterations per pass,
+    """sf_batch_autotune times the wide passes per grid size (two to four iterations per pass,
     several tile heights) and keeps the fastest; a batched solve afterwards is still bitwise equal
     to one solve per instance, and the caller's arrays are untouched by the tuning."""
     sizes = [(24, 24), (33, 47), (24, 24), (64, 64)]
@@ -255,7 +255,7 @@ def test_batch_autotune_keeps_results(gpu):
         for G, B in zip(gs, before):
             assert all(torch.equal(G[k], B[k]) for k in G)
         assert {(t["nx"], t["ny"]) for t in table} == {(24, 24), (33, 47), (64, 64)}
-        assert {t["iters_per_launch"] for t in table} == {2, 3}
+        assert {t["iters_per_launch"] for t in table} == {2, 3, 4}
         assert all(t["ns_per_node_iter"] > 0 for t in table)
         assert [t["instances"] for t in table if (t["nx"], t["ny"]) == (24, 24)][0] == 2
         outs = sfhip.solveStokes_batched(problems, mode="wide")

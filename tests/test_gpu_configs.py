@@ -1,8 +1,8 @@
 """The benchmark workloads at their own shapes: the kernels the bench times, checked where it runs
 them.
 
-* 4095^2 cells (4096^2 nodes): the three-iteration FP64 pass with the size-default tile
-  height (48 rows), bitwise against the oracle; the five-sweep FP32 pass of the refined solver
+* 4095^2 cells (4096^2 nodes): the four- and three-iteration FP64 passes with the size-default
+  tile heights (52 and 48 rows), bitwise against the oracle; the five-sweep FP32 pass of the refined solver
   bitwise against one sweep per launch, and its state against the FP64 solver's.
 * 256 mixed-size plume instances (64..512 nodes per side): the batched solver (FP64 and refined)
   bitwise against one solve per instance, and one instance per size up to 256^2 against the
@@ -42,29 +42,30 @@ def device_plume(n, seed, cx=0.5, cy=0.8, r=0.1, mu_plume=0.1, mu_air=0.01):
 
 # ------------------------------------------------------------------------------ 4095^2 cells
 
-def test_config3_triple_pass_4096_bitwise(gpu):
-    """7 PT iterations at 4096^2 nodes with the library's default launch plan (two three-iteration
-    passes of 48-row tiles, then one iteration), bitwise against the oracle."""
+def test_config3_fp64_passes_4096_bitwise(gpu):
+    """11 PT iterations at 4096^2 nodes with the library's default launch plan (two four-iteration
+    passes of 52-row tiles, then a three-iteration pass of 48-row tiles), bitwise against the
+    oracle."""
     n = 4096
-    print("config 3: triple pass at 4096^2", flush=True)
+    print("config 3: four- and three-iteration passes at 4096^2", flush=True)
     S = cases.random_state(n, n, seed=4096, scale=1e-3, contrast=4.0)
     G = {k: sfhip.from_numpy(S[k]) for k in STATE + ("rho_vy", "mu_b", "mu_p")}
     dx = 1.0 / (n - 1)
     sfhip.set_pt_fusion(n, n, 0, 0)
     sfhip.time_passes(n, n, True)
     try:
-        sfhip.pt_iterations(7, G["P"], G["Vx_s"], G["Vy_s"], G["dVx"], G["dVy"], G["rho_vy"], G["mu_b"], G["mu_p"],
+        sfhip.pt_iterations(11, G["P"], G["Vx_s"], G["Vy_s"], G["dVx"], G["dVy"], G["rho_vy"], G["mu_b"], G["mu_p"],
                             1.0, dx, dx, n, n, 0.01)
         torch.cuda.synchronize()
         launches = sfhip.pass_times(n, n)
     finally:
         sfhip.time_passes(n, n, False)
-    assert [(t["precision"], t["iters_per_launch"], t["launches"]) for t in launches] == [(0, 3, 2)]
+    assert sorted((t["precision"], t["iters_per_launch"], t["launches"]) for t in launches) == [(0, 3, 1), (0, 4, 2)]
     got = {k: sfhip.to_numpy(G[k]) for k in STATE}
     del G
     torch.cuda.empty_cache()
     O.compute_timesteps(S, n, n, dx, dx)
-    O.pt_iterations(S, n, n, 7, 1.0, dx, dx, 0.01)
+    O.pt_iterations(S, n, n, 11, 1.0, dx, dx, 0.01)
     for k in STATE:
         assert np.array_equal(got[k], S[k]), (k, np.max(np.abs(got[k] - S[k])))
 

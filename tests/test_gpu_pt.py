@@ -94,6 +94,40 @@ def test_triple_pass_bitwise(gpu, nx, ny, rows, n_iter):
         assert np.array_equal(got[k], exp[k]), (nx, ny, rows, n_iter, k, np.max(np.abs(got[k] - exp[k])))
 
 
+# four iterations per pass: 56-column strips; n_iter 4, 5 = 4 + 1, 7 = 4 + 3, 10 = 4 + 4 + 2 launches
+@pytest.mark.parametrize("nx,ny,rows", [(3, 3, 0), (4, 5, 1), (5, 9, 2), (55, 6, 3), (56, 8, 8), (57, 13, 16),
+                                        (58, 4, 4), (113, 21, 5), (114, 40, 16), (169, 9, 0), (169, 150, 24),
+                                        (169, 150, 40), (170, 203, 56)])
+@pytest.mark.parametrize("n_iter", [4, 5, 7, 10])
+def test_quad_pass_bitwise(gpu, nx, ny, rows, n_iter):
+    S = cases.random_state(nx, ny, seed=nx * 6007 + ny * 41 + rows)
+    try:
+        sfhip.set_pt_fusion(nx, ny, 4, rows)
+        got = run_gpu_iterations(S, nx, ny, n_iter, 1.2, 0.08, 0.05, 0.011)
+    finally:
+        sfhip.set_pt_fusion(nx, ny, 0, 0)
+    exp = oracle_iterations(S, nx, ny, n_iter, 1.2, 0.08, 0.05, 0.011)
+    for k in STATE:
+        assert np.array_equal(got[k], exp[k]), (nx, ny, rows, n_iter, k, np.max(np.abs(got[k] - exp[k])))
+
+
+def test_quad_pass_solve_golden(gpu, golden):
+    """A full solve with four iterations per pass is bitwise the oracle's (checks, dt refreshes and
+    the timing mark split the passes into four, three, two and one)."""
+    g = golden["stokes_solver"]
+    nx, ny = g["nx"], g["ny"]
+    S = cases.stokes_solver_inputs(g)
+    try:
+        sfhip.set_pt_fusion(nx, ny, 4, 0)
+        R, _, st = gpu_solve(S, nx, ny, g["g_y"], g["dx"], g["dy"], 0.0, g["maxdisp"], True, g["eps"], 100000)
+    finally:
+        sfhip.set_pt_fusion(nx, ny, 0, 0)
+    exp = O.solve_stokes(S, g["g_y"], g["dx"], g["dy"], nx, ny, 0.0, g["maxdisp"], True, g["eps"])
+    assert st["niter"] == exp["niter"]
+    for k in S:
+        assert np.array_equal(R[k], S[k]), k
+
+
 def test_triple_pass_solve_golden(gpu, golden):
     """A full solve with three iterations per pass is bitwise the oracle's (checks, dt refreshes
     and the timing mark split the passes)."""
@@ -112,7 +146,8 @@ def test_triple_pass_solve_golden(gpu, golden):
 
 
 def test_pair_and_single_launch_agree(gpu):
-    """Forcing one iteration per launch gives the same bits as the default two-iteration passes."""
+    """Forcing one iteration per launch gives the same bits as the size-default passes (four
+    iterations each at this size)."""
     nx, ny = 97, 83
     S = cases.random_state(nx, ny, seed=5)
     try:
@@ -371,13 +406,13 @@ def test_pass_timer_counts_launches_and_keeps_results(gpu):
 
 
 def test_autotune_keeps_results(gpu, golden):
-    """The autotuner only picks launch shapes (one-iteration kernel and 2- or 3-iteration passes):
+    """The autotuner only picks launch shapes (one-iteration kernel and 2- to 4-iteration passes):
     iterations after tuning are bitwise the oracle's."""
     try:
         best, table = sfhip.autotune_pt(63, 47, iters=5)
         singles = [t for t in table if t["iters_per_launch"] == 1]
         passes = [t for t in table if t["iters_per_launch"] > 1]
-        assert len(singles) == 20 and {t["iters_per_launch"] for t in passes} == {2, 3}
+        assert len(singles) == 20 and {t["iters_per_launch"] for t in passes} == {2, 3, 4}
         assert all(t["rows"] <= 47 for t in passes)
         assert best["us_per_iter"] == min(t["us_per_iter"] for t in table)
         S = cases.smooth_state(63, 47, seed=4)
