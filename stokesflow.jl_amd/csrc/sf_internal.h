@@ -73,7 +73,8 @@ hipError_t launch_pt_fused(const PTParams& p, hipStream_t s);
 // dvx_o, dvy_o); p.seg = rows per tile (<= 0: default).  Bit-identical to two launch_pt_fused.
 hipError_t launch_pt_pair(const PTParams& p, hipStream_t s);
 int pt2_default_rows(int nx, int ny);
-// k = 2 or 3 PT iterations per pass (k = 3: ptk_tile<3>, 58 output columns per strip).
+// k = 2, 3 or 4 PT iterations per pass (k >= 3: ptk_tile<k>, 64 - 2k output columns per strip;
+// p.seg <= 0: ptk_default_rows).
 hipError_t launch_pt_multi(const PTParams& p, int k, hipStream_t s);
 // Snapshot staging: dst = (float)(src + shift) (sf_snapshot.hip).
 hipError_t launch_snap_cast(const double* src, float* dst, int64_t n, double shift, hipStream_t s);
