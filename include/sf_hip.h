@@ -83,7 +83,7 @@ int sf_ctx_set_pt_tuning(sf_ctx *ctx, int rows, int depth, int max_waves);
  * iterations (no check iteration, no dt refresh before the last of them, not the last
  * iteration or the first timed one) as one pass that reads and writes the state once (where
  * they do not fit, shorter passes); 1 runs one iteration per launch; 0 is the size default (4
- * above 96 nodes per side, else 2).  The FP32 sweeps of sf_solve_stokes_refined take the same
+ * above 448 nodes per side, else 2).  The FP32 sweeps of sf_solve_stokes_refined take the same
  * setting with 5 also allowed (the FP64 passes cap it at 4); their default is 5 above 3072
  * nodes per side, 4 above 1536, 3 above 768, else 2.
  * rows: grid rows per tile of the pass (0 = size default).  Results are bit-identical either

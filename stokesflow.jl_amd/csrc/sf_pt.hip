@@ -284,11 +284,14 @@ hipError_t launch_pt_pair(const PTParams& p0, hipStream_t s)
     return hipGetLastError();
 }
 
-// Iterations per pass by grid size (pass timer, us per iteration, best tile height of each;
-// K = 4 / 3 / 2): 4096^2 108.7 / 129.1 / 190.4; 2048^2 30.0 / 34.3 / 50.7; 1024^2 11.2 / 11.5 / 12.7;
-// 512^2 5.5 / 6.7 / 6.4; 256^2 3.79 / 3.83 / 4.07; 128^2 3.56 / 3.57 / 4.23; 64^2 3.52 / 3.54 / 3.34.
-// Four iterations per pass from 97 nodes per side, two below (where a pass is mostly launch).
-int pt_default_fusion(int nx, int ny) { return std::max(nx, ny) > 96 ? 4 : 2; }
+// Iterations per pass by grid size.  Pass timer, us per iteration at each K's best tile height,
+// K = 4 / 3 / 2: 4096^2 108.7 / 129.1 / 190.4; 2048^2 30.0 / 34.3 / 50.7; 1024^2 11.2 / 11.5 / 12.7.
+// Small grids are launch- and latency-bound, where the pass timer's own events weigh on the
+// shorter passes, so there the solver's timing window decides (us per iteration, 2000 iterations,
+// K = 4 / 3 / 2): 1024^2 9.70 / 9.80 / 11.90; 768^2 7.22 / 7.25 / 8.49; 512^2 4.96 / 5.03 / 5.53;
+// 384^2 4.09 / 4.02 / 3.96; 256^2 3.18 / 3.04 / 3.00; 128^2 3.03 / 2.85 / 2.89; 64^2 3.05 / 2.87
+// / 2.79.  Four iterations per pass from 449 nodes per side, two below.
+int pt_default_fusion(int nx, int ny) { return std::max(nx, ny) > 448 ? 4 : 2; }
 
 // Sweeps per pass of the FP32 correction solver when the context leaves it open (three from 769
 // nodes per side; us per sweep, K=3 against K=2: 1024^2 7.82 (8-row tiles) against 8.8, 1280^2

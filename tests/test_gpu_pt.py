@@ -146,7 +146,7 @@ def test_triple_pass_solve_golden(gpu, golden):
 
 
 def test_pair_and_single_launch_agree(gpu):
-    """Forcing one iteration per launch gives the same bits as the size-default passes (four
+    """Forcing one iteration per launch gives the same bits as the size-default passes (two
     iterations each at this size)."""
     nx, ny = 97, 83
     S = cases.random_state(nx, ny, seed=5)
