@@ -204,6 +204,13 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         bool checking;
         int k;            // iterations of this step's launch (1 to 4)
         int64_t nlaunch;  // launches so far: their parity says where the state (V, P, dV) is
+        // A check's err is read one step later: the host waits for the step before the one it
+        // has just enqueued, so the GPU never drains at a check.  Meanwhile the instance may run
+        // one plain pass (no check, no last iteration, no dt refresh); it reads the checked
+        // state and writes the other ping-pong buffers, so if the check converged, the checked
+        // state is still intact and the pass is dropped.
+        bool pend, skip;
+        int64_t chk_iter, chk_nlaunch;  // iter and nlaunch right after the pending check
     };
     std::vector<Run> act;
     size_t q = 0;
@@ -213,7 +220,7 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         const sf_batch_instance& a = in[id];
         const int mn = std::min(a.nx, a.ny);
         Run r{id, 1, std::min<int64_t>(5 * (int64_t)mn, 2000), 10 * (int64_t)mn, 2.0 * a.eps,
-              batch_working_set(a.nx, a.ny), false, 1, 0};
+              batch_working_set(a.nx, a.ny), false, 1, 0, false, false, 0, 0};
         HIPCHK(launch_set_scalar(desc[(size_t)id].scal + SC_DT, a.use_fss ? a.dt : 0.0, s));
         act.push_back(r);
         ws_active += r.ws;
@@ -246,6 +253,17 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         return SF_OK;
     };
     int rc = SF_OK;
+    struct Events {  // one per step parity: recorded after a step's check reads
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    for (int k = 0; k < 2; ++k) HIPCHK(hipEventCreateWithFlags(&ev.e[k], hipEventDisableTiming));
+    bool step_checked[2] = {false, false};
+    int64_t step = 0;
     WideArgs wa, w2, w3, w4;
     std::memset(&wa, 0, sizeof(wa));
     std::memset(&w2, 0, sizeof(w2));
@@ -286,6 +304,10 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
             for (int kk = kmax; kk >= 2 && r.k == 1; --kk)
                 if (fusable(kk)) r.k = kk;
             r.checking = check;
+            // while its check is pending an instance runs only a plain pass
+            r.skip = r.pend && (check || last || r.iter > a.iter_max ||
+                                (a.use_fss && (r.iter + r.k - 1) % r.ndtupdate == 0));
+            if (r.skip) continue;
             const int par = (int)(r.nlaunch & 1);
             if (r.k == 4) {
                 w4.tile_start[w4.n_active] = tiles4;
@@ -324,6 +346,7 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
         if (wa.n_active) HIPCHK(launch_wide_pt(wa, depth, s));
         bool any_check = false;
         for (Run& r : act) {
+            if (r.skip) continue;
             const sf_batch_instance& a = in[r.id];
             const BatchInstance& d = desc[(size_t)r.id];
             const PTParams& p = d.wpt[r.nlaunch & 1];  // its outputs are now the current state
@@ -376,16 +399,41 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
             HIPCHK(hipMemcpyAsync(env.hscal + (size_t)SC_COUNT * r.id, d.scal, SC_COUNT * sizeof(double),
                                   hipMemcpyDeviceToHost, s));
         }
-        if (any_check) HIPCHK(hipStreamSynchronize(s));
-        // advance, retire converged / capped instances
+        const int sp = (int)(step & 1);
+        step_checked[sp] = any_check;
+        if (any_check) HIPCHK(hipEventRecord(ev.e[sp], s));
+        // the previous step's checks: wait for that step only (this one stays queued)
+        if (step_checked[sp ^ 1]) HIPCHK(hipEventSynchronize(ev.e[sp ^ 1]));
+        step_checked[sp ^ 1] = false;
+        // advance, resolve the previous step's checks, retire converged / capped instances
         std::vector<Run> keep;
         keep.reserve(act.size());
         for (Run& r : act) {
-            if (r.checking) r.err = env.hscal[(size_t)SC_COUNT * r.id + SC_ERR];
-            r.iter += r.k;
-            r.nlaunch += 1;
             const sf_batch_instance& a = in[r.id];
-            if (r.err > a.eps && r.iter <= a.iter_max) {
+            const bool was_pending = r.pend;
+            if (!r.skip) {
+                r.iter += r.k;
+                r.nlaunch += 1;
+            }
+            if (was_pending) {
+                r.err = env.hscal[(size_t)SC_COUNT * r.id + SC_ERR];
+                r.pend = false;
+                if (r.err <= a.eps || r.chk_iter > a.iter_max) {
+                    Run f = r;  // the checked state; a plain pass after it is dropped
+                    f.iter = r.chk_iter;
+                    f.nlaunch = r.chk_nlaunch;
+                    rc = finish(f);
+                    if (rc) return rc;
+                    ws_active -= r.ws;
+                    continue;
+                }
+            }
+            if (!r.skip && r.checking) {
+                r.pend = true;
+                r.chk_iter = r.iter;
+                r.chk_nlaunch = r.nlaunch;
+                keep.push_back(r);
+            } else if (r.iter <= a.iter_max) {
                 keep.push_back(r);
             } else {
                 rc = finish(r);
@@ -394,6 +442,7 @@ int batch_run_wide(const WideEnv& env, const sf_batch_instance* in, int n,
             }
         }
         act.swap(keep);
+        ++step;
     }
     HIPCHK(hipStreamSynchronize(s));
     for (int i = 0; i < n; ++i) {
