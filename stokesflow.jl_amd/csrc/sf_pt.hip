@@ -114,15 +114,23 @@ __global__ __launch_bounds__(256) void reduce_abs_partials(RedParams p)
 }
 
 // dt = maxdisp * min(dx / max|Vx|, dy / max|Vy|)  (compute_dt, :118-121)
+// One wave: each lane takes the maximum of every 64th block partial, then the lanes combine.
+// jl_max is commutative and associative (NaN wins, +0 beats -0 either way round), so the result
+// does not depend on the order, and equals a serial pass over the partials.
 __global__ void finalize_dt(const double* __restrict__ part, int nb, double dx, double dy, double maxdisp,
                             double* __restrict__ scal, int update_dt)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int lane = threadIdx.x;
     double mx = -__builtin_inf(), my = -__builtin_inf();
-    for (int b = 0; b < nb; ++b) {
+    for (int b = lane; b < nb; b += 64) {
         mx = jl_max(mx, part[b]);
         my = jl_max(my, part[nb + b]);
     }
+    for (int d = 32; d >= 1; d >>= 1) {
+        mx = jl_max(mx, __shfl_xor(mx, d, 64));
+        my = jl_max(my, __shfl_xor(my, d, 64));
+    }
+    if (lane != 0) return;
     const double dtc = maxdisp * jl_min(dx / mx, dy / my);
     scal[SC_DT_CHECK] = dtc;
     if (update_dt) scal[SC_DT] = dtc;
