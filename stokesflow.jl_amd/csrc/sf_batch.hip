@@ -258,9 +258,9 @@ hipError_t launch_wide_pt(const WideArgs& a, int depth, hipStream_t s)
 }
 
 // K = 3 or 4 PT iterations of every active instance per launch (ptk_tile<K>; descriptor
-// wpt[2K - 2 + parity]).  K = 3 runs the split loop form (214 VGPRs); K = 4 the plain one (238
-// VGPRs; its split form needs 260, past the two waves per SIMD).
-template <int K, bool SPLIT>
+// wpt[2K - 2 + parity]).  K = 3 runs the split loop form with two middle steps per trip (214
+// VGPRs), K = 4 with one (252 VGPRs; two would need 260, past two waves per SIMD).
+template <int K, int SPLIT>
 __global__ __launch_bounds__(64) void wide_ptk_kernel(WideArgs a)
 {
     const int total = a.tile_start[a.n_active];
@@ -283,7 +283,7 @@ hipError_t launch_wide_pt3(const WideArgs& a, hipStream_t s)
 {
     const int total = a.tile_start[a.n_active];
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL((wide_ptk_kernel<3, true>), dim3(total), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((wide_ptk_kernel<3, 2>), dim3(total), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
@@ -291,7 +291,7 @@ hipError_t launch_wide_pt4(const WideArgs& a, hipStream_t s)
 {
     const int total = a.tile_start[a.n_active];
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL((wide_ptk_kernel<4, false>), dim3(total), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((wide_ptk_kernel<4, 1>), dim3(total), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

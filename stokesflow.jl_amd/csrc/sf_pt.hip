@@ -60,12 +60,13 @@ __global__ __launch_bounds__(64) void pt2_tile_f64(PTParams p)
     pt2_tile(p, logical % p.nstrips, logical / p.nstrips);
 }
 
-// K PT iterations per pass, one wave per tile (sf_ptk_tile.h).
+// K PT iterations per pass, one wave per tile (sf_ptk_tile.h): the plain loop form for K = 3,
+// the split form with one middle step per trip for K = 4.
 template <int K>
 __global__ __launch_bounds__(64) void ptk_tile_f64(PTParams p)
 {
     const int logical = xcd_remap(blockIdx.x, gridDim.x);
-    ptk_tile<K, false>(p, logical % p.nstrips, logical / p.nstrips);
+    ptk_tile<K, K == 4 ? 1 : 0>(p, logical % p.nstrips, logical / p.nstrips);
 }
 
 // compute_ResY with post-update V and dt_check (compute_Residuals!, :241-249).

@@ -10,15 +10,17 @@
 // the free-slip copy (Vx row ny = row ny-1, Vy zero).  All stages evaluate pt_row, so the
 // stored iteration k+K is bit-identical to K pt_tile launches.  dV goes to dvx_o / dvy_o.
 //
-// Two loop forms, with the same results.  The plain form runs every outer step through one
-// body that checks each stage's row range, starts windows and runs the virtual row.  The
-// split form (SPLIT) runs those checks only on the steps at the ends of the tile, and the
-// steps in between, where every stage has a row of its range other than its first and no
-// stage reaches row ny-1, through a body without them, two steps per loop trip: the
-// hand-over then needs no copies between loop-carried registers, about a quarter fewer VALU
-// instructions per step.  The batched three-iteration pass over many small tiles gains from it
-// (5% on a mixed ensemble of 64-512 node grids); the 2048^2-4096^2 single pass does not, and
-// keeps the plain form, as does every four-iteration pass (the split form needs 260 VGPRs).
+// Three loop forms, with the same results.  The plain form (SPLIT = 0) runs every outer step
+// through one body that checks each stage's row range, starts windows and runs the virtual row.
+// The split forms run those checks only on the steps at the ends of the tile, and the steps in
+// between, where every stage has a row of its range other than its first and no stage reaches
+// row ny-1, through a body without them: one step per loop trip (SPLIT = 1) or two (SPLIT = 2,
+// which also spares the copies between loop-carried registers at the back edge).  Measured
+// (same box, us per iteration; 256-instance wide FP64 solve): four iterations per pass, one step
+// per trip (251-252 VGPRs) against plain (238): 4096^2 106.5 / 108.7, 2048^2 28.2 / 30.0, 512^2
+// 5.25 / 5.53, wide 12.84 / 13.74 s (two steps per trip need 260 VGPRs, past two waves per
+// SIMD); three iterations per pass, two steps per trip (214 VGPRs): the wide solve 5% faster,
+// the 2048^2-4096^2 single pass no faster, so the single three-iteration pass stays plain.
 #pragma once
 
 #include "sf_pt2_tile.h"
@@ -31,7 +33,7 @@ struct PTHand {
     PTTau tau;                                   // the row's pseudo-time steps (stage 0's)
 };
 
-template <int K, bool SPLIT, class PP>
+template <int K, int SPLIT, class PP>
 __device__ __forceinline__ void ptk_tile(const PP& p, int strip, int segid)
 {
     constexpr int OUT = 64 - 2 * K;
@@ -75,7 +77,7 @@ __device__ __forceinline__ void ptk_tile(const PP& p, int strip, int segid)
                 (p.rho + (int64_t)r * (nx + 1))[lc.vy]);
     RowIn nxt = load_row(p, lc, r, r >= r0 - (K - 1));
 
-    if constexpr (!SPLIT) {
+    if constexpr (SPLIT == 0) {
         // The plain form: edge_step's body written out in the loop (called through the lambda
         // the same code ran 3% slower at 2048^2-4096^2, from a different scalar layout).
         for (; r <= r_last; ++r) {
@@ -285,11 +287,15 @@ __device__ __forceinline__ void ptk_tile(const PP& p, int strip, int segid)
             r_mid1 = min(r_mid1, rs1[s] + s);
         }
         for (; r < r_mid0 && r <= r_last; ++r) edge_step(r);
-        for (; r + 1 <= r_mid1; r += 2) {
-            mid_step(r);
-            mid_step(r + 1);
+        if constexpr (SPLIT == 2) {
+            for (; r + 1 <= r_mid1; r += 2) {
+                mid_step(r);
+                mid_step(r + 1);
+            }
+            if (r <= r_mid1) mid_step(r++);
+        } else {
+            for (; r <= r_mid1; ++r) mid_step(r);
         }
-        if (r <= r_mid1) mid_step(r++);
         for (; r <= r_last; ++r) edge_step(r);
     }
 }
