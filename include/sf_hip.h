@@ -96,7 +96,8 @@ int sf_ctx_set_pt_fusion(sf_ctx *ctx, int iters_per_launch, int rows);
  * modified) or, when any of them is NULL, on a synthetic one (materials 1, state 0), in scratch
  * it allocates and frees.  Candidates: the one-iteration kernel (rows 1, 2, 4, 8, 16 x depth
  * 1, 2 x one wave per tile or persistent; iters_per_launch = 1), then passes of 2, 3 and 4
- * iterations (rows 2, 4, 8, 12, 16, 24, 32, 48).  The fastest one-iteration shape is set on the
+ * iterations (rows 2, 4, 8, 12, 16, 24, 32, 48, 64 and the pass kind's size default; at most 50
+ * candidates in all).  The fastest one-iteration shape is set on the
  * context (check iterations use it), and the fastest overall decides the iterations per
  * launch and the pass rows of the FP64 solver (sf_solve_stokes, sf_pt_iterations); the FP32
  * sweeps of sf_solve_stokes_refined keep their own size defaults (or the explicit

@@ -792,9 +792,12 @@ extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, const double* P_in, cons
         c->seg = bestv.rows;
         c->depth = bestv.depth;
         c->max_waves = bestv.max_waves;
-        const int prows_c[] = {2, 4, 8, 12, 16, 24, 32, 48};
+        // fixed tile heights plus each pass kind's size default (the wave-round model's pick)
         const int n_it = std::max(12, (iters + 11) / 12 * 12);  // whole passes of every kind
-        for (int k : {2, 3, 4})
+        for (int k : {2, 3, 4}) {
+            std::vector<int> prows_c = {2, 4, 8, 12, 16, 24, 32, 48, 64};
+            const int dflt = k == 2 ? pt2_default_rows(nx, ny) : ptk_default_rows(k, nx, ny);
+            if (std::find(prows_c.begin(), prows_c.end(), dflt) == prows_c.end()) prows_c.push_back(dflt);
             for (int rows : prows_c) {
                 if (rows > ny && rows != prows_c[0]) continue;
                 c->fuse64 = k;
@@ -805,6 +808,7 @@ extern "C" int sf_ctx_autotune_pt(sf_ctx* c, int iters, const double* P_in, cons
                 record(t);
                 if (t.us_per_iter < best_all.us_per_iter) best_all = t;
             }
+        }
         return SF_OK;
     }();
     (void)hipStreamSynchronize(s);

@@ -317,17 +317,18 @@ int corr_default_fusion(int nx, int ny)
 
 //
 // The tile height of a K-iteration pass comes from a wave-round model.  A launch runs in rounds of
-// resident waves (two per SIMD: 198 VGPRs at K = 3, 238 at K = 4) and a partly filled last round
-// costs a whole one, so the height minimises rounds x (rows + 2K - 1 halo rows); a launch of at most
-// one wave per SIMD runs each wave 1.38x faster (measured at 512^2, K = 4: 6-row tiles, 860 waves,
-// 1.70 us per row step; 4-row tiles, 1280 waves, 2.35 us), which is what small grids reach with
-// short tiles.  Candidates: 2-12 rows in steps of 2, then 16-64 in steps of 4.  Measured (pass
-// timer, us per iteration, model pick first), K = 4: 4096^2 52 rows 108.7, 44 111.7, 40 112.2, 60
-// 110.1, 48 118.1; 3584^2 56 rows 80.2, 28 84.6; 3072^2 44 rows 63.5, 28 64.2, 24 66.5; 2560^2 60
-// rows 43.5, 32 47.7; 2048^2 40 rows 30.0, 16 34.7, 48 34.1; 1536^2 24 rows 18.2, 16 19.8; 1280^2
-// 16 rows 13.5, 8 16.0; 1024^2 12 rows 11.2, 24 12.6; 768^2 12 rows 7.9, 8 8.9; 512^2 6 rows 5.5,
-// 8 6.3, 4 6.5, 12 7.9; 64^2-256^2 2 rows.  K = 3 in the solver loop: 4096^2 48 rows 136.0, 32
-// 141.8; 3072^2 40 rows 74.0, 28 76.7; 2048^2 40 rows 34.4, 20 36.6; 1536^2 24 rows 19.7, 12 21.2.
+// resident waves (two per SIMD: 198 VGPRs at K = 3, 251 at K = 4) and a partly filled last round
+// costs a whole one, so the height minimises rounds x (rows + 2K - 1 halo rows); a launch of at
+// most one wave per SIMD runs each wave 1.38x faster (measured at 512^2, K = 4: 6-row tiles, 860
+// waves, 1.70 us per row step; 4-row tiles, 1280 waves, 2.35 us), which is what small grids reach
+// with short tiles.  Candidates: 2-12 rows in steps of 2, then 16-64 in steps of 4.  Measured (pass
+// timer, us per iteration, model pick first), K = 4 in the plain loop form: 4096^2 52 rows 108.7,
+// 44 111.7, 40 112.2, 60 110.1, 48 118.1; 3584^2 56 rows 80.2, 28 84.6; 3072^2 44 rows 63.5, 28
+// 64.2, 24 66.5; 2560^2 60 rows 43.5, 32 47.7; 2048^2 40 rows 30.0, 16 34.7, 48 34.1; 1536^2 24
+// rows 18.2, 16 19.8; 1280^2 16 rows 13.5, 8 16.0; 1024^2 10 rows 10.1, 12 11.2, 8 12.7; 768^2 6
+// rows 7.75, 12 7.9, 8 8.9; 512^2 6 rows 5.5, 8 6.3, 4 6.5, 12 7.9; 64^2-256^2 2 rows.  K = 3 in
+// the solver loop: 4096^2 48 rows 136.0, 32 141.8; 3072^2 40 rows 74.0, 28 76.7; 2048^2 40 rows
+// 34.4, 20 36.6; 1536^2 24 rows 19.7, 12 21.2.
 int ptk_default_rows(int k, int nx, int ny)
 {
     static const int simds = [] {
