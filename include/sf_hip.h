@@ -228,6 +228,12 @@ int sf_solve_stokes_refined(sf_ctx *ctx, double *P, double *Vx_pad, double *Vy_p
  * WIDE: one launch per PT iteration covers every instance of an active set whose working
  * sets fit budget_mb; each instance is checked at its own ncheck iterations and retired on
  * convergence, and the set is refilled from the largest-first queue.
+ * GRID: the instances one after another, each solved by ONE launch whose waves span the chip
+ * (at most as many one-wave workgroups as are resident at once): the whole PT loop with its
+ * checks, dt refreshes and convergence test runs on the device, the passes (up to three PT
+ * iterations each, or prm->depth) separated by a grid barrier;
+ * no host synchronisation, so it can be captured (sf_graph_*).  A barrier wait longer than 5 s
+ * (waves not all resident) ends the launch and the call returns SF_ERR_HIP.
  * With precision 1 (WIDE only) every instance is solved as sf_solve_stokes_refined solves it,
  * with one launch per FP32 sweep of all active instances.
  * In FP64 every PT iteration is bit-identical to sf_solve_stokes.  outs[i] gets instance i's dt_new,
@@ -243,10 +249,11 @@ typedef struct {
     int reserved;
 } sf_batch_instance;
 
-enum { SF_BATCH_AUTO = 0, SF_BATCH_PERSISTENT = 1, SF_BATCH_WIDE = 2 };
+enum { SF_BATCH_AUTO = 0, SF_BATCH_PERSISTENT = 1, SF_BATCH_WIDE = 2, SF_BATCH_GRID = 3 };
 typedef struct {
     int rows;         /* grid rows per tile (<= 0: per instance, as the single solver's default) */
-    int depth;        /* wide: rows of operands prefetched, 1 or 2 (<= 0: 1) */
+    int depth;        /* wide: rows of operands prefetched, 1 or 2 (<= 0: 1); grid: PT iterations per
+                         pass, 1 to 3 (<= 0: 3) */
     int threads;      /* persistent: threads per instance workgroup, multiple of 64, <= 768 (<= 0: 768) */
     int max_blocks;   /* persistent: cap on concurrently resident instances (<= 0: as many as fit) */
     int mode;         /* SF_BATCH_*; AUTO = WIDE */
@@ -352,7 +359,9 @@ int sf_step(sf_ctx *ctx, sf_step_args *args, sf_solve_out *solve_out, sf_m2g_out
  * and RK4 per member with the dt that the solve produced, read on the device -- the loop body
  * of scripts/StokesFlow2D_multixpu.jl:164-181 for each member.
  * prm->mode PERSISTENT: the whole step is one hipGraph (the persistent batched kernel in the
- * middle), replayed with no host synchronisation; best for small members.  WIDE (= AUTO):
+ * middle), replayed with no host synchronisation; best for small members.  GRID: the same
+ * one-graph step with one chip-wide grid-mode solve per member (sf_solve_stokes_batched's GRID
+ * mode, prm->depth PT iterations per pass).  WIDE (= AUTO):
  * the marker -> grid passes and the RK4 passes of all members are two hipGraphs, and the
  * wide batched solve runs between them; best when members reach 256^2 and beyond.
  * Without use_graph the same kernels are enqueued every step (identical results).  The
@@ -365,6 +374,21 @@ int sf_ensemble_create(sf_ctx *ctx, const sf_step_args *members, int n, const sf
                        sf_ensemble **ens);
 int sf_ensemble_step(sf_ensemble *ens, int n_steps, sf_solve_out *outs, double *dt);
 int sf_ensemble_destroy(sf_ensemble *ens);
+
+/* Captured timestep of ONE problem -- the loop body of scripts/StokesFlow2D_multixpu.jl:164-181
+ * (the three bilinearMarkerToGrid! calls, solveStokes!, moveMarkersRK4!) as one hipGraph: the
+ * solve runs in the GRID mode of sf_solve_stokes_batched (one launch over the whole chip with the
+ * PT loop, its checks and its convergence test on the device), and RK4 reads the new dt on the
+ * device, so sf_graph_step replays n_steps timesteps with no host synchronisation in between.
+ * args as for sf_step (its dt is the first step's input dt); iters_per_pass 1 to 3 (0: 3).
+ * Fields, markers and dt are bit-identical to n_steps sf_step calls; err agrees to rounding
+ * (its sum runs in another order).  The arrays must outlive the graph.  out / dt (may be NULL)
+ * receive the last step's solve results and the dt after it.  Replaces sf_step's host loop; the
+ * reference has no counterpart (its loop is host-driven, StokesFlow2D_multixpu.jl:157-218). */
+typedef struct sf_graph sf_graph;
+int sf_graph_create(sf_ctx *ctx, const sf_step_args *args, int iters_per_pass, sf_graph **g);
+int sf_graph_step(sf_graph *g, int n_steps, sf_solve_out *out, double *dt);
+int sf_graph_destroy(sf_graph *g);
 
 /* Seeded marker set-up: md x md markers per cell, x outer / y inner (the order of
  * common.jl:76-83), uniform jitter of +-jitter_frac cells from splitmix64(seed, index),

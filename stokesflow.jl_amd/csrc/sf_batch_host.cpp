@@ -183,6 +183,40 @@ int batch_threads(const sf_batch_params* prm, int* threads, int* rows, const cha
     return SF_OK;
 }
 
+// Grid mode: the single solver's tile heights for every pass kind (one instance fills the chip).
+BatchShape grid_shape(int nx, int ny)
+{
+    BatchShape sh;
+    sh.kmax = 3;
+    sh.rows2 = std::min(pt2_default_rows(nx, ny), ny);
+    sh.rows3 = std::min(ptk_default_rows(3, nx, ny), ny);
+    sh.rows4 = std::min(ptk_default_rows(4, nx, ny), ny);
+    return sh;
+}
+
+// Grid mode: waves of an instance's launch, one per tile of its largest pass, at most all
+// resident ones.
+int grid_waves(const BatchInstance& d, int k)
+{
+    int t = d.wpt[0].ntiles;
+    if (k >= 2) t = std::max(t, d.wpt[2 * k - 2].ntiles);
+    return std::max(1, std::min(t, grid_capacity(k)));
+}
+
+// Grid mode: the barrier words (two per instance) and the partials of the context.
+static int grid_reserve(sf_ctx* c, int n)
+{
+    if (!c->g_part) HIPCHK(dalloc(&c->g_part, 3 * (size_t)GRID_MAX_WAVES));
+    if (n > c->g_cap) {
+        if (c->g_sync) HIPCHK(hipFree(c->g_sync));
+        c->g_sync = nullptr;
+        c->g_cap = 0;
+        HIPCHK(dalloc(&c->g_sync, 2 * (size_t)n));
+        c->g_cap = n;
+    }
+    return SF_OK;
+}
+
 // Bytes an instance keeps live while it iterates: both ping-pong copies of Vx, Vy, P, plus
 // dVx, dVy and the three material grids.
 static double batch_working_set(int nx_, int ny_)
@@ -927,9 +961,13 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
         HIPCHK(dalloc(&c->b_res, (size_t)n));
         c->b_cap = n;
     }
+    const int mode = prm ? prm->mode : SF_BATCH_AUTO;
+    const int precision = prm ? prm->precision : 0;
     std::vector<BatchInstance> host((size_t)n);
     for (int i = 0; i < n; ++i)
-        host[(size_t)i] = batch_descriptor(in[i], rows, c->b_pool + off[(size_t)i], nullptr, batch_shape(c, in[i].nx, in[i].ny));
+        host[(size_t)i] = batch_descriptor(in[i], rows, c->b_pool + off[(size_t)i], nullptr,
+                                           mode == SF_BATCH_GRID ? grid_shape(in[i].nx, in[i].ny)
+                                                                 : batch_shape(c, in[i].nx, in[i].ny));
     std::vector<std::pair<int, int>> dims((size_t)n);
     for (int i = 0; i < n; ++i) dims[(size_t)i] = {in[i].nx, in[i].ny};
     const std::vector<int> order = batch_order(dims);
@@ -943,10 +981,8 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
         HIPCHK(hipHostMalloc((void**)&c->b_hscal, sizeof(double) * SC_COUNT * (size_t)n, hipHostMallocDefault));
         c->b_hscal_cap = n;
     }
-    const int mode = prm ? prm->mode : SF_BATCH_AUTO;
-    const int precision = prm ? prm->precision : 0;
     if (precision != 0 && precision != 1) return set_err(SF_ERR_ARG, "%s: precision %d not 0 or 1", who, precision);
-    if (precision == 1 && mode == SF_BATCH_PERSISTENT)
+    if (precision == 1 && (mode == SF_BATCH_PERSISTENT || mode == SF_BATCH_GRID))
         return set_err(SF_ERR_ARG, "%s: FP32 refinement runs in the wide mode only", who);
     if (precision == 1) {
         // the refined working sets are FP32; 256 mixed 64..512 instances: 11.10 s at 256 MiB, 11.07 s at
@@ -977,6 +1013,25 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
         HIPCHK(launch_batch_solve(c->b_inst, c->b_order, n, c->b_queue, c->b_res, threads, nblocks, s));
         HIPCHK(hipMemcpyAsync(res.data(), c->b_res, sizeof(BatchResult) * (size_t)n, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+    } else if (mode == SF_BATCH_GRID) {
+        // one instance after another, each solved by one launch over the whole chip
+        rc = grid_reserve(c, n);
+        if (rc) return rc;
+        const int k = (prm && prm->depth >= 1 && prm->depth <= 3) ? prm->depth : 3;
+        for (int qi = 0; qi < n; ++qi) {
+            const int i = order[(size_t)qi];
+            const GridArgs ga{c->b_inst + i, c->b_res + i, c->g_sync + 2 * (size_t)i, c->g_part,
+                              grid_waves(host[(size_t)i], k), 0, grid_wait_limit()};
+            HIPCHK(launch_grid_solve(ga, k, s));
+        }
+        std::vector<unsigned long long> sync(2 * (size_t)n);
+        HIPCHK(hipMemcpyAsync(res.data(), c->b_res, sizeof(BatchResult) * (size_t)n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(sync.data(), c->g_sync, sizeof(unsigned long long) * 2 * (size_t)n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int i = 0; i < n; ++i)
+            if (sync[2 * (size_t)i + 1])
+                return set_err(SF_ERR_HIP, "%s: instance %d: a grid barrier wait exceeded %d s (waves not all resident?)",
+                               who, i, (int)GRID_WAIT_SECONDS);
     } else if (mode == SF_BATCH_WIDE || mode == SF_BATCH_AUTO) {
         // a working set past the Infinity Cache still pays with multi-iteration passes: 256 mixed
         // 64..512 instances, one box, one process: 17.69 s at 256 MiB, 16.93 at 384, 16.24 at 768, 15.35-

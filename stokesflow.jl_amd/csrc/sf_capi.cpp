@@ -42,7 +42,8 @@ static void free_ctx(sf_ctx* c)
         if (b) (void)hipFree(b);
     for (void* b : {(void*)c->b_hpart, (void*)c->b_hdt})
         if (b) (void)hipHostFree(b);
-    for (void* b : {(void*)c->b_pool, (void*)c->b_inst, (void*)c->b_order, (void*)c->b_queue, (void*)c->b_res})
+    for (void* b : {(void*)c->b_pool, (void*)c->b_inst, (void*)c->b_order, (void*)c->b_queue, (void*)c->b_res,
+                    (void*)c->g_sync, (void*)c->g_part})
         if (b) (void)hipFree(b);
     if (c->hscal) (void)hipHostFree(c->hscal);
     if (c->hcounters) (void)hipHostFree(c->hcounters);

@@ -1,6 +1,7 @@
 // sf_ensemble.cpp -- ensemble timesteps (sf_ensemble_*): the StokesFlow2D loop body for many
 // independent problems, with the marker passes and RK4 captured in hipGraphs around one
-// batched solve (or the whole step in one graph with the persistent batched kernel).
+// batched solve (or the whole step in one graph with the persistent batched kernel, or with one
+// chip-wide grid-mode solve per member), and the captured timestep of one problem (sf_graph_*).
 #include "sf_host.h"
 
 using namespace sf;
@@ -10,6 +11,7 @@ struct sf_ensemble {
     hipStream_t stream = nullptr;  // own non-blocking stream: capture needs one
     hipEvent_t ev = nullptr;
     int n = 0, threads = 768, nblocks = 0, use_graph = 1, wide = 1, depth = 1;
+    int grid = 0, gk = 3;  // grid mode: one chip-wide launch per member, gk PT iterations per pass
     double budget = 0.0;
     std::vector<sf_step_args> mem;
     std::vector<sf_batch_instance> bi;     // the members as batched-solver instances (dt updated per step)
@@ -26,6 +28,10 @@ struct sf_ensemble {
     unsigned long long* cnt = nullptr;   // per member: 3 x 4 marker -> grid counters, 4 RK4 counters
     unsigned long long* hcnt = nullptr;  // pinned
     BatchResult* hres = nullptr;         // pinned
+    unsigned long long* gsync = nullptr;  // grid mode: two barrier words per member
+    unsigned long long* hsync = nullptr;  // pinned
+    double* gpart = nullptr;              // grid mode: 3 * GRID_MAX_WAVES partials
+    std::vector<int> gwaves;              // grid mode: waves of each member's launch
     hipGraph_t graph[2] = {nullptr, nullptr};  // persistent: [0] whole step; wide: [0] marker -> grid, [1] RK4
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     M2GScratch m2g;  // own sort scratch: the captured graphs hold its addresses
@@ -50,9 +56,10 @@ static void free_ensemble(sf_ensemble* e)
         if (e->graph[k]) (void)hipGraphDestroy(e->graph[k]);
     }
     for (void* p : {(void*)e->pool, (void*)e->inst, (void*)e->order_dev, (void*)e->queue, (void*)e->res,
-                    (void*)e->cnt, (void*)e->part, (void*)e->pool32, (void*)e->rpart, (void*)e->cdesc_dev})
+                    (void*)e->cnt, (void*)e->part, (void*)e->pool32, (void*)e->rpart, (void*)e->cdesc_dev,
+                    (void*)e->gsync, (void*)e->gpart})
         if (p) (void)hipFree(p);
-    for (void* p : {(void*)e->hcnt, (void*)e->hres, (void*)e->hscal, (void*)e->hpart, (void*)e->hdt})
+    for (void* p : {(void*)e->hcnt, (void*)e->hres, (void*)e->hscal, (void*)e->hpart, (void*)e->hdt, (void*)e->hsync})
         if (p) (void)hipHostFree(p);
     if (e->ev) (void)hipEventDestroy(e->ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -107,12 +114,22 @@ static hipError_t ensemble_enqueue_rk4(sf_ensemble* e)
     return hipSuccess;
 }
 
-// Persistent form: the whole step, with the persistent batched kernel in the middle.
+// Persistent and grid forms: the whole step, with the persistent batched kernel (or one grid-mode
+// launch per member, largest first) in the middle.
 static hipError_t ensemble_enqueue_persistent(sf_ensemble* e)
 {
     hipError_t r = ensemble_enqueue_m2g(e);
     if (r != hipSuccess) return r;
-    r = launch_batch_solve(e->inst, e->order_dev, e->n, e->queue, e->res, e->threads, e->nblocks, e->stream);
+    if (e->grid) {
+        for (int q = 0; q < e->n && r == hipSuccess; ++q) {
+            const int i = e->order[(size_t)q];
+            const GridArgs ga{e->inst + i, e->res + i, e->gsync + 2 * (size_t)i, e->gpart, e->gwaves[(size_t)i], 0,
+                              grid_wait_limit()};
+            r = launch_grid_solve(ga, e->gk, e->stream);
+        }
+    } else {
+        r = launch_batch_solve(e->inst, e->order_dev, e->n, e->queue, e->res, e->threads, e->nblocks, e->stream);
+    }
     if (r != hipSuccess) return r;
     return ensemble_enqueue_rk4(e);
 }
@@ -139,7 +156,7 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
     int rc = batch_threads(prm, &threads, &rows, who);
     if (rc) return rc;
     const int mode = prm ? prm->mode : SF_BATCH_AUTO;
-    if (mode != SF_BATCH_AUTO && mode != SF_BATCH_WIDE && mode != SF_BATCH_PERSISTENT)
+    if (mode != SF_BATCH_AUTO && mode != SF_BATCH_WIDE && mode != SF_BATCH_PERSISTENT && mode != SF_BATCH_GRID)
         return set_err(SF_ERR_ARG, "%s: unknown mode %d", who, mode);
     std::vector<sf_batch_instance> bi((size_t)n);
     int64_t max_nm = 0, max_cells = 0;
@@ -191,7 +208,9 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
     e->n = n;
     e->threads = threads;
     e->use_graph = use_graph ? 1 : 0;
-    e->wide = (mode != SF_BATCH_PERSISTENT);
+    e->wide = (mode != SF_BATCH_PERSISTENT && mode != SF_BATCH_GRID);
+    e->grid = (mode == SF_BATCH_GRID);
+    e->gk = (prm && prm->depth >= 1 && prm->depth <= 3) ? prm->depth : 3;
     e->refined = prm ? prm->precision : 0;
     if (e->refined != 0 && (e->refined != 1 || !e->wide)) {
         free_ensemble(e);
@@ -214,6 +233,10 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
     if (er == hipSuccess) er = hipHostMalloc((void**)&e->hcnt, sizeof(unsigned long long) * ENS_CNT * n, hipHostMallocDefault);
     if (er == hipSuccess) er = hipHostMalloc((void**)&e->hres, sizeof(BatchResult) * n, hipHostMallocDefault);
     if (er == hipSuccess) er = hipHostMalloc((void**)&e->hscal, sizeof(double) * SC_COUNT * n, hipHostMallocDefault);
+    if (er == hipSuccess && e->grid) er = dalloc(&e->gsync, 2 * (size_t)n);
+    if (er == hipSuccess && e->grid) er = dalloc(&e->gpart, 3 * (size_t)GRID_MAX_WAVES);
+    if (er == hipSuccess && e->grid)
+        er = hipHostMalloc((void**)&e->hsync, sizeof(unsigned long long) * 2 * n, hipHostMallocDefault);
     if (er != hipSuccess) {
         free_ensemble(e);
         return set_err(SF_ERR_HIP, "%s: %s", who, hipGetErrorString(er));
@@ -223,13 +246,18 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
     std::vector<std::pair<int, int>> dims((size_t)n);
     for (int i = 0; i < n; ++i) {
         e->desc[(size_t)i] = batch_descriptor(bi[(size_t)i], rows, e->pool + off[(size_t)i], e->wide ? nullptr : &e->res[i].dt_new,
-                                              batch_shape(c, members[i].nx, members[i].ny));
+                                              e->grid ? grid_shape(members[i].nx, members[i].ny)
+                                                      : batch_shape(c, members[i].nx, members[i].ny));
         dims[(size_t)i] = {members[i].nx, members[i].ny};
         std::memset(&e->hres[i], 0, sizeof(BatchResult));
         e->hres[i].dt_new = members[i].use_fss ? members[i].dt : 0.0;
         e->dt[(size_t)i] = members[i].dt;
     }
     e->order = batch_order(dims);
+    if (e->grid) {
+        e->gwaves.resize((size_t)n);
+        for (int i = 0; i < n; ++i) e->gwaves[(size_t)i] = grid_waves(e->desc[(size_t)i], e->gk);
+    }
     e->nblocks = batch_grid(c, n, threads, prm ? prm->max_blocks : 0);
     if (e->refined) {
         size_t nf = 0;
@@ -327,10 +355,18 @@ extern "C" int sf_ensemble_step(sf_ensemble* e, int n_steps, sf_solve_out* outs,
     }
     if (!e->wide)
         HIPCHK(hipMemcpyAsync(e->hres, e->res, sizeof(BatchResult) * (size_t)e->n, hipMemcpyDeviceToHost, e->stream));
+    if (e->grid)
+        HIPCHK(hipMemcpyAsync(e->hsync, e->gsync, sizeof(unsigned long long) * 2 * (size_t)e->n, hipMemcpyDeviceToHost,
+                              e->stream));
     HIPCHK(hipMemcpyAsync(e->hcnt, e->cnt, sizeof(unsigned long long) * ENS_CNT * (size_t)e->n, hipMemcpyDeviceToHost,
                           e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    if (n_steps > 0 && e->grid)
+        for (int i = 0; i < e->n; ++i)
+            if (e->hsync[2 * (size_t)i + 1])
+                return set_err(SF_ERR_HIP, "%s: member %d: a grid barrier wait exceeded %d s (waves not all resident?)",
+                               who, i, (int)GRID_WAIT_SECONDS);
     if (n_steps > 0)
         for (int i = 0; i < e->n; ++i) {
             const unsigned long long* q = e->hcnt + (size_t)ENS_CNT * i;
@@ -361,4 +397,47 @@ extern "C" int sf_ensemble_destroy(sf_ensemble* e)
     (void)hipStreamSynchronize(e->stream);
     free_ensemble(e);
     return SF_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// The captured timestep of one problem: a one-member ensemble in the grid form, so the whole
+// step (three marker -> grid passes, the solve with its loop on the device, RK4 with the new
+// dt read on the device) is one hipGraph.
+
+struct sf_graph {
+    sf_ensemble* e = nullptr;
+};
+
+extern "C" int sf_graph_create(sf_ctx* c, const sf_step_args* args, int iters_per_pass, sf_graph** out)
+{
+    const char* who = "sf_graph_create";
+    if (!c || !args || !out) return set_err(SF_ERR_ARG, "%s: NULL argument", who);
+    *out = nullptr;
+    if (iters_per_pass < 0 || iters_per_pass > 3) return set_err(SF_ERR_ARG, "%s: iters_per_pass %d", who, iters_per_pass);
+    sf_batch_params prm;
+    std::memset(&prm, 0, sizeof(prm));
+    prm.mode = SF_BATCH_GRID;
+    prm.depth = iters_per_pass;
+    sf_graph* g = new sf_graph();
+    const int rc = sf_ensemble_create(c, args, 1, &prm, 1, &g->e);
+    if (rc) {
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return SF_OK;
+}
+
+extern "C" int sf_graph_step(sf_graph* g, int n_steps, sf_solve_out* out, double* dt)
+{
+    if (!g || !g->e) return set_err(SF_ERR_ARG, "sf_graph_step: NULL graph");
+    return sf_ensemble_step(g->e, n_steps, out, dt);
+}
+
+extern "C" int sf_graph_destroy(sf_graph* g)
+{
+    if (!g) return SF_OK;
+    const int rc = sf_ensemble_destroy(g->e);
+    delete g;
+    return rc;
 }

@@ -1,0 +1,341 @@
+// sf_grid.hip -- one problem's whole PT solve in ONE launch that spans the chip.
+//
+// solveStokes! (StokesSolver_multixpu.jl:43-95) has a data-dependent loop (convergence is tested
+// every ncheck iterations), which a hipGraph cannot hold.  Here the loop runs on the device: a
+// launch of at most as many one-wave workgroups as the chip holds at once walks the same pass
+// schedule as sf_solve_stokes (one to four PT iterations per pass, the single iterations at
+// checks, dt refreshes and iterMax), each wave taking tiles t = wave, wave + G, ... of every pass,
+// and a grid barrier separates the passes.  The dt refresh and the err check reduce over the
+// whole grid in two steps (a partial per wave, then every wave sums the partials in the same
+// order, so every wave takes the same decision).  Each tile is the code of the single solver's
+// pass kernels (sf_pt_tile.h, sf_pt2_tile.h, sf_ptk_tile.h), so every field is bit-identical to
+// sf_solve_stokes; err is a sum of the same terms in another order (equal to rounding).
+//
+// The barrier is a monotone arrival counter (zeroed by a kernel before each launch) that each
+// wave's first lane increments and then polls, with release / acquire fences at agent scope
+// around it.  The waves must all be resident, which the host ensures by sizing the launch from
+// the kernel's occupancy; should one not be, no wave waits forever: a wait longer than the time
+// limit (or a flag raised by another wave) sets the failure flag, every wave leaves the kernel,
+// and the host reports the error.
+#include "sf_ptk_tile.h"
+
+namespace sf {
+
+#define GRID_CONST __attribute__((address_space(4)))
+typedef const BatchInstance GRID_CONST CGrid;
+
+namespace {
+
+struct GridBarrier {
+    unsigned long long* cnt;   // arrivals since the launch
+    unsigned long long* fail;  // nonzero: a wait timed out, every wave leaves
+    unsigned long long target; // arrivals that complete the next barrier
+    unsigned long long nw;     // waves in the launch
+    uint64_t limit;            // wall-clock ticks a wave may wait
+
+    __device__ bool sync()
+    {
+        target += nw;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        int ok = 1;
+        if (threadIdx.x == 0) {
+            __hip_atomic_fetch_add(cnt, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t t0 = wall_clock64();
+            while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull) {
+                    ok = 0;
+                    break;
+                }
+                if (wall_clock64() - t0 > limit) {
+                    __hip_atomic_store(fail, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        ok = __shfl(ok, 0);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        return ok != 0;
+    }
+};
+
+// Sum over the wave in a fixed order; lane 0's value in every lane.
+__device__ inline double wave_sum(double v)
+{
+    for (int d = 32; d > 0; d >>= 1) v = v + __shfl_down(v, d);
+    return __shfl(v, 0);
+}
+
+__device__ inline double wave_max(double v)
+{
+    for (int d = 32; d > 0; d >>= 1) v = jl_max(v, __shfl_down(v, d));
+    return __shfl(v, 0);
+}
+
+// Every wave reduces the nw partials part[0..nw) in the same order.
+__device__ inline double partials_sum(const double* part, int nw)
+{
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nw; k += 64) s = s + part[k];
+    return wave_sum(s);
+}
+
+__device__ inline double partials_max(const double* part, int nw)
+{
+    double m = -__builtin_inf();
+    for (int k = threadIdx.x; k < nw; k += 64) m = jl_max(m, part[k]);
+    return wave_max(m);
+}
+
+// compute_Residuals! y part (StokesSolver_multixpu.jl:189-202) on the post-update state, with
+// dt = dt_check; grid-stride over the waves.
+__device__ inline void grid_resid_y(const PTParams GRID_CONST& p, double* __restrict__ ry, double dt, int64_t gtid,
+                                    int64_t gstride)
+{
+    const int nx = p.nx, ny = p.ny;
+    const int64_t lp = nx - 1, lvx = nx, lvy = nx + 1, lr = nx + 1;
+    const int64_t n = (int64_t)(nx - 1) * (ny - 2);
+    const double rdx = p.rdx, rdy = p.rdy, g = p.g;
+    const double* __restrict__ P = p.p_b;
+    const double* __restrict__ vx = p.vx_b;
+    const double* __restrict__ vy = p.vy_b;
+    const double* __restrict__ rho = p.rho;
+    const double* __restrict__ tyy = p.tyy;
+    const double* __restrict__ txy = p.txy;
+    for (int64_t k = gtid; k < n; k += gstride) {
+        const int i = (int)(k % (nx - 1)), j = (int)(k / (nx - 1));
+        const double av = 0.25 * (((vx[ix2(i, j + 1, lvx)] + vx[ix2(i + 1, j + 1, lvx)]) + vx[ix2(i, j + 2, lvx)]) +
+                                  vx[ix2(i + 1, j + 2, lvx)]);
+        const double d_xi = rho[ix2(i + 2, j + 1, lr)] - rho[ix2(i, j + 1, lr)];
+        const double d_yi = rho[ix2(i + 1, j + 2, lr)] - rho[ix2(i + 1, j, lr)];
+        ry[ix2(i, j, nx - 1)] =
+            (((tyy[ix2(i, j + 1, lp)] - tyy[ix2(i, j, lp)]) * rdy + (txy[ix2(i + 1, j + 1, lvx)] - txy[ix2(i, j + 1, lvx)]) * rdx) -
+             (P[ix2(i, j + 1, lp)] - P[ix2(i, j, lp)]) * rdy) +
+            g * (rho[ix2(i + 1, j + 1, lr)] - dt * (((av * d_xi) * 0.5) * rdx + ((vy[ix2(i + 1, j + 1, lvy)] * d_yi) * 0.5) * rdy));
+    }
+}
+
+__device__ inline double lane_absmax(const double* __restrict__ a, int64_t n, int64_t gtid, int64_t gstride)
+{
+    double m = -__builtin_inf();
+    for (int64_t k = gtid; k < n; k += gstride) m = jl_max(m, fabs(a[k]));
+    return m;
+}
+
+__device__ inline double lane_abssum(const double* __restrict__ a, int64_t n, int64_t gtid, int64_t gstride)
+{
+    double s = 0.0;
+    for (int64_t k = gtid; k < n; k += gstride) s = s + fabs(a[k]);
+    return s;
+}
+
+__device__ inline void grid_copy(double* __restrict__ dst, const double* __restrict__ src, int64_t n, int64_t gtid,
+                                 int64_t gstride)
+{
+    for (int64_t k = gtid; k < n; k += gstride) dst[k] = src[k];
+}
+
+}  // namespace
+
+template <int K>
+__global__ __launch_bounds__(64) void grid_solve_kernel(GridArgs a)
+{
+    CGrid& I = *(CGrid*)a.desc;
+    const int nw = a.nwaves, w = blockIdx.x;
+    const int64_t gtid = (int64_t)w * 64 + threadIdx.x, gstride = (int64_t)nw * 64;
+    GridBarrier bar{a.sync, a.sync + 1, 0ull, (unsigned long long)nw, a.limit};
+    double* part_a = a.part;           // nw partials (max of |Vx|, sum of |Rx|)
+    double* part_b = a.part + nw;      // |Vy|, |Ry|
+    double* part_c = a.part + 2 * nw;  // |divV|
+
+    const int nx = I.nx, ny = I.ny;
+    const int mn = nx < ny ? nx : ny;
+    const int64_t ncheck = (5 * (int64_t)mn < 2000) ? 5 * (int64_t)mn : 2000;
+    const int64_t ndtupdate = 10 * (int64_t)mn;
+    const int64_t nvx = (int64_t)nx * (ny + 1), nvy = (int64_t)(nx + 1) * ny, np = (int64_t)(nx - 1) * (ny - 1);
+    const double n_rx = (double)(nx - 2) * (ny - 1), n_ry = (double)(nx - 1) * (ny - 2), n_dv = (double)np;
+    const double eps = I.eps;
+    const int64_t iter_max = I.iter_max;
+    const bool fss = I.use_fss != 0;
+
+    if (w == 0 && threadIdx.x == 0) {
+        const double* dd = I.dt_dev;
+        I.scal[SC_DT] = fss ? (dd ? *dd : I.dt_in) : 0.0;
+    }
+    if (!bar.sync()) return;
+
+    // max(|Vx|), max(|Vy|) of the state in p -> maxdisp * min(dx / mx, dy / my) (compute_dt!)
+    auto grid_dt = [&](const double* vx, const double* vy, double* dtc) -> bool {
+        const double mx = wave_max(lane_absmax(vx, nvx, gtid, gstride));
+        const double my = wave_max(lane_absmax(vy, nvy, gtid, gstride));
+        if (threadIdx.x == 0) {
+            part_a[w] = mx;
+            part_b[w] = my;
+        }
+        if (!bar.sync()) return false;
+        const double gx = partials_max(part_a, nw), gy = partials_max(part_b, nw);
+        *dtc = I.maxdisp * jl_min(I.dx / gx, I.dy / gy);
+        return true;
+    };
+
+    double err = 2.0 * eps;
+    int64_t iter = 1;
+    int pf = 0;  // passes so far, mod 2: the state (V, P, dV) is in the caller's buffers when 0
+    while (err > eps && iter <= iter_max) {
+        // iterations iter..iter+K-1 in one pass when none is a check iteration, none but the last
+        // is followed by a dt refresh and the last is not iter_max (the rule of sf_solve_stokes;
+        // where K do not fit, single iterations follow, which gives the same bits)
+        bool fused = K > 1 && iter + K - 1 < iter_max;
+        for (int64_t j = iter; fused && j < iter + K; ++j) {
+            if (j % ncheck == 0) fused = false;
+            if (j < iter + K - 1 && fss && j % ndtupdate == 0) fused = false;
+        }
+        if (fused) {
+            const PTParams GRID_CONST& pp = I.wpt[2 * K - 2 + pf];
+            const int ntiles = pp.ntiles, nstrips = pp.nstrips;
+            if constexpr (K == 4) {
+                for (int t = w; t < ntiles; t += nw) ptk_tile<4, 1>(pp, t % nstrips, t / nstrips);
+            } else if constexpr (K == 3) {
+                for (int t = w; t < ntiles; t += nw) ptk_tile<3, 0>(pp, t % nstrips, t / nstrips);
+            } else if constexpr (K == 2) {
+                for (int t = w; t < ntiles; t += nw) pt2_tile(pp, t % nstrips, t / nstrips);
+            }
+            pf ^= 1;
+            if (!bar.sync()) return;
+            if (fss && (iter + K - 1) % ndtupdate == 0) {
+                double dtc;
+                if (!grid_dt(pp.vx_b, pp.vy_b, &dtc)) return;
+                if (w == 0 && threadIdx.x == 0) I.scal[SC_DT] = dtc;
+                if (!bar.sync()) return;
+            }
+            iter += K;
+            continue;
+        }
+        const PTParams GRID_CONST& pp = I.wpt[pf];
+        const bool check = (iter % ncheck) == 0;
+        const bool last = iter == iter_max;
+        const int flags = ((check || last) ? PT_WRITE_TAU : 0) | (check ? PT_WRITE_RX : 0);
+        const int ntiles = pp.ntiles, nstrips = pp.nstrips;
+        if (flags) {
+            for (int t = w; t < ntiles; t += nw) pt_tile<1, true>(pp, t % nstrips, t / nstrips, flags);
+        } else {
+            for (int t = w; t < ntiles; t += nw) pt_tile<1, false>(pp, t % nstrips, t / nstrips, 0);
+        }
+        pf ^= 1;
+        if (!bar.sync()) return;
+
+        const bool dt_upd = fss && (iter % ndtupdate) == 0;
+        double dtc = 0.0;
+        if (dt_upd || (check && fss)) {
+            if (!grid_dt(pp.vx_b, pp.vy_b, &dtc)) return;
+            if (dt_upd && w == 0 && threadIdx.x == 0) I.scal[SC_DT] = dtc;
+        }
+        if (check) {
+            grid_resid_y(pp, I.ry, fss ? dtc : 0.0, gtid, gstride);
+            if (!bar.sync()) return;
+            const double s0 = wave_sum(lane_abssum(pp.rx, (int64_t)n_rx, gtid, gstride));
+            const double s1 = wave_sum(lane_abssum(I.ry, (int64_t)n_ry, gtid, gstride));
+            const double s2 = wave_sum(lane_abssum(pp.divv, (int64_t)n_dv, gtid, gstride));
+            if (threadIdx.x == 0) {
+                part_a[w] = s0;
+                part_b[w] = s1;
+                part_c[w] = s2;
+            }
+            if (!bar.sync()) return;
+            const double g0 = partials_sum(part_a, nw), g1 = partials_sum(part_b, nw), g2 = partials_sum(part_c, nw);
+            err = jl_max(jl_max(g0 / n_rx, g1 / n_ry), g2 / n_dv);
+        }
+        if ((dt_upd || check) && !bar.sync()) return;  // dt written, partials read
+        iter += 1;
+    }
+    const int64_t niter = iter - 1;
+    if (pf) {  // the last iterate (state and dV) is in the scratch buffers
+        const PTParams GRID_CONST& p0 = I.wpt[0];
+        grid_copy(I.Vx_s, p0.vx_b, nvx, gtid, gstride);
+        grid_copy(I.Vy_s, p0.vy_b, nvy, gtid, gstride);
+        grid_copy(I.P, p0.p_b, np, gtid, gstride);
+        grid_copy(p0.dvx, p0.dvx_o, (int64_t)(nx - 2) * (ny - 1), gtid, gstride);
+        grid_copy(p0.dvy, p0.dvy_o, (int64_t)(nx - 1) * (ny - 2), gtid, gstride);
+        if (!bar.sync()) return;
+    }
+    // Vx_pad[2:end-1,:] .= Vx_s; Vy_pad[:,2:end-1] .= Vy_s; bc_x_mirror!; bc_y_mirror!  (:89-92)
+    const double* __restrict__ Vx_s = I.Vx_s;
+    const double* __restrict__ Vy_s = I.Vy_s;
+    double* __restrict__ Vx_pad = I.Vx_pad;
+    double* __restrict__ Vy_pad = I.Vy_pad;
+    const int64_t npx = (int64_t)(nx + 2) * (ny + 1), npy = (int64_t)(nx + 1) * (ny + 2);
+    for (int64_t k = gtid; k < npx; k += gstride) {
+        const int i = (int)(k % (nx + 2)), j = (int)(k / (nx + 2));
+        double v;
+        if (i == 0) v = -Vx_s[ix2(1, j, nx)];
+        else if (i == nx + 1) v = -Vx_s[ix2(nx - 2, j, nx)];
+        else v = Vx_s[ix2(i - 1, j, nx)];
+        Vx_pad[k] = v;
+    }
+    for (int64_t k = gtid; k < npy; k += gstride) {
+        const int i = (int)(k % (nx + 1)), j = (int)(k / (nx + 1));
+        double v;
+        if (j == 0) v = -Vy_s[ix2(i, 1, nx + 1)];
+        else if (j == ny + 1) v = -Vy_s[ix2(i, ny - 2, nx + 1)];
+        else v = Vy_s[ix2(i, j - 1, nx + 1)];
+        Vy_pad[k] = v;
+    }
+    double dt_new;
+    if (!grid_dt(Vx_s, Vy_s, &dt_new)) return;
+    if (w == 0 && threadIdx.x == 0) {
+        BatchResult* R = a.res;
+        R->dt_new = dt_new;
+        R->err = err;
+        R->niter = niter;
+        R->converged = err <= eps;
+        R->dt_last = I.scal[SC_DT];
+    }
+}
+
+__global__ void grid_zero_sync(unsigned long long* p)
+{
+    if (threadIdx.x < 2) p[threadIdx.x] = 0ull;
+}
+
+// Resident one-wave workgroups of grid_solve_kernel<k> on the current device.
+int grid_capacity(int k)
+{
+    static int cap[4] = {0, 0, 0, 0};
+    if (k < 1 || k > 3) return 0;
+    if (cap[k] == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+        const void* f = k == 3 ? (const void*)grid_solve_kernel<3>
+                               : (k == 2 ? (const void*)grid_solve_kernel<2> : (const void*)grid_solve_kernel<1>);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+        cap[k] = std::min(cus * per_cu, (int)GRID_MAX_WAVES);
+    }
+    return cap[k];
+}
+
+uint64_t grid_wait_limit()
+{
+    static const uint64_t lim = [] {
+        int dev = 0, khz = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz < 1)
+            khz = 100000;
+        return (uint64_t)khz * 1000ull * GRID_WAIT_SECONDS;
+    }();
+    return lim;
+}
+
+hipError_t launch_grid_solve(const GridArgs& a, int k, hipStream_t s)
+{
+    if (a.nwaves < 1 || a.nwaves > grid_capacity(k)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(grid_zero_sync, dim3(1), dim3(64), 0, s, a.sync);
+    if (k == 3) hipLaunchKernelGGL(grid_solve_kernel<3>, dim3(a.nwaves), dim3(64), 0, s, a);
+    else if (k == 2) hipLaunchKernelGGL(grid_solve_kernel<2>, dim3(a.nwaves), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL(grid_solve_kernel<1>, dim3(a.nwaves), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace sf

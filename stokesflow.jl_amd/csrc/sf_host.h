@@ -85,6 +85,8 @@ struct BatchTune {
     BatchShape sh;
 };
 BatchShape batch_shape(const sf_ctx* c, int nx, int ny);
+BatchShape grid_shape(int nx, int ny);
+int grid_waves(const BatchInstance& d, int k);
 BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, const double* dt_dev,
                                const BatchShape& sh = BatchShape());
 std::vector<int> batch_order(const std::vector<std::pair<int, int>>& dims);
@@ -258,4 +260,8 @@ struct sf_ctx {
     double *dvx_b = nullptr, *dvy_b = nullptr;  // second dVx/dVy (two-iteration pass)
     sf::PassTimer timer;  // sf_ctx_time_passes
     std::vector<sf::BatchTune> btune;  // sf_batch_autotune: wide-pass shapes per grid size
+    // grid mode of the batched solver
+    unsigned long long* g_sync = nullptr;  // two barrier words per instance
+    int g_cap = 0;
+    double* g_part = nullptr;  // 3 * GRID_MAX_WAVES partials
 };

@@ -181,6 +181,24 @@ struct BatchResult {
     int converged, pad;
 };
 
+// Grid mode (sf_grid.hip): one instance's whole solve in one launch of nwaves one-wave
+// workgroups (at most grid_capacity(), all resident), the passes separated by a grid barrier.
+// sync[0] (arrivals) and sync[1] (failure flag: a barrier wait exceeded the limit) are zeroed by
+// launch_grid_solve; part holds 3 * nwaves partials.
+enum { GRID_MAX_WAVES = 4096, GRID_WAIT_SECONDS = 5 };
+struct GridArgs {
+    const BatchInstance* desc;
+    BatchResult* res;
+    unsigned long long* sync;
+    double* part;
+    int nwaves, pad;
+    uint64_t limit;  // wall-clock ticks a wave may wait at a barrier
+};
+// k = PT iterations per pass (1, 2 or 3; four need more than 256 VGPRs in this kernel).
+int grid_capacity(int k);
+uint64_t grid_wait_limit();
+hipError_t launch_grid_solve(const GridArgs& a, int k, hipStream_t s);
+
 // Wide mode of the batched solver: the active set of one PT-iteration launch.
 #define WIDE_MAX 64
 struct WideArgs {

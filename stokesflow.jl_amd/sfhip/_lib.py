@@ -27,7 +27,7 @@ EXPORTS = (
     "sf_marker_cell_keys", "sf_ensemble_create", "sf_ensemble_step", "sf_ensemble_destroy",
     "sf_ctx_autotune_pt", "sf_probe_dram", "sf_markers_to_grids3", "sf_ctx_time_passes", "sf_ctx_pass_times",
     "sf_solve_stokes_noalloc", "sf_lattice_markers", "sf_marker_properties", "sf_batch_autotune",
-    "sf_batch_tuning_clear",
+    "sf_batch_tuning_clear", "sf_graph_create", "sf_graph_step", "sf_graph_destroy",
 )
 
 
@@ -128,7 +128,7 @@ class Shape(C.Structure):
 SF_SHAPE_Y_BELOW, SF_SHAPE_Y_ABOVE, SF_SHAPE_CIRCLE, SF_SHAPE_RECT, SF_SHAPE_SINE_BELOW = 1, 2, 3, 4, 5
 SF_MAX_SHAPES = 16
 
-SF_BATCH_AUTO, SF_BATCH_PERSISTENT, SF_BATCH_WIDE = 0, 1, 2
+SF_BATCH_AUTO, SF_BATCH_PERSISTENT, SF_BATCH_WIDE, SF_BATCH_GRID = 0, 1, 2, 3
 
 
 class BatchParams(C.Structure):
@@ -190,6 +190,9 @@ def load():
     L.sf_ensemble_create.argtypes = [_vp, C.POINTER(StepArgs), _i, C.POINTER(BatchParams), _i, C.POINTER(_vp)]
     L.sf_ensemble_step.argtypes = [_vp, _i, C.POINTER(SolveOut), C.POINTER(_d)]
     L.sf_ensemble_destroy.argtypes = [_vp]
+    L.sf_graph_create.argtypes = [_vp, C.POINTER(StepArgs), _i, C.POINTER(_vp)]
+    L.sf_graph_step.argtypes = [_vp, _i, C.POINTER(SolveOut), C.POINTER(_d)]
+    L.sf_graph_destroy.argtypes = [_vp]
     L.sf_marker_cell_keys.argtypes = [_vp, _d, _d, _i, _i, _vp, _vp, _i64, _d, _d, _vp, _vp, C.POINTER(M2GOut)]
     L.sf_move_markers_rk4.argtypes = [_vp, _vp, _vp, _i64, _vp, _i, _i, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d,
                                       _d, _d, C.POINTER(RK4Out)]

@@ -263,6 +263,10 @@ def solveStokes(P, Vx_pad, Vy_pad, rho_vy, mu_b, mu_p, txx, tyy, txy, divV, dtau
 _BATCH_ARRAYS = ("P", "Vx_pad", "Vy_pad", "dVxdtau", "dVydtau", "Vx_s", "Vy_s", "rho_vy", "mu_b", "mu_p")
 
 
+_BATCH_MODES = {"auto": _lib.SF_BATCH_AUTO, "persistent": _lib.SF_BATCH_PERSISTENT, "wide": _lib.SF_BATCH_WIDE,
+                "grid": _lib.SF_BATCH_GRID}
+
+
 def solveStokes_batched(problems, *, mode="auto", rows=0, depth=0, threads=0, max_blocks=0, budget_mb=0.0,
                         mixed_precision=False):
     """Many independent solveStokes! problems in one persistent launch (sf_solve_stokes_batched).
@@ -275,16 +279,18 @@ def solveStokes_batched(problems, *, mode="auto", rows=0, depth=0, threads=0, ma
     runtime = wall time of the whole batch).
 
     ``mode`` "persistent": one launch, one workgroup per instance from a device work queue;
-    "wide" (= "auto"): one to three PT iterations per launch over an active set of instances
+    "wide" (= "auto"): one to four PT iterations per launch over an active set of instances
     whose working sets fit ``budget_mb`` (default 1536 MiB; 1024 MiB with mixed precision),
-    refilled as instances converge.
+    refilled as instances converge; "grid": one instance after another, each solved by one
+    launch over the whole chip with the PT loop on the device (``depth`` = PT iterations per
+    pass, 1 to 3, default 3).
     ``mixed_precision=True`` (wide mode): every instance is solved as
     ``solveStokes(..., mixed_precision=True)`` solves it (FP32 sweeps inside FP64 defect
     correction), one to three FP32 sweeps per launch of all active instances.
     """
     n = len(problems)
     arr = _batch_instances(problems)
-    modes = {"auto": _lib.SF_BATCH_AUTO, "persistent": _lib.SF_BATCH_PERSISTENT, "wide": _lib.SF_BATCH_WIDE}
+    modes = _BATCH_MODES
     prm = BatchParams(int(rows), int(depth), int(threads), int(max_blocks), modes[mode], int(bool(mixed_precision)),
                       float(budget_mb))
     outs = (SolveOut * max(n, 1))()
@@ -622,6 +628,45 @@ def step(state, x_m, y_m, rho_m, mu_m, dx, dy, dt, *, g_y=9.81, maxdisp=0.5, eps
     return a.dt, stats
 
 
+class StepGraph:
+    """The captured timestep of one StokesFlow2D problem (sf_graph_*): the loop body of
+    StokesFlow2D_multixpu.jl:164-181 as one hipGraph, the solve with its PT loop on the device
+    (one chip-wide launch), RK4 with the new dt read on the device.  Arguments as for
+    :func:`step` (``dt``: the first step's input dt); ``iters_per_pass`` 1 to 3 (0: 3).  The
+    tensors must stay alive while the graph exists.  Bit-identical to repeated :func:`step` calls
+    (err to rounding)."""
+
+    def __init__(self, state, x_m, y_m, rho_m, mu_m, dx, dy, dt=0.0, *, g_y=9.81, maxdisp=0.5, eps=1e-8,
+                 iterMax=100000, use_free_surface_stabilization=True, iters_per_pass=0):
+        self.lib = _lib.load()
+        self._keep = (state, x_m, y_m, rho_m, mu_m)
+        a = _step_args(state, x_m, y_m, rho_m, mu_m, dx, dy, dt, g_y, maxdisp, eps, iterMax,
+                       use_free_surface_stabilization)
+        self._ctx = context(0, 0)
+        self._ctx.use_current_stream()
+        self.handle = C.c_void_p()
+        check(self.lib.sf_graph_create(self._ctx.handle, C.byref(a), int(iters_per_pass), C.byref(self.handle)))
+
+    def step(self, n_steps=1):
+        """Replay n_steps timesteps; returns (dt, stats of the last step's solve)."""
+        self._ctx.use_current_stream()
+        so = SolveOut()
+        dt = C.c_double()
+        check(self.lib.sf_graph_step(self.handle, int(n_steps), C.byref(so), C.byref(dt)))
+        return dt.value, so.as_dict()
+
+    def close(self):
+        if self.handle:
+            self.lib.sf_graph_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Ensemble:
     """Many independent StokesFlow2D problems stepped together (sf_ensemble_*).
 
@@ -630,6 +675,7 @@ class Ensemble:
     use_free_surface_stabilization (True).  One :meth:`step` runs, for every member, the three
     marker -> grid passes, then one batched solve of all members, then RK4 with each member's
     new dt.  ``mode`` "persistent": the whole step is one hipGraph replay (small members);
+    "grid": the same with one chip-wide grid-mode solve per member;
     "wide" (= "auto"): marker -> grid and RK4 are hipGraph replays around the wide batched
     solve; ``mixed_precision=True`` (wide form) solves with FP32 sweeps inside FP64 defect
     correction.  The members' tensors must stay alive while the ensemble exists.
@@ -648,8 +694,7 @@ class Ensemble:
         ctx = context(0, 0)
         self._ctx = ctx
         self.handle = C.c_void_p()
-        modes = {"auto": _lib.SF_BATCH_AUTO, "persistent": _lib.SF_BATCH_PERSISTENT, "wide": _lib.SF_BATCH_WIDE}
-        prm = BatchParams(int(rows), 0, int(threads), int(max_blocks), modes[mode], int(bool(mixed_precision)),
+        prm = BatchParams(int(rows), 0, int(threads), int(max_blocks), _BATCH_MODES[mode], int(bool(mixed_precision)),
                           float(budget_mb))
         check(self.lib.sf_ensemble_create(ctx.handle, arr, self.n, C.byref(prm), int(bool(use_graph)),
                                           C.byref(self.handle)))

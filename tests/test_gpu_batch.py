@@ -1,7 +1,7 @@
 """Batched ensemble solver (sf_solve_stokes_batched) against per-instance solves.
 
-Each instance of the persistent launch runs the same PT tile code as the single-instance
-solver, so its iterates, iteration count and dt must be bit-identical to
+Each instance of the persistent, wide and grid modes runs the same PT tile code as the
+single-instance solver, so its iterates, iteration count and dt must be bit-identical to
 ``solveStokes`` on the same inputs; err is a sum in another fixed order (rtol 1e-12).
 One instance is also checked against the CPU oracle directly."""
 import numpy as np
@@ -55,7 +55,8 @@ SIZES = [(24, 24), (33, 47), (40, 29), (64, 64), (17, 90)]
 @pytest.mark.parametrize("prm", [{"mode": "persistent"}, {"mode": "persistent", "max_blocks": 1},
                                  {"mode": "persistent", "threads": 64},
                                  {"mode": "persistent", "threads": 320, "rows": 7},
-                                 {"mode": "wide"}, {"mode": "wide", "budget_mb": 1e-3}, {"mode": "wide", "rows": 7}])
+                                 {"mode": "wide"}, {"mode": "wide", "budget_mb": 1e-3}, {"mode": "wide", "rows": 7},
+                                 {"mode": "grid"}, {"mode": "grid", "depth": 2}, {"mode": "grid", "depth": 1}])
 def test_batched_matches_single_solves(gpu, prm):
     insts = [(nx, ny, *plume_instance(nx, ny, seed=10 + k)) for k, (nx, ny) in enumerate(SIZES)]
     problems, gs = [], []
@@ -75,7 +76,7 @@ def test_batched_matches_single_solves(gpu, prm):
             assert np.array_equal(sfhip.to_numpy(G[k]), exp[k]), (nx, ny, k)
 
 
-@pytest.mark.parametrize("mode", ["persistent", "wide"])
+@pytest.mark.parametrize("mode", ["persistent", "wide", "grid"])
 def test_batched_against_oracle(gpu, mode):
     nx, ny = 40, 29
     S, dx, dy = plume_instance(nx, ny, seed=5)
@@ -88,7 +89,7 @@ def test_batched_against_oracle(gpu, mode):
         assert np.array_equal(sfhip.to_numpy(G[k]), S[k]), k
 
 
-@pytest.mark.parametrize("mode", ["persistent", "wide"])
+@pytest.mark.parametrize("mode", ["persistent", "wide", "grid"])
 @pytest.mark.parametrize("cap", [37, 250])
 def test_batched_iteration_cap(gpu, cap, mode):
     """An odd cap leaves the last iterate in the scratch buffers: it must be copied back."""
@@ -107,7 +108,7 @@ def test_batched_iteration_cap(gpu, cap, mode):
             assert np.array_equal(sfhip.to_numpy(G[k]), exp[k]), (nx, ny, k)
 
 
-@pytest.mark.parametrize("mode", ["persistent", "wide"])
+@pytest.mark.parametrize("mode", ["persistent", "wide", "grid"])
 def test_batched_no_stabilisation_and_mixed_eps(gpu, mode):
     nx, ny = 33, 47
     S, dx, dy = plume_instance(nx, ny, seed=8)
@@ -196,9 +197,11 @@ def test_wide_refined_iteration_cap_and_errors(gpu):
         assert np.array_equal(sfhip.to_numpy(G[k]), exp[k]), k
     with pytest.raises(sfhip.SFError):
         sfhip.solveStokes_batched([pb], mode="persistent", mixed_precision=True)
+    with pytest.raises(sfhip.SFError):
+        sfhip.solveStokes_batched([pb], mode="grid", mixed_precision=True)
 
 
-@pytest.mark.parametrize("mode", ["persistent", "wide"])
+@pytest.mark.parametrize("mode", ["persistent", "wide", "grid"])
 def test_batched_minimal_grids(gpu, mode):
     """The smallest grids the solver accepts, batched with a larger one (tile edge cases)."""
     sizes = [(3, 3), (4, 7), (5, 3), (63, 65)]

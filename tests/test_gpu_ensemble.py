@@ -32,7 +32,7 @@ def same(a, b):
     assert torch.equal(a["x_m"], b["x_m"]) and torch.equal(a["y_m"], b["y_m"])
 
 
-@pytest.mark.parametrize("mode", ["persistent", "wide"])
+@pytest.mark.parametrize("mode", ["persistent", "wide", "grid"])
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_ensemble_matches_single_steps(gpu, use_graph, mode):
     ens_m = [member(nx, ny, 20 + k) for k, (nx, ny) in enumerate(SIZES)]
@@ -53,7 +53,7 @@ def test_ensemble_matches_single_steps(gpu, use_graph, mode):
     ens.close()
 
 
-@pytest.mark.parametrize("mode", ["persistent", "wide"])
+@pytest.mark.parametrize("mode", ["persistent", "wide", "grid"])
 def test_ensemble_multi_step_replay(gpu, mode):
     """n_steps replays in one call equal n single-step calls."""
     a = [member(nx, ny, 40 + k) for k, (nx, ny) in enumerate(SIZES)]
@@ -70,7 +70,7 @@ def test_ensemble_multi_step_replay(gpu, mode):
     eb.close()
 
 
-@pytest.mark.parametrize("mode", ["persistent", "wide"])
+@pytest.mark.parametrize("mode", ["persistent", "wide", "grid"])
 def test_ensemble_nonfinite_marker(gpu, mode):
     m = member(24, 24, 3)
     m["x_m"][5] = float("nan")
@@ -115,7 +115,7 @@ def test_ensemble_refined_matches_single_refined_steps(gpu, use_graph):
     ens.close()
 
 
-@pytest.mark.parametrize("mode", ["persistent", "wide"])
+@pytest.mark.parametrize("mode", ["persistent", "wide", "grid"])
 def test_ensemble_graph_against_oracle_loop(gpu, mode):
     """The captured ensemble step against the oracle's own timestep loop (StokesFlow2D_multixpu.jl:
     155-218 at one rank) per member: 2 FP64 steps, fields and markers bitwise, same iterations and
