@@ -194,13 +194,13 @@ BatchShape grid_shape(int nx, int ny)
     return sh;
 }
 
-// Grid mode: waves of an instance's launch, one per tile of its largest pass, at most all
-// resident ones.
-int grid_waves(const BatchInstance& d, int k)
+// Grid mode: workgroups of an instance's launch, one wave per tile of its largest pass, at most
+// all resident ones.
+int grid_workgroups(const BatchInstance& d, int k)
 {
     int t = d.wpt[0].ntiles;
     if (k >= 2) t = std::max(t, d.wpt[2 * k - 2].ntiles);
-    return std::max(1, std::min(t, grid_capacity(k)));
+    return std::max(1, std::min((t + GRID_WG_WAVES - 1) / GRID_WG_WAVES, grid_capacity(k)));
 }
 
 // Grid mode: the barrier words (two per instance) and the partials of the context.
@@ -211,7 +211,7 @@ static int grid_reserve(sf_ctx* c, int n)
         if (c->g_sync) HIPCHK(hipFree(c->g_sync));
         c->g_sync = nullptr;
         c->g_cap = 0;
-        HIPCHK(dalloc(&c->g_sync, 2 * (size_t)n));
+        HIPCHK(dalloc(&c->g_sync, (size_t)GRID_SYNC_WORDS * n));
         c->g_cap = n;
     }
     return SF_OK;
@@ -1020,16 +1020,17 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
         const int k = (prm && prm->depth >= 1 && prm->depth <= 3) ? prm->depth : 3;
         for (int qi = 0; qi < n; ++qi) {
             const int i = order[(size_t)qi];
-            const GridArgs ga{c->b_inst + i, c->b_res + i, c->g_sync + 2 * (size_t)i, c->g_part,
-                              grid_waves(host[(size_t)i], k), 0, grid_wait_limit()};
+            const GridArgs ga{c->b_inst + i, c->b_res + i, c->g_sync + (size_t)GRID_SYNC_WORDS * i, c->g_part,
+                              grid_workgroups(host[(size_t)i], k), 0, grid_wait_limit()};
             HIPCHK(launch_grid_solve(ga, k, s));
         }
-        std::vector<unsigned long long> sync(2 * (size_t)n);
+        std::vector<unsigned long long> sync((size_t)GRID_SYNC_WORDS * n);
         HIPCHK(hipMemcpyAsync(res.data(), c->b_res, sizeof(BatchResult) * (size_t)n, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(sync.data(), c->g_sync, sizeof(unsigned long long) * 2 * (size_t)n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(sync.data(), c->g_sync, sizeof(unsigned long long) * GRID_SYNC_WORDS * (size_t)n,
+                              hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         for (int i = 0; i < n; ++i)
-            if (sync[2 * (size_t)i + 1])
+            if (sync[(size_t)GRID_SYNC_WORDS * i + GRID_SYNC_FAIL])
                 return set_err(SF_ERR_HIP, "%s: instance %d: a grid barrier wait exceeded %d s (waves not all resident?)",
                                who, i, (int)GRID_WAIT_SECONDS);
     } else if (mode == SF_BATCH_WIDE || mode == SF_BATCH_AUTO) {

@@ -28,10 +28,10 @@ struct sf_ensemble {
     unsigned long long* cnt = nullptr;   // per member: 3 x 4 marker -> grid counters, 4 RK4 counters
     unsigned long long* hcnt = nullptr;  // pinned
     BatchResult* hres = nullptr;         // pinned
-    unsigned long long* gsync = nullptr;  // grid mode: two barrier words per member
+    unsigned long long* gsync = nullptr;  // grid mode: GRID_SYNC_WORDS barrier words per member
     unsigned long long* hsync = nullptr;  // pinned
     double* gpart = nullptr;              // grid mode: 3 * GRID_MAX_WAVES partials
-    std::vector<int> gwaves;              // grid mode: waves of each member's launch
+    std::vector<int> gwg;                 // grid mode: workgroups of each member's launch
     hipGraph_t graph[2] = {nullptr, nullptr};  // persistent: [0] whole step; wide: [0] marker -> grid, [1] RK4
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     M2GScratch m2g;  // own sort scratch: the captured graphs hold its addresses
@@ -123,7 +123,7 @@ static hipError_t ensemble_enqueue_persistent(sf_ensemble* e)
     if (e->grid) {
         for (int q = 0; q < e->n && r == hipSuccess; ++q) {
             const int i = e->order[(size_t)q];
-            const GridArgs ga{e->inst + i, e->res + i, e->gsync + 2 * (size_t)i, e->gpart, e->gwaves[(size_t)i], 0,
+            const GridArgs ga{e->inst + i, e->res + i, e->gsync + (size_t)GRID_SYNC_WORDS * i, e->gpart, e->gwg[(size_t)i], 0,
                               grid_wait_limit()};
             r = launch_grid_solve(ga, e->gk, e->stream);
         }
@@ -233,10 +233,10 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
     if (er == hipSuccess) er = hipHostMalloc((void**)&e->hcnt, sizeof(unsigned long long) * ENS_CNT * n, hipHostMallocDefault);
     if (er == hipSuccess) er = hipHostMalloc((void**)&e->hres, sizeof(BatchResult) * n, hipHostMallocDefault);
     if (er == hipSuccess) er = hipHostMalloc((void**)&e->hscal, sizeof(double) * SC_COUNT * n, hipHostMallocDefault);
-    if (er == hipSuccess && e->grid) er = dalloc(&e->gsync, 2 * (size_t)n);
+    if (er == hipSuccess && e->grid) er = dalloc(&e->gsync, (size_t)GRID_SYNC_WORDS * n);
     if (er == hipSuccess && e->grid) er = dalloc(&e->gpart, 3 * (size_t)GRID_MAX_WAVES);
     if (er == hipSuccess && e->grid)
-        er = hipHostMalloc((void**)&e->hsync, sizeof(unsigned long long) * 2 * n, hipHostMallocDefault);
+        er = hipHostMalloc((void**)&e->hsync, sizeof(unsigned long long) * GRID_SYNC_WORDS * n, hipHostMallocDefault);
     if (er != hipSuccess) {
         free_ensemble(e);
         return set_err(SF_ERR_HIP, "%s: %s", who, hipGetErrorString(er));
@@ -255,8 +255,8 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
     }
     e->order = batch_order(dims);
     if (e->grid) {
-        e->gwaves.resize((size_t)n);
-        for (int i = 0; i < n; ++i) e->gwaves[(size_t)i] = grid_waves(e->desc[(size_t)i], e->gk);
+        e->gwg.resize((size_t)n);
+        for (int i = 0; i < n; ++i) e->gwg[(size_t)i] = grid_workgroups(e->desc[(size_t)i], e->gk);
     }
     e->nblocks = batch_grid(c, n, threads, prm ? prm->max_blocks : 0);
     if (e->refined) {
@@ -356,15 +356,15 @@ extern "C" int sf_ensemble_step(sf_ensemble* e, int n_steps, sf_solve_out* outs,
     if (!e->wide)
         HIPCHK(hipMemcpyAsync(e->hres, e->res, sizeof(BatchResult) * (size_t)e->n, hipMemcpyDeviceToHost, e->stream));
     if (e->grid)
-        HIPCHK(hipMemcpyAsync(e->hsync, e->gsync, sizeof(unsigned long long) * 2 * (size_t)e->n, hipMemcpyDeviceToHost,
-                              e->stream));
+        HIPCHK(hipMemcpyAsync(e->hsync, e->gsync, sizeof(unsigned long long) * GRID_SYNC_WORDS * (size_t)e->n,
+                              hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipMemcpyAsync(e->hcnt, e->cnt, sizeof(unsigned long long) * ENS_CNT * (size_t)e->n, hipMemcpyDeviceToHost,
                           e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     if (n_steps > 0 && e->grid)
         for (int i = 0; i < e->n; ++i)
-            if (e->hsync[2 * (size_t)i + 1])
+            if (e->hsync[(size_t)GRID_SYNC_WORDS * i + GRID_SYNC_FAIL])
                 return set_err(SF_ERR_HIP, "%s: member %d: a grid barrier wait exceeded %d s (waves not all resident?)",
                                who, i, (int)GRID_WAIT_SECONDS);
     if (n_steps > 0)

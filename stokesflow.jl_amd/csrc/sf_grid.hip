@@ -11,12 +11,18 @@
 // pass kernels (sf_pt_tile.h, sf_pt2_tile.h, sf_ptk_tile.h), so every field is bit-identical to
 // sf_solve_stokes; err is a sum of the same terms in another order (equal to rounding).
 //
-// The barrier is a monotone arrival counter (zeroed by a kernel before each launch) that each
-// wave's first lane increments and then polls, with release / acquire fences at agent scope
-// around it.  The waves must all be resident, which the host ensures by sizing the launch from
-// the kernel's occupancy; should one not be, no wave waits forever: a wait longer than the time
-// limit (or a flag raised by another wave) sets the failure flag, every wave leaves the kernel,
-// and the host reports the error.
+// The barrier is two-level, because an agent-scope release writes back the whole L2 of its XCD
+// and agent-scope atomics on one word are serialised (measured, 2,048 one-wave workgroups: 98 us
+// per barrier with a release, an arrival and a poll per wave on one counter; 24 us with one
+// counter per XCD and one release per XCD).  A wave's first lane waits for its stores to reach
+// the XCD's L2 (a workgroup-scope release), counts the wave in on its XCD's counter, and the
+// XCD's last wave writes that L2 back (release, agent scope) and counts the XCD in on the global
+// counter, which every wave polls before invalidating its caches (acquire, agent scope).  The
+// counters are monotone within a launch and zeroed by a kernel before it; the waves per XCD are
+// counted at the start.  The waves must all be resident, which the host ensures by sizing the
+// launch from the kernel's occupancy; should one not be, no wave waits forever: a wait longer
+// than the time limit (or a flag raised by another wave) sets the failure flag, every wave leaves
+// the kernel, and the host reports the error.
 #include "sf_ptk_tile.h"
 
 namespace sf {
@@ -26,33 +32,77 @@ typedef const BatchInstance GRID_CONST CGrid;
 
 namespace {
 
+typedef unsigned long long u64;
+
+__device__ inline u64 ld_agent(const u64* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline u64 add_agent(u64* p) { return __hip_atomic_fetch_add(p, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Polls p until it reaches target; false (and the failure flag set) after the time limit or when
+// another wave has raised the flag.
+__device__ inline bool wait_agent(const u64* p, u64 target, u64* fail, uint64_t limit)
+{
+    const uint64_t t0 = wall_clock64();
+    while (ld_agent(p) < target) {
+        if (ld_agent(fail) != 0ull) return false;
+        if (wall_clock64() - t0 > limit) {
+            __hip_atomic_store(fail, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return true;
+}
+
+__device__ inline int xcd_id()
+{
+    int v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return v & (GRID_MAX_XCD - 1);
+}
+
 struct GridBarrier {
-    unsigned long long* cnt;   // arrivals since the launch
-    unsigned long long* fail;  // nonzero: a wait timed out, every wave leaves
-    unsigned long long target; // arrivals that complete the next barrier
-    unsigned long long nw;     // waves in the launch
-    uint64_t limit;            // wall-clock ticks a wave may wait
+    u64* s;          // sync words (layout in sf_internal.h)
+    u64 gtarget;     // global arrivals that complete the next barrier
+    u64 xtarget;     // arrivals on this XCD's counter that complete it
+    u64 n_xcd;       // XCDs with waves of this launch
+    u64 w_here;      // waves of this launch on this XCD
+    uint64_t limit;  // wall-clock ticks a wave may wait
+    int xcd;
+
+    // Counts the waves per XCD (once, at the start).
+    __device__ bool init(u64* sync, uint64_t lim)
+    {
+        s = sync;
+        limit = lim;
+        xcd = xcd_id();
+        gtarget = xtarget = 0ull;
+        int ok = 1;
+        u64 nx = 0, here = 0;
+        if (threadIdx.x == 0) {
+            // waiting for the returned count orders this add before the start arrival
+            if (add_agent(s + GRID_SYNC_XCOUNT + 16 * xcd) >= (u64)gridDim.x) ok = 0;
+            add_agent(s + GRID_SYNC_START);
+            ok = wait_agent(s + GRID_SYNC_START, gridDim.x, s + GRID_SYNC_FAIL, limit) && ok;
+            for (int x = 0; x < GRID_MAX_XCD; ++x) nx += ld_agent(s + GRID_SYNC_XCOUNT + 16 * x) > 0ull;
+            here = ld_agent(s + GRID_SYNC_XCOUNT + 16 * xcd);
+        }
+        n_xcd = __shfl(nx, 0);
+        w_here = __shfl(here, 0);
+        return __shfl(ok, 0) != 0;
+    }
 
     __device__ bool sync()
     {
-        target += nw;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        gtarget += n_xcd;
+        xtarget += w_here;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // this wave's stores are in the L2
         int ok = 1;
         if (threadIdx.x == 0) {
-            __hip_atomic_fetch_add(cnt, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const uint64_t t0 = wall_clock64();
-            while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull) {
-                    ok = 0;
-                    break;
-                }
-                if (wall_clock64() - t0 > limit) {
-                    __hip_atomic_store(fail, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    ok = 0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
+            if (add_agent(s + GRID_SYNC_XARRIVE + 16 * xcd) + 1ull == xtarget) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this XCD's L2 written back
+                add_agent(s + GRID_SYNC_ARRIVE);
             }
+            ok = wait_agent(s + GRID_SYNC_ARRIVE, gtarget, s + GRID_SYNC_FAIL, limit);
         }
         ok = __shfl(ok, 0);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -142,9 +192,10 @@ template <int K>
 __global__ __launch_bounds__(64) void grid_solve_kernel(GridArgs a)
 {
     CGrid& I = *(CGrid*)a.desc;
-    const int nw = a.nwaves, w = blockIdx.x;
+    const int nw = gridDim.x, w = blockIdx.x;
     const int64_t gtid = (int64_t)w * 64 + threadIdx.x, gstride = (int64_t)nw * 64;
-    GridBarrier bar{a.sync, a.sync + 1, 0ull, (unsigned long long)nw, a.limit};
+    GridBarrier bar;
+    if (!bar.init(a.sync, a.limit)) return;
     double* part_a = a.part;           // nw partials (max of |Vx|, sum of |Rx|)
     double* part_b = a.part + nw;      // |Vy|, |Ry|
     double* part_c = a.part + 2 * nw;  // |divV|
@@ -295,7 +346,7 @@ __global__ __launch_bounds__(64) void grid_solve_kernel(GridArgs a)
 
 __global__ void grid_zero_sync(unsigned long long* p)
 {
-    if (threadIdx.x < 2) p[threadIdx.x] = 0ull;
+    for (int k = threadIdx.x; k < GRID_SYNC_WORDS; k += blockDim.x) p[k] = 0ull;
 }
 
 // Resident one-wave workgroups of grid_solve_kernel<k> on the current device.
@@ -330,11 +381,11 @@ uint64_t grid_wait_limit()
 
 hipError_t launch_grid_solve(const GridArgs& a, int k, hipStream_t s)
 {
-    if (a.nwaves < 1 || a.nwaves > grid_capacity(k)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(grid_zero_sync, dim3(1), dim3(64), 0, s, a.sync);
-    if (k == 3) hipLaunchKernelGGL(grid_solve_kernel<3>, dim3(a.nwaves), dim3(64), 0, s, a);
-    else if (k == 2) hipLaunchKernelGGL(grid_solve_kernel<2>, dim3(a.nwaves), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL(grid_solve_kernel<1>, dim3(a.nwaves), dim3(64), 0, s, a);
+    if (a.nwg < 1 || a.nwg > grid_capacity(k)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(grid_zero_sync, dim3(1), dim3(256), 0, s, a.sync);
+    if (k == 3) hipLaunchKernelGGL(grid_solve_kernel<3>, dim3(a.nwg), dim3(64), 0, s, a);
+    else if (k == 2) hipLaunchKernelGGL(grid_solve_kernel<2>, dim3(a.nwg), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL(grid_solve_kernel<1>, dim3(a.nwg), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

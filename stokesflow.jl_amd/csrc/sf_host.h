@@ -86,7 +86,7 @@ struct BatchTune {
 };
 BatchShape batch_shape(const sf_ctx* c, int nx, int ny);
 BatchShape grid_shape(int nx, int ny);
-int grid_waves(const BatchInstance& d, int k);
+int grid_workgroups(const BatchInstance& d, int k);
 BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, const double* dt_dev,
                                const BatchShape& sh = BatchShape());
 std::vector<int> batch_order(const std::vector<std::pair<int, int>>& dims);
@@ -261,7 +261,7 @@ struct sf_ctx {
     sf::PassTimer timer;  // sf_ctx_time_passes
     std::vector<sf::BatchTune> btune;  // sf_batch_autotune: wide-pass shapes per grid size
     // grid mode of the batched solver
-    unsigned long long* g_sync = nullptr;  // two barrier words per instance
+    unsigned long long* g_sync = nullptr;  // GRID_SYNC_WORDS barrier words per instance
     int g_cap = 0;
     double* g_part = nullptr;  // 3 * GRID_MAX_WAVES partials
 };

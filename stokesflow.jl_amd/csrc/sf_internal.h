@@ -181,17 +181,30 @@ struct BatchResult {
     int converged, pad;
 };
 
-// Grid mode (sf_grid.hip): one instance's whole solve in one launch of nwaves one-wave
-// workgroups (at most grid_capacity(), all resident), the passes separated by a grid barrier.
-// sync[0] (arrivals) and sync[1] (failure flag: a barrier wait exceeded the limit) are zeroed by
-// launch_grid_solve; part holds 3 * nwaves partials.
-enum { GRID_MAX_WAVES = 4096, GRID_WAIT_SECONDS = 5 };
+// Grid mode (sf_grid.hip): one instance's whole solve in one launch of nwg one-wave workgroups
+// (at most grid_capacity(k), all resident), the passes separated by a grid barrier.  sync holds
+// GRID_SYNC_WORDS words, zeroed by launch_grid_solve: [GRID_SYNC_ARRIVE] XCDs arrived,
+// [GRID_SYNC_FAIL] failure flag (a barrier wait exceeded the limit), [GRID_SYNC_START] waves
+// started, [GRID_SYNC_XARRIVE + 16 x] waves of XCD x arrived, [GRID_SYNC_XCOUNT + 16 x] waves on
+// XCD x (one 128-byte line each); part holds 3 * nwg partials.
+enum {
+    GRID_WG_WAVES = 1,
+    GRID_MAX_WAVES = 4096,
+    GRID_WAIT_SECONDS = 5,
+    GRID_MAX_XCD = 16,
+    GRID_SYNC_ARRIVE = 0,
+    GRID_SYNC_FAIL = 1,
+    GRID_SYNC_START = 2,
+    GRID_SYNC_XARRIVE = 16,
+    GRID_SYNC_XCOUNT = 16 + 16 * GRID_MAX_XCD,
+    GRID_SYNC_WORDS = 16 + 32 * GRID_MAX_XCD,
+};
 struct GridArgs {
     const BatchInstance* desc;
     BatchResult* res;
     unsigned long long* sync;
     double* part;
-    int nwaves, pad;
+    int nwg, pad;
     uint64_t limit;  // wall-clock ticks a wave may wait at a barrier
 };
 // k = PT iterations per pass (1, 2 or 3; four need more than 256 VGPRs in this kernel).
