@@ -196,7 +196,8 @@ def time_to_solution(n, seed=3000, progress=False):
 def timestep_stages(n, iters=31, seed=3000):
     """One StokesFlow2D timestep at n x n nodes (16 markers per cell) through the reference-named
     calls, each stage timed with HIP events: the three marker -> grid passes, a solve capped at
-    `iters` PT iterations, and RK4 (StokesFlow2D_multixpu.jl:164-181)."""
+    `iters` PT iterations, and RK4 (StokesFlow2D_multixpu.jl:164-181); then the same step as one
+    captured hipGraph replay (sfhip.StepGraph), timed whole."""
     import torch
 
     import sfhip
@@ -230,7 +231,18 @@ def timestep_stages(n, iters=31, seed=3000):
         out = {"n": n, "markers": nm, "solve_iters": iters,
                "m2g_ms": {"rho_vy": ms[0], "mu_b": ms[1], "mu_p": ms[2]}, "solve_ms": ms[3], "rk4_ms": ms[4],
                "m2g_markers_per_s": 3 * nm / (1e-3 * (ms[0] + ms[1] + ms[2])), "rk4_markers_per_s": nm / (1e-3 * ms[4]),
-               "rk4_gbs": 32 * nm / (1e-3 * ms[4]) / 1e9}
+               "rk4_gbs": 32 * nm / (1e-3 * ms[4]) / 1e9, "eager_step_ms": sum(ms)}
+    # the same step as one hipGraph replay (sf_graph_*: the PT loop on the device), same cap
+    g = sfhip.StepGraph(st, x, y, rho, mu, dx, dy, 0.0, g_y=1.0, maxdisp=0.5, eps=1e-300, iterMax=iters)
+    g.step(1)  # warm-up replay
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    _, gst = g.step(1)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    out["captured_step_ms"] = ev[0].elapsed_time(ev[1])
+    out["captured_solve_iters"] = gst["niter"]
+    g.close()
     del x, y, rho, mu, st
     torch.cuda.empty_cache()
     return out

@@ -228,12 +228,12 @@ int sf_solve_stokes_refined(sf_ctx *ctx, double *P, double *Vx_pad, double *Vy_p
  * WIDE: one launch per PT iteration covers every instance of an active set whose working
  * sets fit budget_mb; each instance is checked at its own ncheck iterations and retired on
  * convergence, and the set is refilled from the largest-first queue.
- * GRID: the instances one after another, each solved by ONE launch whose waves span the chip
- * (at most as many one-wave workgroups as are resident at once): the whole PT loop with its
- * checks, dt refreshes and convergence test runs on the device, the passes (up to three PT
- * iterations each, or prm->depth) separated by a grid barrier;
- * no host synchronisation, so it can be captured (sf_graph_*).  A barrier wait longer than 5 s
- * (waves not all resident) ends the launch and the call returns SF_ERR_HIP.
+ * GRID: the instances one after another, each solved by ONE launch that spans the chip (at most
+ * one workgroup of one to eight waves per CU): the whole PT loop with its checks, dt refreshes
+ * and convergence test runs on the device, the passes (up to three PT iterations each, or
+ * prm->depth) separated by a grid barrier; no host synchronisation, so it can be captured
+ * (sf_graph_*).  A barrier wait longer than 5 s (workgroups not all resident) ends the launch
+ * and the call returns SF_ERR_HIP.
  * With precision 1 (WIDE only) every instance is solved as sf_solve_stokes_refined solves it,
  * with one launch per FP32 sweep of all active instances.
  * In FP64 every PT iteration is bit-identical to sf_solve_stokes.  outs[i] gets instance i's dt_new,

@@ -194,13 +194,20 @@ BatchShape grid_shape(int nx, int ny)
     return sh;
 }
 
-// Grid mode: workgroups of an instance's launch, one wave per tile of its largest pass, at most
-// all resident ones.
-int grid_workgroups(const BatchInstance& d, int k)
+// Grid mode: the launch of an instance, one wave per tile of its largest pass spread one wave per
+// CU first, then up to GRID_WG_WAVES waves per workgroup (at most one workgroup per CU).
+GridShape grid_launch_shape(const BatchInstance& d, int k)
 {
     int t = d.wpt[0].ntiles;
     if (k >= 2) t = std::max(t, d.wpt[2 * k - 2].ntiles);
-    return std::max(1, std::min((t + GRID_WG_WAVES - 1) / GRID_WG_WAVES, grid_capacity(k)));
+    int cus = 256;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
+    const int waves = std::max(1, std::min((t + cus - 1) / cus, (int)GRID_WG_WAVES));
+    const int nwg = std::max(1, std::min((t + waves - 1) / waves, grid_capacity(k, waves)));
+    return GridShape{nwg, waves};
 }
 
 // Grid mode: the barrier words (two per instance) and the partials of the context.
@@ -1020,8 +1027,9 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
         const int k = (prm && prm->depth >= 1 && prm->depth <= 3) ? prm->depth : 3;
         for (int qi = 0; qi < n; ++qi) {
             const int i = order[(size_t)qi];
+            const GridShape gs = grid_launch_shape(host[(size_t)i], k);
             const GridArgs ga{c->b_inst + i, c->b_res + i, c->g_sync + (size_t)GRID_SYNC_WORDS * i, c->g_part,
-                              grid_workgroups(host[(size_t)i], k), 0, grid_wait_limit()};
+                              gs.nwg, gs.waves, grid_wait_limit()};
             HIPCHK(launch_grid_solve(ga, k, s));
         }
         std::vector<unsigned long long> sync((size_t)GRID_SYNC_WORDS * n);

@@ -31,7 +31,7 @@ struct sf_ensemble {
     unsigned long long* gsync = nullptr;  // grid mode: GRID_SYNC_WORDS barrier words per member
     unsigned long long* hsync = nullptr;  // pinned
     double* gpart = nullptr;              // grid mode: 3 * GRID_MAX_WAVES partials
-    std::vector<int> gwg;                 // grid mode: workgroups of each member's launch
+    std::vector<GridShape> gshape;        // grid mode: each member's launch
     hipGraph_t graph[2] = {nullptr, nullptr};  // persistent: [0] whole step; wide: [0] marker -> grid, [1] RK4
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     M2GScratch m2g;  // own sort scratch: the captured graphs hold its addresses
@@ -123,7 +123,8 @@ static hipError_t ensemble_enqueue_persistent(sf_ensemble* e)
     if (e->grid) {
         for (int q = 0; q < e->n && r == hipSuccess; ++q) {
             const int i = e->order[(size_t)q];
-            const GridArgs ga{e->inst + i, e->res + i, e->gsync + (size_t)GRID_SYNC_WORDS * i, e->gpart, e->gwg[(size_t)i], 0,
+            const GridShape& gs = e->gshape[(size_t)i];
+            const GridArgs ga{e->inst + i, e->res + i, e->gsync + (size_t)GRID_SYNC_WORDS * i, e->gpart, gs.nwg, gs.waves,
                               grid_wait_limit()};
             r = launch_grid_solve(ga, e->gk, e->stream);
         }
@@ -255,8 +256,8 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
     }
     e->order = batch_order(dims);
     if (e->grid) {
-        e->gwg.resize((size_t)n);
-        for (int i = 0; i < n; ++i) e->gwg[(size_t)i] = grid_workgroups(e->desc[(size_t)i], e->gk);
+        e->gshape.resize((size_t)n);
+        for (int i = 0; i < n; ++i) e->gshape[(size_t)i] = grid_launch_shape(e->desc[(size_t)i], e->gk);
     }
     e->nblocks = batch_grid(c, n, threads, prm ? prm->max_blocks : 0);
     if (e->refined) {

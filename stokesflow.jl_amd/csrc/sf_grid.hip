@@ -2,27 +2,29 @@
 //
 // solveStokes! (StokesSolver_multixpu.jl:43-95) has a data-dependent loop (convergence is tested
 // every ncheck iterations), which a hipGraph cannot hold.  Here the loop runs on the device: a
-// launch of at most as many one-wave workgroups as the chip holds at once walks the same pass
-// schedule as sf_solve_stokes (one to four PT iterations per pass, the single iterations at
-// checks, dt refreshes and iterMax), each wave taking tiles t = wave, wave + G, ... of every pass,
-// and a grid barrier separates the passes.  The dt refresh and the err check reduce over the
+// launch of at most one workgroup per CU, of one to eight waves, walks the same pass schedule as
+// sf_solve_stokes (passes of up to three PT iterations, the single iterations at checks, dt
+// refreshes and iterMax), each wave taking tiles t = wave, wave + G, ... of every pass, and a grid
+// barrier separates the passes.  Small grids get one wave per CU (a wave alone on a SIMD runs its
+// tile faster), larger ones up to eight (fewer tiles per wave).  The dt refresh and the err check reduce over the
 // whole grid in two steps (a partial per wave, then every wave sums the partials in the same
 // order, so every wave takes the same decision).  Each tile is the code of the single solver's
 // pass kernels (sf_pt_tile.h, sf_pt2_tile.h, sf_ptk_tile.h), so every field is bit-identical to
 // sf_solve_stokes; err is a sum of the same terms in another order (equal to rounding).
 //
 // The barrier is two-level, because an agent-scope release writes back the whole L2 of its XCD
-// and agent-scope atomics on one word are serialised (measured, 2,048 one-wave workgroups: 98 us
-// per barrier with a release, an arrival and a poll per wave on one counter; 24 us with one
-// counter per XCD and one release per XCD).  A wave's first lane waits for its stores to reach
-// the XCD's L2 (a workgroup-scope release), counts the wave in on its XCD's counter, and the
-// XCD's last wave writes that L2 back (release, agent scope) and counts the XCD in on the global
-// counter, which every wave polls before invalidating its caches (acquire, agent scope).  The
-// counters are monotone within a launch and zeroed by a kernel before it; the waves per XCD are
-// counted at the start.  The waves must all be resident, which the host ensures by sizing the
-// launch from the kernel's occupancy; should one not be, no wave waits forever: a wait longer
-// than the time limit (or a flag raised by another wave) sets the failure flag, every wave leaves
-// the kernel, and the host reports the error.
+// and agent-scope atomics on one word are serialised (measured with one-wave workgroups: 98 us per
+// barrier at 2,048 waves with a release, an arrival and a poll per wave on one counter; 24 us
+// with one counter per XCD and one release per XCD; 4 us at 256 arrivals).  A workgroup's waves
+// meet at the workgroup barrier (their stores are then in the XCD's L2); its first thread counts
+// the workgroup in on its XCD's counter; the XCD's last workgroup writes that L2 back (release,
+// agent scope) and counts the XCD in on the global counter, which every workgroup's first thread
+// polls before invalidating the caches (acquire, agent scope) and letting its waves go.  The
+// counters are monotone within a launch and zeroed by a kernel before it; the workgroups per XCD
+// are counted at the start.  The workgroups must all be resident, which the host ensures by
+// sizing the launch from the kernel's occupancy; should one not be, nothing waits forever: a wait
+// longer than the time limit (or a flag raised by another workgroup) sets the failure flag, every
+// workgroup leaves the kernel, and the host reports the error.
 #include "sf_ptk_tile.h"
 
 namespace sf {
@@ -64,12 +66,12 @@ struct GridBarrier {
     u64* s;          // sync words (layout in sf_internal.h)
     u64 gtarget;     // global arrivals that complete the next barrier
     u64 xtarget;     // arrivals on this XCD's counter that complete it
-    u64 n_xcd;       // XCDs with waves of this launch
-    u64 w_here;      // waves of this launch on this XCD
-    uint64_t limit;  // wall-clock ticks a wave may wait
+    u64 n_xcd;       // XCDs with workgroups of this launch
+    u64 w_here;      // workgroups of this launch on this XCD
+    uint64_t limit;  // wall-clock ticks a workgroup may wait
     int xcd;
 
-    // Counts the waves per XCD (once, at the start).
+    // Counts the workgroups per XCD (once, at the start).
     __device__ bool init(u64* sync, uint64_t lim)
     {
         s = sync;
@@ -86,16 +88,19 @@ struct GridBarrier {
             for (int x = 0; x < GRID_MAX_XCD; ++x) nx += ld_agent(s + GRID_SYNC_XCOUNT + 16 * x) > 0ull;
             here = ld_agent(s + GRID_SYNC_XCOUNT + 16 * xcd);
         }
-        n_xcd = __shfl(nx, 0);
-        w_here = __shfl(here, 0);
-        return __shfl(ok, 0) != 0;
+        __shared__ u64 sh[2];
+        if (threadIdx.x == 0) { sh[0] = nx; sh[1] = here; }
+        const int bad = __syncthreads_or(threadIdx.x == 0 && !ok);
+        n_xcd = sh[0];
+        w_here = sh[1];
+        return bad == 0;
     }
 
     __device__ bool sync()
     {
         gtarget += n_xcd;
         xtarget += w_here;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // this wave's stores are in the L2
+        __syncthreads();  // the workgroup's stores are in this XCD's L2
         int ok = 1;
         if (threadIdx.x == 0) {
             if (add_agent(s + GRID_SYNC_XARRIVE + 16 * xcd) + 1ull == xtarget) {
@@ -104,9 +109,8 @@ struct GridBarrier {
             }
             ok = wait_agent(s + GRID_SYNC_ARRIVE, gtarget, s + GRID_SYNC_FAIL, limit);
         }
-        ok = __shfl(ok, 0);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        return ok != 0;
+        if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        return __syncthreads_or(threadIdx.x == 0 && !ok) == 0;
     }
 };
 
@@ -127,14 +131,14 @@ __device__ inline double wave_max(double v)
 __device__ inline double partials_sum(const double* part, int nw)
 {
     double s = 0.0;
-    for (int k = threadIdx.x; k < nw; k += 64) s = s + part[k];
+    for (int k = threadIdx.x & 63; k < nw; k += 64) s = s + part[k];
     return wave_sum(s);
 }
 
 __device__ inline double partials_max(const double* part, int nw)
 {
     double m = -__builtin_inf();
-    for (int k = threadIdx.x; k < nw; k += 64) m = jl_max(m, part[k]);
+    for (int k = threadIdx.x & 63; k < nw; k += 64) m = jl_max(m, part[k]);
     return wave_max(m);
 }
 
@@ -189,11 +193,13 @@ __device__ inline void grid_copy(double* __restrict__ dst, const double* __restr
 }  // namespace
 
 template <int K>
-__global__ __launch_bounds__(64) void grid_solve_kernel(GridArgs a)
+__global__ __launch_bounds__(64 * GRID_WG_WAVES) void grid_solve_kernel(GridArgs a)
 {
     CGrid& I = *(CGrid*)a.desc;
-    const int nw = gridDim.x, w = blockIdx.x;
-    const int64_t gtid = (int64_t)w * 64 + threadIdx.x, gstride = (int64_t)nw * 64;
+    // w is uniform in the wave (saying so keeps the tile indices in scalar registers)
+    const int nw = gridDim.x * (blockDim.x >> 6),
+              w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+    const int64_t gtid = (int64_t)w * 64 + (threadIdx.x & 63), gstride = (int64_t)nw * 64;
     GridBarrier bar;
     if (!bar.init(a.sync, a.limit)) return;
     double* part_a = a.part;           // nw partials (max of |Vx|, sum of |Rx|)
@@ -210,7 +216,7 @@ __global__ __launch_bounds__(64) void grid_solve_kernel(GridArgs a)
     const int64_t iter_max = I.iter_max;
     const bool fss = I.use_fss != 0;
 
-    if (w == 0 && threadIdx.x == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         const double* dd = I.dt_dev;
         I.scal[SC_DT] = fss ? (dd ? *dd : I.dt_in) : 0.0;
     }
@@ -220,7 +226,7 @@ __global__ __launch_bounds__(64) void grid_solve_kernel(GridArgs a)
     auto grid_dt = [&](const double* vx, const double* vy, double* dtc) -> bool {
         const double mx = wave_max(lane_absmax(vx, nvx, gtid, gstride));
         const double my = wave_max(lane_absmax(vy, nvy, gtid, gstride));
-        if (threadIdx.x == 0) {
+        if ((threadIdx.x & 63) == 0) {
             part_a[w] = mx;
             part_b[w] = my;
         }
@@ -257,7 +263,7 @@ __global__ __launch_bounds__(64) void grid_solve_kernel(GridArgs a)
             if (fss && (iter + K - 1) % ndtupdate == 0) {
                 double dtc;
                 if (!grid_dt(pp.vx_b, pp.vy_b, &dtc)) return;
-                if (w == 0 && threadIdx.x == 0) I.scal[SC_DT] = dtc;
+                if (blockIdx.x == 0 && threadIdx.x == 0) I.scal[SC_DT] = dtc;
                 if (!bar.sync()) return;
             }
             iter += K;
@@ -288,7 +294,7 @@ __global__ __launch_bounds__(64) void grid_solve_kernel(GridArgs a)
             const double s0 = wave_sum(lane_abssum(pp.rx, (int64_t)n_rx, gtid, gstride));
             const double s1 = wave_sum(lane_abssum(I.ry, (int64_t)n_ry, gtid, gstride));
             const double s2 = wave_sum(lane_abssum(pp.divv, (int64_t)n_dv, gtid, gstride));
-            if (threadIdx.x == 0) {
+            if ((threadIdx.x & 63) == 0) {
                 part_a[w] = s0;
                 part_b[w] = s1;
                 part_c[w] = s2;
@@ -334,7 +340,7 @@ __global__ __launch_bounds__(64) void grid_solve_kernel(GridArgs a)
     }
     double dt_new;
     if (!grid_dt(Vx_s, Vy_s, &dt_new)) return;
-    if (w == 0 && threadIdx.x == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         BatchResult* R = a.res;
         R->dt_new = dt_new;
         R->err = err;
@@ -349,22 +355,25 @@ __global__ void grid_zero_sync(unsigned long long* p)
     for (int k = threadIdx.x; k < GRID_SYNC_WORDS; k += blockDim.x) p[k] = 0ull;
 }
 
-// Resident one-wave workgroups of grid_solve_kernel<k> on the current device.
-int grid_capacity(int k)
+// Resident workgroups of `waves` waves of grid_solve_kernel<k> on the current device, at most
+// one per CU.
+int grid_capacity(int k, int waves)
 {
-    static int cap[4] = {0, 0, 0, 0};
-    if (k < 1 || k > 3) return 0;
-    if (cap[k] == 0) {
+    static int cap[4][GRID_WG_WAVES + 1] = {};
+    if (k < 1 || k > 3 || waves < 1 || waves > GRID_WG_WAVES) return 0;
+    if (cap[k][waves] == 0) {
         int dev = 0, cus = 0, per_cu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
             cus = 256;
         const void* f = k == 3 ? (const void*)grid_solve_kernel<3>
                                : (k == 2 ? (const void*)grid_solve_kernel<2> : (const void*)grid_solve_kernel<1>);
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64, 0) != hipSuccess || per_cu < 1) per_cu = 1;
-        cap[k] = std::min(cus * per_cu, (int)GRID_MAX_WAVES);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64 * waves, 0) != hipSuccess || per_cu < 1)
+            cap[k][waves] = -1;
+        else
+            cap[k][waves] = cus;
     }
-    return cap[k];
+    return std::max(cap[k][waves], 0);
 }
 
 uint64_t grid_wait_limit()
@@ -381,11 +390,12 @@ uint64_t grid_wait_limit()
 
 hipError_t launch_grid_solve(const GridArgs& a, int k, hipStream_t s)
 {
-    if (a.nwg < 1 || a.nwg > grid_capacity(k)) return hipErrorInvalidValue;
+    if (a.nwg < 1 || a.nwg > grid_capacity(k, a.wg_waves)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(grid_zero_sync, dim3(1), dim3(256), 0, s, a.sync);
-    if (k == 3) hipLaunchKernelGGL(grid_solve_kernel<3>, dim3(a.nwg), dim3(64), 0, s, a);
-    else if (k == 2) hipLaunchKernelGGL(grid_solve_kernel<2>, dim3(a.nwg), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL(grid_solve_kernel<1>, dim3(a.nwg), dim3(64), 0, s, a);
+    const dim3 g(a.nwg), b(64 * a.wg_waves);
+    if (k == 3) hipLaunchKernelGGL(grid_solve_kernel<3>, g, b, 0, s, a);
+    else if (k == 2) hipLaunchKernelGGL(grid_solve_kernel<2>, g, b, 0, s, a);
+    else hipLaunchKernelGGL(grid_solve_kernel<1>, g, b, 0, s, a);
     return hipGetLastError();
 }
 

@@ -86,7 +86,10 @@ struct BatchTune {
 };
 BatchShape batch_shape(const sf_ctx* c, int nx, int ny);
 BatchShape grid_shape(int nx, int ny);
-int grid_workgroups(const BatchInstance& d, int k);
+struct GridShape {
+    int nwg, waves;  // workgroups, waves per workgroup
+};
+GridShape grid_launch_shape(const BatchInstance& d, int k);
 BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, const double* dt_dev,
                                const BatchShape& sh = BatchShape());
 std::vector<int> batch_order(const std::vector<std::pair<int, int>>& dims);

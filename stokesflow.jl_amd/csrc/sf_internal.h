@@ -181,14 +181,15 @@ struct BatchResult {
     int converged, pad;
 };
 
-// Grid mode (sf_grid.hip): one instance's whole solve in one launch of nwg one-wave workgroups
-// (at most grid_capacity(k), all resident), the passes separated by a grid barrier.  sync holds
-// GRID_SYNC_WORDS words, zeroed by launch_grid_solve: [GRID_SYNC_ARRIVE] XCDs arrived,
-// [GRID_SYNC_FAIL] failure flag (a barrier wait exceeded the limit), [GRID_SYNC_START] waves
-// started, [GRID_SYNC_XARRIVE + 16 x] waves of XCD x arrived, [GRID_SYNC_XCOUNT + 16 x] waves on
-// XCD x (one 128-byte line each); part holds 3 * nwg partials.
+// Grid mode (sf_grid.hip): one instance's whole solve in one launch of nwg workgroups of wg_waves
+// waves (at most one workgroup per CU, grid_capacity(k, wg_waves), all resident), the passes
+// separated by a grid barrier.  sync holds GRID_SYNC_WORDS words, zeroed by launch_grid_solve:
+// [GRID_SYNC_ARRIVE] XCDs arrived, [GRID_SYNC_FAIL] failure flag (a barrier wait exceeded the
+// limit), [GRID_SYNC_START] workgroups started, [GRID_SYNC_XARRIVE + 16 x] workgroups of XCD x
+// arrived, [GRID_SYNC_XCOUNT + 16 x] workgroups on XCD x (one 128-byte line each); part holds
+// 3 * wg_waves * nwg partials.
 enum {
-    GRID_WG_WAVES = 1,
+    GRID_WG_WAVES = 8,
     GRID_MAX_WAVES = 4096,
     GRID_WAIT_SECONDS = 5,
     GRID_MAX_XCD = 16,
@@ -204,11 +205,12 @@ struct GridArgs {
     BatchResult* res;
     unsigned long long* sync;
     double* part;
-    int nwg, pad;
-    uint64_t limit;  // wall-clock ticks a wave may wait at a barrier
+    int nwg, wg_waves;
+    uint64_t limit;  // wall-clock ticks a workgroup may wait at a barrier
 };
-// k = PT iterations per pass (1, 2 or 3; four need more than 256 VGPRs in this kernel).
-int grid_capacity(int k);
+// k = PT iterations per pass (1, 2 or 3; four need more than 256 VGPRs in this kernel); waves =
+// waves per workgroup (1 to GRID_WG_WAVES).
+int grid_capacity(int k, int waves);
 uint64_t grid_wait_limit();
 hipError_t launch_grid_solve(const GridArgs& a, int k, hipStream_t s);
 
