@@ -400,8 +400,8 @@ def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 1.0), eps=1e-5,
     cell): upper (small-y) layer rho 1.1 and viscosity 10^c, lower layer rho 1 and viscosity 1,
     interface y = 0.5 + 0.02 sin(2 pi k x), k uniform over 1..8, c uniform over ``log10c``.
     The same steps run several ways from the same initial state: the ensemble step in its
-    wide and persistent forms, each with hipGraphs and enqueued eagerly, and sf_step per
-    instance in a loop.  All must agree bit for bit.  eps defaults to the reference's timestep
+    wide and persistent forms, each with hipGraphs and enqueued eagerly, in its grid form (the
+    whole step one hipGraph, one chip-wide solve per member), and sf_step per instance in a loop.  All must agree bit for bit.  eps defaults to the reference's timestep
     loop tolerance (stokes_eps = 1e-5, StokesFlow2D_multixpu.jl:56).  ``modes`` restricts the
     runs to the named ones (for large ensembles)."""
     import numpy as np
@@ -427,7 +427,7 @@ def run_ensemble_steps(n_inst, n, steps, seed=5000, log10c=(0.0, 1.0), eps=1e-5,
            "log10_contrast": list(log10c), "eps": eps}
     finals = {}
     runs = [("wide_graph", "wide", True, False), ("wide_eager", "wide", False, False),
-            ("wide_refined_graph", "wide", True, True)]
+            ("wide_refined_graph", "wide", True, True), ("grid_graph", "grid", True, False)]
     if n <= 128:  # one CU per member: only for small members
         runs += [("persistent_graph", "persistent", True, False), ("persistent_eager", "persistent", False, False)]
     runs += [("loop", None, None, False)]
@@ -520,8 +520,8 @@ def main():
     ap.add_argument("--ens-steps-eps", type=float, default=1e-5,
                     help="solver tolerance of the --ens-steps timesteps (the reference's timestep loop uses 1e-5)")
     ap.add_argument("--ens-steps-modes", nargs="+", default=None,
-                    choices=["wide_graph", "wide_eager", "wide_refined_graph", "persistent_graph", "persistent_eager",
-                             "loop"], help="run only these forms of the --ens-steps comparison")
+                    choices=["wide_graph", "wide_eager", "wide_refined_graph", "grid_graph", "persistent_graph",
+                             "persistent_eager", "loop"], help="run only these forms of the --ens-steps comparison")
     ap.add_argument("--progress", action="store_true", help="print the solvers' convergence checks (long runs)")
     ap.add_argument("--refined-iters", type=int, default=4000,
                     help="FP32 correction sweeps of the refined solver timed on the same state (roofline_fp32); 0 "

@@ -115,13 +115,21 @@ BatchInstance batch_descriptor(const sf_batch_instance& a, int rows, double* q, 
         w.nstrips = (nx - 1 + 59) / 60;
         w.ntiles = w.nstrips * ((ny + rows2 - 1) / rows2);
         b.wpt[2 + par] = w;
-        const int rows3 = std::min(ny, sh.rows3 > 0 ? sh.rows3 : (std::max(nx, ny) > 256 ? 16 : 8));
+        // few instances fill the chip only with shorter tiles: below WIDE_MAX instances the height
+        // comes from the wave-round model over that many copies of the grid (16 x 256^2, one box:
+        // 48-row four-iteration tiles ran 480 waves per launch)
+        const bool model = sh.copies > 0 && sh.copies < WIDE_MAX;
+        const int rows3 = std::min(ny, sh.rows3 > 0 ? sh.rows3
+                                                    : (model ? ptk_default_rows(3, nx, ny, sh.copies)
+                                                             : (std::max(nx, ny) > 256 ? 16 : 8)));
         w.seg = rows3;
         w.nstrips = (nx - 1 + 57) / 58;
         w.ntiles = w.nstrips * ((ny + rows3 - 1) / rows3);
         b.wpt[4 + par] = w;
         const int n = std::max(nx, ny);
-        const int rows4 = std::min(ny, sh.rows4 > 0 ? sh.rows4 : (n > 384 ? 64 : (n > 192 ? 48 : 8)));
+        const int rows4 = std::min(ny, sh.rows4 > 0 ? sh.rows4
+                                                    : (model ? ptk_default_rows(4, nx, ny, sh.copies)
+                                                             : (n > 384 ? 64 : (n > 192 ? 48 : 8))));
         w.seg = rows4;
         w.nstrips = (nx - 1 + 55) / 56;
         w.ntiles = w.nstrips * ((ny + rows4 - 1) / rows4);
@@ -971,10 +979,11 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
     const int mode = prm ? prm->mode : SF_BATCH_AUTO;
     const int precision = prm ? prm->precision : 0;
     std::vector<BatchInstance> host((size_t)n);
-    for (int i = 0; i < n; ++i)
-        host[(size_t)i] = batch_descriptor(in[i], rows, c->b_pool + off[(size_t)i], nullptr,
-                                           mode == SF_BATCH_GRID ? grid_shape(in[i].nx, in[i].ny)
-                                                                 : batch_shape(c, in[i].nx, in[i].ny));
+    for (int i = 0; i < n; ++i) {
+        BatchShape sh = mode == SF_BATCH_GRID ? grid_shape(in[i].nx, in[i].ny) : batch_shape(c, in[i].nx, in[i].ny);
+        if (mode != SF_BATCH_GRID) sh.copies = n;
+        host[(size_t)i] = batch_descriptor(in[i], rows, c->b_pool + off[(size_t)i], nullptr, sh);
+    }
     std::vector<std::pair<int, int>> dims((size_t)n);
     for (int i = 0; i < n; ++i) dims[(size_t)i] = {in[i].nx, in[i].ny};
     const std::vector<int> order = batch_order(dims);

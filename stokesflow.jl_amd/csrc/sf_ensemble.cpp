@@ -246,9 +246,10 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
     e->dt.resize((size_t)n);
     std::vector<std::pair<int, int>> dims((size_t)n);
     for (int i = 0; i < n; ++i) {
+        BatchShape sh = e->grid ? grid_shape(members[i].nx, members[i].ny) : batch_shape(c, members[i].nx, members[i].ny);
+        if (!e->grid) sh.copies = n;
         e->desc[(size_t)i] = batch_descriptor(bi[(size_t)i], rows, e->pool + off[(size_t)i], e->wide ? nullptr : &e->res[i].dt_new,
-                                              e->grid ? grid_shape(members[i].nx, members[i].ny)
-                                                      : batch_shape(c, members[i].nx, members[i].ny));
+                                              sh);
         dims[(size_t)i] = {members[i].nx, members[i].ny};
         std::memset(&e->hres[i], 0, sizeof(BatchResult));
         e->hres[i].dt_new = members[i].use_fss ? members[i].dt : 0.0;

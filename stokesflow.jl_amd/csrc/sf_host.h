@@ -79,6 +79,9 @@ int batch_check(const sf_batch_instance& a, int i, const char* who);
 // passes (0: size default).  sf_batch_autotune keeps one per grid size on the context.
 struct BatchShape {
     int kmax = 0, rows2 = 0, rows3 = 0, rows4 = 0;
+    // instances of the batch: below WIDE_MAX the default three- and four-iteration tile heights
+    // come from the wave-round model over that many copies of the grid (0: the fixed table)
+    int copies = 0;
 };
 struct BatchTune {
     int nx, ny;

@@ -76,7 +76,8 @@ int pt2_default_rows(int nx, int ny);
 // k = 2, 3 or 4 PT iterations per pass (k >= 3: ptk_tile<k>, 64 - 2k output columns per strip;
 // p.seg <= 0: ptk_default_rows).
 hipError_t launch_pt_multi(const PTParams& p, int k, hipStream_t s);
-int ptk_default_rows(int k, int nx, int ny);  // tile height of a k-iteration pass (k >= 3)
+// tile height of a k-iteration pass (k >= 3) when `copies` grids of this size share the launch
+int ptk_default_rows(int k, int nx, int ny, int copies = 1);
 // Snapshot staging: dst = (float)(src + shift) (sf_snapshot.hip).
 hipError_t launch_snap_cast(const double* src, float* dst, int64_t n, double shift, hipStream_t s);
 int pt_default_fusion(int nx, int ny);  // iterations per pass when the context leaves it open

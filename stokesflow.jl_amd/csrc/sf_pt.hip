@@ -329,7 +329,7 @@ int corr_default_fusion(int nx, int ny)
 // rows 7.75, 12 7.9, 8 8.9; 512^2 6 rows 5.5, 8 6.3, 4 6.5, 12 7.9; 64^2-256^2 2 rows.  K = 3 in
 // the solver loop: 4096^2 48 rows 136.0, 32 141.8; 3072^2 40 rows 74.0, 28 76.7; 2048^2 40 rows
 // 34.4, 20 36.6; 1536^2 24 rows 19.7, 12 21.2.
-int ptk_default_rows(int k, int nx, int ny)
+int ptk_default_rows(int k, int nx, int ny, int copies)
 {
     static const int simds = [] {
         int dev = 0, cus = 0;
@@ -343,7 +343,7 @@ int ptk_default_rows(int k, int nx, int ny)
     int best = 16;
     double best_cost = 1e300;
     for (int rows = 2; rows <= 64; rows += rows < 12 ? 2 : 4) {
-        const int64_t waves = strips * ((ny + rows - 1) / rows);
+        const int64_t waves = (int64_t)copies * strips * ((ny + rows - 1) / rows);
         const double steps = rows + 2 * k - 1;
         const double cost = waves <= simds ? steps : 1.38 * (double)((waves + 2 * simds - 1) / (2 * simds)) * steps;
         if (cost < best_cost) {
