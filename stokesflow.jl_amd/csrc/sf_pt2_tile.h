@@ -40,7 +40,7 @@ namespace sf {
 // lane's own values are carried; their neighbours one or two lanes down are moved in again
 // on every row (a DPP move is exact, so the values are the same), which keeps the window at
 // 9 doubles instead of 16: a three-stage pass takes 198 VGPRs instead of 236, and a four-stage
-// pass 238, still two waves per SIMD.
+// pass 238 (251 in the split loop form it runs), still two waves per SIMD.
 struct PTWin {
     double vx_r, vy_r;
     double rho_r, rho_rm1_p1;
