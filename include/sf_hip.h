@@ -254,7 +254,7 @@ typedef struct {
     int rows;         /* grid rows per tile (<= 0: per instance, as the single solver's default) */
     int depth;        /* wide: rows of operands prefetched, 1 or 2 (<= 0: 1); grid: PT iterations per
                          pass, 1 to 3 (<= 0: 3) */
-    int threads;      /* persistent: threads per instance workgroup, multiple of 64, <= 768 (<= 0: 768) */
+    int threads;      /* persistent: threads per instance workgroup, multiple of 64, <= 512 (<= 0: 512) */
     int max_blocks;   /* persistent: cap on concurrently resident instances (<= 0: as many as fit) */
     int mode;         /* SF_BATCH_*; AUTO = WIDE */
     int precision;    /* wide: 0 FP64 PT iterations; 1 FP32 PT sweeps inside FP64 defect correction (the

@@ -1,14 +1,15 @@
 // sf_batch.hip -- batched ensemble of independent Stokes solves in one persistent launch.
 //
-// Each workgroup (12 waves) takes an instance from a device work queue (largest first),
-// runs that instance's whole PT solve -- the loop of solveStokes!
+// Each workgroup (eight waves: one CU) takes an instance from a device work queue (largest
+// first), runs that instance's whole PT solve -- the loop of solveStokes!
 // (StokesSolver_multixpu.jl:43-77) with its ncheck / ndtupdate rules, the err and dt
 // reductions and the padded/mirror epilogue (:89-95) -- and takes the next instance when
-// it converged or hit iterMax.  The per-iteration barrier is the workgroup barrier, so no
-// workgroup ever waits for another and every instance exits on its own.  The waves of the
-// workgroup share the instance's tiles; each tile is the same code as the single-solve
-// kernel (sf_pt_tile.h), so every iteration is bit-identical to a single solve.  Reductions
-// are block-wide in a fixed order (deterministic).
+// it converged or hit iterMax.  Plain iterations run three per pass (ptk_tile<3>, the rule of
+// sf_solve_stokes), the others one per pass; the barrier between passes is the workgroup
+// barrier, so no workgroup ever waits for another and every instance exits on its own.  The
+// waves of the workgroup share the instance's tiles; each tile is the same code as the single
+// solver's pass kernels, so every iteration is bit-identical to a single solve.  Reductions are
+// block-wide in a fixed order (deterministic).
 //
 // The instance descriptors (with the tile parameters of both ping-pong directions,
 // prepared on the host) are read through the constant address space: scalar loads that the
@@ -18,9 +19,9 @@
 
 namespace sf {
 
-// 12 waves per instance workgroup: the PT tile needs ~150 VGPRs, which leaves 3 waves per SIMD
-#define BATCH_MAX_THREADS 768
-#define BATCH_RED_POW2 1024  // power of two >= BATCH_MAX_THREADS for the reduction tree
+// 8 waves per instance workgroup: the three-iteration tile needs ~200 VGPRs, two waves per SIMD
+#define BATCH_MAX_THREADS 512
+#define BATCH_RED_POW2 512  // power of two >= BATCH_MAX_THREADS for the reduction tree
 #define SF_CONST __attribute__((address_space(4)))
 
 typedef const BatchInstance SF_CONST CBatch;
@@ -89,7 +90,7 @@ __device__ inline void block_copy(double* __restrict__ dst, const double* __rest
 __device__ void solve_instance(CBatch& I, BatchResult* __restrict__ R, double* sh)
 {
     const int nx = I.nx, ny = I.ny;
-    const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), nwaves = blockDim.x >> 6;
     const int mn = nx < ny ? nx : ny;
     const int64_t ncheck = (5 * (int64_t)mn < 2000) ? 5 * (int64_t)mn : 2000;
     const int64_t ndtupdate = 10 * (int64_t)mn;
@@ -107,9 +108,31 @@ __device__ void solve_instance(CBatch& I, BatchResult* __restrict__ R, double* s
 
     double err = 2.0 * eps;
     int64_t iter = 1;
+    int pf = 0;  // passes so far, mod 2: the state (V, P, dV) is in the caller's buffers when 0
     while (err > eps && iter <= iter_max) {
-        const int par = (int)((iter - 1) & 1);  // odd iterations: caller -> scratch; even: back
-        const PTParams SF_CONST& pp = I.pt[par];
+        // iterations iter..iter+2 in one pass when none is a check iteration, none but the last is
+        // followed by a dt refresh and the last is not iter_max (the rule of sf_solve_stokes)
+        bool fused = iter + 2 < iter_max;
+        for (int64_t j = iter; fused && j < iter + 3; ++j) {
+            if (j % ncheck == 0) fused = false;
+            if (j < iter + 2 && fss && j % ndtupdate == 0) fused = false;
+        }
+        if (fused) {
+            const PTParams SF_CONST& pp = I.wpt[4 + pf];
+            const int ntiles = pp.ntiles, nstrips = pp.nstrips;
+            for (int t = wave; t < ntiles; t += nwaves) ptk_tile<3, 0>(pp, t % nstrips, t / nstrips);
+            pf ^= 1;
+            __syncthreads();
+            if (fss && (iter + 2) % ndtupdate == 0) {
+                const double mx = block_absmax(pp.vx_b, nvx, sh);
+                const double my = block_absmax(pp.vy_b, nvy, sh);
+                if (threadIdx.x == 0) I.scal[SC_DT] = I.maxdisp * jl_min(I.dx / mx, I.dy / my);
+                __syncthreads();
+            }
+            iter += 3;
+            continue;
+        }
+        const PTParams SF_CONST& pp = I.wpt[pf];
         const bool check = (iter % ncheck) == 0;
         const bool last = iter == iter_max;
         const int flags = ((check || last) ? PT_WRITE_TAU : 0) | (check ? PT_WRITE_RX : 0);
@@ -119,6 +142,7 @@ __device__ void solve_instance(CBatch& I, BatchResult* __restrict__ R, double* s
         } else {
             for (int t = wave; t < ntiles; t += nwaves) pt_tile<1, false>(pp, t % nstrips, t / nstrips, 0);
         }
+        pf ^= 1;
         __syncthreads();
 
         const bool dt_upd = fss && (iter % ndtupdate) == 0;
@@ -141,11 +165,13 @@ __device__ void solve_instance(CBatch& I, BatchResult* __restrict__ R, double* s
         iter += 1;
     }
     const int64_t niter = iter - 1;
-    if (niter & 1) {  // the final state is in the scratch buffers
-        const PTParams SF_CONST& p0 = I.pt[0];
+    if (pf) {  // the last iterate (state and dV) is in the scratch buffers
+        const PTParams SF_CONST& p0 = I.wpt[0];
         block_copy(I.Vx_s, p0.vx_b, nvx);
         block_copy(I.Vy_s, p0.vy_b, nvy);
         block_copy(I.P, p0.p_b, np);
+        block_copy(p0.dvx, p0.dvx_o, (int64_t)(nx - 2) * (ny - 1));
+        block_copy(p0.dvy, p0.dvy_o, (int64_t)(nx - 1) * (ny - 2));
     }
     __syncthreads();
     // Vx_pad[2:end-1,:] .= Vx_s; Vy_pad[:,2:end-1] .= Vy_s; bc_x_mirror!; bc_y_mirror!  (:89-92)

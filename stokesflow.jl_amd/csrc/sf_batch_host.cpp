@@ -185,9 +185,9 @@ int batch_grid(sf_ctx* c, int n, int threads, int max_blocks)
 
 int batch_threads(const sf_batch_params* prm, int* threads, int* rows, const char* who)
 {
-    *threads = (prm && prm->threads > 0) ? prm->threads : 768;
+    *threads = (prm && prm->threads > 0) ? prm->threads : 512;
     *rows = (prm && prm->rows > 0) ? prm->rows : 0;  // 0: per instance, pt_default_rows
-    if (*threads % 64 != 0 || *threads > 768) return set_err(SF_ERR_ARG, "%s: threads %d", who, *threads);
+    if (*threads % 64 != 0 || *threads > 512) return set_err(SF_ERR_ARG, "%s: threads %d", who, *threads);
     return SF_OK;
 }
 

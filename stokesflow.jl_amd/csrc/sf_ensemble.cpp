@@ -10,7 +10,7 @@ struct sf_ensemble {
     sf_ctx* c = nullptr;
     hipStream_t stream = nullptr;  // own non-blocking stream: capture needs one
     hipEvent_t ev = nullptr;
-    int n = 0, threads = 768, nblocks = 0, use_graph = 1, wide = 1, depth = 1;
+    int n = 0, threads = 512, nblocks = 0, use_graph = 1, wide = 1, depth = 1;
     int grid = 0, gk = 3;  // grid mode: one chip-wide launch per member, gk PT iterations per pass
     double budget = 0.0;
     std::vector<sf_step_args> mem;
