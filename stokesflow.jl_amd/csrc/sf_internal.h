@@ -165,8 +165,8 @@ hipError_t launch_probe_r3w1(const double* a, const double* b, const double* c, 
 // runs it back (tile parameters complete, flags unused); the rest is what the solve loop
 // and the epilogue need.
 struct BatchInstance {
-    PTParams pt[2];   // one iteration, state ping-pong by parity, dV in place (persistent mode)
-    PTParams wpt[8];  // wide mode, dV ping-pong too: [parity] one iteration, [2 + parity] two, [4 + parity]
+    PTParams pt[2];   // one iteration, state ping-pong by parity, dV in place (the host's template of wpt)
+    PTParams wpt[8];  // every mode, dV ping-pong too: [parity] one iteration, [2 + parity] two, [4 + parity]
                       // three, [6 + parity] four
     double *P, *Vx_s, *Vy_s, *Vx_pad, *Vy_pad;  // caller state (in/out)
     double *ry, *scal;                          // library scratch (scal[SC_DT] = stabilisation dt)
