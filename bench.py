@@ -248,6 +248,56 @@ def timestep_stages(n, iters=31, seed=3000):
     return out
 
 
+def grid_solve(sizes=(64, 128, 256, 512, 1024, 2048, 4096), seed=3100):
+    """One problem's PT loop on the host (sfhip.solveStokes, size-default passes) against the
+    grid mode (the whole loop in one chip-wide launch, three-iteration passes) from the same plume
+    state (materials from 4 x 4 markers per cell), a fixed number of iterations per size (eps
+    never reached); best of two runs each, microseconds per PT iteration, and whether the two
+    final states are bitwise equal."""
+    import torch
+
+    import sfhip
+
+    out = {}
+    for n in sizes:
+        iters = 4000 if n <= 256 else (3000 if n <= 512 else (2000 if n <= 1024 else (800 if n <= 2048 else 400)))
+        dx = dy = 1.0 / (n - 1)
+        x, y = sfhip.init_markers(n, n, 4, dx, dy, 0.25, seed=seed)
+        rho, mu = sfhip.plume_properties(x, y, 0.1, 0.5, 0.8, 0.1, 0.001, 0.9, 1.0, 0.01, 0.1, 1.0)
+        base = sfhip.FlowState(n, n)
+        sfhip.markers_to_grids3(base, x, y, rho, mu, dx, dy)
+        del x, y, rho, mu
+        row, finals = {"iters": iters}, {}
+        for label in ("host", "grid"):
+            best = 1e30
+            for _ in range(2):
+                st = sfhip.FlowState(n, n)
+                for f in ("rho_vy", "mu_b", "mu_p"):
+                    getattr(st, f).copy_(getattr(base, f))
+                pb = {"P": st.P, "Vx_pad": st.Vx, "Vy_pad": st.Vy, "dVxdtau": st.dVxdtau, "dVydtau": st.dVydtau,
+                      "Vx_s": st.Vx_small, "Vy_s": st.Vy_small, "rho_vy": st.rho_vy, "mu_b": st.mu_b, "mu_p": st.mu_p,
+                      "Nx": n, "Ny": n, "dx": dx, "dy": dy, "g_y": 1.0, "dt": 0.0, "maxdisp": 0.5, "eps": 1e-300,
+                      "iterMax": iters}
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                if label == "host":
+                    sfhip.solveStokes(st.P, st.Vx, st.Vy, st.rho_vy, st.mu_b, st.mu_p, None, None, None, None, None,
+                                      None, None, st.dVxdtau, st.dVydtau, None, None, st.Vx_small, st.Vy_small, 1.0,
+                                      dx, dy, n, n, 0.0, 0.5, eps=1e-300, iterMax=iters, print_info=False)
+                else:
+                    sfhip.solveStokes_batched([pb], mode="grid")
+                torch.cuda.synchronize()
+                best = min(best, time.perf_counter() - t0)
+            row[label + "_us_per_iter"] = 1e6 * best / iters
+            finals[label] = (st.P.clone(), st.Vx.clone())
+        row["grid_over_host"] = row["grid_us_per_iter"] / row["host_us_per_iter"]
+        row["bitwise_equal"] = all(torch.equal(a, b) for a, b in zip(finals["host"], finals["grid"]))
+        out[str(n)] = row
+        del base, finals
+        torch.cuda.empty_cache()
+    return out
+
+
 def ensemble_specs(n_inst, seed=4000, sizes=(64, 128, 256, 512), log10c=(1.0, 4.0)):
     """The mixed-size plume ensemble: n uniform over ``sizes``, plume centre uniform over
     [0.2,0.8]x[0.6,0.9], radius uniform over [0.05,0.15], plume viscosity 10^-c with c
@@ -522,6 +572,9 @@ def main():
     ap.add_argument("--ens-steps-modes", nargs="+", default=None,
                     choices=["wide_graph", "wide_eager", "wide_refined_graph", "grid_graph", "persistent_graph",
                              "persistent_eager", "loop"], help="run only these forms of the --ens-steps comparison")
+    ap.add_argument("--grid-solve", type=int, nargs="+", default=None, metavar="N",
+                    help="time one problem's PT loop on the host against the grid mode (loop on the device) at "
+                         "these sizes (not in the default line)")
     ap.add_argument("--progress", action="store_true", help="print the solvers' convergence checks (long runs)")
     ap.add_argument("--refined-iters", type=int, default=4000,
                     help="FP32 correction sweeps of the refined solver timed on the same state (roofline_fp32); 0 "
@@ -680,6 +733,8 @@ def main():
     if a.tts_n > 0 and fits("time_to_1e-8", 190.0 * (a.tts_n / 4096.0) ** 3):
         line["time_to_1e-8"] = time_to_solution(a.tts_n, progress=a.progress)
         torch.cuda.empty_cache()
+    if a.grid_solve and fits("grid_solve", 0.0):
+        line["grid_solve"] = grid_solve(tuple(a.grid_solve))
     if a.ens_steps and fits("ensemble_steps", 0.0):
         line["ensemble_steps"] = run_ensemble_steps(*a.ens_steps, modes=a.ens_steps_modes, progress=a.progress,
                                                     eps=a.ens_steps_eps)
