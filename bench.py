@@ -714,8 +714,8 @@ def main():
         note("timestep stages done")
     torch.cuda.empty_cache()
 
-    # the north-star records, each only if the remaining budget holds its expected time (seconds
-    # measured on MI355X: 256-instance ensemble about 90, time to 1e-8 at 4096^2 about 190)
+    # the ensemble and time-to-solution records, each only if the remaining budget holds its expected
+    # time (seconds measured on MI355X: 256-instance ensemble about 90, time to 1e-8 at 4096^2 about 190)
     skipped = []
 
     def fits(name, expect_s):
