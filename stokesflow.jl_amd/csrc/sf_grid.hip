@@ -6,11 +6,12 @@
 // sf_solve_stokes (passes of up to three PT iterations, the single iterations at checks, dt
 // refreshes and iterMax), each wave taking tiles t = wave, wave + G, ... of every pass, and a grid
 // barrier separates the passes.  Small grids get one wave per CU (a wave alone on a SIMD runs its
-// tile faster), larger ones up to eight (fewer tiles per wave).  The dt refresh and the err check reduce over the
-// whole grid in two steps (a partial per wave, then every wave sums the partials in the same
-// order, so every wave takes the same decision).  Each tile is the code of the single solver's
-// pass kernels (sf_pt_tile.h, sf_pt2_tile.h, sf_ptk_tile.h), so every field is bit-identical to
-// sf_solve_stokes; err is a sum of the same terms in another order (equal to rounding).
+// tile faster), larger ones up to eight (fewer tiles per wave).  The dt refresh and the err check
+// reduce over the whole grid in two steps (a partial per wave, then every wave sums the partials
+// in the same order, so every wave takes the same decision).  Each tile is the code of the single
+// solver's pass kernels (sf_pt_tile.h, sf_pt2_tile.h, sf_ptk_tile.h), so every field is
+// bit-identical to sf_solve_stokes; err is a sum of the same terms in another order (equal to
+// rounding).
 //
 // The barrier is two-level, because an agent-scope release writes back the whole L2 of its XCD
 // and agent-scope atomics on one word are serialised (measured with one-wave workgroups: 98 us per
@@ -40,7 +41,7 @@ __device__ inline u64 ld_agent(const u64* p) { return __hip_atomic_load(p, __ATO
 __device__ inline u64 add_agent(u64* p) { return __hip_atomic_fetch_add(p, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 // Polls p until it reaches target; false (and the failure flag set) after the time limit or when
-// another wave has raised the flag.
+// another workgroup has raised the flag.
 __device__ inline bool wait_agent(const u64* p, u64 target, u64* fail, uint64_t limit)
 {
     const uint64_t t0 = wall_clock64();
