@@ -193,6 +193,7 @@ __device__ inline void grid_copy(double* __restrict__ dst, const double* __restr
 
 }  // namespace
 
+// K = PT iterations per pass, 1 to 3 (the four-iteration tile spills beside the loop control).
 template <int K>
 __global__ __launch_bounds__(64 * GRID_WG_WAVES) void grid_solve_kernel(GridArgs a)
 {
@@ -252,9 +253,7 @@ __global__ __launch_bounds__(64 * GRID_WG_WAVES) void grid_solve_kernel(GridArgs
         if (fused) {
             const PTParams GRID_CONST& pp = I.wpt[2 * K - 2 + pf];
             const int ntiles = pp.ntiles, nstrips = pp.nstrips;
-            if constexpr (K == 4) {
-                for (int t = w; t < ntiles; t += nw) ptk_tile<4, 1>(pp, t % nstrips, t / nstrips);
-            } else if constexpr (K == 3) {
+            if constexpr (K == 3) {
                 for (int t = w; t < ntiles; t += nw) ptk_tile<3, 0>(pp, t % nstrips, t / nstrips);
             } else if constexpr (K == 2) {
                 for (int t = w; t < ntiles; t += nw) pt2_tile(pp, t % nstrips, t / nstrips);
