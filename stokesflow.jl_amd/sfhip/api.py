@@ -753,7 +753,7 @@ def saveGrid(path, Nx, Ny, dx, dy, mu_b, P, Vx, Vy):
 
 
 def StokesFlow2D(Nt, Nx, Ny, Lx_glob, Ly_glob, x_m, y_m, rho_m, mu_m, *, stokes_eps=1e-5, g_y=9.81, maxdisp=0.5,
-                 iterMax=100000, print_info=False, state=None, save_to_file=False, out_dir="viz_out"):
+                 iterMax=100000, print_info=False, state=None, save_to_file=False, out_dir="viz_out", captured=False):
     """Timestep loop of StokesFlow2D (StokesFlow2D_multixpu.jl:55-235) at one rank.
 
     The reference evaluates density/viscosity callbacks on a marker lattice it builds
@@ -761,7 +761,8 @@ def StokesFlow2D(Nt, Nx, Ny, Lx_glob, Ly_glob, x_m, y_m, rho_m, mu_m, *, stokes_
     material helpers).  Returns (Vx, Vy, t_tot) with Vx, Vy the padded velocity arrays.
     save_to_file writes the reference's snapshot files into out_dir (stats.mat, and
     markers_%04d_0000.mat / grid_%04d_0000.mat for step 0 and after every step, :137-141,
-    :196-200), read by viz.jl.
+    :196-200), read by viz.jl.  captured=True replays each timestep as one hipGraph with the PT loop
+    on the device (:class:`StepGraph`); the results are the same bits.
     """
     dx = Lx_glob / (Nx - 1)
     dy = Ly_glob / (Ny - 1)
@@ -776,12 +777,21 @@ def StokesFlow2D(Nt, Nx, Ny, Lx_glob, Ly_glob, x_m, y_m, rho_m, mu_m, *, stokes_
         os.makedirs(out_dir, exist_ok=True)
         snapshot(0)
     dt, t_tot = 0.0, 0.0
-    for t in range(Nt):
-        if print_info:
-            print(f"(t, t_tot) = ({t + 1}, {t_tot})")
-        dt, stats = step(st, x_m, y_m, rho_m, mu_m, dx, dy, dt, g_y=g_y, maxdisp=maxdisp, eps=stokes_eps,
-                         iterMax=iterMax)
-        if save_to_file:
-            snapshot(t + 1)
-        t_tot += dt
+    g = (StepGraph(st, x_m, y_m, rho_m, mu_m, dx, dy, 0.0, g_y=g_y, maxdisp=maxdisp, eps=stokes_eps, iterMax=iterMax)
+         if captured else None)
+    try:
+        for t in range(Nt):
+            if print_info:
+                print(f"(t, t_tot) = ({t + 1}, {t_tot})")
+            if g is not None:
+                dt, _ = g.step(1)
+            else:
+                dt, _ = step(st, x_m, y_m, rho_m, mu_m, dx, dy, dt, g_y=g_y, maxdisp=maxdisp, eps=stokes_eps,
+                             iterMax=iterMax)
+            if save_to_file:
+                snapshot(t + 1)
+            t_tot += dt
+    finally:
+        if g is not None:
+            g.close()
     return st.Vx, st.Vy, t_tot
