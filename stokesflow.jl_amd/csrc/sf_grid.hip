@@ -4,8 +4,8 @@
 // every ncheck iterations), which a hipGraph cannot hold.  Here the loop runs on the device: a
 // launch of at most one workgroup per CU, of one to eight waves, walks the same pass schedule as
 // sf_solve_stokes (passes of up to three PT iterations, the single iterations at checks, dt
-// refreshes and iterMax), each wave taking tiles t = wave, wave + G, ... of every pass, and a grid
-// barrier separates the passes.  Small grids get one wave per CU (a wave alone on a SIMD runs its
+// refreshes and iterMax), each wave taking tiles t = band, band + G, ... of every pass (the waves
+// of one XCD hold consecutive bands), and a grid barrier separates the passes.  Small grids get one wave per CU (a wave alone on a SIMD runs its
 // tile faster), larger ones up to eight (fewer tiles per wave).  The dt refresh and the err check
 // reduce over the whole grid in two steps (a partial per wave, then every wave sums the partials
 // in the same order, so every wave takes the same decision).  Each tile is the code of the single
@@ -71,6 +71,7 @@ struct GridBarrier {
     u64 w_here;      // workgroups of this launch on this XCD
     uint64_t limit;  // wall-clock ticks a workgroup may wait
     int xcd;
+    int band;        // workgroups before this one when they are ordered by XCD, then by arrival
 
     // Counts the workgroups per XCD (once, at the start).
     __device__ bool init(u64* sync, uint64_t lim)
@@ -80,20 +81,32 @@ struct GridBarrier {
         xcd = xcd_id();
         gtarget = xtarget = 0ull;
         int ok = 1;
-        u64 nx = 0, here = 0;
+        u64 nx = 0, here = 0, before = 0;
         if (threadIdx.x == 0) {
-            // waiting for the returned count orders this add before the start arrival
-            if (add_agent(s + GRID_SYNC_XCOUNT + 16 * xcd) >= (u64)gridDim.x) ok = 0;
+            // the returned count is this workgroup's rank on its XCD; waiting for it also orders
+            // this add before the start arrival
+            const u64 rank = add_agent(s + GRID_SYNC_XCOUNT + 16 * xcd);
+            if (rank >= (u64)gridDim.x) ok = 0;
             add_agent(s + GRID_SYNC_START);
             ok = wait_agent(s + GRID_SYNC_START, gridDim.x, s + GRID_SYNC_FAIL, limit) && ok;
-            for (int x = 0; x < GRID_MAX_XCD; ++x) nx += ld_agent(s + GRID_SYNC_XCOUNT + 16 * x) > 0ull;
+            for (int x = 0; x < GRID_MAX_XCD; ++x) {
+                const u64 c = ld_agent(s + GRID_SYNC_XCOUNT + 16 * x);
+                nx += c > 0ull;
+                if (x < xcd) before += c;
+            }
             here = ld_agent(s + GRID_SYNC_XCOUNT + 16 * xcd);
+            before += rank;
         }
-        __shared__ u64 sh[2];
-        if (threadIdx.x == 0) { sh[0] = nx; sh[1] = here; }
+        __shared__ u64 sh[3];
+        if (threadIdx.x == 0) {
+            sh[0] = nx;
+            sh[1] = here;
+            sh[2] = before;
+        }
         const int bad = __syncthreads_or(threadIdx.x == 0 && !ok);
         n_xcd = sh[0];
         w_here = sh[1];
+        band = (int)sh[2];
         return bad == 0;
     }
 
@@ -204,6 +217,9 @@ __global__ __launch_bounds__(64 * GRID_WG_WAVES) void grid_solve_kernel(GridArgs
     const int64_t gtid = (int64_t)w * 64 + (threadIdx.x & 63), gstride = (int64_t)nw * 64;
     GridBarrier bar;
     if (!bar.init(a.sync, a.limit)) return;
+    // the tiles of a pass go out in bands: the waves of one XCD take consecutive tiles, so the
+    // strips that share halo columns mostly share an L2
+    const int wt = __builtin_amdgcn_readfirstlane(bar.band * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
     double* part_a = a.part;           // nw partials (max of |Vx|, sum of |Rx|)
     double* part_b = a.part + nw;      // |Vy|, |Ry|
     double* part_c = a.part + 2 * nw;  // |divV|
@@ -254,9 +270,9 @@ __global__ __launch_bounds__(64 * GRID_WG_WAVES) void grid_solve_kernel(GridArgs
             const PTParams GRID_CONST& pp = I.wpt[2 * K - 2 + pf];
             const int ntiles = pp.ntiles, nstrips = pp.nstrips;
             if constexpr (K == 3) {
-                for (int t = w; t < ntiles; t += nw) ptk_tile<3, 0>(pp, t % nstrips, t / nstrips);
+                for (int t = wt; t < ntiles; t += nw) ptk_tile<3, 0>(pp, t % nstrips, t / nstrips);
             } else if constexpr (K == 2) {
-                for (int t = w; t < ntiles; t += nw) pt2_tile(pp, t % nstrips, t / nstrips);
+                for (int t = wt; t < ntiles; t += nw) pt2_tile(pp, t % nstrips, t / nstrips);
             }
             pf ^= 1;
             if (!bar.sync()) return;
@@ -275,9 +291,9 @@ __global__ __launch_bounds__(64 * GRID_WG_WAVES) void grid_solve_kernel(GridArgs
         const int flags = ((check || last) ? PT_WRITE_TAU : 0) | (check ? PT_WRITE_RX : 0);
         const int ntiles = pp.ntiles, nstrips = pp.nstrips;
         if (flags) {
-            for (int t = w; t < ntiles; t += nw) pt_tile<1, true>(pp, t % nstrips, t / nstrips, flags);
+            for (int t = wt; t < ntiles; t += nw) pt_tile<1, true>(pp, t % nstrips, t / nstrips, flags);
         } else {
-            for (int t = w; t < ntiles; t += nw) pt_tile<1, false>(pp, t % nstrips, t / nstrips, 0);
+            for (int t = wt; t < ntiles; t += nw) pt_tile<1, false>(pp, t % nstrips, t / nstrips, 0);
         }
         pf ^= 1;
         if (!bar.sync()) return;
