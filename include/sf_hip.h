@@ -382,8 +382,11 @@ int sf_ensemble_destroy(sf_ensemble *ens);
  * device, so sf_graph_step replays n_steps timesteps with no host synchronisation in between.
  * args as for sf_step (its dt is the first step's input dt); iters_per_pass 1 to 3 (0: 3).
  * Fields, markers and dt are bit-identical to n_steps sf_step calls; err agrees to rounding
- * (its sum runs in another order).  The arrays must outlive the graph.  out / dt (may be NULL)
- * receive the last step's solve results and the dt after it.  Replaces sf_step's host loop; the
+ * (its sum runs in another order).  The arrays must outlive the graph; it runs on a stream of its
+ * own, ordered after the context's stream at each call, and sf_graph_step returns once the steps
+ * are done.  out / dt (may be NULL) receive the last step's solve results and the dt after it.
+ * Other work on the GPU while a step runs can keep the solve's workgroups from being resident
+ * together; the step then returns SF_ERR_HIP after a 5 s wait.  Replaces sf_step's host loop; the
  * reference has no counterpart (its loop is host-driven, StokesFlow2D_multixpu.jl:157-218). */
 typedef struct sf_graph sf_graph;
 int sf_graph_create(sf_ctx *ctx, const sf_step_args *args, int iters_per_pass, sf_graph **g);
