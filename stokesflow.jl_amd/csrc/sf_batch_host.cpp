@@ -1034,6 +1034,7 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
         rc = grid_reserve(c, n);
         if (rc) return rc;
         const int k = (prm && prm->depth >= 1 && prm->depth <= 3) ? prm->depth : 3;
+        HIPCHK(hipMemsetAsync(c->g_sync, 0, sizeof(unsigned long long) * GRID_SYNC_WORDS * (size_t)n, s));
         for (int qi = 0; qi < n; ++qi) {
             const int i = order[(size_t)qi];
             const GridShape gs = grid_launch_shape(host[(size_t)i], k);
@@ -1047,8 +1048,8 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
                               hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         for (int i = 0; i < n; ++i)
-            if (sync[(size_t)GRID_SYNC_WORDS * i + GRID_SYNC_FAIL])
-                return set_err(SF_ERR_HIP, "%s: instance %d: a grid barrier wait exceeded %d s (waves not all resident?)",
+            if (sync[(size_t)GRID_SYNC_WORDS * i + GRID_SYNC_STICKY])
+                return set_err(SF_ERR_HIP, "%s: instance %d: a grid barrier wait exceeded %d s (workgroups not all resident?)",
                                who, i, (int)GRID_WAIT_SECONDS);
     } else if (mode == SF_BATCH_WIDE || mode == SF_BATCH_AUTO) {
         // a working set past the Infinity Cache still pays with multi-iteration passes: 256 mixed

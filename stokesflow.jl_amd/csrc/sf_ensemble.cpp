@@ -295,6 +295,8 @@ extern "C" int sf_ensemble_create(sf_ctx* c, const sf_step_args* members, int n,
     // make the caller's earlier work on its stream visible, then upload on our stream
     er = hipEventRecord(e->ev, c->stream);
     if (er == hipSuccess) er = hipStreamWaitEvent(e->stream, e->ev, 0);
+    if (er == hipSuccess && e->grid)  // the sticky failure flags start clear (outside any graph)
+        er = hipMemsetAsync(e->gsync, 0, sizeof(unsigned long long) * GRID_SYNC_WORDS * (size_t)n, e->stream);
     if (er == hipSuccess)
         er = hipMemcpyAsync(e->inst, e->desc.data(), sizeof(BatchInstance) * (size_t)n, hipMemcpyHostToDevice, e->stream);
     if (er == hipSuccess)
@@ -366,8 +368,8 @@ extern "C" int sf_ensemble_step(sf_ensemble* e, int n_steps, sf_solve_out* outs,
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     if (n_steps > 0 && e->grid)
         for (int i = 0; i < e->n; ++i)
-            if (e->hsync[(size_t)GRID_SYNC_WORDS * i + GRID_SYNC_FAIL])
-                return set_err(SF_ERR_HIP, "%s: member %d: a grid barrier wait exceeded %d s (waves not all resident?)",
+            if (e->hsync[(size_t)GRID_SYNC_WORDS * i + GRID_SYNC_STICKY])
+                return set_err(SF_ERR_HIP, "%s: member %d: a grid barrier wait exceeded %d s (workgroups not all resident?)",
                                who, i, (int)GRID_WAIT_SECONDS);
     if (n_steps > 0)
         for (int i = 0; i < e->n; ++i) {

@@ -40,15 +40,16 @@ typedef unsigned long long u64;
 __device__ inline u64 ld_agent(const u64* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ inline u64 add_agent(u64* p) { return __hip_atomic_fetch_add(p, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
-// Polls p until it reaches target; false (and the failure flag set) after the time limit or when
-// another workgroup has raised the flag.
-__device__ inline bool wait_agent(const u64* p, u64 target, u64* fail, uint64_t limit)
+// Polls p until it reaches target; false after the time limit (the failure flag and its sticky
+// copy then set) or when another workgroup has raised the flag.  s: the launch's sync words.
+__device__ inline bool wait_agent(const u64* p, u64 target, u64* s, uint64_t limit)
 {
     const uint64_t t0 = wall_clock64();
     while (ld_agent(p) < target) {
-        if (ld_agent(fail) != 0ull) return false;
+        if (ld_agent(s + GRID_SYNC_FAIL) != 0ull) return false;
         if (wall_clock64() - t0 > limit) {
-            __hip_atomic_store(fail, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(s + GRID_SYNC_FAIL, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(s + GRID_SYNC_STICKY, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
         }
         __builtin_amdgcn_s_sleep(2);
@@ -88,7 +89,7 @@ struct GridBarrier {
             const u64 rank = add_agent(s + GRID_SYNC_XCOUNT + 16 * xcd);
             if (rank >= (u64)gridDim.x) ok = 0;
             add_agent(s + GRID_SYNC_START);
-            ok = wait_agent(s + GRID_SYNC_START, gridDim.x, s + GRID_SYNC_FAIL, limit) && ok;
+            ok = wait_agent(s + GRID_SYNC_START, gridDim.x, s, limit) && ok;
             for (int x = 0; x < GRID_MAX_XCD; ++x) {
                 const u64 c = ld_agent(s + GRID_SYNC_XCOUNT + 16 * x);
                 nx += c > 0ull;
@@ -121,7 +122,7 @@ struct GridBarrier {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this XCD's L2 written back
                 add_agent(s + GRID_SYNC_ARRIVE);
             }
-            ok = wait_agent(s + GRID_SYNC_ARRIVE, gtarget, s + GRID_SYNC_FAIL, limit);
+            ok = wait_agent(s + GRID_SYNC_ARRIVE, gtarget, s, limit);
         }
         if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         return __syncthreads_or(threadIdx.x == 0 && !ok) == 0;
@@ -366,9 +367,11 @@ __global__ __launch_bounds__(64 * GRID_WG_WAVES) void grid_solve_kernel(GridArgs
     }
 }
 
+// Every word but the sticky failure flag, which only the host clears.
 __global__ void grid_zero_sync(unsigned long long* p)
 {
-    for (int k = threadIdx.x; k < GRID_SYNC_WORDS; k += blockDim.x) p[k] = 0ull;
+    for (int k = threadIdx.x; k < GRID_SYNC_WORDS; k += blockDim.x)
+        if (k != GRID_SYNC_STICKY) p[k] = 0ull;
 }
 
 // Resident workgroups of `waves` waves of grid_solve_kernel<k> on the current device, at most

@@ -186,7 +186,8 @@ struct BatchResult {
 // waves (at most one workgroup per CU, grid_capacity(k, wg_waves), all resident), the passes
 // separated by a grid barrier.  sync holds GRID_SYNC_WORDS words, zeroed by launch_grid_solve:
 // [GRID_SYNC_ARRIVE] XCDs arrived, [GRID_SYNC_FAIL] failure flag (a barrier wait exceeded the
-// limit), [GRID_SYNC_START] workgroups started, [GRID_SYNC_XARRIVE + 16 x] workgroups of XCD x
+// limit), [GRID_SYNC_STICKY] its copy, which launch_grid_solve leaves alone (the host clears it),
+// [GRID_SYNC_START] workgroups started, [GRID_SYNC_XARRIVE + 16 x] workgroups of XCD x
 // arrived, [GRID_SYNC_XCOUNT + 16 x] workgroups on XCD x (one 128-byte line each); part holds
 // 3 * wg_waves * nwg partials.
 enum {
@@ -197,6 +198,7 @@ enum {
     GRID_SYNC_ARRIVE = 0,
     GRID_SYNC_FAIL = 1,
     GRID_SYNC_START = 2,
+    GRID_SYNC_STICKY = 3,
     GRID_SYNC_XARRIVE = 16,
     GRID_SYNC_XCOUNT = 16 + 16 * GRID_MAX_XCD,
     GRID_SYNC_WORDS = 16 + 32 * GRID_MAX_XCD,
