@@ -388,6 +388,12 @@ int sf_ensemble_destroy(sf_ensemble *ens);
  * Other work on the GPU while a step runs can keep the solve's workgroups from being resident
  * together; the step then returns SF_ERR_HIP after a 5 s wait.  Replaces sf_step's host loop; the
  * reference has no counterpart (its loop is host-driven, StokesFlow2D_multixpu.jl:157-218). */
+/* Time limit of one grid-barrier wait in the GRID mode (default 5 s): a launch whose wait exceeds
+ * it ends and its call returns SF_ERR_HIP.  For grids whose passes take longer, or to test the
+ * error path (0: every wait that does not find its barrier complete fails at once).  Launches
+ * already enqueued or captured keep the limit they were made with.  No reference counterpart. */
+int sf_set_grid_wait_limit(double seconds);
+
 typedef struct sf_graph sf_graph;
 int sf_graph_create(sf_ctx *ctx, const sf_step_args *args, int iters_per_pass, sf_graph **g);
 int sf_graph_step(sf_graph *g, int n_steps, sf_solve_out *out, double *dt);

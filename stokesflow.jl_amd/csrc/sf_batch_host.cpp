@@ -218,7 +218,19 @@ GridShape grid_launch_shape(const BatchInstance& d, int k)
     return GridShape{nwg, waves};
 }
 
-// Grid mode: the barrier words (two per instance) and the partials of the context.
+}  // namespace sf
+
+extern "C" int sf_set_grid_wait_limit(double seconds)
+{
+    if (!(seconds >= 0.0) || seconds > 3600.0)
+        return set_err(SF_ERR_ARG, "sf_set_grid_wait_limit: %g s is not in [0, 3600]", seconds);
+    grid_set_wait_seconds(seconds);
+    return SF_OK;
+}
+
+namespace sf {
+
+// Grid mode: the barrier words and the partials of the context.
 static int grid_reserve(sf_ctx* c, int n)
 {
     if (!c->g_part) HIPCHK(dalloc(&c->g_part, 3 * (size_t)GRID_MAX_WAVES));

@@ -26,6 +26,8 @@
 // sizing the launch from the kernel's occupancy; should one not be, nothing waits forever: a wait
 // longer than the time limit (or a flag raised by another workgroup) sets the failure flag, every
 // workgroup leaves the kernel, and the host reports the error.
+#include <atomic>
+
 #include "sf_ptk_tile.h"
 
 namespace sf {
@@ -395,17 +397,22 @@ int grid_capacity(int k, int waves)
     return std::max(cap[k][waves], 0);
 }
 
+static std::atomic<double> grid_wait_s{(double)GRID_WAIT_SECONDS};
+
 uint64_t grid_wait_limit()
 {
-    static const uint64_t lim = [] {
-        int dev = 0, khz = 0;
+    static const uint64_t khz = [] {
+        int dev = 0, k = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz < 1)
-            khz = 100000;
-        return (uint64_t)khz * 1000ull * GRID_WAIT_SECONDS;
+            hipDeviceGetAttribute(&k, hipDeviceAttributeWallClockRate, dev) != hipSuccess || k < 1)
+            k = 100000;
+        return (uint64_t)k;
     }();
-    return lim;
+    return (uint64_t)(grid_wait_s.load() * 1000.0 * (double)khz);
 }
+
+// Launches already enqueued (or captured in a graph) keep the limit they were made with.
+void grid_set_wait_seconds(double seconds) { grid_wait_s.store(seconds); }
 
 hipError_t launch_grid_solve(const GridArgs& a, int k, hipStream_t s)
 {

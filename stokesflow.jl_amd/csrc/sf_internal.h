@@ -214,7 +214,8 @@ struct GridArgs {
 // k = PT iterations per pass (1, 2 or 3; four need more than 256 VGPRs in this kernel); waves =
 // waves per workgroup (1 to GRID_WG_WAVES).
 int grid_capacity(int k, int waves);
-uint64_t grid_wait_limit();
+uint64_t grid_wait_limit();  // wall-clock ticks of the current limit (sf_set_grid_wait_limit)
+void grid_set_wait_seconds(double seconds);
 hipError_t launch_grid_solve(const GridArgs& a, int k, hipStream_t s);
 
 // Wide mode of the batched solver: the active set of one PT-iteration launch.

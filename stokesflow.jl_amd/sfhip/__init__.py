@@ -9,7 +9,7 @@ from .api import (  # noqa: F401
     init_markers, marker_cell_keys, markers_to_grids3, moveMarkersRK4, probe_dram, plume_properties, pt_iterations, rt_properties, solveStokes, solveStokes_batched,
     saveGrid, saveMarkers, saveStats, set_pt_fusion, set_pt_tuning, step, to_numpy, time_passes, pass_times,
     Materials, autotune_batched, clear_batched_tuning, initializeMarkers, lattice_markers, marker_properties, setInitialMarkerProperties,
-    StepGraph,
+    StepGraph, set_grid_wait_limit,
 )
 
 __version__ = "0.1.0"

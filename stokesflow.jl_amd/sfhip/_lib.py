@@ -27,7 +27,7 @@ EXPORTS = (
     "sf_marker_cell_keys", "sf_ensemble_create", "sf_ensemble_step", "sf_ensemble_destroy",
     "sf_ctx_autotune_pt", "sf_probe_dram", "sf_markers_to_grids3", "sf_ctx_time_passes", "sf_ctx_pass_times",
     "sf_solve_stokes_noalloc", "sf_lattice_markers", "sf_marker_properties", "sf_batch_autotune",
-    "sf_batch_tuning_clear", "sf_graph_create", "sf_graph_step", "sf_graph_destroy",
+    "sf_batch_tuning_clear", "sf_graph_create", "sf_graph_step", "sf_graph_destroy", "sf_set_grid_wait_limit",
 )
 
 
@@ -193,6 +193,7 @@ def load():
     L.sf_graph_create.argtypes = [_vp, C.POINTER(StepArgs), _i, C.POINTER(_vp)]
     L.sf_graph_step.argtypes = [_vp, _i, C.POINTER(SolveOut), C.POINTER(_d)]
     L.sf_graph_destroy.argtypes = [_vp]
+    L.sf_set_grid_wait_limit.argtypes = [_d]
     L.sf_marker_cell_keys.argtypes = [_vp, _d, _d, _i, _i, _vp, _vp, _i64, _d, _d, _vp, _vp, C.POINTER(M2GOut)]
     L.sf_move_markers_rk4.argtypes = [_vp, _vp, _vp, _i64, _vp, _i, _i, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d,
                                       _d, _d, C.POINTER(RK4Out)]

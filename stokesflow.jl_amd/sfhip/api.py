@@ -628,6 +628,11 @@ def step(state, x_m, y_m, rho_m, mu_m, dx, dy, dt, *, g_y=9.81, maxdisp=0.5, eps
     return a.dt, stats
 
 
+def set_grid_wait_limit(seconds):
+    """Time limit of one grid-barrier wait in the grid mode (sf_set_grid_wait_limit; default 5 s)."""
+    check(_lib.load().sf_set_grid_wait_limit(float(seconds)))
+
+
 class StepGraph:
     """The captured timestep of one StokesFlow2D problem (sf_graph_*): the loop body of
     StokesFlow2D_multixpu.jl:164-181 as one hipGraph, the solve with its PT loop on the device

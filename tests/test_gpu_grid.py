@@ -142,3 +142,25 @@ def test_stokesflow2d_captured_matches_eager(gpu):
     assert a_t == b_t
     assert np.array_equal(a_vx, b_vx) and np.array_equal(a_vy, b_vy)
     assert np.array_equal(a_x, b_x) and np.array_equal(a_y, b_y)
+
+
+def test_grid_wait_limit_error_path(gpu):
+    """A zero wait limit makes the first barrier wait that finds its barrier incomplete fail: every
+    workgroup leaves the kernel and the call returns an error; the default limit then solves
+    normally (the failure is forced from the host; no kernel waits without bound)."""
+    nx, ny = 600, 520
+    S = cases.smooth_state(nx, ny, seed=630)
+    dx, dy = 1.0 / (nx - 1), 1.0 / (ny - 1)
+    pb, _ = problem(S, nx, ny, dx, dy, iterMax=10)
+    try:
+        sfhip.set_grid_wait_limit(0.0)
+        with pytest.raises(sfhip.SFError):
+            sfhip.solveStokes_batched([pb], mode="grid")
+    finally:
+        sfhip.set_grid_wait_limit(5.0)
+    with pytest.raises(sfhip.SFError):
+        sfhip.set_grid_wait_limit(-1.0)
+    pb, G = problem(S, nx, ny, dx, dy, iterMax=10)
+    outs = sfhip.solveStokes_batched([pb], mode="grid")
+    torch.cuda.synchronize()
+    check_against_single([(nx, ny, S, dx, dy)], outs, [G], iterMax=10)
