@@ -1047,11 +1047,13 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
         if (rc) return rc;
         const int k = (prm && prm->depth >= 1 && prm->depth <= 3) ? prm->depth : 3;
         HIPCHK(hipMemsetAsync(c->g_sync, 0, sizeof(unsigned long long) * GRID_SYNC_WORDS * (size_t)n, s));
+        const double wait_s = grid_wait_seconds();
+        const uint64_t wait_ticks = grid_wait_limit();
         for (int qi = 0; qi < n; ++qi) {
             const int i = order[(size_t)qi];
             const GridShape gs = grid_launch_shape(host[(size_t)i], k);
             const GridArgs ga{c->b_inst + i, c->b_res + i, c->g_sync + (size_t)GRID_SYNC_WORDS * i, c->g_part,
-                              gs.nwg, gs.waves, grid_wait_limit()};
+                              gs.nwg, gs.waves, wait_ticks};
             HIPCHK(launch_grid_solve(ga, k, s));
         }
         std::vector<unsigned long long> sync((size_t)GRID_SYNC_WORDS * n);
@@ -1061,8 +1063,8 @@ extern "C" int sf_solve_stokes_batched(sf_ctx* c, const sf_batch_instance* in, i
         HIPCHK(hipStreamSynchronize(s));
         for (int i = 0; i < n; ++i)
             if (sync[(size_t)GRID_SYNC_WORDS * i + GRID_SYNC_STICKY])
-                return set_err(SF_ERR_HIP, "%s: instance %d: a grid barrier wait exceeded %d s (workgroups not all resident?)",
-                               who, i, (int)GRID_WAIT_SECONDS);
+                return set_err(SF_ERR_HIP, "%s: instance %d: a grid barrier wait exceeded %g s (workgroups not all resident?)",
+                               who, i, wait_s);
     } else if (mode == SF_BATCH_WIDE || mode == SF_BATCH_AUTO) {
         // a working set past the Infinity Cache still pays with multi-iteration passes: 256 mixed
         // 64..512 instances, one box, one process: 17.69 s at 256 MiB, 16.93 at 384, 16.24 at 768, 15.35-

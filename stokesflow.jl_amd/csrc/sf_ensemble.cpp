@@ -369,8 +369,10 @@ extern "C" int sf_ensemble_step(sf_ensemble* e, int n_steps, sf_solve_out* outs,
     if (n_steps > 0 && e->grid)
         for (int i = 0; i < e->n; ++i)
             if (e->hsync[(size_t)GRID_SYNC_WORDS * i + GRID_SYNC_STICKY])
-                return set_err(SF_ERR_HIP, "%s: member %d: a grid barrier wait exceeded %d s (workgroups not all resident?)",
-                               who, i, (int)GRID_WAIT_SECONDS);
+                return set_err(SF_ERR_HIP,
+                               "%s: member %d: a grid barrier wait exceeded the limit its launch was made with "
+                               "(workgroups not all resident?)",
+                               who, i);
     if (n_steps > 0)
         for (int i = 0; i < e->n; ++i) {
             const unsigned long long* q = e->hcnt + (size_t)ENS_CNT * i;

@@ -413,6 +413,7 @@ uint64_t grid_wait_limit()
 
 // Launches already enqueued (or captured in a graph) keep the limit they were made with.
 void grid_set_wait_seconds(double seconds) { grid_wait_s.store(seconds); }
+double grid_wait_seconds() { return grid_wait_s.load(); }
 
 hipError_t launch_grid_solve(const GridArgs& a, int k, hipStream_t s)
 {

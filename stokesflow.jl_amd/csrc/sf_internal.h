@@ -216,6 +216,7 @@ struct GridArgs {
 int grid_capacity(int k, int waves);
 uint64_t grid_wait_limit();  // wall-clock ticks of the current limit (sf_set_grid_wait_limit)
 void grid_set_wait_seconds(double seconds);
+double grid_wait_seconds();  // the current limit in seconds
 hipError_t launch_grid_solve(const GridArgs& a, int k, hipStream_t s);
 
 // Wide mode of the batched solver: the active set of one PT-iteration launch.

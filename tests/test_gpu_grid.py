@@ -154,7 +154,7 @@ def test_grid_wait_limit_error_path(gpu):
     pb, _ = problem(S, nx, ny, dx, dy, iterMax=10)
     try:
         sfhip.set_grid_wait_limit(0.0)
-        with pytest.raises(sfhip.SFError):
+        with pytest.raises(sfhip.SFError, match="exceeded 0 s"):
             sfhip.solveStokes_batched([pb], mode="grid")
     finally:
         sfhip.set_grid_wait_limit(5.0)
