@@ -232,8 +232,8 @@ int sf_solve_stokes_refined(sf_ctx *ctx, double *P, double *Vx_pad, double *Vy_p
  * one workgroup of one to eight waves per CU): the whole PT loop with its checks, dt refreshes
  * and convergence test runs on the device, the passes (up to three PT iterations each, or
  * prm->depth) separated by a grid barrier; no host synchronisation, so it can be captured
- * (sf_graph_*).  A barrier wait longer than 5 s (workgroups not all resident) ends the launch
- * and the call returns SF_ERR_HIP.
+ * (sf_graph_*).  A barrier wait longer than the limit (5 s unless sf_set_grid_wait_limit sets
+ * another; workgroups not all resident) ends the launch and the call returns SF_ERR_HIP.
  * With precision 1 (WIDE only) every instance is solved as sf_solve_stokes_refined solves it,
  * with one launch per FP32 sweep of all active instances.
  * In FP64 every PT iteration is bit-identical to sf_solve_stokes.  outs[i] gets instance i's dt_new,
@@ -386,7 +386,8 @@ int sf_ensemble_destroy(sf_ensemble *ens);
  * own, ordered after the context's stream at each call, and sf_graph_step returns once the steps
  * are done.  out / dt (may be NULL) receive the last step's solve results and the dt after it.
  * Other work on the GPU while a step runs can keep the solve's workgroups from being resident
- * together; the step then returns SF_ERR_HIP after a 5 s wait.  Replaces sf_step's host loop; the
+ * together; the step then returns SF_ERR_HIP after the barrier wait limit (5 s by default,
+ * fixed when the graph is created).  Replaces sf_step's host loop; the
  * reference has no counterpart (its loop is host-driven, StokesFlow2D_multixpu.jl:157-218). */
 /* Time limit of one grid-barrier wait in the GRID mode (default 5 s): a launch whose wait exceeds
  * it ends and its call returns SF_ERR_HIP.  For grids whose passes take longer, or to test the
